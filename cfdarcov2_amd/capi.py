@@ -1,0 +1,297 @@
+"""ctypes binding of the C-ABI in include/cfdarco.h (libcfdarco_hip.so).
+
+PyTorch supplies device memory and streams only: field arrays are CUDA
+(HIP) tensors whose data pointers go straight to the C-ABI, and calls are
+ordered on torch's current stream.  There is no CPU fallback: if the HIP
+library is missing or no GPU is visible, loading fails with an error.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcfdarco_hip.so")
+
+F32, F64 = 0, 1
+H2D, D2H, D2D = 1, 2, 3
+
+EW = {"add": 0, "sub": 1, "mul": 2, "div": 3, "neg": 4, "muls": 5, "divs": 6, "smul": 7,
+      "axpy": 8, "leap": 9, "fill": 10, "copy": 11}
+
+TABLES = {"ids": 0, "ids_bf": 1, "nx": 2, "ny": 3, "nz": 4, "nax": 5, "nay": 6, "naz": 7,
+          "area": 8, "alpha": 9, "len": 10, "vol": 11, "is_bnd": 12, "ccx": 13, "ccy": 14, "ccz": 15}
+
+# Every symbol declared in include/cfdarco.h (checked by the CPU test suite).
+EXPORTS = [
+    "cfd_init", "cfd_destroy", "cfd_last_error", "cfd_version",
+    "cfd_alloc", "cfd_free", "cfd_pool_set_limit", "cfd_pool_stats",
+    "cfd_memcpy", "cfd_stream_sync",
+    "cfd_mesh_create_hex", "cfd_mesh_destroy", "cfd_mesh_get_info", "cfd_mesh_table", "cfd_hex_table",
+    "cfd_grad", "cfd_lapl", "cfd_stab", "cfd_ewise", "cfd_cfl_min", "cfd_count_nonfinite",
+    "cfd_heat_run", "cfd_wave_run",
+]
+
+
+class CfdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class HexDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("dtype", ctypes.c_int), ("force_general", ctypes.c_int),
+                ("vertex_offsets", ctypes.c_void_p)]
+
+
+class MeshInfo(ctypes.Structure):
+    _fields_ = [("n_cells", ctypes.c_int64), ("n_faces", ctypes.c_int64), ("n_vertices", ctypes.c_int64),
+                ("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64),
+                ("dtype", ctypes.c_int), ("uniform", ctypes.c_int), ("device_bytes", ctypes.c_int64)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libcfdarco_hip.so and declare argument types (no device calls)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library not built: {path} is missing (run __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    vp, i64, i, d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "cfd_init": [i, pp], "cfd_destroy": [vp], "cfd_version": [],
+        "cfd_alloc": [vp, i64, pp, vp], "cfd_free": [vp, vp, vp],
+        "cfd_pool_set_limit": [vp, i64], "cfd_pool_stats": [vp, vp, vp],
+        "cfd_memcpy": [vp, vp, vp, i64, i, vp], "cfd_stream_sync": [vp],
+        "cfd_mesh_create_hex": [vp, ctypes.POINTER(HexDesc), pp], "cfd_mesh_destroy": [vp],
+        "cfd_mesh_get_info": [vp, ctypes.POINTER(MeshInfo)], "cfd_mesh_table": [vp, i, vp],
+        "cfd_hex_table": [ctypes.POINTER(HexDesc), i, vp],
+        "cfd_grad": [vp, vp, vp, i, i, vp], "cfd_lapl": [vp, vp, vp, i, vp], "cfd_stab": [vp, vp, vp, i, vp],
+        "cfd_ewise": [vp, i, i, vp, vp, vp, d, vp, i64, vp],
+        "cfd_cfl_min": [vp, vp, vp, vp, vp, vp, d, d, ctypes.POINTER(ctypes.c_double), vp],
+        "cfd_count_nonfinite": [vp, i, vp, i64, ctypes.POINTER(ctypes.c_int64), vp],
+        "cfd_heat_run": [vp, vp, vp, d, vp, d, i64, vp],
+        "cfd_wave_run": [vp, ctypes.c_void_p * 4, ctypes.c_void_p * 4, d, vp, d, i64, vp],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    L.cfd_last_error.restype = ctypes.c_char_p
+    L.cfd_last_error.argtypes = []
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != 0:
+        raise CfdError(rc, load_library().cfd_last_error().decode(errors="replace"))
+
+
+def _dtype_code(dtype) -> int:
+    import torch
+    if dtype in (np.float32, torch.float32, "f32", "float32"):
+        return F32
+    if dtype in (np.float64, torch.float64, "f64", "float64"):
+        return F64
+    raise ValueError(f"unsupported dtype {dtype!r}")
+
+
+def hex_table(nx, ny, nz, dx, dy, dz, dtype, name, vertex_offsets=None):
+    """Reference-layout mesh table computed on the host (no device needed)."""
+    L = load_library()
+    code = _dtype_code(dtype)
+    off = None
+    if vertex_offsets is not None:
+        off = np.ascontiguousarray(vertex_offsets, dtype=np.float64)
+    desc = HexDesc(nx, ny, nz, dx, dy, dz, code, 0, None if off is None else off.ctypes.data)
+    n = nx * ny * nz
+    tid = TABLES[name]
+    np_dt = np.int64 if name in ("ids", "ids_bf") else (np.float32 if code == F32 else np.float64)
+    count = n if name in ("vol", "is_bnd", "ccx", "ccy", "ccz") else 6 * n
+    out = np.empty(count, dtype=np_dt)
+    check(L.cfd_hex_table(ctypes.byref(desc), tid, out.ctypes.data))
+    return out.reshape(6, n) if count == 6 * n else out
+
+
+def _stream_ptr():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class Context:
+    """cfd_init / cfd_destroy on one device."""
+
+    def __init__(self, device: int = 0):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU visible: the HIP path has no CPU fallback")
+        self.lib = load_library()
+        self.device = device
+        h = ctypes.c_void_p()
+        check(self.lib.cfd_init(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(self.lib.cfd_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pool_set_limit(self, nbytes: int):
+        check(self.lib.cfd_pool_set_limit(self.h, int(nbytes)))
+
+    def pool_stats(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        check(self.lib.cfd_pool_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def alloc(self, nbytes: int):
+        p = ctypes.c_void_p()
+        check(self.lib.cfd_alloc(self.h, int(nbytes), ctypes.byref(p), _stream_ptr()))
+        return p
+
+    def free(self, p):
+        check(self.lib.cfd_free(self.h, p, _stream_ptr()))
+
+    def ewise(self, op: str, a=None, b=None, aux=None, s=0.0, out=None, n=None):
+        import torch
+        ref = out if out is not None else a
+        code = _dtype_code(ref.dtype)
+        n = ref.numel() if n is None else n
+        check(self.lib.cfd_ewise(self.h, EW[op], code, _dptr(a), _dptr(b), _dptr(aux), float(s), _dptr(out),
+                                 int(n), _stream_ptr()))
+        return out
+
+    def count_nonfinite(self, x) -> int:
+        bad = ctypes.c_int64()
+        check(self.lib.cfd_count_nonfinite(self.h, _dtype_code(x.dtype), _dptr(x), x.numel(), ctypes.byref(bad),
+                                           _stream_ptr()))
+        return bad.value
+
+
+@dataclass
+class MeshShape:
+    nx: int
+    ny: int
+    nz: int
+
+
+class HexMesh:
+    """cfd_mesh_create_hex: reference geometry of Mesh3D(nx,ny,nz,dx,dy,dz)."""
+
+    def __init__(self, ctx: Context, nx, ny, nz, dx, dy, dz, dtype="f64", vertex_offsets=None,
+                 force_general=False):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.code = _dtype_code(dtype)
+        self.shape = MeshShape(int(nx), int(ny), int(nz))
+        self.spacing = (float(dx), float(dy), float(dz))
+        self.N = int(nx) * int(ny) * int(nz)
+        off = None
+        if vertex_offsets is not None:
+            off = np.ascontiguousarray(vertex_offsets, dtype=np.float64)
+            assert off.size == 3 * (nx + 1) * (ny + 1) * (nz + 1)
+        self._off = off
+        desc = HexDesc(int(nx), int(ny), int(nz), float(dx), float(dy), float(dz), self.code, int(force_general),
+                       None if off is None else off.ctypes.data)
+        h = ctypes.c_void_p()
+        check(self.lib.cfd_mesh_create_hex(ctx.h, ctypes.byref(desc), ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def torch_dtype(self):
+        import torch
+        return torch.float32 if self.code == F32 else torch.float64
+
+    def info(self) -> MeshInfo:
+        inf = MeshInfo()
+        check(self.lib.cfd_mesh_get_info(self.h, ctypes.byref(inf)))
+        return inf
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(self.lib.cfd_mesh_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def table(self, name):
+        n = self.N
+        np_dt = np.int64 if name in ("ids", "ids_bf") else (np.float32 if self.code == F32 else np.float64)
+        count = n if name in ("vol", "is_bnd", "ccx", "ccy", "ccz") else 6 * n
+        out = np.empty(count, dtype=np_dt)
+        check(self.lib.cfd_mesh_table(self.h, TABLES[name], out.ctypes.data))
+        return out.reshape(6, n) if count == 6 * n else out
+
+    def _check_field(self, t):
+        assert t.is_cuda and t.dtype == self.torch_dtype and t.is_contiguous() and t.numel() == self.N, \
+            f"field must be a contiguous {self.torch_dtype} CUDA tensor of {self.N} cells"
+
+    def grad(self, phi, axis_mask=7, wall_axis=-1, out=None):
+        import torch
+        self._check_field(phi)
+        out = torch.empty_like(phi) if out is None else out
+        check(self.lib.cfd_grad(self.h, _dptr(phi), _dptr(out), int(axis_mask), int(wall_axis), _stream_ptr()))
+        return out
+
+    def lapl(self, phi, axis_mask=7, out=None):
+        import torch
+        self._check_field(phi)
+        out = torch.empty_like(phi) if out is None else out
+        check(self.lib.cfd_lapl(self.h, _dptr(phi), _dptr(out), int(axis_mask), _stream_ptr()))
+        return out
+
+    def stab(self, phi, axis_mask=7, out=None):
+        import torch
+        self._check_field(phi)
+        out = torch.empty_like(phi) if out is None else out
+        check(self.lib.cfd_stab(self.h, _dptr(phi), _dptr(out), int(axis_mask), _stream_ptr()))
+        return out
+
+    def cfl_min(self, u, p, rho, v=None, w=None, cfl=1.0, gamma=5.0 / 3.0) -> float:
+        dt = ctypes.c_double()
+        check(self.lib.cfd_cfl_min(self.h, _dptr(u), _dptr(v), _dptr(w), _dptr(p), _dptr(rho), float(cfl),
+                                   float(gamma), ctypes.byref(dt), _stream_ptr()))
+        return dt.value
+
+    def heat_run(self, T_a, T_b, alpha, dt, nsteps, alpha_field=None):
+        """Fused heat steps; returns the tensor holding the final state."""
+        for t in (T_a, T_b):
+            self._check_field(t)
+        check(self.lib.cfd_heat_run(self.h, _dptr(T_a), _dptr(T_b), float(alpha), _dptr(alpha_field), float(dt),
+                                    int(nsteps), _stream_ptr()))
+        return T_a if nsteps % 2 == 0 else T_b
+
+    def wave_run(self, state_a, state_b, K, dt, nsteps, K_field=None):
+        """Fused acoustic steps on (p, u, v, w); returns the tuple holding the final state."""
+        for t in list(state_a) + list(state_b):
+            self._check_field(t)
+        A = (ctypes.c_void_p * 4)(*[t.data_ptr() for t in state_a])
+        B = (ctypes.c_void_p * 4)(*[t.data_ptr() for t in state_b])
+        check(self.lib.cfd_wave_run(self.h, A, B, float(K), _dptr(K_field), float(dt), int(nsteps),
+                                    _stream_ptr()))
+        return tuple(state_a) if nsteps % 2 == 0 else tuple(state_b)

@@ -1,0 +1,116 @@
+// Internal runtime types shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cfdarco.h"
+
+namespace cfd {
+
+// Error carried from the implementation to the C-ABI wrapper, which turns it
+// into a status code plus a thread-local message.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define CFD_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::cfd::Error(CFD_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CFD_REQUIRE(cond, msg)                                   \
+    do {                                                         \
+        if (!(cond)) throw ::cfd::Error(CFD_E_INVALID, (msg));  \
+    } while (0)
+
+// Runs `body`, mapping exceptions to a status code (no exception crosses the ABI).
+template <class F> int guarded(F&& body) {
+    try {
+        body();
+        return CFD_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return CFD_E_NOMEM;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return CFD_E_INVALID;
+    }
+}
+
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+}  // namespace cfd
+
+// Library context: one device, one stream-ordered memory pool.
+struct cfd_ctx_s {
+    int device = 0;
+    hipMemPool_t pool = nullptr;
+    std::mutex mu;
+    int64_t limit = 0;          // 0 = uncapped
+    int64_t in_use = 0;
+    int64_t peak = 0;
+    std::vector<std::pair<void*, int64_t>> live;  // outstanding pool blocks
+    int num_cu = 0;
+};
+
+namespace cfd {
+
+// Constant geometry of an exactly uniform hex mesh, per axis a = 0 (x), 1 (y), 2 (z).
+template <class T> struct UniformGeo {
+    int64_t X, Y, Z;
+    T area[3];       // face area of the a-normal faces
+    T alpha[3];      // alpha_d on interior a-faces
+    T alpha_b[3];    // alpha_d on boundary a-faces
+    T len[3];        // cell centre to a-face centre distance
+    T vol;           // cell volume
+    T dl;            // min(dx, dy, dz)
+};
+
+// Face-ordered SoA geometry for a general (non-uniform) structured hex mesh.
+// Face families in the reference's local face order: z-faces (fc 0,1), y-faces
+// (fc 2,3), x-faces (fc 4,5).  Family index of the face between cells along:
+//   z: x*Y*(Z+1) + y*(Z+1) + zf       y: x*(Y+1)*Z + yf*Z + z       x: xf*Y*Z + y*Z + z
+// Per face: the axis-direction normal (reference _normals, Q4), area, alpha_d.
+// Per cell: volume and the six centre-to-face distances (stab only).
+template <class T> struct GeneralGeo {
+    int64_t X, Y, Z;
+    const T* fn[3][3];   // [family z,y,x][component x,y,z]
+    const T* farea[3];
+    const T* falpha[3];
+    const T* vol;
+    const T* len;        // 6 x N, fc-major
+    T dl;
+};
+
+}  // namespace cfd
+
+struct cfd_mesh_s {
+    cfd_ctx_t ctx = nullptr;
+    int dtype = CFD_F32;
+    int64_t X = 0, Y = 0, Z = 0, N = 0, F = 0, V = 0;
+    bool uniform = false;
+    double dx = 0, dy = 0, dz = 0;
+    cfd::UniformGeo<float> ug32{};
+    cfd::UniformGeo<double> ug64{};
+    cfd::GeneralGeo<float> gg32{};
+    cfd::GeneralGeo<double> gg64{};
+    void* dev_block = nullptr;     // one allocation holding all device geometry
+    int64_t device_bytes = 0;
+    // host copies of the reference-layout tables are rebuilt on demand
+    std::vector<double> vertex_offsets;
+    bool has_offsets = false;
+};

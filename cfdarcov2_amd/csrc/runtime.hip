@@ -1,0 +1,549 @@
+// C-ABI entry points: context, stream-ordered pool, hex mesh, operator and
+// fused-step dispatch.  Every function maps failures to a status code plus a
+// thread-local message (cfd_last_error); nothing throws across the ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "common.hpp"
+#include "hexmesh.hpp"
+#include "kernels.hpp"
+
+namespace cfd {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+static int elem_size(int dtype) { return dtype == CFD_F64 ? 8 : 4; }
+static void check_dtype(int dtype) {
+    CFD_REQUIRE(dtype == CFD_F32 || dtype == CFD_F64, "dtype must be CFD_F32 or CFD_F64");
+}
+
+// ---------------------------------------------------------------------------
+// stream-ordered pool (hipMemPool with a release threshold of "never", so freed
+// blocks are reused by later allocations on the same device)
+// ---------------------------------------------------------------------------
+void* pool_alloc(cfd_ctx_t ctx, int64_t bytes, hipStream_t s) {
+    CFD_REQUIRE(ctx != nullptr, "null context");
+    CFD_REQUIRE(bytes >= 0, "negative allocation size");
+    if (bytes == 0) bytes = 1;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (ctx->limit > 0 && ctx->in_use + bytes > ctx->limit)
+            throw Error(CFD_E_NOMEM, "pool limit reached: " + std::to_string(ctx->in_use) + " + " +
+                                         std::to_string(bytes) + " > " + std::to_string(ctx->limit) + " bytes");
+    }
+    void* p = nullptr;
+    CFD_HIP(hipSetDevice(ctx->device));
+    hipError_t e = hipMallocFromPoolAsync(&p, size_t(bytes), ctx->pool, s);
+    if (e != hipSuccess || p == nullptr) {
+        (void)hipGetLastError();
+        throw Error(CFD_E_NOMEM, std::string("device pool allocation of ") + std::to_string(bytes) +
+                                     " bytes failed: " + hipGetErrorString(e));
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->live.emplace_back(p, bytes);
+    ctx->in_use += bytes;
+    ctx->peak = std::max(ctx->peak, ctx->in_use);
+    return p;
+}
+
+void pool_free(cfd_ctx_t ctx, void* p, hipStream_t s) {
+    if (p == nullptr) return;
+    int64_t bytes = -1;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        for (size_t i = 0; i < ctx->live.size(); ++i)
+            if (ctx->live[i].first == p) {
+                bytes = ctx->live[i].second;
+                ctx->live[i] = ctx->live.back();
+                ctx->live.pop_back();
+                break;
+            }
+    }
+    CFD_REQUIRE(bytes >= 0, "cfd_free: pointer was not allocated by this context");
+    CFD_HIP(hipFreeAsync(p, s));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->in_use -= bytes;
+}
+
+// ---------------------------------------------------------------------------
+// mesh construction
+// ---------------------------------------------------------------------------
+template <class T> static bool all_equal(const std::vector<T>& v, T ref) {
+    for (const T& x : v)
+        if (!(x == ref)) return false;
+    return true;
+}
+
+// Determines whether the geometry is exactly uniform and, if so, fills the
+// constants; returns false otherwise.
+template <class T> static bool detect_uniform(const HexGeometry<T>& g, UniformGeo<T>& u) {
+    const int64_t X = g.X, Y = g.Y, Z = g.Z;
+    u.X = X; u.Y = Y; u.Z = Z;
+    // axis-normal of each family must be exactly +e_axis
+    for (int fam = 0; fam < 3; ++fam) {
+        const int axis = 2 - fam;
+        const int fc = 2 * fam;  // a local face of this family (lower)
+        for (int64_t f = 0; f < g.nf[fam]; ++f) {
+            const P3<T> ax = HexGeometry<T>::axis_normal(g.fnorm(fam, f), fc);
+            const T comp[3] = {ax.x, ax.y, ax.z};
+            for (int d = 0; d < 3; ++d)
+                if (!(comp[d] == (d == axis ? T(1) : T(0)))) return false;
+        }
+        if (!all_equal(g.farea[fam], g.farea[fam][0])) return false;
+        u.area[axis] = g.farea[fam][0];
+    }
+    if (!all_equal(g.vol, g.vol[0])) return false;
+    u.vol = g.vol[0];
+    // alpha on interior and boundary faces of each family
+    for (int fam = 0; fam < 3; ++fam) {
+        const int axis = 2 - fam;
+        const int64_t ext[3] = {X, Y, Z};
+        bool have_i = false, have_b = false;
+        T ai = 0, ab = 0;
+        for (int64_t x = 0; x < X + (axis == 0); ++x)
+            for (int64_t y = 0; y < Y + (axis == 1); ++y)
+                for (int64_t z = 0; z < Z + (axis == 2); ++z) {
+                    const int64_t k = axis == 0 ? x : axis == 1 ? y : z;
+                    const int64_t f = fam == 0 ? g.zface(x, y, z) : fam == 1 ? g.yface(x, y, z) : g.xface(x, y, z);
+                    const bool bnd = (k == 0) || (k == ext[axis]);
+                    const T a = g.falpha[fam][f];
+                    if (bnd) {
+                        if (!have_b) { ab = a; have_b = true; }
+                        else if (!(a == ab)) return false;
+                    } else {
+                        if (!have_i) { ai = a; have_i = true; }
+                        else if (!(a == ai)) return false;
+                    }
+                }
+        u.alpha_b[axis] = ab;
+        u.alpha[axis] = have_i ? ai : ab;
+    }
+    // centre-to-face distance per axis (both faces of the pair)
+    for (int axis = 0; axis < 3; ++axis) {
+        const int fc_lo = 2 * (2 - axis), fc_hi = fc_lo + 1;
+        const T ref = g.len[int64_t(fc_lo) * g.N];
+        for (int64_t i = 0; i < g.N; ++i)
+            if (!(g.len[int64_t(fc_lo) * g.N + i] == ref) || !(g.len[int64_t(fc_hi) * g.N + i] == ref)) return false;
+        u.len[axis] = ref;
+    }
+    return true;
+}
+
+template <class T> static T min3(T a, T b, T c) { return std::min(std::min(a, b), c); }
+
+template <class T>
+static void upload_general(cfd_mesh_s* m, const HexGeometry<T>& g, GeneralGeo<T>& gg, hipStream_t s) {
+    // one block: per family [ax(3), area, alpha] * nf, then vol (N), len (6N)
+    int64_t total = 0;
+    for (int fam = 0; fam < 3; ++fam) total += 5 * g.nf[fam];
+    total += 7 * g.N;
+    const int64_t bytes = total * int64_t(sizeof(T));
+    m->dev_block = pool_alloc(m->ctx, bytes, s);
+    m->device_bytes = bytes;
+    std::vector<T> host(static_cast<size_t>(total));
+    T* dev = (T*)m->dev_block;
+    int64_t off = 0;
+    gg.X = g.X; gg.Y = g.Y; gg.Z = g.Z;
+    for (int fam = 0; fam < 3; ++fam) {
+        const int fc = 2 * fam;
+        const int64_t nf = g.nf[fam];
+        for (int64_t f = 0; f < nf; ++f) {
+            const P3<T> ax = HexGeometry<T>::axis_normal(g.fnorm(fam, f), fc);
+            host[off + 0 * nf + f] = ax.x;
+            host[off + 1 * nf + f] = ax.y;
+            host[off + 2 * nf + f] = ax.z;
+            host[off + 3 * nf + f] = g.farea[fam][f];
+            host[off + 4 * nf + f] = g.falpha[fam][f];
+        }
+        for (int d = 0; d < 3; ++d) gg.fn[fam][d] = dev + off + d * nf;
+        gg.farea[fam] = dev + off + 3 * nf;
+        gg.falpha[fam] = dev + off + 4 * nf;
+        off += 5 * nf;
+    }
+    std::copy(g.vol.begin(), g.vol.end(), host.begin() + off);
+    gg.vol = dev + off;
+    off += g.N;
+    std::copy(g.len.begin(), g.len.end(), host.begin() + off);
+    gg.len = dev + off;
+    off += 6 * g.N;
+    CFD_HIP(hipMemcpyAsync(dev, host.data(), size_t(bytes), hipMemcpyHostToDevice, s));
+    CFD_HIP(hipStreamSynchronize(s));
+}
+
+template <class T> static void create_mesh(cfd_mesh_s* m, const cfd_hex_desc& d) {
+    HexGeometry<T> g;
+    build_hex_geometry<T>(g, d.nx, d.ny, d.nz, T(d.dx), T(d.dy), T(d.dz), d.vertex_offsets);
+    UniformGeo<T> u{};
+    const bool uni = !d.force_general && detect_uniform(g, u);
+    const T dl = min3(T(d.dx), T(d.dy), T(d.dz));
+    if (uni) {
+        u.dl = dl;
+        m->uniform = true;
+        if constexpr (sizeof(T) == 4) m->ug32 = u; else m->ug64 = u;
+    } else {
+        m->uniform = false;
+        GeneralGeo<T> gg{};
+        gg.dl = dl;
+        upload_general(m, g, gg, nullptr);
+        if constexpr (sizeof(T) == 4) m->gg32 = gg; else m->gg64 = gg;
+    }
+}
+
+// Reference-layout table reconstruction (Mesh3D::compute, src/mesh3d.cpp:202-315).
+template <class T>
+static void hex_table(int64_t X_, int64_t Y_, int64_t Z_, double dx, double dy, double dz, const double* offsets,
+                      int which, void* out) {
+    HexGeometry<T> g;
+    build_hex_geometry<T>(g, X_, Y_, Z_, T(dx), T(dy), T(dz), offsets);
+    const int64_t N = g.N, X = g.X, Y = g.Y, Z = g.Z;
+    if (which == CFD_T_VOL || which == CFD_T_CCX || which == CFD_T_CCY || which == CFD_T_CCZ) {
+        const std::vector<T>& src = which == CFD_T_VOL ? g.vol : which == CFD_T_CCX ? g.ccx
+                                  : which == CFD_T_CCY ? g.ccy : g.ccz;
+        std::memcpy(out, src.data(), size_t(N) * sizeof(T));
+        return;
+    }
+    int64_t* oi = (int64_t*)out;
+    T* ot = (T*)out;
+    for (int64_t x = 0; x < X; ++x)
+        for (int64_t y = 0; y < Y; ++y)
+            for (int64_t z = 0; z < Z; ++z) {
+                const int64_t i = Z * Y * x + Z * y + z;
+                const int64_t crd[3] = {x, y, z}, ext[3] = {X, Y, Z}, str[3] = {Y * Z, Z, 1};
+                bool bnd = false;
+                for (int fc = 0; fc < 6; ++fc) {
+                    const int axis = 2 - (fc >> 1);
+                    const bool upper = fc & 1;
+                    const bool b = upper ? crd[axis] == ext[axis] - 1 : crd[axis] == 0;
+                    bnd = bnd || b;
+                    const int64_t nb = b ? i : (upper ? i + str[axis] : i - str[axis]);
+                    int fam; int64_t f;
+                    g.cell_face(x, y, z, fc, fam, f);
+                    const P3<T> ax = HexGeometry<T>::axis_normal(g.fnorm(fam, f), fc);
+                    const P3<T> na = (fc & 1) ? ax : P3<T>{-ax.x, -ax.y, -ax.z};
+                    const int64_t k = int64_t(fc) * N + i;
+                    switch (which) {
+                        case CFD_T_IDS: oi[k] = nb; break;
+                        case CFD_T_IDS_BF: oi[k] = b ? -1 : nb; break;
+                        case CFD_T_NX: ot[k] = ax.x; break;
+                        case CFD_T_NY: ot[k] = ax.y; break;
+                        case CFD_T_NZ: ot[k] = ax.z; break;
+                        case CFD_T_NAX: ot[k] = na.x; break;
+                        case CFD_T_NAY: ot[k] = na.y; break;
+                        case CFD_T_NAZ: ot[k] = na.z; break;
+                        case CFD_T_AREA: ot[k] = g.farea[fam][f]; break;
+                        case CFD_T_ALPHA: ot[k] = g.falpha[fam][f]; break;
+                        case CFD_T_LEN: ot[k] = g.len[k]; break;
+                        default: break;
+                    }
+                }
+                if (which == CFD_T_IS_BND) ot[i] = bnd ? T(1) : T(0);
+            }
+    CFD_REQUIRE(which >= CFD_T_IDS && which <= CFD_T_CCZ, "unknown mesh table id");
+}
+
+}  // namespace cfd
+
+using namespace cfd;
+
+extern "C" {
+
+const char* cfd_last_error(void) { return g_last_error.c_str(); }
+int cfd_version(void) { return 1; }
+
+int cfd_init(int device, cfd_ctx_t* out) {
+    return guarded([&] {
+        CFD_REQUIRE(out != nullptr, "cfd_init: null output");
+        int n = 0;
+        CFD_HIP(hipGetDeviceCount(&n));
+        CFD_REQUIRE(device >= 0 && device < n, "cfd_init: device index out of range");
+        CFD_HIP(hipSetDevice(device));
+        auto ctx = std::make_unique<cfd_ctx_s>();
+        ctx->device = device;
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = device;
+        CFD_HIP(hipMemPoolCreate(&ctx->pool, &props));
+        uint64_t thr = UINT64_MAX;
+        CFD_HIP(hipMemPoolSetAttribute(ctx->pool, hipMemPoolAttrReleaseThreshold, &thr));
+        hipDeviceProp_t prop;
+        CFD_HIP(hipGetDeviceProperties(&prop, device));
+        ctx->num_cu = prop.multiProcessorCount;
+        *out = ctx.release();
+    });
+}
+
+int cfd_destroy(cfd_ctx_t ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        CFD_HIP(hipSetDevice(ctx->device));
+        CFD_HIP(hipDeviceSynchronize());
+        for (auto& b : ctx->live) (void)hipFreeAsync(b.first, nullptr);
+        ctx->live.clear();
+        CFD_HIP(hipDeviceSynchronize());
+        CFD_HIP(hipMemPoolDestroy(ctx->pool));
+        delete ctx;
+    });
+}
+
+int cfd_alloc(cfd_ctx_t ctx, int64_t bytes, void** dptr, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(dptr != nullptr, "cfd_alloc: null output");
+        *dptr = pool_alloc(ctx, bytes, as_stream(stream));
+    });
+}
+
+int cfd_free(cfd_ctx_t ctx, void* dptr, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(ctx != nullptr, "null context");
+        pool_free(ctx, dptr, as_stream(stream));
+    });
+}
+
+int cfd_pool_set_limit(cfd_ctx_t ctx, int64_t bytes) {
+    return guarded([&] {
+        CFD_REQUIRE(ctx != nullptr, "null context");
+        CFD_REQUIRE(bytes >= 0, "negative pool limit");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->limit = bytes;
+    });
+}
+
+int cfd_pool_stats(cfd_ctx_t ctx, int64_t* in_use, int64_t* peak) {
+    return guarded([&] {
+        CFD_REQUIRE(ctx != nullptr, "null context");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (in_use) *in_use = ctx->in_use;
+        if (peak) *peak = ctx->peak;
+    });
+}
+
+int cfd_memcpy(cfd_ctx_t ctx, void* dst, const void* src, int64_t bytes, int kind, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(ctx != nullptr, "null context");
+        CFD_REQUIRE(bytes >= 0, "negative copy size");
+        if (bytes == 0) return;
+        CFD_REQUIRE(dst != nullptr && src != nullptr, "cfd_memcpy: null pointer");
+        hipMemcpyKind k = kind == CFD_H2D ? hipMemcpyHostToDevice
+                        : kind == CFD_D2H ? hipMemcpyDeviceToHost
+                        : kind == CFD_D2D ? hipMemcpyDeviceToDevice : hipMemcpyDefault;
+        CFD_REQUIRE(kind == CFD_H2D || kind == CFD_D2H || kind == CFD_D2D, "cfd_memcpy: bad kind");
+        CFD_HIP(hipMemcpyAsync(dst, src, size_t(bytes), k, as_stream(stream)));
+        if (kind != CFD_D2D) CFD_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+
+int cfd_stream_sync(void* stream) {
+    return guarded([&] { CFD_HIP(hipStreamSynchronize(as_stream(stream))); });
+}
+
+int cfd_mesh_create_hex(cfd_ctx_t ctx, const cfd_hex_desc* desc, cfd_mesh_t* out) {
+    return guarded([&] {
+        CFD_REQUIRE(ctx != nullptr && desc != nullptr && out != nullptr, "cfd_mesh_create_hex: null argument");
+        check_dtype(desc->dtype);
+        CFD_REQUIRE(desc->nx > 0 && desc->ny > 0 && desc->nz > 0, "mesh extents must be positive");
+        CFD_REQUIRE(desc->ny <= 65535 * 4 && desc->nx <= 65535, "mesh extent exceeds launch grid limits");
+        CFD_REQUIRE(desc->dx > 0 && desc->dy > 0 && desc->dz > 0, "mesh spacing must be positive");
+        CFD_HIP(hipSetDevice(ctx->device));
+        auto m = std::make_unique<cfd_mesh_s>();
+        m->ctx = ctx;
+        m->dtype = desc->dtype;
+        m->X = desc->nx; m->Y = desc->ny; m->Z = desc->nz;
+        m->N = m->X * m->Y * m->Z;
+        m->V = (m->X + 1) * (m->Y + 1) * (m->Z + 1);
+        m->F = m->X * m->Y * (m->Z + 1) + m->Z * m->Y * (m->X + 1) + m->X * m->Z * (m->Y + 1);
+        m->dx = desc->dx; m->dy = desc->dy; m->dz = desc->dz;
+        if (desc->vertex_offsets) {
+            m->has_offsets = true;
+            m->vertex_offsets.assign(desc->vertex_offsets, desc->vertex_offsets + 3 * m->V);
+        }
+        if (m->dtype == CFD_F32) create_mesh<float>(m.get(), *desc);
+        else create_mesh<double>(m.get(), *desc);
+        *out = m.release();
+    });
+}
+
+int cfd_mesh_destroy(cfd_mesh_t mesh) {
+    return guarded([&] {
+        if (!mesh) return;
+        if (mesh->dev_block) {
+            pool_free(mesh->ctx, mesh->dev_block, nullptr);
+            CFD_HIP(hipStreamSynchronize(nullptr));
+        }
+        delete mesh;
+    });
+}
+
+int cfd_mesh_get_info(cfd_mesh_t m, cfd_mesh_info* info) {
+    return guarded([&] {
+        CFD_REQUIRE(m != nullptr && info != nullptr, "null argument");
+        info->n_cells = m->N; info->n_faces = m->F; info->n_vertices = m->V;
+        info->nx = m->X; info->ny = m->Y; info->nz = m->Z;
+        info->dtype = m->dtype;
+        info->uniform = m->uniform ? 1 : 0;
+        info->device_bytes = m->device_bytes;
+    });
+}
+
+int cfd_mesh_table(cfd_mesh_t m, int table, void* host_out) {
+    return guarded([&] {
+        CFD_REQUIRE(m != nullptr && host_out != nullptr, "null argument");
+        CFD_REQUIRE(table >= CFD_T_IDS && table <= CFD_T_CCZ, "unknown mesh table id");
+        const double* off = m->has_offsets ? m->vertex_offsets.data() : nullptr;
+        if (m->dtype == CFD_F32) hex_table<float>(m->X, m->Y, m->Z, m->dx, m->dy, m->dz, off, table, host_out);
+        else hex_table<double>(m->X, m->Y, m->Z, m->dx, m->dy, m->dz, off, table, host_out);
+    });
+}
+
+int cfd_hex_table(const cfd_hex_desc* d, int table, void* host_out) {
+    return guarded([&] {
+        CFD_REQUIRE(d != nullptr && host_out != nullptr, "null argument");
+        check_dtype(d->dtype);
+        CFD_REQUIRE(d->nx > 0 && d->ny > 0 && d->nz > 0, "mesh extents must be positive");
+        CFD_REQUIRE(table >= CFD_T_IDS && table <= CFD_T_CCZ, "unknown mesh table id");
+        if (d->dtype == CFD_F32)
+            hex_table<float>(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->vertex_offsets, table, host_out);
+        else
+            hex_table<double>(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->vertex_offsets, table, host_out);
+    });
+}
+
+#define DISPATCH(m, F, ...) ((m)->dtype == CFD_F32 ? F<float>(__VA_ARGS__) : F<double>(__VA_ARGS__))
+
+int cfd_grad(cfd_mesh_t m, const void* phi, void* out, int mask, int wall, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(m && phi && out, "cfd_grad: null argument");
+        CFD_REQUIRE(mask >= 0 && mask <= 7 && wall >= -1 && wall <= 2, "cfd_grad: bad axis_mask or wall_axis");
+        DISPATCH(m, launch_grad, m, phi, out, mask, wall, as_stream(stream));
+    });
+}
+
+int cfd_lapl(cfd_mesh_t m, const void* phi, void* out, int mask, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(m && phi && out, "cfd_lapl: null argument");
+        CFD_REQUIRE(mask >= 0 && mask <= 7, "cfd_lapl: bad axis_mask");
+        DISPATCH(m, launch_lapl, m, phi, out, mask, as_stream(stream));
+    });
+}
+
+int cfd_stab(cfd_mesh_t m, const void* phi, void* out, int mask, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(m && phi && out, "cfd_stab: null argument");
+        CFD_REQUIRE(mask >= 0 && mask <= 7, "cfd_stab: bad axis_mask");
+        hipStream_t s = as_stream(stream);
+        void* scratch = pool_alloc(m->ctx, 3 * m->N * elem_size(m->dtype), s);
+        try {
+            DISPATCH(m, launch_stab, m, phi, out, mask, scratch, s);
+        } catch (...) {
+            pool_free(m->ctx, scratch, s);
+            throw;
+        }
+        pool_free(m->ctx, scratch, s);
+    });
+}
+
+int cfd_ewise(cfd_ctx_t ctx, int op, int dtype, const void* a, const void* b, const void* aux, double s,
+              void* out, int64_t n, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(ctx != nullptr && out != nullptr, "cfd_ewise: null argument");
+        check_dtype(dtype);
+        CFD_REQUIRE(op >= CFD_EW_ADD && op <= CFD_EW_COPY, "cfd_ewise: unknown op");
+        CFD_REQUIRE(n >= 0, "cfd_ewise: negative length");
+        const bool needs_a = op != CFD_EW_FILL;
+        const bool needs_b = op == CFD_EW_ADD || op == CFD_EW_SUB || op == CFD_EW_MUL || op == CFD_EW_DIV ||
+                             op == CFD_EW_AXPY || op == CFD_EW_LEAP;
+        CFD_REQUIRE(!needs_a || a != nullptr, "cfd_ewise: operand a is null");
+        CFD_REQUIRE(!needs_b || b != nullptr, "cfd_ewise: operand b is null");
+        CFD_REQUIRE(op != CFD_EW_LEAP || aux != nullptr, "cfd_ewise: LEAP needs aux");
+        if (dtype == CFD_F32) launch_ewise<float>(op, a, b, aux, s, out, n, as_stream(stream));
+        else launch_ewise<double>(op, a, b, aux, s, out, n, as_stream(stream));
+    });
+}
+
+int cfd_cfl_min(cfd_mesh_t m, const void* u, const void* v, const void* w, const void* p, const void* rho,
+                double cfl, double gamma, double* dt_out, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(m && u && p && rho && dt_out, "cfd_cfl_min: null argument");
+        hipStream_t s = as_stream(stream);
+        const int nb = int(std::min<int64_t>(std::max<int64_t>((m->N + 255) / 256, 1), 2048));
+        const int es = elem_size(m->dtype);
+        void* scratch = pool_alloc(m->ctx, int64_t(nb + 1) * es, s);
+        char* base = (char*)scratch;
+        double host = 0;
+        try {
+            if (m->dtype == CFD_F32) {
+                const float dl = std::min(std::min(float(m->dx), float(m->dy)), float(m->dz));
+                launch_cfl<float>(u, v, w, p, rho, dl, float(gamma), m->N, base, nb, base + int64_t(nb) * es, s);
+                float r = 0;
+                CFD_HIP(hipMemcpyAsync(&r, base + int64_t(nb) * es, 4, hipMemcpyDeviceToHost, s));
+                CFD_HIP(hipStreamSynchronize(s));
+                host = double(float(cfl) * r);
+            } else {
+                const double dl = std::min(std::min(m->dx, m->dy), m->dz);
+                launch_cfl<double>(u, v, w, p, rho, dl, gamma, m->N, base, nb, base + int64_t(nb) * es, s);
+                double r = 0;
+                CFD_HIP(hipMemcpyAsync(&r, base + int64_t(nb) * es, 8, hipMemcpyDeviceToHost, s));
+                CFD_HIP(hipStreamSynchronize(s));
+                host = cfl * r;
+            }
+        } catch (...) {
+            pool_free(m->ctx, scratch, s);
+            throw;
+        }
+        pool_free(m->ctx, scratch, s);
+        *dt_out = host;
+    });
+}
+
+int cfd_count_nonfinite(cfd_ctx_t ctx, int dtype, const void* x, int64_t n, int64_t* bad, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(ctx && x && bad, "cfd_count_nonfinite: null argument");
+        check_dtype(dtype);
+        hipStream_t s = as_stream(stream);
+        void* scratch = pool_alloc(ctx, 1025 * 8, s);
+        unsigned long long total = 0;
+        try {
+            if (dtype == CFD_F32) launch_nonfinite<float>(x, n, (unsigned long long*)scratch, s);
+            else launch_nonfinite<double>(x, n, (unsigned long long*)scratch, s);
+            CFD_HIP(hipMemcpyAsync(&total, (char*)scratch + 1024 * 8, 8, hipMemcpyDeviceToHost, s));
+            CFD_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            pool_free(ctx, scratch, s);
+            throw;
+        }
+        pool_free(ctx, scratch, s);
+        *bad = int64_t(total);
+    });
+}
+
+int cfd_heat_run(cfd_mesh_t m, void* Ta, void* Tb, double alpha, const void* af, double dt, int64_t nsteps,
+                 void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(m && Ta && Tb, "cfd_heat_run: null argument");
+        CFD_REQUIRE(Ta != Tb, "cfd_heat_run: ping-pong buffers must differ");
+        CFD_REQUIRE(nsteps >= 0, "cfd_heat_run: negative step count");
+        DISPATCH(m, launch_heat, m, Ta, Tb, alpha, af, dt, nsteps, as_stream(stream));
+    });
+}
+
+int cfd_wave_run(cfd_mesh_t m, void* const a[4], void* const b[4], double K, const void* Kf, double dt,
+                 int64_t nsteps, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(m && a && b, "cfd_wave_run: null argument");
+        for (int i = 0; i < 4; ++i) {
+            CFD_REQUIRE(a[i] && b[i], "cfd_wave_run: null state pointer");
+            for (int j = 0; j < 4; ++j) CFD_REQUIRE(a[i] != b[j], "cfd_wave_run: state_a and state_b overlap");
+        }
+        CFD_REQUIRE(nsteps >= 0, "cfd_wave_run: negative step count");
+        DISPATCH(m, launch_wave, m, a, b, K, Kf, dt, nsteps, as_stream(stream));
+    });
+}
+
+}  // extern "C"

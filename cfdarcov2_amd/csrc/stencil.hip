@@ -1,0 +1,496 @@
+// Stencil kernels of the explicit finite-volume step on the structured hex
+// mesh: Green-Gauss gradient, Laplacian, stabilisation, and the fused heat and
+// acoustic steps.  Two geometry forms:
+//   uniform  - exactly uniform meshes; geometry is a handful of constants and
+//              neighbours are implicit, so only state crosses HBM;
+//   general  - per-face SoA geometry (face families z, y, x) for perturbed
+//              meshes; neighbours are still implicit (structured topology).
+// All arithmetic follows the reference operation order (see the oracle and
+// src/fvm3d.cpp / include/grad_utils.hpp cited per kernel) and the library is
+// built with -ffp-contract=off, so FP64 results are bit-identical to a CPU
+// evaluation in the same order.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace cfd {
+
+// ---------------------------------------------------------------------------
+// indexing helpers
+// ---------------------------------------------------------------------------
+struct Cell {
+    int64_t x, y, z, id;
+    bool valid;
+};
+
+template <class G> __device__ inline Cell cell_of_thread(const G& g) {
+    Cell c;
+    c.z = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    c.y = int64_t(blockIdx.y) * blockDim.y + threadIdx.y;
+    c.x = int64_t(blockIdx.z);
+    c.valid = c.z < g.Z && c.y < g.Y && c.x < g.X;
+    c.id = (c.x * g.Y + c.y) * g.Z + c.z;
+    return c;
+}
+
+// Neighbour of cell c across its lower / upper face on axis a (0 = x, 1 = y,
+// 2 = z); the cell itself on a boundary face (reference rule, Q6).
+template <class G> __device__ inline int64_t nb_lo(const G& g, const Cell& c, int a) {
+    if (a == 0) return c.x > 0 ? c.id - g.Y * g.Z : c.id;
+    if (a == 1) return c.y > 0 ? c.id - g.Z : c.id;
+    return c.z > 0 ? c.id - 1 : c.id;
+}
+template <class G> __device__ inline int64_t nb_hi(const G& g, const Cell& c, int a) {
+    if (a == 0) return c.x < g.X - 1 ? c.id + g.Y * g.Z : c.id;
+    if (a == 1) return c.y < g.Y - 1 ? c.id + g.Z : c.id;
+    return c.z < g.Z - 1 ? c.id + 1 : c.id;
+}
+template <class G> __device__ inline bool at_lo(const G& g, const Cell& c, int a) {
+    return a == 0 ? c.x == 0 : a == 1 ? c.y == 0 : c.z == 0;
+}
+template <class G> __device__ inline bool at_hi(const G& g, const Cell& c, int a) {
+    return a == 0 ? c.x == g.X - 1 : a == 1 ? c.y == g.Y - 1 : c.z == g.Z - 1;
+}
+
+// Face-family index of the lower / upper face of cell c on axis a.
+template <class T> __device__ inline int64_t face_lo(const GeneralGeo<T>& g, const Cell& c, int a) {
+    if (a == 2) return c.x * g.Y * (g.Z + 1) + c.y * (g.Z + 1) + c.z;
+    if (a == 1) return c.x * (g.Y + 1) * g.Z + c.y * g.Z + c.z;
+    return c.x * g.Y * g.Z + c.y * g.Z + c.z;
+}
+template <class T> __device__ inline int64_t face_hi(const GeneralGeo<T>& g, const Cell& c, int a) {
+    if (a == 2) return face_lo(g, c, a) + 1;
+    if (a == 1) return face_lo(g, c, a) + g.Z;
+    return face_lo(g, c, a) + g.Y * g.Z;
+}
+// local face fc -> axis: fc 0,1 z; 2,3 y; 4,5 x
+__device__ inline int fc_axis(int fc) { return 2 - (fc >> 1); }
+__device__ inline int axis_family(int a) { return 2 - a; }  // family 0 = z-faces, 1 = y, 2 = x
+
+// ---------------------------------------------------------------------------
+// uniform-mesh building blocks
+// ---------------------------------------------------------------------------
+
+// Green-Gauss component along axis a from the two face values on that axis:
+// acc = (f_lo * -1) * A + (f_hi * +1) * A, then / V (gauss_grad_cu_k order,
+// include/custom_cuda_functions.hpp:277-295; the other four faces add zeros).
+template <class T> __device__ inline T gg_pair(T f_lo, T f_hi, T A, T V) {
+    const T t_lo = (-f_lo) * A;
+    const T t_hi = f_hi * A;
+    return (t_lo + t_hi) / V;
+}
+
+// Linear face interpolation, (phi_c + phi_nb) * 0.5 (grad_utils.hpp:23-36).
+template <class T> __device__ inline T lin(T c, T nb) { return (c + nb) * T(0.5); }
+
+// ---------------------------------------------------------------------------
+// GradVar / Grad2Var / StabVar operator kernels
+// ---------------------------------------------------------------------------
+
+template <class T>
+__global__ void __launch_bounds__(256) grad_uniform_k(UniformGeo<T> g, const T* __restrict__ phi,
+                                                      T* __restrict__ out, int mask, int wall_axis) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    const T pc = phi[c.id];
+    T res = T(0);
+    for (int a = 0; a < 3; ++a) {
+        if (!(mask & (1 << a))) continue;
+        T f_lo = lin(pc, phi[nb_lo(g, c, a)]);
+        T f_hi = lin(pc, phi[nb_hi(g, c, a)]);
+        if (wall_axis == a) {
+            if (at_lo(g, c, a)) f_lo = T(0);
+            if (at_hi(g, c, a)) f_hi = T(0);
+        }
+        res = res + gg_pair(f_lo, f_hi, g.area[a], g.vol);
+    }
+    out[c.id] = res;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) lapl_uniform_k(UniformGeo<T> g, const T* __restrict__ phi,
+                                                      T* __restrict__ out, int mask) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    const T pc = phi[c.id];
+    T res = T(0);
+    for (int a = 0; a < 3; ++a) {
+        if (!(mask & (1 << a))) continue;
+        const T al_lo = at_lo(g, c, a) ? g.alpha_b[a] : g.alpha[a];
+        const T al_hi = at_hi(g, c, a) ? g.alpha_b[a] : g.alpha[a];
+        const T s_lo = (-pc + phi[nb_lo(g, c, a)]) * al_lo;
+        const T s_hi = (-pc + phi[nb_hi(g, c, a)]) * al_hi;
+        // axis normals are +1 on both faces of the pair (Q4)
+        res = res + (s_lo * g.area[a] + s_hi * g.area[a]) / g.vol;
+    }
+    out[c.id] = res;
+}
+
+// Outward Green-Gauss gradient, all three components (first stage of StabVar).
+template <class T>
+__global__ void __launch_bounds__(256) grad3_uniform_k(UniformGeo<T> g, const T* __restrict__ phi,
+                                                       T* __restrict__ gx, T* __restrict__ gy, T* __restrict__ gz) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    const T pc = phi[c.id];
+    T* outs[3] = {gx, gy, gz};
+    for (int a = 0; a < 3; ++a)
+        outs[a][c.id] = gg_pair(lin(pc, phi[nb_lo(g, c, a)]), lin(pc, phi[nb_hi(g, c, a)]), g.area[a], g.vol);
+}
+
+// Second stage of StabVar on a uniform mesh (src/fvm3d.cpp:915-936): upwind
+// face values phi + (len * g_d) * n_out_d, jump against the neighbour's
+// opposite face (self's opposite face on the boundary, Q8), 0.5 * jump, Gauss
+// with axis normals for component d.
+template <class T>
+__global__ void __launch_bounds__(256) stab_uniform_k(UniformGeo<T> g, const T* __restrict__ phi,
+                                                      const T* __restrict__ gx, const T* __restrict__ gy,
+                                                      const T* __restrict__ gz, T* __restrict__ out, int mask) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    const T* gs[3] = {gx, gy, gz};
+    T res = T(0);
+    for (int d = 0; d < 3; ++d) {
+        if (!(mask & (1 << d))) continue;
+        const T L = g.len[d];
+        const int64_t lo = nb_lo(g, c, d), hi = nb_hi(g, c, d);
+        const T pc = phi[c.id], gc = gs[d][c.id];
+        // lower face: own outward normal -1, neighbour's upper face +1
+        const T own_lo = pc + (L * gc) * T(-1);
+        const T col_lo = phi[lo] + (L * gs[d][lo]) * T(1);
+        // upper face: own +1, neighbour's lower face -1
+        const T own_hi = pc + (L * gc) * T(1);
+        const T col_hi = phi[hi] + (L * gs[d][hi]) * T(-1);
+        const T f_lo = T(0.5) * (own_lo - col_lo);
+        const T f_hi = T(0.5) * (own_hi - col_hi);
+        res = res + (f_lo * g.area[d] + f_hi * g.area[d]) / g.vol;
+    }
+    out[c.id] = res;
+}
+
+// --- general (per-face geometry) forms -------------------------------------
+
+// value of face fc's stored axis normal component d and the outward sign
+template <class T>
+__device__ inline void face_geo(const GeneralGeo<T>& g, const Cell& c, int fc, int64_t& f, int& fam) {
+    const int a = fc_axis(fc);
+    fam = axis_family(a);
+    f = (fc & 1) ? face_hi(g, c, a) : face_lo(g, c, a);
+}
+template <class G> __device__ inline int64_t nb_of(const G& g, const Cell& c, int fc) {
+    const int a = fc_axis(fc);
+    return (fc & 1) ? nb_hi(g, c, a) : nb_lo(g, c, a);
+}
+template <class G> __device__ inline bool is_bnd_face(const G& g, const Cell& c, int fc) {
+    const int a = fc_axis(fc);
+    return (fc & 1) ? at_hi(g, c, a) : at_lo(g, c, a);
+}
+
+// Gauss sum of six face values for component d, fc = 0..5 from 0, outward
+// (use_alt=false in the reference's inverted flag) or axis normals.
+template <class T>
+__device__ inline T gauss_general(const GeneralGeo<T>& g, const Cell& c, const T fv[6], int d, bool axis_normals) {
+    T acc = T(0);
+    for (int fc = 0; fc < 6; ++fc) {
+        int64_t f; int fam;
+        face_geo(g, c, fc, f, fam);
+        T n = g.fn[fam][d][f];
+        if (!axis_normals && !(fc & 1)) n = -n;
+        T t = fv[fc] * n;
+        t = t * g.farea[fam][f];
+        acc = acc + t;
+    }
+    return acc / g.vol[c.id];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) grad_general_k(GeneralGeo<T> g, const T* __restrict__ phi,
+                                                      T* __restrict__ out, int mask, int wall_axis) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    const T pc = phi[c.id];
+    T fv[6];
+    for (int fc = 0; fc < 6; ++fc) {
+        fv[fc] = lin(pc, phi[nb_of(g, c, fc)]);
+        if (wall_axis == fc_axis(fc) && is_bnd_face(g, c, fc)) fv[fc] = T(0);
+    }
+    T res = T(0);
+    for (int d = 0; d < 3; ++d)
+        if (mask & (1 << d)) res = res + gauss_general(g, c, fv, d, false);
+    out[c.id] = res;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) lapl_general_k(GeneralGeo<T> g, const T* __restrict__ phi,
+                                                      T* __restrict__ out, int mask) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    const T pc = phi[c.id];
+    T fv[6];
+    for (int fc = 0; fc < 6; ++fc) {
+        int64_t f; int fam;
+        face_geo(g, c, fc, f, fam);
+        fv[fc] = (-pc + phi[nb_of(g, c, fc)]) * g.falpha[fam][f];
+    }
+    T res = T(0);
+    for (int d = 0; d < 3; ++d)
+        if (mask & (1 << d)) res = res + gauss_general(g, c, fv, d, true);
+    out[c.id] = res;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) grad3_general_k(GeneralGeo<T> g, const T* __restrict__ phi,
+                                                       T* __restrict__ gx, T* __restrict__ gy, T* __restrict__ gz) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    const T pc = phi[c.id];
+    T fv[6];
+    for (int fc = 0; fc < 6; ++fc) fv[fc] = lin(pc, phi[nb_of(g, c, fc)]);
+    gx[c.id] = gauss_general(g, c, fv, 0, false);
+    gy[c.id] = gauss_general(g, c, fv, 1, false);
+    gz[c.id] = gauss_general(g, c, fv, 2, false);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) stab_general_k(GeneralGeo<T> g, const T* __restrict__ phi,
+                                                      const T* __restrict__ gx, const T* __restrict__ gy,
+                                                      const T* __restrict__ gz, T* __restrict__ out, int mask) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    const T* gs[3] = {gx, gy, gz};
+    const int64_t N = g.X * g.Y * g.Z;
+    T res = T(0);
+    for (int d = 0; d < 3; ++d) {
+        if (!(mask & (1 << d))) continue;
+        T fv[6];
+        for (int fc = 0; fc < 6; ++fc) {
+            int64_t f; int fam;
+            face_geo(g, c, fc, f, fam);
+            const T n = g.fn[fam][d][f];
+            const int ofc = fc ^ 1;
+            const int64_t nb = nb_of(g, c, fc);
+            // outward normal of (cell, fc) = (fc odd ? +n : -n); the neighbour
+            // sees the same face as its opposite local face.  On a boundary the
+            // neighbour is the cell itself and the opposite face is its own.
+            const T n_own = (fc & 1) ? n : -n;
+            T n_col;
+            if (nb == c.id) {
+                int64_t f2; int fam2;
+                face_geo(g, c, ofc, f2, fam2);
+                const T n2 = g.fn[fam2][d][f2];
+                n_col = (ofc & 1) ? n2 : -n2;
+            } else {
+                n_col = (ofc & 1) ? n : -n;
+            }
+            const T own = phi[c.id] + (g.len[int64_t(fc) * N + c.id] * gs[d][c.id]) * n_own;
+            const T col = phi[nb] + (g.len[int64_t(ofc) * N + nb] * gs[d][nb]) * n_col;
+            fv[fc] = T(0.5) * (own - col);
+        }
+        res = res + gauss_general(g, c, fv, d, true);
+    }
+    out[c.id] = res;
+}
+
+// ---------------------------------------------------------------------------
+// Fused heat step: T' = dt * (alpha * lapl(T)) + T   (DtVar::extract order,
+// src/fvm3d.cpp:690-692, with lapl = ((0 + L_x) + L_y) + L_z).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) heat_uniform_k(UniformGeo<T> g, const T* __restrict__ Tin,
+                                                      T* __restrict__ Tout, T alpha, const T* __restrict__ alpha_f,
+                                                      T dt) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    const T pc = Tin[c.id];
+    T L = T(0);
+    for (int a = 0; a < 3; ++a) {
+        const T al_lo = at_lo(g, c, a) ? g.alpha_b[a] : g.alpha[a];
+        const T al_hi = at_hi(g, c, a) ? g.alpha_b[a] : g.alpha[a];
+        const T s_lo = (-pc + Tin[nb_lo(g, c, a)]) * al_lo;
+        const T s_hi = (-pc + Tin[nb_hi(g, c, a)]) * al_hi;
+        L = L + (s_lo * g.area[a] + s_hi * g.area[a]) / g.vol;
+    }
+    const T k = alpha_f ? alpha_f[c.id] : alpha;
+    Tout[c.id] = dt * (k * L) + pc;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) heat_general_k(GeneralGeo<T> g, const T* __restrict__ Tin,
+                                                      T* __restrict__ Tout, T alpha, const T* __restrict__ alpha_f,
+                                                      T dt) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    const T pc = Tin[c.id];
+    T fv[6];
+    for (int fc = 0; fc < 6; ++fc) {
+        int64_t f; int fam;
+        face_geo(g, c, fc, f, fam);
+        fv[fc] = (-pc + Tin[nb_of(g, c, fc)]) * g.falpha[fam][f];
+    }
+    T L = T(0);
+    for (int d = 0; d < 3; ++d) L = L + gauss_general(g, c, fv, d, true);
+    const T k = alpha_f ? alpha_f[c.id] : alpha;
+    Tout[c.id] = dt * (k * L) + pc;
+}
+
+// ---------------------------------------------------------------------------
+// Fused acoustic step (uniform mesh), single pass per step.
+//   u' = dt*(-Gx(p)) + u, v' = dt*(-Gy(p)) + v, w' = dt*(-Gz(p)) + w
+//   p' = dt*(K*((Gx(u') + Gy(v')) + Gz(w'))) + p
+// p' needs u' at x+-1, v' at y+-1, w' at z+-1, which each thread recomputes
+// from p (distance 2) and the old velocities; the recomputation is bitwise
+// identical to the neighbour's own update, so the result equals the
+// sequential (Gauss-Seidel across equations) evaluation.  Walls: the normal
+// velocity's face value is zero; p is zero-gradient (neighbour = self).
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ inline T vel_update(const UniformGeo<T>& g, const T* __restrict__ p, const T* __restrict__ q,
+                               int64_t j, int64_t stride, bool lo_wall, bool hi_wall, int a, T dt) {
+    const T pc = p[j];
+    const T pl = lo_wall ? pc : p[j - stride];
+    const T ph = hi_wall ? pc : p[j + stride];
+    const T G = gg_pair(lin(pc, pl), lin(pc, ph), g.area[a], g.vol);
+    return dt * (-G) + q[j];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) wave_uniform_k(UniformGeo<T> g, const T* __restrict__ p,
+                                                      const T* __restrict__ u, const T* __restrict__ v,
+                                                      const T* __restrict__ w, T* __restrict__ po,
+                                                      T* __restrict__ uo, T* __restrict__ vo, T* __restrict__ wo,
+                                                      T K, const T* __restrict__ Kf, T dt) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    const int64_t sx = g.Y * g.Z, sy = g.Z, sz = 1;
+    const T* vel[3] = {u, v, w};
+    T* velo[3] = {uo, vo, wo};
+    const int64_t crd[3] = {c.x, c.y, c.z};
+    const int64_t ext[3] = {g.X, g.Y, g.Z};
+    const int64_t str[3] = {sx, sy, sz};
+    T S = T(0);
+    for (int a = 0; a < 3; ++a) {
+        const bool lo = crd[a] == 0, hi = crd[a] == ext[a] - 1;
+        const T qc = vel_update(g, p, vel[a], c.id, str[a], lo, hi, a, dt);
+        velo[a][c.id] = qc;
+        T f_lo = T(0), f_hi = T(0);  // wall faces carry zero normal velocity
+        if (!lo) {
+            const int64_t j = c.id - str[a];
+            const T ql = vel_update(g, p, vel[a], j, str[a], crd[a] - 1 == 0, false, a, dt);
+            f_lo = lin(qc, ql);
+        }
+        if (!hi) {
+            const int64_t j = c.id + str[a];
+            const T qh = vel_update(g, p, vel[a], j, str[a], false, crd[a] + 1 == ext[a] - 1, a, dt);
+            f_hi = lin(qc, qh);
+        }
+        const T G = gg_pair(f_lo, f_hi, g.area[a], g.vol);
+        S = (a == 0) ? G : S + G;
+    }
+    const T k = Kf ? Kf[c.id] : K;
+    po[c.id] = dt * (k * S) + p[c.id];
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+template <class G> inline dim3 grid_for(const G& g, dim3 blk) {
+    return dim3(unsigned((g.Z + blk.x - 1) / blk.x), unsigned((g.Y + blk.y - 1) / blk.y), unsigned(g.X));
+}
+static const dim3 kBlock(64, 4, 1);
+
+template <class T> const UniformGeo<T>& ugeo(const cfd_mesh_s* m);
+template <> const UniformGeo<float>& ugeo<float>(const cfd_mesh_s* m) { return m->ug32; }
+template <> const UniformGeo<double>& ugeo<double>(const cfd_mesh_s* m) { return m->ug64; }
+template <class T> const GeneralGeo<T>& ggeo(const cfd_mesh_s* m);
+template <> const GeneralGeo<float>& ggeo<float>(const cfd_mesh_s* m) { return m->gg32; }
+template <> const GeneralGeo<double>& ggeo<double>(const cfd_mesh_s* m) { return m->gg64; }
+
+inline void check_launch() { CFD_HIP(hipGetLastError()); }
+
+template <class T> void launch_grad(const cfd_mesh_s* m, const void* phi, void* out, int mask, int wall, hipStream_t s) {
+    if (m->uniform) {
+        const auto& g = ugeo<T>(m);
+        grad_uniform_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)phi, (T*)out, mask, wall);
+    } else {
+        const auto& g = ggeo<T>(m);
+        grad_general_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)phi, (T*)out, mask, wall);
+    }
+    check_launch();
+}
+
+template <class T> void launch_lapl(const cfd_mesh_s* m, const void* phi, void* out, int mask, hipStream_t s) {
+    if (m->uniform) {
+        const auto& g = ugeo<T>(m);
+        lapl_uniform_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)phi, (T*)out, mask);
+    } else {
+        const auto& g = ggeo<T>(m);
+        lapl_general_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)phi, (T*)out, mask);
+    }
+    check_launch();
+}
+
+template <class T>
+void launch_stab(const cfd_mesh_s* m, const void* phi, void* out, int mask, void* scratch3, hipStream_t s) {
+    T* gx = (T*)scratch3;
+    T* gy = gx + m->N;
+    T* gz = gy + m->N;
+    if (m->uniform) {
+        const auto& g = ugeo<T>(m);
+        grad3_uniform_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)phi, gx, gy, gz);
+        check_launch();
+        stab_uniform_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)phi, gx, gy, gz, (T*)out, mask);
+    } else {
+        const auto& g = ggeo<T>(m);
+        grad3_general_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)phi, gx, gy, gz);
+        check_launch();
+        stab_general_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)phi, gx, gy, gz, (T*)out, mask);
+    }
+    check_launch();
+}
+
+template <class T>
+void launch_heat(const cfd_mesh_s* m, void* Ta, void* Tb, double alpha, const void* af, double dt, int64_t nsteps,
+                 hipStream_t s) {
+    for (int64_t k = 0; k < nsteps; ++k) {
+        const T* in = (const T*)((k & 1) ? Tb : Ta);
+        T* out = (T*)((k & 1) ? Ta : Tb);
+        if (m->uniform) {
+            const auto& g = ugeo<T>(m);
+            heat_uniform_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, in, out, T(alpha), (const T*)af, T(dt));
+        } else {
+            const auto& g = ggeo<T>(m);
+            heat_general_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, in, out, T(alpha), (const T*)af, T(dt));
+        }
+        check_launch();
+    }
+}
+
+template <class T>
+void launch_wave(const cfd_mesh_s* m, void* const a[4], void* const b[4], double K, const void* Kf, double dt,
+                 int64_t nsteps, hipStream_t s) {
+    if (!m->uniform)
+        throw Error(CFD_E_UNSUPPORTED, "cfd_wave_run: the fused acoustic step needs a uniform mesh");
+    const auto& g = ugeo<T>(m);
+    for (int64_t k = 0; k < nsteps; ++k) {
+        void* const* in = (k & 1) ? b : a;
+        void* const* out = (k & 1) ? a : b;
+        wave_uniform_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(
+            g, (const T*)in[0], (const T*)in[1], (const T*)in[2], (const T*)in[3], (T*)out[0], (T*)out[1],
+            (T*)out[2], (T*)out[3], T(K), (const T*)Kf, T(dt));
+        check_launch();
+    }
+}
+
+#define CFD_INSTANTIATE(T)                                                                                       \
+    template void launch_grad<T>(const cfd_mesh_s*, const void*, void*, int, int, hipStream_t);                \
+    template void launch_lapl<T>(const cfd_mesh_s*, const void*, void*, int, hipStream_t);                     \
+    template void launch_stab<T>(const cfd_mesh_s*, const void*, void*, int, void*, hipStream_t);              \
+    template void launch_heat<T>(const cfd_mesh_s*, void*, void*, double, const void*, double, int64_t,         \
+                                 hipStream_t);                                                                   \
+    template void launch_wave<T>(const cfd_mesh_s*, void* const*, void* const*, double, const void*, double,    \
+                                 int64_t, hipStream_t);
+CFD_INSTANTIATE(float)
+CFD_INSTANTIATE(double)
+
+}  // namespace cfd

@@ -1,0 +1,124 @@
+"""Synthetic inputs for the benchmark workloads.
+
+All randomness comes from one project-defined generator, SplitMix64
+(Steele, Lea & Flood 2014), with doubles drawn as (x >> 11) * 2**-53, so the
+Python and C++ sides produce identical streams.  Initial states are evaluated
+in float64 at cell centres ((i + 1/2) * d on the unperturbed lattice) and cast
+to the run dtype once.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+_MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & _MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+
+    def uniform_array(self, n: int) -> np.ndarray:
+        """n draws, vectorised (same stream as n calls of uniform())."""
+        with np.errstate(over="ignore"):
+            k = np.arange(1, n + 1, dtype=np.uint64)
+            z = np.uint64(self.state) + k * np.uint64(0x9E3779B97F4A7C15)
+            self.state = int(z[-1]) if n else self.state
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def cell_centres(nx, ny, nz, dx, dy, dz):
+    """Cell-centre coordinates in the reference cell order (z fastest), float64."""
+    x = (np.arange(nx) + 0.5) * dx
+    y = (np.arange(ny) + 0.5) * dy
+    z = (np.arange(nz) + 0.5) * dz
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    return X.ravel(), Y.ravel(), Z.ravel()
+
+
+def gaussian(nx, ny, nz, dx, dy, dz, centre, sigma, amplitude=1.0):
+    X, Y, Z = cell_centres(nx, ny, nz, dx, dy, dz)
+    r2 = (X - centre[0]) ** 2 + (Y - centre[1]) ** 2 + (Z - centre[2]) ** 2
+    return amplitude * np.exp(-r2 / (2.0 * sigma * sigma))
+
+
+def perturbed_vertex_offsets(nx, ny, nz, dx, dy, dz, seed, frac):
+    """Seeded offsets, uniform in [-frac*d, frac*d] per component, for interior
+    vertices only (boundary vertices stay on the box), in the reference vertex
+    order (z fastest).  Returns (V, 3) float64."""
+    V = (nx + 1) * (ny + 1) * (nz + 1)
+    g = SplitMix64(seed)
+    r = g.uniform_array(3 * V).reshape(V, 3)
+    off = (2.0 * r - 1.0) * frac * np.array([dx, dy, dz])
+    ix, iy, iz = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    interior = ((ix > 0) & (ix < nx) & (iy > 0) & (iy < ny) & (iz > 0) & (iz < nz)).ravel()
+    off[~interior] = 0.0
+    return off
+
+
+@dataclass
+class HeatCase:
+    n: tuple
+    spacing: tuple
+    alpha: float
+    dt: float
+    steps: int
+    T0: np.ndarray
+
+
+def heat_config(n=64, steps=200, dt_factor=0.2, alpha=1e-3, sigma=0.1):
+    """Plain 3D heat on n^3 over the unit cube, Gaussian amplitude 1, sigma 0.1,
+    centred at (0.5, 0.5, 0.5); dt = dt_factor * dx^2 / alpha."""
+    d = 1.0 / n
+    T0 = gaussian(n, n, n, d, d, d, (0.5, 0.5, 0.5), sigma)
+    return HeatCase((n, n, n), (d, d, d), alpha, dt_factor * d * d / alpha, steps, T0)
+
+
+@dataclass
+class WaveCase:
+    n: tuple
+    spacing: tuple
+    c: float
+    dt: float
+    steps: int
+    p0: np.ndarray
+    centre: tuple
+
+
+def wave_config(n=256, steps=1000, cfl=0.4, c=1.0, sigma=0.05, seed=0):
+    """First-order acoustic system on n^3 over the unit cube: Gaussian p pulse
+    (amplitude 1, sigma 0.05) at a seeded centre in [0.25, 0.75]^3, u=v=w=0,
+    dt = cfl * dx / c."""
+    d = 1.0 / n
+    g = SplitMix64(seed)
+    centre = tuple(0.25 + 0.5 * g.uniform() for _ in range(3))
+    p0 = gaussian(n, n, n, d, d, d, centre, sigma)
+    return WaveCase((n, n, n), (d, d, d), c, cfl * d / c, steps, p0, centre)
+
+
+def wave_energy(p, u, v, w, c):
+    """Discrete acoustic energy 0.5 * sum(u^2 + v^2 + w^2 + p^2 / c^2)."""
+    return 0.5 * float(np.sum(u * u + v * v + w * w + (p * p) / (c * c)))
+
+
+def heat_free_space(nx, ny, nz, dx, dy, dz, centre, sigma, alpha, t):
+    """Free-space heat kernel: a Gaussian of variance sigma^2 spreads to
+    sigma^2 + 2 alpha t with amplitude (sigma^2 / (sigma^2 + 2 alpha t))^(3/2)."""
+    s2 = sigma * sigma + 2.0 * alpha * t
+    amp = (sigma * sigma / s2) ** 1.5
+    return gaussian(nx, ny, nz, dx, dy, dz, centre, math.sqrt(s2), amp)

@@ -1,0 +1,189 @@
+/*
+ * cfdarco.h — C-ABI of the MI355X explicit finite-volume library
+ * (libcfdarco_hip.so).
+ *
+ * Every entry point returns int: CFD_OK (0) on success, a negative CFD_E* code
+ * on failure, with a message available from cfd_last_error() on the calling
+ * thread.  Nothing here calls exit() or lets a C++ exception cross the ABI.
+ * Sizes are int64_t.  Device work is ordered on the caller's stream (a
+ * hipStream_t passed as void*, NULL = the null stream); the library only
+ * synchronises where a function returns a host value (noted per function).
+ * Field arrays are device pointers of the mesh's dtype, N = nx*ny*nz cells in
+ * the reference cell order id = nz*ny*x + nz*y + z.
+ *
+ * Each declaration names the reference interface it replaces
+ * (int0vrt/cfdARCOV2 file:line).
+ */
+#ifndef CFDARCO_H
+#define CFDARCO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFD_OK 0
+#define CFD_E_INVALID (-1)    /* bad argument (null pointer, size mismatch, bad enum) */
+#define CFD_E_HIP (-2)        /* HIP runtime error */
+#define CFD_E_NOMEM (-3)      /* pool or device allocation failed / pool limit reached */
+#define CFD_E_UNSUPPORTED (-4)/* valid request this build does not implement */
+#define CFD_E_COMPILE (-5)    /* hipRTC compilation failed (log in cfd_last_error) */
+#define CFD_E_NONFINITE (-6)  /* a state check found NaN/Inf */
+
+#define CFD_F32 0
+#define CFD_F64 1
+
+typedef struct cfd_ctx_s* cfd_ctx_t;
+typedef struct cfd_mesh_s* cfd_mesh_t;
+typedef struct cfd_program_s* cfd_program_t;
+typedef struct cfd_batch_s* cfd_batch_t;
+
+/* ---- lifetime --------------------------------------------------------------
+ * Replaces CFDArcoGlobalInit::enable_cuda device selection and RMM pool set-up
+ * (src/cfdarcho_main_3d.cpp:304-347) and gpuErrchk/exit error handling
+ * (include/cuda_operators.hpp:29-36). */
+int cfd_init(int device, cfd_ctx_t* out);
+int cfd_destroy(cfd_ctx_t ctx);
+const char* cfd_last_error(void);
+int cfd_version(void);
+
+/* ---- memory ----------------------------------------------------------------
+ * Stream-ordered pool allocation; replaces CudaDataMatrix's RMM pool
+ * allocate/deallocate (include/cuda_data_matrix.hpp:35-83) and
+ * Allocator::cuda_mem_pool (include/pool_allocator.hpp:29-33).
+ * cfd_pool_set_limit caps the bytes the pool may hand out (0 = no cap); an
+ * allocation beyond it fails with CFD_E_NOMEM instead of reaching the device. */
+int cfd_alloc(cfd_ctx_t ctx, int64_t bytes, void** dptr, void* stream);
+int cfd_free(cfd_ctx_t ctx, void* dptr, void* stream);
+int cfd_pool_set_limit(cfd_ctx_t ctx, int64_t bytes);
+int cfd_pool_stats(cfd_ctx_t ctx, int64_t* bytes_in_use, int64_t* peak_bytes);
+
+#define CFD_H2D 1
+#define CFD_D2H 2
+#define CFD_D2D 3
+/* Replaces CudaDataMatrix::from_eigen / to_eigen / deep copy
+ * (include/cuda_data_matrix.hpp:85-177).  D2H waits for the copy. */
+int cfd_memcpy(cfd_ctx_t ctx, void* dst, const void* src, int64_t bytes, int kind, void* stream);
+int cfd_stream_sync(void* stream);
+
+/* ---- mesh ------------------------------------------------------------------
+ * Replaces Mesh3D(x,y,z,dx,dy,dz) + init_basic_internals + compute
+ * (include/mesh3d.hpp:120-124, src/mesh3d.cpp:151-385) and
+ * CudaMesh3D::host_to_device (include/mesh3d.hpp:209-236).
+ * vertex_offsets (nullable) holds V = (nx+1)(ny+1)(nz+1) xyz triples, in the
+ * reference vertex order, added to the lattice coordinates x*dx before any
+ * geometry is computed (the non-uniform-mesh hook).  The library computes the
+ * reference geometry in the mesh dtype.  When every face of a family has the
+ * same geometry the mesh is stored as constants (uniform kernels, no geometry
+ * traffic); otherwise as face-ordered SoA arrays.  force_general = 1 forces the
+ * SoA form. */
+typedef struct cfd_hex_desc {
+    int64_t nx, ny, nz;
+    double dx, dy, dz;
+    int dtype;
+    int force_general;
+    const double* vertex_offsets;
+} cfd_hex_desc;
+
+typedef struct cfd_mesh_info {
+    int64_t n_cells, n_faces, n_vertices;
+    int64_t nx, ny, nz;
+    int dtype;
+    int uniform;            /* 1 = constant-geometry kernels */
+    int64_t device_bytes;   /* bytes of geometry held on the device */
+} cfd_mesh_info;
+
+int cfd_mesh_create_hex(cfd_ctx_t ctx, const cfd_hex_desc* desc, cfd_mesh_t* out);
+int cfd_mesh_destroy(cfd_mesh_t mesh);
+int cfd_mesh_get_info(cfd_mesh_t mesh, cfd_mesh_info* info);
+
+/* Reference-layout tables (N x 6 column-major, element (i,fc) at fc*N+i) copied
+ * to host memory, for parity checks against Mesh3D's tables
+ * (include/mesh3d.hpp:248-269).  ids/ids_bf are int64; the others are in the mesh
+ * dtype; vol/is_bnd/ccx/ccy/ccz are N long. */
+#define CFD_T_IDS 0
+#define CFD_T_IDS_BF 1
+#define CFD_T_NX 2
+#define CFD_T_NY 3
+#define CFD_T_NZ 4
+#define CFD_T_NAX 5
+#define CFD_T_NAY 6
+#define CFD_T_NAZ 7
+#define CFD_T_AREA 8
+#define CFD_T_ALPHA 9
+#define CFD_T_LEN 10
+#define CFD_T_VOL 11
+#define CFD_T_IS_BND 12
+#define CFD_T_CCX 13
+#define CFD_T_CCY 14
+#define CFD_T_CCZ 15
+int cfd_mesh_table(cfd_mesh_t mesh, int table, void* host_out);
+/* The same tables computed on the host straight from a descriptor, without a
+ * device or context (host-only; used for CPU-side parity checks). */
+int cfd_hex_table(const cfd_hex_desc* desc, int table, void* host_out);
+
+/* ---- operators (out is caller-owned, N long, mesh dtype) --------------------
+ * axis_mask bit 0/1/2 selects x/y/z; the selected components are summed.
+ * GradVar (d1dx/d1dy/d1dz) evaluate_cu + eval_grad (src/fvm3d.cpp:825-829,
+ * src/cuda_operators.cu:237-305).  wall_axis >= 0: reflective wall, the face
+ * value on boundary faces normal to that axis is zero; -1: reference rule. */
+int cfd_grad(cfd_mesh_t mesh, const void* phi, void* out, int axis_mask, int wall_axis, void* stream);
+/* Grad2Var (lapl, d2dx..) evaluate_cu + eval_grad2 (src/fvm3d.cpp:902-906,
+ * src/cuda_operators.cu:490-569). */
+int cfd_lapl(cfd_mesh_t mesh, const void* phi, void* out, int axis_mask, void* stream);
+/* StabVar (stab_tot, stabx..) evaluate_cu + eval_stab (src/fvm3d.cpp:938-942,
+ * src/cuda_operators.cu:307-487).  Uses a 3N scratch from the pool. */
+int cfd_stab(cfd_mesh_t mesh, const void* phi, void* out, int axis_mask, void* stream);
+
+/* Elementwise kernels: add_mtrx/sub_mtrx/mul_mtrx/div_mtrx/neg_mtrx/
+ * mul_mtrx(float)/div_const (src/cuda_operators.cu:27-136) plus the DtVar /
+ * D2tVar updates (src/fvm3d.cpp:694-697,742-751). */
+#define CFD_EW_ADD 0        /* out = a + b */
+#define CFD_EW_SUB 1        /* out = a - b */
+#define CFD_EW_MUL 2        /* out = a * b */
+#define CFD_EW_DIV 3        /* out = a / b */
+#define CFD_EW_NEG 4        /* out = -a */
+#define CFD_EW_MULS 5       /* out = a * s */
+#define CFD_EW_DIVS 6       /* out = a / s */
+#define CFD_EW_SMUL 7       /* out = s * a */
+#define CFD_EW_AXPY 8       /* out = s * a + b        (DtVar extract)   */
+#define CFD_EW_LEAP 9       /* out = (s*s) * a + 2*b - c (D2tVar extract), c passed as aux */
+#define CFD_EW_FILL 10      /* out = s */
+#define CFD_EW_COPY 11      /* out = a */
+int cfd_ewise(cfd_ctx_t ctx, int op, int dtype, const void* a, const void* b, const void* aux, double s,
+              void* out, int64_t n, void* stream);
+
+/* CFL time step, UpdatePolicies::CourantFriedrichsLewy3D/1D(Cu)
+ * (src/fvm3d.cpp:497-567, src/cuda_operators.cu:138-235):
+ * CFL * min_i dl / (sqrt(gamma p/rho) + |u|), dl = min(dx,dy,dz).  v and w may
+ * be NULL (1D form).  Synchronises the stream to return the host value. */
+int cfd_cfl_min(cfd_mesh_t mesh, const void* u, const void* v, const void* w, const void* p,
+                const void* rho, double cfl, double gamma, double* dt_out, void* stream);
+
+/* Non-finite check: *bad = number of NaN/Inf entries in x (synchronises). */
+int cfd_count_nonfinite(cfd_ctx_t ctx, int dtype, const void* x, int64_t n, int64_t* bad, void* stream);
+
+/* ---- fused explicit steps ---------------------------------------------------
+ * Heat: {d1t(T), '=', alpha * lapl(T)} (examples/main_3d_heat.cpp:89-91 with
+ * the JIT kernel of src/fvm3d.cpp:1219-1394): T <- dt*(alpha*lapl(T)) + T.
+ * alpha_field (nullable) is a per-cell coefficient.  Ping-pong between T_a and
+ * T_b: after nsteps the state is in T_a when nsteps is even, T_b when odd. */
+int cfd_heat_run(cfd_mesh_t mesh, void* T_a, void* T_b, double alpha, const void* alpha_field, double dt,
+                 int64_t nsteps, void* stream);
+
+/* First-order acoustic system, one fused launch per step, equations in order
+ *   d1t(u) = -d1dx(p); d1t(v) = -d1dy(p); d1t(w) = -d1dz(p);
+ *   d1t(p) = K * (d1dx(u) + d1dy(v) + d1dz(w))      (K = -c^2)
+ * with reflective walls (normal velocity face value zero) and zero-gradient p.
+ * state_a/state_b each hold {p, u, v, w}; after nsteps the state is in state_a
+ * when nsteps is even, state_b when odd.  K_field (nullable) is per cell.
+ * Replaces four KernelBuilder launches per step (src/fvm3d.cpp:1019-1032). */
+int cfd_wave_run(cfd_mesh_t mesh, void* const state_a[4], void* const state_b[4], double K,
+                 const void* K_field, double dt, int64_t nsteps, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFDARCO_H */

@@ -1,0 +1,462 @@
+// ORACLE — test infrastructure only.
+//
+// CPU restatement of cfdARCO's explicit finite-volume path (mesh topology and
+// geometry, face interpolation, Green-Gauss gradient, Laplacian, stabilisation
+// and the explicit time integrators), templated on the scalar type so the same
+// code is the FP32 reference-semantics checker and the FP64 oracle.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+// this code, and only as the checker or as the timed CPU baseline.  The product
+// library (cfdarcov2_amd/) never includes, links or calls it.
+//
+// Every function cites the reference file:line it restates.  Arithmetic is
+// written in the reference's operation order; compile with -ffp-contract=off so
+// no multiply-add is fused behind our back.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+namespace cfdo {
+
+template <class T> struct V3 { T x, y, z; };
+
+template <class T> inline V3<T> vsub(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+// Eigen's 3-vector cross product component formulas.
+template <class T> inline V3<T> vcross(V3<T> a, V3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// Fixed-size (3) reductions in Eigen unroll as a0 + (a1 + a2); dynamic-size
+// vectors reduce left to right, (a0 + a1) + a2.  The reference uses fixed-size
+// vectors in Face3D/Quadrangle3D::compute and dynamic ones in Mesh3D::compute.
+template <class T> inline T sum3_fixed(T a, T b, T c) { return a + (b + c); }
+template <class T> inline T sum3_dyn(T a, T b, T c) { return (a + b) + c; }
+template <class T> inline T dot_fixed(V3<T> a, V3<T> b) { return sum3_fixed(a.x * b.x, a.y * b.y, a.z * b.z); }
+template <class T> inline T dot_dyn(V3<T> a, V3<T> b) { return sum3_dyn(a.x * b.x, a.y * b.y, a.z * b.z); }
+
+// opposite local face: src/mesh3d.cpp:140-148
+inline int opposite_fc(int fc) { return fc ^ 1; }
+
+// Hex mesh with the reference's numbering and N x 6 column-major tables
+// (element (i, fc) at fc * N + i, include/decls.hpp:25-28).
+template <class T> struct HexMesh {
+    int64_t X = 0, Y = 0, Z = 0;
+    T dx = 0, dy = 0, dz = 0;
+    int64_t N = 0, F = 0, V = 0;
+
+    std::vector<T> vx, vy, vz;              // vertex coordinates
+    std::vector<int64_t> face_verts;        // F x 4
+    std::vector<int64_t> face_cells;        // F x 2 (-1 when absent)
+    std::vector<int32_t> face_ncells;       // 1 on the boundary, 2 inside
+    std::vector<T> fcx, fcy, fcz;           // face centre
+    std::vector<T> fnx, fny, fnz;           // face normal (from the first three vertices)
+    std::vector<T> farea;                   // face area (sum-of-components form)
+    std::vector<int64_t> cell_faces;        // N x 6 (row-major per cell)
+    std::vector<int64_t> cell_verts;        // N x 8
+    std::vector<T> ccx, ccy, ccz;           // cell centre
+
+    // N x 6 column-major tables
+    std::vector<int64_t> ids, ids_bf;
+    std::vector<T> nx, ny, nz;              // _normals (axis direction, Q4)
+    std::vector<T> nax, nay, naz;           // _normals_alt (outward)
+    std::vector<T> area6, alpha_d, len6;
+    std::vector<T> vol, is_bnd, is_bnd_rev;
+
+    // Mesh3D::square_node_coord_to_idx, src/mesh3d.cpp:387-389
+    int64_t cell_idx(int64_t x, int64_t y, int64_t z) const { return Z * Y * x + Z * y + z; }
+
+    // Mesh3D::square_node_coord_to_face_idx, src/mesh3d.cpp:391-403
+    void face_idx(int64_t x, int64_t y, int64_t z, int64_t out[6]) const {
+        const int64_t nzf = X * Y * (Z + 1);
+        const int64_t nyf = X * (Y + 1) * Z;
+        out[0] = (Z + 1) * Y * x + (Z + 1) * y + z;
+        out[1] = (Z + 1) * Y * x + (Z + 1) * y + (z + 1);
+        out[2] = nzf + (X * (Y + 1) * z + (Y + 1) * x + y);
+        out[3] = nzf + (X * (Y + 1) * z + (Y + 1) * x + (y + 1));
+        out[4] = nzf + nyf + ((X + 1) * Z * y + (X + 1) * z + x);
+        out[5] = nzf + nyf + ((X + 1) * Z * y + (X + 1) * z + (x + 1));
+    }
+
+    // Mesh3D::square_node_coord_to_vertex_idx, src/mesh3d.cpp:405-416
+    void vertex_idx(int64_t x, int64_t y, int64_t z, int64_t out[8]) const {
+        const int64_t sx = (Z + 1) * (Y + 1), sy = (Z + 1);
+        int k = 0;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int c = 0; c < 2; ++c) out[k++] = (x + a) * sx + (y + b) * sy + (z + c);
+    }
+};
+
+// Mesh3D::init_basic_internals, src/mesh3d.cpp:323-385.  Vertex coordinates are
+// x_ * dx in T (:328).  `perturb` (nullable, V x 3, in T) is added to the
+// coordinates before any geometry is computed: the seeded non-uniform-mesh hook.
+template <class T>
+void build_topology(HexMesh<T>& m, int64_t X, int64_t Y, int64_t Z, T dx, T dy, T dz, const T* perturb) {
+    m.X = X; m.Y = Y; m.Z = Z; m.dx = dx; m.dy = dy; m.dz = dz;
+    m.N = X * Y * Z;
+    m.V = (X + 1) * (Y + 1) * (Z + 1);
+    m.F = X * Y * (Z + 1) + Z * Y * (X + 1) + X * Z * (Y + 1);
+    m.vx.resize(m.V); m.vy.resize(m.V); m.vz.resize(m.V);
+    int64_t vid = 0;
+    for (int64_t x = 0; x < X + 1; ++x)
+        for (int64_t y = 0; y < Y + 1; ++y)
+            for (int64_t z = 0; z < Z + 1; ++z) {
+                m.vx[vid] = T(x) * dx; m.vy[vid] = T(y) * dy; m.vz[vid] = T(z) * dz;
+                if (perturb) {
+                    m.vx[vid] = m.vx[vid] + perturb[3 * vid + 0];
+                    m.vy[vid] = m.vy[vid] + perturb[3 * vid + 1];
+                    m.vz[vid] = m.vz[vid] + perturb[3 * vid + 2];
+                }
+                ++vid;
+            }
+    m.face_verts.assign(4 * m.F, 0);
+    m.face_cells.assign(2 * m.F, -1);
+    m.face_ncells.assign(m.F, 0);
+    m.cell_faces.resize(6 * m.N);
+    m.cell_verts.resize(8 * m.N);
+    // per-cell local face -> vertex quads, src/mesh3d.cpp:361-374
+    static const int quad[6][4] = {{0, 2, 6, 4}, {1, 3, 7, 5}, {0, 1, 5, 4},
+                                   {2, 3, 7, 6}, {1, 3, 2, 0}, {5, 7, 6, 4}};
+    for (int64_t x = 0; x < X; ++x)
+        for (int64_t y = 0; y < Y; ++y)
+            for (int64_t z = 0; z < Z; ++z) {
+                const int64_t c = m.cell_idx(x, y, z);
+                int64_t fid[6], vids[8];
+                m.face_idx(x, y, z, fid);
+                m.vertex_idx(x, y, z, vids);
+                for (int j = 0; j < 6; ++j) m.cell_faces[6 * c + j] = fid[j];
+                for (int j = 0; j < 8; ++j) m.cell_verts[8 * c + j] = vids[j];
+                for (int j = 0; j < 6; ++j)
+                    for (int q = 0; q < 4; ++q) m.face_verts[4 * fid[j] + q] = vids[quad[j][q]];
+                for (int j = 0; j < 6; ++j) {
+                    const int64_t f = fid[j];
+                    m.face_cells[2 * f + m.face_ncells[f]] = c;
+                    m.face_ncells[f] += 1;
+                }
+            }
+}
+
+// Face3D::compute (src/mesh3d.cpp:43-62), Quadrangle3D::compute (:69-105) and the
+// table assembly of Mesh3D::compute (:151-321).
+template <class T> void compute_geometry(HexMesh<T>& m) {
+    const int64_t N = m.N, F = m.F;
+    m.fcx.resize(F); m.fcy.resize(F); m.fcz.resize(F);
+    m.fnx.resize(F); m.fny.resize(F); m.fnz.resize(F); m.farea.resize(F);
+    auto vert = [&](int64_t v) { return V3<T>{m.vx[v], m.vy[v], m.vz[v]}; };
+#pragma omp parallel for schedule(static)
+    for (int64_t f = 0; f < F; ++f) {
+        V3<T> p[4];
+        V3<T> c{0, 0, 0};
+        for (int q = 0; q < 4; ++q) {
+            p[q] = vert(m.face_verts[4 * f + q]);
+            c = {c.x + p[q].x, c.y + p[q].y, c.z + p[q].z};
+        }
+        const V3<T> AB = vsub(p[1], p[0]), BC = vsub(p[2], p[1]), CD = vsub(p[3], p[2]);
+        const V3<T> AD = vsub(p[3], p[0]), CB = vsub(p[1], p[2]);
+        V3<T> n = vcross(AB, BC);
+        const T sq = dot_fixed(n, n);  // normalized(): divide by sqrt(squaredNorm) when > 0
+        if (sq > T(0)) {
+            const T s = std::sqrt(sq);
+            n = {n.x / s, n.y / s, n.z / s};
+        }
+        m.fnx[f] = n.x; m.fny[f] = n.y; m.fnz[f] = n.z;
+        const T four = T(4);
+        m.fcx[f] = c.x / four; m.fcy[f] = c.y / four; m.fcz[f] = c.z / four;
+        const V3<T> c1 = vcross(AB, AD), c2 = vcross(CB, CD);
+        const T s1 = std::abs(sum3_fixed(c1.x, c1.y, c1.z));
+        const T s2 = std::abs(sum3_fixed(c2.x, c2.y, c2.z));
+        m.farea[f] = T(0.5 * (s1 + s2));  // Q2: |sum of components|, src/mesh3d.cpp:61
+    }
+
+    m.ccx.resize(N); m.ccy.resize(N); m.ccz.resize(N); m.vol.resize(N);
+    const size_t n6 = size_t(6) * size_t(N);
+    m.nx.resize(n6); m.ny.resize(n6); m.nz.resize(n6);
+    m.nax.resize(n6); m.nay.resize(n6); m.naz.resize(n6);
+    m.ids.resize(n6); m.ids_bf.resize(n6); m.area6.resize(n6); m.alpha_d.resize(n6); m.len6.resize(n6);
+    m.is_bnd.resize(N); m.is_bnd_rev.resize(N);
+
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        V3<T> c{0, 0, 0};
+        for (int q = 0; q < 8; ++q) {
+            const V3<T> p = vert(m.cell_verts[8 * i + q]);
+            c = {c.x + p.x, c.y + p.y, c.z + p.z};
+        }
+        const T eight = T(8);
+        c = {c.x / eight, c.y / eight, c.z / eight};
+        m.ccx[i] = c.x; m.ccy[i] = c.y; m.ccz[i] = c.z;
+        T volume = 0;
+        for (int j = 0; j < 6; ++j) {
+            const int64_t f = m.cell_faces[6 * i + j];
+            const V3<T> n{m.fnx[f], m.fny[f], m.fnz[f]};
+            const V3<T> fc{m.fcx[f], m.fcy[f], m.fcz[f]};
+            const T a = m.farea[f];
+            V3<T> good = n;
+            if (!(dot_fixed(n, vsub(fc, c)) >= T(0))) good = {-n.x, -n.y, -n.z};
+            const size_t k = size_t(j) * size_t(N) + size_t(i);
+            // Q4: axis-direction normals, src/mesh3d.cpp:92-96
+            if (j == 4 || j == 5 || j == 0 || j == 1) {
+                m.nx[k] = -n.x; m.ny[k] = -n.y; m.nz[k] = -n.z;
+            } else {
+                m.nx[k] = n.x; m.ny[k] = n.y; m.nz[k] = n.z;
+            }
+            m.nax[k] = good.x; m.nay[k] = good.y; m.naz[k] = good.z;
+            volume += dot_fixed(fc, good) * a;  // :101
+        }
+        m.vol[i] = volume / T(3);
+    }
+
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        bool bnd = false;
+        for (int j = 0; j < 6; ++j) bnd = bnd || (m.face_ncells[m.cell_faces[6 * i + j]] == 1);
+        m.is_bnd[i] = bnd ? T(1) : T(0);
+        m.is_bnd_rev[i] = bnd ? T(0) : T(1);
+        const V3<T> c1{m.ccx[i], m.ccy[i], m.ccz[i]};
+        for (int j = 0; j < 6; ++j) {
+            const int64_t f = m.cell_faces[6 * i + j];
+            const size_t k = size_t(j) * size_t(N) + size_t(i);
+            m.area6[k] = m.farea[f];
+            const V3<T> fn{m.nx[k], m.ny[k], m.nz[k]};
+            const V3<T> fcen{m.fcx[f], m.fcy[f], m.fcz[f]};
+            const V3<T> to_face = vsub(fcen, c1);
+            if (m.face_ncells[f] > 1) {  // :240-265
+                const int64_t a0 = m.face_cells[2 * f + 0], a1 = m.face_cells[2 * f + 1];
+                const int64_t nb = (a0 == i) ? a1 : a0;
+                m.ids[k] = nb;
+                m.ids_bf[k] = nb;
+                const V3<T> d = vsub(V3<T>{m.ccx[nb], m.ccy[nb], m.ccz[nb]}, c1);
+                const T dn = std::sqrt(dot_dyn(d, d));
+                const T cos_phi = dot_dyn(fn, d) / (std::sqrt(dot_dyn(fn, fn)) * dn);
+                const T alpha = T(1) / cos_phi;
+                m.alpha_d[k] = std::abs(alpha / dn);
+                m.len6[k] = std::abs(std::sqrt(dot_dyn(to_face, to_face)));
+            } else {  // boundary: neighbour = self, bound-free id = -1 (:266-312)
+                m.ids[k] = i;
+                m.ids_bf[k] = -1;
+                const V3<T> d = to_face;
+                const T dn = std::sqrt(dot_dyn(d, d));
+                const T cos_phi = dot_dyn(fn, d) / (std::sqrt(dot_dyn(fn, fn)) * dn);
+                const T alpha = T(1) / cos_phi;
+                m.alpha_d[k] = std::abs(alpha / (dn * T(2)));
+                m.len6[k] = std::abs(std::sqrt(dot_dyn(to_face, to_face)));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Operators (include/grad_utils.hpp and src/fvm3d.cpp evaluate())
+// ---------------------------------------------------------------------------
+
+// Face value of `phi` on face fc of cell i under linear interpolation
+// (grad_utils.hpp:23-36).  `wall_axis` >= 0 selects the reflective-wall rule used
+// by the first-order acoustic system: on boundary faces normal to that axis the
+// face value is zero (antisymmetric mirror).  wall_axis < 0 is the reference
+// rule, neighbour = self, which gives phi_f = phi_c.
+template <class T>
+inline T face_linear(const HexMesh<T>& m, const T* phi, int64_t i, int fc, int wall_axis) {
+    const size_t k = size_t(fc) * size_t(m.N) + size_t(i);
+    if (wall_axis >= 0 && m.ids_bf[k] < 0 && (fc >> 1) == (2 - wall_axis)) return T(0);
+    return T((phi[i] + phi[m.ids[k]]) * 0.5);
+}
+
+// Green-Gauss sum for component d over the six faces of cell i
+// (grad_utils.hpp:82-99).  `axis_normals` = the reference's use_alt_normals=true
+// (Q5: that flag selects the axis normals).  The row sum runs fc = 0..5 from 0.
+template <class T>
+inline T gauss_component(const HexMesh<T>& m, const T* face_val6_i /*6 values*/, int64_t i, int d, bool axis_normals) {
+    const std::vector<T>* nrm;
+    if (axis_normals) nrm = d == 0 ? &m.nx : d == 1 ? &m.ny : &m.nz;
+    else nrm = d == 0 ? &m.nax : d == 1 ? &m.nay : &m.naz;
+    T acc = T(0);
+    for (int fc = 0; fc < 6; ++fc) {
+        const size_t k = size_t(fc) * size_t(m.N) + size_t(i);
+        T t = face_val6_i[fc] * (*nrm)[k];
+        t = t * m.area6[k];
+        acc = acc + t;
+    }
+    return acc / m.vol[i];
+}
+
+// GradVar::evaluate (src/fvm3d.cpp:808-823): linear interpolation, Gauss sum with
+// the outward normals, then res = 0 + g_x (+ g_y) (+ g_z) over the selected axes.
+template <class T>
+void grad(const HexMesh<T>& m, const T* phi, int axis_mask, int wall_axis, T* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m.N; ++i) {
+        T fv[6];
+        for (int fc = 0; fc < 6; ++fc) fv[fc] = face_linear(m, phi, i, fc, wall_axis);
+        T res = T(0);
+        for (int d = 0; d < 3; ++d)
+            if (axis_mask & (1 << d)) res = res + gauss_component(m, fv, i, d, false);
+        out[i] = res;
+    }
+}
+
+// Grad2Var::evaluate (src/fvm3d.cpp:880-900) with corrected_surface_normal_grad
+// (grad_utils.hpp:101-125): (-phi_c + phi_nb) * alpha_d, Gauss sum with the axis
+// normals, res = 0 + L_x (+ L_y) (+ L_z).
+template <class T>
+void lapl(const HexMesh<T>& m, const T* phi, int axis_mask, T* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m.N; ++i) {
+        T fv[6];
+        for (int fc = 0; fc < 6; ++fc) {
+            const size_t k = size_t(fc) * size_t(m.N) + size_t(i);
+            fv[fc] = (-phi[i] + phi[m.ids[k]]) * m.alpha_d[k];
+        }
+        T res = T(0);
+        for (int d = 0; d < 3; ++d)
+            if (axis_mask & (1 << d)) res = res + gauss_component(m, fv, i, d, true);
+        out[i] = res;
+    }
+}
+
+// StabVar::evaluate (src/fvm3d.cpp:915-936): linear interpolation, outward Gauss
+// gradient, upwind face extrapolation phi + len * g_d * n_out_d
+// (grad_utils.hpp:38-59), the neighbour's opposite-face value
+// (collect_vals_neigh_faces<.., false>, grad_utils.hpp:61-80; boundary reads
+// self's opposite face, Q8), 0.5 * (own - collected), Gauss with axis normals,
+// component d, summed over the selected axes from 0.
+template <class T>
+void stab(const HexMesh<T>& m, const T* phi, int axis_mask, T* out) {
+    const int64_t N = m.N;
+    std::vector<T> g[3];
+    for (int d = 0; d < 3; ++d) g[d].resize(N);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        T fv[6];
+        for (int fc = 0; fc < 6; ++fc) fv[fc] = face_linear(m, phi, i, fc, -1);
+        for (int d = 0; d < 3; ++d) g[d][i] = gauss_component(m, fv, i, d, false);
+    }
+    auto upw = [&](int d, int64_t c, int fc) {
+        const size_t k = size_t(fc) * size_t(N) + size_t(c);
+        const std::vector<T>& na = d == 0 ? m.nax : d == 1 ? m.nay : m.naz;
+        return phi[c] + m.len6[k] * g[d][c] * na[k];
+    };
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        T res = T(0);
+        for (int d = 0; d < 3; ++d) {
+            if (!(axis_mask & (1 << d))) continue;
+            T fv[6];
+            for (int fc = 0; fc < 6; ++fc) {
+                const size_t k = size_t(fc) * size_t(N) + size_t(i);
+                const int64_t nb = m.ids[k];
+                const T own = upw(d, i, fc);
+                const T col = upw(d, nb, opposite_fc(fc));
+                fv[fc] = T(0.5) * (own - col);
+            }
+            res = res + gauss_component(m, fv, i, d, true);
+        }
+        out[i] = res;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Time integration (src/fvm3d.cpp:690-765, 944-1069)
+// ---------------------------------------------------------------------------
+
+// DtVar::extract (src/fvm3d.cpp:690-692): dt * rhs + current.
+template <class T> inline T euler_update(T dt, T rhs, T cur) { return dt * rhs + cur; }
+
+// D2tVar::extract (src/fvm3d.cpp:735-740) once two history entries exist:
+// dt * dt * rhs + 2 * current - previous.
+template <class T> inline T leapfrog_update(T dt, T rhs, T cur, T prev) {
+    return ((dt * dt) * rhs + T(2) * cur) - prev;
+}
+
+// UpdatePolicies::CourantFriedrichsLewy3D (src/fvm3d.cpp:512-527): gamma = 5/3
+// regardless of the driver (Q11), dl = min(dx, dy, dz).
+template <class T>
+T cfl3d(T CFL, const HexMesh<T>& m, const T* u, const T* v, const T* w, const T* p, const T* rho) {
+    const T gamma = T(5. / 3.);
+    const T dl = std::min({m.dx, m.dy, m.dz});
+    T mn = std::numeric_limits<T>::infinity();
+    for (int64_t i = 0; i < m.N; ++i) {
+        const T a = std::sqrt((gamma * p[i]) / rho[i]);
+        const T s = std::sqrt((u[i] * u[i] + v[i] * v[i]) + w[i] * w[i]);
+        const T val = dl * (T(1) / (a + s));
+        mn = std::min(mn, val);
+    }
+    return CFL * mn;
+}
+
+// UpdatePolicies::CourantFriedrichsLewy1D (src/fvm3d.cpp:543-555).
+template <class T>
+T cfl1d(T CFL, const HexMesh<T>& m, const T* u, const T* p, const T* rho) {
+    const T gamma = T(5. / 3.);
+    const T dl = std::min({m.dx, m.dy, m.dz});
+    T mn = std::numeric_limits<T>::infinity();
+    for (int64_t i = 0; i < m.N; ++i) {
+        const T a = std::sqrt((gamma * p[i]) / rho[i]);
+        const T s = std::sqrt(u[i] * u[i]);
+        const T val = dl * (T(1) / (a + s));
+        mn = std::min(mn, val);
+    }
+    return CFL * mn;
+}
+
+// ---------------------------------------------------------------------------
+// Whole-step solvers for the benchmark systems, written as the reference's
+// Equation::evaluate would run them on the CPU: each equation's right-hand
+// side is evaluated over the whole mesh and then assigned (Jacobi within an
+// equation), the equations run in listed order (Gauss-Seidel across
+// equations), boundary callbacks after all equations (src/fvm3d.cpp:1008-1049).
+// ---------------------------------------------------------------------------
+
+// Heat: {d1t(T), '=', alpha * lapl(T)} with no boundary callback; zero-gradient
+// walls come from neighbour = self (Q6).  alpha_field (nullable) replaces the
+// constant by a per-cell coefficient.  Operators are materialised per node, as the
+// reference CPU path does.
+template <class T>
+void heat_run(const HexMesh<T>& m, T* Tf, T alpha, const T* alpha_field, T dt, int64_t steps) {
+    const int64_t N = m.N;
+    std::vector<T> L(N);
+    for (int64_t s = 0; s < steps; ++s) {
+        lapl(m, Tf, 7, L.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < N; ++i) {
+            const T a = alpha_field ? alpha_field[i] : alpha;
+            Tf[i] = euler_update(dt, a * L[i], Tf[i]);
+        }
+    }
+}
+
+// First-order acoustic system, equations in order
+//   d1t(u) = -d1dx(p); d1t(v) = -d1dy(p); d1t(w) = -d1dz(p);
+//   d1t(p) = K * (d1dx(u) + d1dy(v) + d1dz(w)),  K = -c^2 (constant or per cell)
+// with reflective walls: the normal velocity's face value is zero on the walls
+// normal to it, p is zero-gradient.  Sequential equations make this the
+// symplectic Euler scheme.
+template <class T>
+void wave_run(const HexMesh<T>& m, T* p, T* u, T* v, T* w, T K, const T* K_field, T dt, int64_t steps) {
+    const int64_t N = m.N;
+    std::vector<T> g(N), gx(N), gy(N), gz(N);
+    T* vel[3] = {u, v, w};
+    for (int64_t s = 0; s < steps; ++s) {
+        for (int d = 0; d < 3; ++d) {
+            grad(m, p, 1 << d, -1, g.data());
+            T* q = vel[d];
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < N; ++i) q[i] = euler_update(dt, -g[i], q[i]);
+        }
+        grad(m, u, 1, 0, gx.data());
+        grad(m, v, 2, 1, gy.data());
+        grad(m, w, 4, 2, gz.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < N; ++i) {
+            const T div = (gx[i] + gy[i]) + gz[i];
+            const T k = K_field ? K_field[i] : K;
+            p[i] = euler_update(dt, k * div, p[i]);
+        }
+    }
+}
+
+}  // namespace cfdo

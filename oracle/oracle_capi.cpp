@@ -1,0 +1,153 @@
+// ORACLE — test infrastructure only (see cfd_oracle.hpp).
+// extern "C" surface of the CPU oracle so pytest and the CPU-baseline leg of
+// bench.py can drive it through ctypes.  Scalars arrive as double and are
+// converted to the mesh's scalar type once, at entry.
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "cfd_oracle.hpp"
+
+namespace {
+struct OMesh {
+    int dtype;  // 0 = float, 1 = double
+    cfdo::HexMesh<float> f;
+    cfdo::HexMesh<double> d;
+};
+
+template <class T> cfdo::HexMesh<T>& pick(OMesh* m);
+template <> cfdo::HexMesh<float>& pick<float>(OMesh* m) { return m->f; }
+template <> cfdo::HexMesh<double>& pick<double>(OMesh* m) { return m->d; }
+
+template <class T>
+void build(OMesh* om, int64_t X, int64_t Y, int64_t Z, double dx, double dy, double dz, const double* perturb) {
+    auto& m = pick<T>(om);
+    std::vector<T> pt;
+    if (perturb) {
+        const int64_t V = (X + 1) * (Y + 1) * (Z + 1);
+        pt.resize(3 * V);
+        for (int64_t k = 0; k < 3 * V; ++k) pt[k] = T(perturb[k]);
+    }
+    cfdo::build_topology<T>(m, X, Y, Z, T(dx), T(dy), T(dz), perturb ? pt.data() : nullptr);
+    cfdo::compute_geometry<T>(m);
+}
+
+template <class V> int copy_vec(const std::vector<V>& v, void* out) {
+    std::memcpy(out, v.data(), v.size() * sizeof(V));
+    return 0;
+}
+
+template <class T> int table(OMesh* om, const std::string& nm, void* out) {
+    auto& m = pick<T>(om);
+    if (nm == "ids") return copy_vec(m.ids, out);
+    if (nm == "ids_bf") return copy_vec(m.ids_bf, out);
+    if (nm == "nx") return copy_vec(m.nx, out);
+    if (nm == "ny") return copy_vec(m.ny, out);
+    if (nm == "nz") return copy_vec(m.nz, out);
+    if (nm == "nax") return copy_vec(m.nax, out);
+    if (nm == "nay") return copy_vec(m.nay, out);
+    if (nm == "naz") return copy_vec(m.naz, out);
+    if (nm == "area") return copy_vec(m.area6, out);
+    if (nm == "alpha") return copy_vec(m.alpha_d, out);
+    if (nm == "len") return copy_vec(m.len6, out);
+    if (nm == "vol") return copy_vec(m.vol, out);
+    if (nm == "is_bnd") return copy_vec(m.is_bnd, out);
+    if (nm == "ccx") return copy_vec(m.ccx, out);
+    if (nm == "ccy") return copy_vec(m.ccy, out);
+    if (nm == "ccz") return copy_vec(m.ccz, out);
+    if (nm == "vx") return copy_vec(m.vx, out);
+    if (nm == "vy") return copy_vec(m.vy, out);
+    if (nm == "vz") return copy_vec(m.vz, out);
+    if (nm == "face_verts") return copy_vec(m.face_verts, out);
+    if (nm == "face_cells") return copy_vec(m.face_cells, out);
+    if (nm == "cell_faces") return copy_vec(m.cell_faces, out);
+    if (nm == "cell_verts") return copy_vec(m.cell_verts, out);
+    if (nm == "fnx") return copy_vec(m.fnx, out);
+    if (nm == "fny") return copy_vec(m.fny, out);
+    if (nm == "fnz") return copy_vec(m.fnz, out);
+    if (nm == "farea") return copy_vec(m.farea, out);
+    return -1;
+}
+}  // namespace
+
+extern "C" {
+
+void* oracle_mesh_create(int64_t X, int64_t Y, int64_t Z, double dx, double dy, double dz, int dtype,
+                         const double* perturb) {
+    auto* om = new OMesh();
+    om->dtype = dtype;
+    if (dtype == 0) build<float>(om, X, Y, Z, dx, dy, dz, perturb);
+    else build<double>(om, X, Y, Z, dx, dy, dz, perturb);
+    return om;
+}
+
+void oracle_mesh_destroy(void* h) { delete static_cast<OMesh*>(h); }
+
+void oracle_mesh_sizes(void* h, int64_t* out) {
+    auto* om = static_cast<OMesh*>(h);
+    if (om->dtype == 0) { out[0] = om->f.N; out[1] = om->f.F; out[2] = om->f.V; }
+    else { out[0] = om->d.N; out[1] = om->d.F; out[2] = om->d.V; }
+}
+
+int oracle_mesh_table(void* h, const char* name, void* out) {
+    auto* om = static_cast<OMesh*>(h);
+    return om->dtype == 0 ? table<float>(om, name, out) : table<double>(om, name, out);
+}
+
+#define ORACLE_DISPATCH(om, CALL_F, CALL_D) \
+    do { if ((om)->dtype == 0) { CALL_F; } else { CALL_D; } } while (0)
+
+void oracle_grad(void* h, const void* phi, int mask, int wall_axis, void* out) {
+    auto* om = static_cast<OMesh*>(h);
+    ORACLE_DISPATCH(om, cfdo::grad(om->f, (const float*)phi, mask, wall_axis, (float*)out),
+                    cfdo::grad(om->d, (const double*)phi, mask, wall_axis, (double*)out));
+}
+
+void oracle_lapl(void* h, const void* phi, int mask, void* out) {
+    auto* om = static_cast<OMesh*>(h);
+    ORACLE_DISPATCH(om, cfdo::lapl(om->f, (const float*)phi, mask, (float*)out),
+                    cfdo::lapl(om->d, (const double*)phi, mask, (double*)out));
+}
+
+void oracle_stab(void* h, const void* phi, int mask, void* out) {
+    auto* om = static_cast<OMesh*>(h);
+    ORACLE_DISPATCH(om, cfdo::stab(om->f, (const float*)phi, mask, (float*)out),
+                    cfdo::stab(om->d, (const double*)phi, mask, (double*)out));
+}
+
+void oracle_heat_run(void* h, void* Tf, double alpha, const void* alpha_field, double dt, int64_t steps) {
+    auto* om = static_cast<OMesh*>(h);
+    ORACLE_DISPATCH(om,
+                    cfdo::heat_run(om->f, (float*)Tf, float(alpha), (const float*)alpha_field, float(dt), steps),
+                    cfdo::heat_run(om->d, (double*)Tf, alpha, (const double*)alpha_field, dt, steps));
+}
+
+void oracle_wave_run(void* h, void* p, void* u, void* v, void* w, double K, const void* K_field, double dt,
+                     int64_t steps) {
+    auto* om = static_cast<OMesh*>(h);
+    ORACLE_DISPATCH(om,
+                    cfdo::wave_run(om->f, (float*)p, (float*)u, (float*)v, (float*)w, float(K),
+                                   (const float*)K_field, float(dt), steps),
+                    cfdo::wave_run(om->d, (double*)p, (double*)u, (double*)v, (double*)w, K,
+                                   (const double*)K_field, dt, steps));
+}
+
+double oracle_cfl3d(void* h, double CFL, const void* u, const void* v, const void* w, const void* p,
+                    const void* rho) {
+    auto* om = static_cast<OMesh*>(h);
+    if (om->dtype == 0)
+        return cfdo::cfl3d(float(CFL), om->f, (const float*)u, (const float*)v, (const float*)w, (const float*)p,
+                           (const float*)rho);
+    return cfdo::cfl3d(CFL, om->d, (const double*)u, (const double*)v, (const double*)w, (const double*)p,
+                       (const double*)rho);
+}
+
+double oracle_cfl1d(void* h, double CFL, const void* u, const void* p, const void* rho) {
+    auto* om = static_cast<OMesh*>(h);
+    if (om->dtype == 0)
+        return cfdo::cfl1d(float(CFL), om->f, (const float*)u, (const float*)p, (const float*)rho);
+    return cfdo::cfl1d(CFL, om->d, (const double*)u, (const double*)p, (const double*)rho);
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+"""ORACLE — test infrastructure only.
+
+ctypes binding of the CPU oracle (``liboracle.so``, built from
+``oracle/cfd_oracle.hpp`` by ``oracle/Makefile``).  Only tests, the smoke check
+and the CPU-baseline leg of the benchmark import this module; the product path
+never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_F32, _F64 = 0, 1
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with make (the recipe sits beside the sources)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, i64, d, i = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int
+        L.oracle_mesh_create.restype = vp
+        L.oracle_mesh_create.argtypes = [i64, i64, i64, d, d, d, i, vp]
+        L.oracle_mesh_destroy.argtypes = [vp]
+        L.oracle_mesh_sizes.argtypes = [vp, vp]
+        L.oracle_mesh_table.argtypes = [vp, ctypes.c_char_p, vp]
+        L.oracle_mesh_table.restype = i
+        L.oracle_grad.argtypes = [vp, vp, i, i, vp]
+        L.oracle_lapl.argtypes = [vp, vp, i, vp]
+        L.oracle_stab.argtypes = [vp, vp, i, vp]
+        L.oracle_heat_run.argtypes = [vp, vp, d, vp, d, i64]
+        L.oracle_wave_run.argtypes = [vp, vp, vp, vp, vp, d, vp, d, i64]
+        L.oracle_cfl3d.restype = d
+        L.oracle_cfl3d.argtypes = [vp, d, vp, vp, vp, vp, vp]
+        L.oracle_cfl1d.restype = d
+        L.oracle_cfl1d.argtypes = [vp, d, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+_INT_TABLES = {"ids", "ids_bf", "face_verts", "face_cells", "cell_faces", "cell_verts"}
+
+
+class OracleMesh:
+    """Reference-semantics hex mesh (Mesh3D::init_basic_internals + compute)."""
+
+    def __init__(self, X, Y, Z, dx, dy, dz, dtype=np.float64, perturb=None):
+        self.dtype = np.dtype(dtype)
+        self.shape = (int(X), int(Y), int(Z))
+        self.spacing = (float(dx), float(dy), float(dz))
+        pert = None
+        if perturb is not None:
+            pert = np.ascontiguousarray(perturb, dtype=np.float64)
+        code = _F32 if self.dtype == np.float32 else _F64
+        self._h = lib().oracle_mesh_create(X, Y, Z, dx, dy, dz, code, _ptr(pert))
+        sizes = np.zeros(3, dtype=np.int64)
+        lib().oracle_mesh_sizes(self._h, _ptr(sizes))
+        self.N, self.F, self.V = (int(s) for s in sizes)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.oracle_mesh_destroy(h)
+            self._h = None
+
+    def table(self, name: str) -> np.ndarray:
+        n6 = 6 * self.N
+        sizes = {"face_verts": 4 * self.F, "face_cells": 2 * self.F, "cell_faces": 6 * self.N,
+                 "cell_verts": 8 * self.N, "vx": self.V, "vy": self.V, "vz": self.V,
+                 "fnx": self.F, "fny": self.F, "fnz": self.F, "farea": self.F,
+                 "vol": self.N, "is_bnd": self.N, "ccx": self.N, "ccy": self.N, "ccz": self.N}
+        n = sizes.get(name, n6)
+        dt = np.int64 if name in _INT_TABLES else self.dtype
+        out = np.empty(n, dtype=dt)
+        rc = lib().oracle_mesh_table(self._h, name.encode(), _ptr(out))
+        if rc != 0:
+            raise KeyError(name)
+        if n == n6 and name not in ("cell_faces",):
+            return out.reshape(6, self.N)  # column-major N x 6: row fc holds face fc of every cell
+        return out
+
+    def _arr(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.shape == (self.N,), a.shape
+        return a
+
+    def grad(self, phi, mask=7, wall_axis=-1):
+        phi = self._arr(phi)
+        out = np.empty_like(phi)
+        lib().oracle_grad(self._h, _ptr(phi), int(mask), int(wall_axis), _ptr(out))
+        return out
+
+    def lapl(self, phi, mask=7):
+        phi = self._arr(phi)
+        out = np.empty_like(phi)
+        lib().oracle_lapl(self._h, _ptr(phi), int(mask), _ptr(out))
+        return out
+
+    def stab(self, phi, mask=7):
+        phi = self._arr(phi)
+        out = np.empty_like(phi)
+        lib().oracle_stab(self._h, _ptr(phi), int(mask), _ptr(out))
+        return out
+
+    def heat_run(self, T, alpha, dt, steps, alpha_field=None):
+        T = self._arr(T).copy()
+        af = None if alpha_field is None else self._arr(alpha_field)
+        lib().oracle_heat_run(self._h, _ptr(T), float(alpha), _ptr(af), float(dt), int(steps))
+        return T
+
+    def wave_run(self, p, u, v, w, K, dt, steps, K_field=None):
+        p, u, v, w = (self._arr(a).copy() for a in (p, u, v, w))
+        kf = None if K_field is None else self._arr(K_field)
+        lib().oracle_wave_run(self._h, _ptr(p), _ptr(u), _ptr(v), _ptr(w), float(K), _ptr(kf),
+                              float(dt), int(steps))
+        return p, u, v, w
+
+    def cfl3d(self, CFL, u, v, w, p, rho):
+        u, v, w, p, rho = (self._arr(a) for a in (u, v, w, p, rho))
+        return lib().oracle_cfl3d(self._h, float(CFL), _ptr(u), _ptr(v), _ptr(w), _ptr(p), _ptr(rho))
+
+    def cfl1d(self, CFL, u, p, rho):
+        u, p, rho = (self._arr(a) for a in (u, p, rho))
+        return lib().oracle_cfl1d(self._h, float(CFL), _ptr(u), _ptr(p), _ptr(rho))

@@ -1,0 +1,236 @@
+"""GPU parity: the HIP library (through the C-ABI) against the CPU oracle on
+the same seeded inputs.
+
+Tolerance policy: the library and the oracle evaluate the same operation
+sequence with no FMA contraction, so results are compared for exact equality
+(==) in FP64 and FP32 alike, which is stronger than the 1e-12 relative L2
+required for FP64.  FP32-vs-FP64 error is reported separately.
+"""
+import numpy as np
+import pytest
+
+from cfdarcov2_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _t(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).cuda()
+
+
+def _np(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+MESHES = [
+    # (name, shape, spacing, perturb seed, force_general)
+    ("uniform", (9, 7, 10), (0.5, 0.25, 0.125), None, False),
+    ("uniform-as-general", (9, 7, 10), (0.5, 0.25, 0.125), None, True),
+    ("non-dyadic", (6, 8, 5), (0.1, 0.3, 0.7), None, False),
+    ("perturbed", (8, 6, 7), (0.125, 0.25, 0.5), 42, False),
+]
+
+
+def _pair(hip_ctx, oracle_lib, case, dtype):
+    from cfdarcov2_amd import capi
+    name, n, d, seed, force = case
+    off = None if seed is None else workloads.perturbed_vertex_offsets(*n, *d, seed=seed, frac=0.2)
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=dtype, vertex_offsets=off, force_general=force)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=dtype, perturb=off)
+    return gm, om
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("case", MESHES, ids=[c[0] for c in MESHES])
+def test_operators_match_oracle(hip_ctx, oracle_lib, case, dtype):
+    print(f"\n[case] operators {case[0]} {np.dtype(dtype).name}", flush=True)
+    gm, om = _pair(hip_ctx, oracle_lib, case, dtype)
+    info = gm.info()
+    if case[0] == "uniform":
+        assert info.uniform == 1
+    if case[0] in ("uniform-as-general", "perturbed"):
+        assert info.uniform == 0
+    rng = np.random.default_rng(7)
+    phi = rng.standard_normal(om.N).astype(dtype)
+    tphi = _t(phi, dtype)
+    for mask in (1, 2, 4, 7, 5):
+        for wall in (-1, 0, 1, 2):
+            got = _np(gm.grad(tphi, mask, wall))
+            ref = om.grad(phi, mask, wall)
+            assert np.array_equal(got, ref), f"grad mask={mask} wall={wall}"
+        assert np.array_equal(_np(gm.lapl(tphi, mask)), om.lapl(phi, mask)), f"lapl mask={mask}"
+        assert np.array_equal(_np(gm.stab(tphi, mask)), om.stab(phi, mask)), f"stab mask={mask}"
+    gm.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_heat_config_bit_exact(hip_ctx, oracle_lib, dtype):
+    """Heat 64^3, Gaussian, alpha 1e-3, dt = 0.2 dx^2/alpha, 200 steps.  r = 0.2
+    exceeds the 3D explicit limit 1/6, so round-off grows ~1.4x per step; only
+    an identical operation order keeps GPU and oracle together."""
+    from cfdarcov2_amd import capi
+    print(f"\n[case] heat 64^3 {np.dtype(dtype).name}", flush=True)
+    case = workloads.heat_config()
+    gm = capi.HexMesh(hip_ctx, *case.n, *case.spacing, dtype=dtype)
+    om = oracle_lib.OracleMesh(*case.n, *case.spacing, dtype=dtype)
+    T0 = case.T0.astype(dtype)
+    a, b = _t(T0, dtype), torch.empty_like(_t(T0, dtype))
+    res = gm.heat_run(a, b, case.alpha, case.dt, case.steps)
+    got = _np(res)
+    ref = om.heat_run(T0, case.alpha, case.dt, case.steps)
+    assert np.array_equal(got, ref)
+    if dtype == np.float32:
+        o64 = oracle_lib.OracleMesh(*case.n, *case.spacing, dtype=np.float64)
+        r64 = o64.heat_run(case.T0, case.alpha, case.dt, case.steps)
+        err = np.linalg.norm(got - r64) / np.linalg.norm(r64)
+        print(f"heat FP32 vs FP64 oracle rel L2 after {case.steps} steps (unstable dt): {err:.3e}")
+
+
+def test_heat_alpha_field_and_general_mesh(hip_ctx, oracle_lib):
+    from cfdarcov2_amd import capi
+    n, d = (10, 9, 8), (0.1, 0.1, 0.1)
+    off = workloads.perturbed_vertex_offsets(*n, *d, seed=9, frac=0.15)
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=np.float64, vertex_offsets=off)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=np.float64, perturb=off)
+    rng = np.random.default_rng(3)
+    T0 = rng.random(om.N)
+    af = 1e-3 * (1 + rng.random(om.N))
+    a, b = _t(T0, np.float64), torch.empty(om.N, dtype=torch.float64, device="cuda")
+    got = _np(gm.heat_run(a, b, 0.0, 0.5, 25, alpha_field=_t(af, np.float64)))
+    ref = om.heat_run(T0, 0.0, 0.5, 25, alpha_field=af)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [(32, 32, 32), (17, 9, 70)])
+def test_wave_fused_matches_sequential_oracle(hip_ctx, oracle_lib, dtype, n):
+    from cfdarcov2_amd import capi
+    print(f"\n[case] wave {n} {np.dtype(dtype).name}", flush=True)
+    d = tuple(1.0 / k for k in n)
+    case = workloads.wave_config(n=n[0], steps=0, seed=3)
+    X, Y, Z = workloads.cell_centres(*n, *d)
+    p0 = np.exp(-((X - 0.4) ** 2 + (Y - 0.55) ** 2 + (Z - 0.5) ** 2) / (2 * 0.1 ** 2))
+    dt = 0.4 * min(d)
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=dtype)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=dtype)
+    N = om.N
+    z = np.zeros(N, dtype=dtype)
+    A = [_t(p0.astype(dtype), dtype)] + [_t(z, dtype) for _ in range(3)]
+    B = [torch.empty_like(A[0]) for _ in range(4)]
+    steps = 60
+    got = [_np(t) for t in gm.wave_run(A, B, -1.0, dt, steps)]
+    ref = om.wave_run(p0.astype(dtype), z, z, z, -1.0, dt, steps)
+    for name, g, r in zip("puvw", got, ref):
+        assert np.array_equal(g, r), f"field {name}: max diff {np.max(np.abs(g - r))}"
+
+
+def test_wave_K_field(hip_ctx, oracle_lib):
+    from cfdarcov2_amd import capi
+    n = (12, 10, 14)
+    d = (0.1, 0.1, 0.1)
+    rng = np.random.default_rng(5)
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=np.float64)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=np.float64)
+    p0, u0, v0, w0 = (rng.standard_normal(om.N) for _ in range(4))
+    K = -(0.5 + rng.random(om.N)) ** 2
+    A = [_t(x, np.float64) for x in (p0, u0, v0, w0)]
+    B = [torch.empty_like(A[0]) for _ in range(4)]
+    got = [_np(t) for t in gm.wave_run(A, B, 0.0, 0.02, 9, K_field=_t(K, np.float64))]
+    ref = om.wave_run(p0, u0, v0, w0, 0.0, 0.02, 9, K_field=K)
+    for g, r in zip(got, ref):
+        assert np.array_equal(g, r)
+
+
+def test_wave_full_size_properties(hip_ctx, oracle_lib):
+    """256^3 FP32 benchmark case: (a) exact linearity, run(2 p0) == 2 run(p0)
+    (scaling by 2 is exact in binary floating point); (b) FP32 stays within
+    the FP64 path's energy to 1e-5 after 100 steps; (c) bounded energy."""
+    from cfdarcov2_amd import capi
+    print("\n[case] wave 256^3 properties", flush=True)
+    case = workloads.wave_config(n=256, steps=100)
+    g32 = capi.HexMesh(hip_ctx, *case.n, *case.spacing, dtype=np.float32)
+    N = g32.N
+
+    def run(mesh, p0, dtype, steps):
+        A = [_t(p0.astype(dtype), dtype)] + [torch.zeros(N, dtype=A0dt(dtype), device="cuda") for _ in range(3)]
+        B = [torch.empty_like(A[0]) for _ in range(4)]
+        return mesh.wave_run(A, B, -case.c ** 2, case.dt, steps)
+
+    def A0dt(dtype):
+        return torch.float32 if dtype == np.float32 else torch.float64
+
+    r1 = run(g32, case.p0, np.float32, case.steps)
+    r2 = run(g32, 2.0 * case.p0, np.float32, case.steps)
+    for a, b in zip(r1, r2):
+        assert torch.equal(2 * a, b)
+    E0 = workloads.wave_energy(case.p0, 0, 0, 0, 1.0)
+    e32 = workloads.wave_energy(*[_np(t).astype(np.float64) for t in r1], 1.0)
+    del r2
+    g64 = capi.HexMesh(hip_ctx, *case.n, *case.spacing, dtype=np.float64)
+    r64 = run(g64, case.p0, np.float64, case.steps)
+    f64 = [_np(t) for t in r64]
+    e64 = workloads.wave_energy(*f64, 1.0)
+    assert abs(e32 - e64) < 1e-5 * E0
+    assert 0.8 * E0 < e64 < 1.1 * E0
+    rel = [np.linalg.norm(_np(a).astype(np.float64) - b) / np.linalg.norm(f64[0]) for a, b in zip(r1, f64)]
+    print("wave 256^3 FP32 vs FP64 rel L2 (p,u,v,w) after 100 steps:", ", ".join(f"{x:.2e}" for x in rel))
+    assert max(rel) < 1e-4
+
+
+def test_ewise_and_reductions(hip_ctx, oracle_lib):
+    from cfdarcov2_amd import capi
+    rng = np.random.default_rng(11)
+    for dtype in (np.float32, np.float64):
+        a, b, c = (rng.standard_normal(1000).astype(dtype) for _ in range(3))
+        b = np.where(np.abs(b) < 0.1, 0.5, b).astype(dtype)
+        ta, tb, tc = _t(a, dtype), _t(b, dtype), _t(c, dtype)
+        out = torch.empty_like(ta)
+        s = dtype(0.37)
+        expect = {"add": a + b, "sub": a - b, "mul": a * b, "div": a / b, "neg": -a, "muls": a * s,
+                  "divs": a / s, "smul": s * a, "axpy": s * a + b, "leap": ((s * s) * a + dtype(2) * b) - c,
+                  "fill": np.full_like(a, s), "copy": a}
+        for op, ref in expect.items():
+            hip_ctx.ewise(op, ta, tb, tc, float(s), out)
+            assert np.array_equal(_np(out), ref.astype(dtype)), op
+    # CFL minimum against the oracle (3D and 1D forms)
+    n, d = (8, 8, 8), (0.5, 0.25, 0.125)
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=np.float64)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=np.float64)
+    rho, p = 1 + rng.random(om.N), 1 + rng.random(om.N)
+    u, v, w = (rng.standard_normal(om.N) for _ in range(3))
+    T = lambda x: _t(x, np.float64)
+    assert gm.cfl_min(T(u), T(p), T(rho), T(v), T(w), 0.4) == om.cfl3d(0.4, u, v, w, p, rho)
+    assert gm.cfl_min(T(u), T(p), T(rho), None, None, 0.4) == om.cfl1d(0.4, u, p, rho)
+    x = rng.standard_normal(5000)
+    x[[3, 77, 4999]] = [np.nan, np.inf, -np.inf]
+    assert hip_ctx.count_nonfinite(T(x)) == 3
+
+
+def test_error_paths(hip_ctx):
+    """C-ABI misuse and pool exhaustion come back as status codes."""
+    from cfdarcov2_amd import capi
+    with pytest.raises(capi.CfdError) as e:
+        capi.HexMesh(hip_ctx, 0, 4, 4, 1.0, 1.0, 1.0)
+    assert e.value.code == -1
+    gm = capi.HexMesh(hip_ctx, 4, 4, 4, 1.0, 1.0, 1.0, dtype=np.float32)
+    t = torch.zeros(64, device="cuda")
+    with pytest.raises(capi.CfdError):
+        gm.lib.cfd_grad(gm.h, None, None, 1, -1, None) and capi.check(-1)
+    rc = gm.lib.cfd_grad(gm.h, capi._dptr(t), capi._dptr(t), 9, -1, None)
+    assert rc == -1 and b"axis_mask" in gm.lib.cfd_last_error()
+    with pytest.raises(capi.CfdError) as e:
+        gm.wave_run([t, t, t, t], [t, t, t, t], -1.0, 0.1, 1)
+    assert e.value.code == -1
+    used, _ = hip_ctx.pool_stats()
+    hip_ctx.pool_set_limit(used + 1024)
+    try:
+        with pytest.raises(capi.CfdError) as e:
+            hip_ctx.alloc(1 << 20)
+        assert e.value.code == -3
+    finally:
+        hip_ctx.pool_set_limit(0)
+    p = hip_ctx.alloc(1 << 20)
+    hip_ctx.free(p)

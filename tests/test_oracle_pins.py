@@ -1,0 +1,318 @@
+"""Pins for the CPU oracle (oracle/cfd_oracle.hpp).
+
+The reference ships no tests, fixtures or golden vectors, so each oracle
+function is pinned by facts that do not restate its formula: brute-force
+enumeration of the hex topology, closed-form geometry, polynomial exactness of
+the operators, the odd-even mode of the stabilisation, discrete conservation,
+the free-space heat kernel and the bounded energy of the symplectic acoustic
+scheme.  Each check is chosen so a plausible slip (a sign, a swapped face, a
+wrong index or a dropped term) fails it.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from cfdarcov2_amd import workloads
+
+
+def _mesh(oracle_lib, n, d, dtype=np.float64, perturb=None):
+    return oracle_lib.OracleMesh(*n, *d, dtype=dtype, perturb=perturb)
+
+
+# ---------------------------------------------------------------------------
+# topology (src/mesh3d.cpp:323-416)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [(1, 1, 1), (2, 3, 4), (5, 1, 1), (3, 4, 5)])
+def test_topology_brute_force(oracle_lib, n):
+    X, Y, Z = n
+    m = _mesh(oracle_lib, n, (1.0, 1.0, 1.0))
+    N = X * Y * Z
+    assert m.N == N
+    assert m.F == X * Y * (Z + 1) + X * (Y + 1) * Z + (X + 1) * Y * Z
+    assert m.V == (X + 1) * (Y + 1) * (Z + 1)
+    ids, idsbf = m.table("ids"), m.table("ids_bf")
+    # neighbour across local face fc: 0 z-, 1 z+, 2 y-, 3 y+, 4 x-, 5 x+
+    step = {0: (0, 0, -1), 1: (0, 0, 1), 2: (0, -1, 0), 3: (0, 1, 0), 4: (-1, 0, 0), 5: (1, 0, 0)}
+    cid = lambda x, y, z: (x * Y + y) * Z + z
+    for x, y, z in itertools.product(range(X), range(Y), range(Z)):
+        i = cid(x, y, z)
+        for fc, (a, b, c) in step.items():
+            q = (x + a, y + b, z + c)
+            inside = 0 <= q[0] < X and 0 <= q[1] < Y and 0 <= q[2] < Z
+            assert ids[fc, i] == (cid(*q) if inside else i)
+            assert idsbf[fc, i] == (cid(*q) if inside else -1)
+    # each face id is used by exactly the cells that share it; face vertices
+    # are the four vertices both cells have in common
+    cf = m.table("cell_faces").reshape(N, 6)
+    cv = m.table("cell_verts").reshape(N, 8)
+    fv = m.table("face_verts").reshape(m.F, 4)
+    use = np.zeros(m.F, dtype=int)
+    for i in range(N):
+        for fc in range(6):
+            use[cf[i, fc]] += 1
+            assert set(fv[cf[i, fc]]) <= set(cv[i])
+    assert use.min() >= 1 and use.max() <= 2
+    n_bnd_faces = 2 * (X * Y + Y * Z + X * Z)
+    assert (use == 1).sum() == n_bnd_faces
+    for i in range(N):
+        for fc in range(6):
+            j = ids[fc, i]
+            if j != i:
+                assert cf[j, fc ^ 1] == cf[i, fc]
+                assert set(fv[cf[i, fc]]) == set(cv[i]) & set(cv[j])
+    # vertex coordinates sit on the lattice in the reference vertex order
+    vx, vy, vz = m.table("vx"), m.table("vy"), m.table("vz")
+    gx, gy, gz = np.meshgrid(np.arange(X + 1), np.arange(Y + 1), np.arange(Z + 1), indexing="ij")
+    assert np.array_equal(vx, gx.ravel()) and np.array_equal(vy, gy.ravel()) and np.array_equal(vz, gz.ravel())
+
+
+# ---------------------------------------------------------------------------
+# geometry closed forms (Face3D/Quadrangle3D/Mesh3D::compute)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [(0.5, 0.25, 0.125), (1.0, 1.0, 1.0), (0.1, 0.2, 0.3)])
+def test_uniform_geometry_closed_form(oracle_lib, dtype, d):
+    n = (4, 3, 5)
+    m = _mesh(oracle_lib, n, d, dtype)
+    dx, dy, dz = d
+    rtol = 1e-6 if dtype == np.float32 else 1e-13
+    area = m.table("area")
+    for fc, a in zip(range(6), [dx * dy, dx * dy, dx * dz, dx * dz, dy * dz, dy * dz]):
+        np.testing.assert_allclose(area[fc], a, rtol=rtol)
+    alpha = m.table("alpha")
+    for fc, h in zip(range(6), [dz, dz, dy, dy, dx, dx]):
+        np.testing.assert_allclose(alpha[fc], 1.0 / h, rtol=rtol)
+    ln = m.table("len")
+    for fc, h in zip(range(6), [dz, dz, dy, dy, dx, dx]):
+        np.testing.assert_allclose(ln[fc], h / 2, rtol=rtol)
+    np.testing.assert_allclose(m.table("vol"), dx * dy * dz, rtol=rtol)
+    # axis normals (Q4) are +e on both faces of a pair; outward normals flip on
+    # the lower face
+    ax = {0: "nz", 1: "nz", 2: "ny", 3: "ny", 4: "nx", 5: "nx"}
+    out = {0: "naz", 1: "naz", 2: "nay", 3: "nay", 4: "nax", 5: "nax"}
+    for fc in range(6):
+        np.testing.assert_allclose(m.table(ax[fc])[fc], 1.0, rtol=rtol)
+        np.testing.assert_allclose(m.table(out[fc])[fc], -1.0 if fc % 2 == 0 else 1.0, rtol=rtol)
+        for other in ({"nx", "ny", "nz"} - {ax[fc]}):
+            assert np.all(m.table(other)[fc] == 0)
+    X, Y, Z = workloads.cell_centres(*n, *d)
+    np.testing.assert_allclose(m.table("ccx"), X, rtol=rtol)
+    np.testing.assert_allclose(m.table("ccz"), Z, rtol=rtol)
+
+
+def test_area_sum_of_components_quirk(oracle_lib):
+    """Q2: face area = |A_vec . (1,1,1)|, not |A_vec|.  Tilting the top face of a
+    unit cube so its normal is along (-s, 0, 1) makes the reference area
+    dx*dy*|1 - s| (the true area is dx*dy*sqrt(1 + s^2))."""
+    s = 0.5
+    off = np.zeros((8, 3))
+    # vertex order: x slowest, z fastest; lift the z=1 vertices by s * x
+    for k, (x, y, z) in enumerate(itertools.product(range(2), range(2), range(2))):
+        if z == 1:
+            off[k, 2] = s * x
+    m = _mesh(oracle_lib, (1, 1, 1), (1.0, 1.0, 1.0), np.float64, perturb=off)
+    top = m.table("area")[1, 0]
+    assert top == pytest.approx(abs(1 - s), rel=1e-14)
+    bottom = m.table("area")[0, 0]
+    assert bottom == pytest.approx(1.0, rel=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# operators (include/grad_utils.hpp, src/fvm3d.cpp evaluate())
+# ---------------------------------------------------------------------------
+N3 = (7, 6, 8)
+D3 = (0.3, 0.2, 0.1)
+
+
+def _coords():
+    return workloads.cell_centres(*N3, *D3)
+
+
+def _idx():
+    return np.meshgrid(np.arange(N3[0]), np.arange(N3[1]), np.arange(N3[2]), indexing="ij")
+
+
+def test_grad_linear_exact(oracle_lib):
+    m = _mesh(oracle_lib, N3, D3)
+    X, Y, Z = _coords()
+    ix, iy, iz = (a.ravel() for a in _idx())
+    a, b, c = 1.5, -2.0, 0.75
+    phi = a * X + b * Y + c * Z + 3.0
+    for mask, k, coef, idx, n in [(1, 0, a, ix, N3[0]), (2, 1, b, iy, N3[1]), (4, 2, c, iz, N3[2])]:
+        g = m.grad(phi, mask)
+        inner = (idx > 0) & (idx < n - 1)
+        np.testing.assert_allclose(g[inner], coef, rtol=1e-12)
+        # boundary cell: one-sided half difference, coef / 2
+        np.testing.assert_allclose(g[idx == 0], coef / 2, rtol=1e-12)
+        np.testing.assert_allclose(g[idx == n - 1], coef / 2, rtol=1e-12)
+    inner = (ix > 0) & (ix < N3[0] - 1) & (iy > 0) & (iy < N3[1] - 1) & (iz > 0) & (iz < N3[2] - 1)
+    np.testing.assert_allclose(m.grad(phi, 7)[inner], a + b + c, rtol=1e-12)
+
+
+def test_grad_reflective_wall(oracle_lib):
+    """With the wall rule the boundary face value is 0: for phi = 1 the wall
+    cells see face values 0 and 1, so d1dx = +1/dx on the lower x-wall,
+    -1/dx on the upper one and 0 inside."""
+    m = _mesh(oracle_lib, N3, D3)
+    ix = _idx()[0].ravel()
+    g = m.grad(np.ones(m.N), 1, wall_axis=0)
+    np.testing.assert_allclose(g[ix == 0], 1.0 / D3[0], rtol=1e-12)
+    np.testing.assert_allclose(g[ix == N3[0] - 1], -1.0 / D3[0], rtol=1e-12)
+    assert np.all(g[(ix > 0) & (ix < N3[0] - 1)] == 0)
+
+
+def test_lapl_quadratic_and_linear(oracle_lib):
+    m = _mesh(oracle_lib, N3, D3)
+    X, Y, Z = _coords()
+    ix, iy, iz = (a.ravel() for a in _idx())
+    dx = D3[0]
+    L = m.lapl(X * X, 1)
+    np.testing.assert_allclose(L[(ix > 0) & (ix < N3[0] - 1)], 2.0, rtol=1e-10)
+    # zero-flux walls: lower wall (x_c = dx/2) gives ((1.5dx)^2 - (0.5dx)^2)/dx^2 = 2,
+    # upper wall gives -((n-0.5)^2 - (n-1.5)^2) = -(2n - 2)
+    np.testing.assert_allclose(L[ix == 0], 2.0, rtol=1e-10)
+    np.testing.assert_allclose(L[ix == N3[0] - 1], -(2 * N3[0] - 2), rtol=1e-10)
+    Ly = m.lapl(Y * Y, 7)
+    np.testing.assert_allclose(Ly[(iy > 0) & (iy < N3[1] - 1)], 2.0, rtol=1e-10)
+    lin = 2.0 * X - 1.0
+    Ll = m.lapl(lin, 7)
+    assert np.max(np.abs(Ll[(ix > 0) & (ix < N3[0] - 1)])) < 1e-10
+    np.testing.assert_allclose(Ll[ix == 0], 2.0 / dx, rtol=1e-10)
+
+
+def test_stab_polynomials_and_odd_even(oracle_lib):
+    m = _mesh(oracle_lib, N3, D3)
+    X, Y, Z = _coords()
+    ix = _idx()[0].ravel()
+    inner2 = (ix >= 2) & (ix <= N3[0] - 3)
+    for phi in (1.5 * X - 0.3, X * X):
+        s = m.stab(phi, 1)
+        assert np.max(np.abs(s[inner2])) < 1e-9
+    # odd-even mode (-1)^x: zero gradient inside, jumps of 2 phi on both
+    # faces, stabx = 2 phi / dx
+    phi = np.where(ix % 2 == 0, 1.0, -1.0)
+    s = m.stab(phi, 1)
+    np.testing.assert_allclose(s[inner2], 2.0 * phi[inner2] / D3[0], rtol=1e-12)
+    # the y/z components of an x-only mode vanish
+    assert np.max(np.abs(m.stab(phi, 6))) < 1e-12
+
+
+def test_constants_are_annihilated(oracle_lib):
+    m = _mesh(oracle_lib, N3, D3)
+    c = np.full(m.N, 2.5)
+    assert np.max(np.abs(m.grad(c))) == 0
+    assert np.max(np.abs(m.lapl(c))) == 0
+    assert np.max(np.abs(m.stab(c))) == 0
+    # On a perturbed mesh only the Laplacian (differences of cell values)
+    # still annihilates constants: the Q2 face areas do not close the cell
+    # surface, so sum_f n_out A != 0 and the Green-Gauss gradient of a
+    # constant is not zero there - a property of the reference geometry.
+    off = workloads.perturbed_vertex_offsets(*N3, *D3, seed=3, frac=0.2)
+    mp = _mesh(oracle_lib, N3, D3, perturb=off)
+    assert np.max(np.abs(mp.lapl(np.full(mp.N, 2.5)))) == 0
+    assert np.max(np.abs(mp.grad(np.full(mp.N, 2.5)))) > 1e-3
+
+
+def test_conservation_on_perturbed_mesh(oracle_lib):
+    """sum_i V_i lapl(phi)_i = 0: interior fluxes are antisymmetric (same face
+    area, alpha and normal from both sides) and boundary fluxes vanish."""
+    off = workloads.perturbed_vertex_offsets(*N3, *D3, seed=5, frac=0.2)
+    m = _mesh(oracle_lib, N3, D3, perturb=off)
+    phi = np.random.default_rng(0).standard_normal(m.N)
+    V = m.table("vol")
+    L = m.lapl(phi)
+    assert abs(np.sum(V * L)) < 1e-12 * np.sum(np.abs(V * L))
+
+
+def test_grad_telescopes_to_boundary(oracle_lib):
+    """sum_i V_i d1dx(phi)_i on a uniform mesh = sum over the two x-walls of
+    (phi_last - phi_first) * A: interior face values cancel."""
+    m = _mesh(oracle_lib, N3, D3)
+    phi = np.random.default_rng(1).standard_normal(m.N)
+    V = m.table("vol")
+    g = m.grad(phi, 1)
+    P = phi.reshape(N3)
+    A = D3[1] * D3[2]
+    expect = np.sum(P[-1] - P[0]) * A
+    assert np.sum(V * g) == pytest.approx(expect, rel=1e-10, abs=1e-12)
+
+
+def test_cfl_closed_form(oracle_lib):
+    m = _mesh(oracle_lib, (4, 4, 4), (0.5, 0.25, 0.125))
+    rng = np.random.default_rng(2)
+    rho = 1.0 + rng.random(m.N)
+    p = 1.0 + rng.random(m.N)
+    u, v, w = (rng.standard_normal(m.N) for _ in range(3))
+    a = np.sqrt(5.0 / 3.0 * p / rho)
+    expect = 0.4 * np.min(0.125 / (a + np.sqrt(u * u + v * v + w * w)))
+    assert m.cfl3d(0.4, u, v, w, p, rho) == pytest.approx(expect, rel=1e-13)
+    expect1 = 0.4 * np.min(0.125 / (a + np.abs(u)))
+    assert m.cfl1d(0.4, u, p, rho) == pytest.approx(expect1, rel=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# time integration
+# ---------------------------------------------------------------------------
+def test_heat_one_step_is_explicit_euler(oracle_lib):
+    m = _mesh(oracle_lib, N3, D3)
+    T = np.random.default_rng(3).standard_normal(m.N)
+    alpha, dt = 0.7, 1e-3
+    T1 = m.heat_run(T, alpha, dt, 1)
+    np.testing.assert_allclose(T1, T + dt * (alpha * m.lapl(T)), rtol=0, atol=1e-14)
+    # per-cell coefficient field equal to the constant gives the same bits
+    T1f = m.heat_run(T, alpha, dt, 1, alpha_field=np.full(m.N, alpha))
+    assert np.array_equal(T1, T1f)
+
+
+def test_heat_matches_free_space_kernel(oracle_lib):
+    n, steps, alpha = 48, 120, 1e-3
+    case = workloads.heat_config(n=n, steps=steps, dt_factor=0.15, alpha=alpha, sigma=0.08)
+    m = _mesh(oracle_lib, case.n, case.spacing)
+    T = m.heat_run(case.T0, alpha, case.dt, steps)
+    exact = workloads.heat_free_space(*case.n, *case.spacing, (0.5, 0.5, 0.5), 0.08, alpha, steps * case.dt)
+    assert np.max(np.abs(T - exact)) < 0.01 * exact.max()
+    # total heat is conserved (zero-flux walls)
+    assert np.sum(T) == pytest.approx(np.sum(case.T0), rel=1e-12)
+
+
+def test_wave_energy_bounded_and_symplectic_order(oracle_lib):
+    case = workloads.wave_config(n=24, steps=0, cfl=0.4, seed=1)
+    m = _mesh(oracle_lib, case.n, case.spacing)
+    z = np.zeros(m.N)
+    E0 = workloads.wave_energy(case.p0, z, z, z, 1.0)
+    p, u, v, w = case.p0, z, z, z
+    Es = []
+    for _ in range(10):
+        p, u, v, w = m.wave_run(p, u, v, w, -1.0, case.dt, 20)
+        Es.append(workloads.wave_energy(p, u, v, w, 1.0))
+    assert max(Es) < 1.1 * E0 and min(Es) > 0.8 * E0
+    # Gauss-Seidel order matters: a simultaneous (Jacobi) update of the same
+    # system at the same CFL grows; the oracle's sequential update does not.
+    pj, uj, vj, wj = case.p0.copy(), z.copy(), z.copy(), z.copy()
+    for _ in range(200):
+        gx, gy, gz = m.grad(pj, 1), m.grad(pj, 2), m.grad(pj, 4)
+        div = (m.grad(uj, 1, 0) + m.grad(vj, 2, 1)) + m.grad(wj, 4, 2)
+        uj, vj, wj, pj = uj - case.dt * gx, vj - case.dt * gy, wj - case.dt * gz, pj - case.dt * div
+    assert workloads.wave_energy(pj, uj, vj, wj, 1.0) > 1.01 * E0
+
+
+def test_float_oracle_tracks_double(oracle_lib):
+    case = workloads.wave_config(n=16, steps=0, seed=2)
+    m64 = _mesh(oracle_lib, case.n, case.spacing, np.float64)
+    m32 = _mesh(oracle_lib, case.n, case.spacing, np.float32)
+    z = np.zeros(m64.N)
+    r64 = m64.wave_run(case.p0, z, z, z, -1.0, case.dt, 50)
+    r32 = m32.wave_run(case.p0.astype(np.float32), z.astype(np.float32), z.astype(np.float32),
+                       z.astype(np.float32), -1.0, case.dt, 50)
+    for a, b in zip(r64, r32):
+        assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(r64[0])
+
+
+def test_splitmix64_known_answers():
+    g = workloads.SplitMix64(0)
+    assert [g.next_u64() for _ in range(3)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    g1, g2 = workloads.SplitMix64(1234), workloads.SplitMix64(1234)
+    scal = np.array([g1.uniform() for _ in range(17)])
+    vec = g2.uniform_array(17)
+    assert np.array_equal(scal, vec) and g1.state == g2.state
