@@ -109,8 +109,8 @@ def test_heat_alpha_field_and_general_mesh(hip_ctx, oracle_lib):
 def test_wave_fused_matches_sequential_oracle(hip_ctx, oracle_lib, dtype, n):
     from cfdarcov2_amd import capi
     print(f"\n[case] wave {n} {np.dtype(dtype).name}", flush=True)
-    d = tuple(1.0 / k for k in n)
-    case = workloads.wave_config(n=n[0], steps=0, seed=3)
+    # dyadic spacings keep x*dx exact, so the mesh is exactly uniform
+    d = (1.0 / 32, 1.0 / 16, 1.0 / 64)
     X, Y, Z = workloads.cell_centres(*n, *d)
     p0 = np.exp(-((X - 0.4) ** 2 + (Y - 0.55) ** 2 + (Z - 0.5) ** 2) / (2 * 0.1 ** 2))
     dt = 0.4 * min(d)
@@ -127,13 +127,16 @@ def test_wave_fused_matches_sequential_oracle(hip_ctx, oracle_lib, dtype, n):
         assert np.array_equal(g, r), f"field {name}: max diff {np.max(np.abs(g - r))}"
 
 
-def test_wave_K_field(hip_ctx, oracle_lib):
+@pytest.mark.parametrize("kind", ["uniform", "non-dyadic", "perturbed"])
+def test_wave_K_field(hip_ctx, oracle_lib, kind):
     from cfdarcov2_amd import capi
     n = (12, 10, 14)
-    d = (0.1, 0.1, 0.1)
+    d = (0.125, 0.125, 0.0625) if kind == "uniform" else (0.1, 0.1, 0.1)
+    off = workloads.perturbed_vertex_offsets(*n, *d, seed=2, frac=0.2) if kind == "perturbed" else None
     rng = np.random.default_rng(5)
-    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=np.float64)
-    om = oracle_lib.OracleMesh(*n, *d, dtype=np.float64)
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=np.float64, vertex_offsets=off)
+    assert gm.info().uniform == (kind == "uniform")
+    om = oracle_lib.OracleMesh(*n, *d, dtype=np.float64, perturb=off)
     p0, u0, v0, w0 = (rng.standard_normal(om.N) for _ in range(4))
     K = -(0.5 + rng.random(om.N)) ** 2
     A = [_t(x, np.float64) for x in (p0, u0, v0, w0)]
@@ -145,33 +148,49 @@ def test_wave_K_field(hip_ctx, oracle_lib):
 
 
 def test_wave_full_size_properties(hip_ctx, oracle_lib):
-    """256^3 FP32 benchmark case: (a) exact linearity, run(2 p0) == 2 run(p0)
-    (scaling by 2 is exact in binary floating point); (b) FP32 stays within
-    the FP64 path's energy to 1e-5 after 100 steps; (c) bounded energy."""
+    """256^3 benchmark case: (a) FP32, two steps equal the oracle bit for bit
+    at full size; (b) FP64 linearity, run(2 p0) == 2 run(p0) exactly over 100
+    steps (scaling by 2 is exact while every value stays in the normal range,
+    which holds in FP64; in FP32 ulp differences born in subnormals near the
+    walls ride the steep wave front into larger values, so FP32 is checked to
+    1e-6); (c) FP32 tracks FP64 and the energy stays bounded."""
     from cfdarcov2_amd import capi
     print("\n[case] wave 256^3 properties", flush=True)
     case = workloads.wave_config(n=256, steps=100)
-    g32 = capi.HexMesh(hip_ctx, *case.n, *case.spacing, dtype=np.float32)
-    N = g32.N
+    N = 256 ** 3
 
-    def run(mesh, p0, dtype, steps):
-        A = [_t(p0.astype(dtype), dtype)] + [torch.zeros(N, dtype=A0dt(dtype), device="cuda") for _ in range(3)]
+    def run(mesh, p0, tdt, steps):
+        A = [torch.from_numpy(p0).to(tdt).cuda()] + [torch.zeros(N, dtype=tdt, device="cuda") for _ in range(3)]
         B = [torch.empty_like(A[0]) for _ in range(4)]
         return mesh.wave_run(A, B, -case.c ** 2, case.dt, steps)
 
-    def A0dt(dtype):
-        return torch.float32 if dtype == np.float32 else torch.float64
-
-    r1 = run(g32, case.p0, np.float32, case.steps)
-    r2 = run(g32, 2.0 * case.p0, np.float32, case.steps)
-    for a, b in zip(r1, r2):
+    g32 = capi.HexMesh(hip_ctx, *case.n, *case.spacing, dtype=np.float32)
+    p0f = case.p0.astype(np.float32)
+    # (a) full-size exactness over two steps
+    om = oracle_lib.OracleMesh(*case.n, *case.spacing, dtype=np.float32)
+    zf = np.zeros(N, dtype=np.float32)
+    ref2 = om.wave_run(p0f, zf, zf, zf, -1.0, case.dt, 2)
+    del om
+    got2 = [_np(t) for t in run(g32, p0f, torch.float32, 2)]
+    for name, g, r in zip("puvw", got2, ref2):
+        assert np.array_equal(g, r), f"256^3 field {name}"
+    del got2, ref2
+    # (b) linearity
+    g64 = capi.HexMesh(hip_ctx, *case.n, *case.spacing, dtype=np.float64)
+    d1 = run(g64, case.p0, torch.float64, case.steps)
+    d2 = run(g64, 2 * case.p0, torch.float64, case.steps)
+    for a, b in zip(d1, d2):
         assert torch.equal(2 * a, b)
+    del d2
+    r1 = run(g32, p0f, torch.float32, case.steps)
+    r2 = run(g32, 2 * p0f, torch.float32, case.steps)
+    for a, b in zip(r1, r2):
+        assert float(torch.max(torch.abs(2 * a - b))) <= 1e-6 * float(torch.max(torch.abs(b)))
+    del r2
+    # (c) FP32 vs FP64, energy
     E0 = workloads.wave_energy(case.p0, 0, 0, 0, 1.0)
     e32 = workloads.wave_energy(*[_np(t).astype(np.float64) for t in r1], 1.0)
-    del r2
-    g64 = capi.HexMesh(hip_ctx, *case.n, *case.spacing, dtype=np.float64)
-    r64 = run(g64, case.p0, np.float64, case.steps)
-    f64 = [_np(t) for t in r64]
+    f64 = [_np(t) for t in d1]
     e64 = workloads.wave_energy(*f64, 1.0)
     assert abs(e32 - e64) < 1e-5 * E0
     assert 0.8 * E0 < e64 < 1.1 * E0
@@ -217,8 +236,7 @@ def test_error_paths(hip_ctx):
     assert e.value.code == -1
     gm = capi.HexMesh(hip_ctx, 4, 4, 4, 1.0, 1.0, 1.0, dtype=np.float32)
     t = torch.zeros(64, device="cuda")
-    with pytest.raises(capi.CfdError):
-        gm.lib.cfd_grad(gm.h, None, None, 1, -1, None) and capi.check(-1)
+    assert gm.lib.cfd_grad(gm.h, None, None, 1, -1, None) == -1
     rc = gm.lib.cfd_grad(gm.h, capi._dptr(t), capi._dptr(t), 9, -1, None)
     assert rc == -1 and b"axis_mask" in gm.lib.cfd_last_error()
     with pytest.raises(capi.CfdError) as e:
