@@ -26,12 +26,12 @@ if ROOT not in sys.path:
 
 METRIC = "MCUPS + % HBM roofline (1 GPU); adjoint/forward cost ratio; batched vs plain"
 HBM_PEAK_GBS = 8000.0
-KERNEL = "wave_uniform_k"
+KERNELS = {0: "wave_uniform_k", 1: "wave_march_k"}
 
 
 def _traffic(n, dtype):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
-    path = os.path.join(ROOT, "profiles", f"{KERNEL}_wave{n}cube_{dtype}_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"wave{n}cube_{dtype}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -59,6 +59,22 @@ def cpu_baseline(case, dtype, steps):
                       f"{t:.2f} s"}
 
 
+def calibrate_copy(args):
+    """A plain device-to-device copy of 4 fields' worth of bytes (what one fused
+    launch reads), repeated --steps times: the known-size reference for
+    scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    esize = 4 if args.dtype == "f32" else 8
+    nbytes = 4 * esize * args.n ** 3
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes_per_launch": nbytes, "launches": args.steps}), flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +84,11 @@ def main(argv=None):
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--shape", default="auto",
+                    help="fused-kernel shape 'variant,tile_rows,chunk,xcd_remap' or 'auto' (autotuner)")
+    ap.add_argument("--tune-verbose", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain device copy of the benchmark's read volume (PMC calibration)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -79,11 +100,22 @@ def main(argv=None):
 
     from cfdarcov2_amd import capi, workloads
 
+    if args.calibrate_copy:
+        calibrate_copy(args)
+        return
+
     case = workloads.wave_config(n=args.n, steps=args.steps)
     tdt = torch.float32 if args.dtype == "f32" else torch.float64
     ctx = capi.Context(0)
     mesh = capi.HexMesh(ctx, *case.n, *case.spacing, dtype=args.dtype)
     N = mesh.N
+    from cfdarcov2_amd import autotune
+    if args.shape == "auto":
+        shape = autotune.tune_wave(mesh, use_cache=False, verbose=args.tune_verbose)
+    else:
+        v_, ty_, L_, r_ = (int(x) for x in args.shape.split(","))
+        mesh.set_wave_shape(v_, ty_, L_, r_)
+        shape = {"variant": v_, "tile_rows": ty_, "chunk": L_, "xcd_remap": r_}
     A = [torch.from_numpy(case.p0).to(tdt).cuda()] + [torch.zeros(N, dtype=tdt, device="cuda") for _ in range(3)]
     B = [torch.empty_like(A[0]) for _ in range(4)]
     K = -case.c ** 2
@@ -129,10 +161,11 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": f"3D first-order acoustic wave (p,u,v,w), {args.n}^3 uniform hex mesh, "
                                f"{args.dtype.upper()}, CFL 0.4, reflective walls, seeded Gaussian pulse",
-                   "cells": N, "parallelism": "single GPU"},
+                   "cells": N, "parallelism": "single GPU",
+                   "kernel_shape": {k: shape[k] for k in ("variant", "tile_rows", "chunk", "xcd_remap")}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": KERNEL, "kernel_ms": kernel_ms,
+                     "kernel": KERNELS[shape["variant"]], "kernel_ms": kernel_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
     mesh.close()

@@ -1,0 +1,85 @@
+"""Autotuner for the fused kernels' block and tile shapes.
+
+For a given mesh it times every candidate shape of the fused acoustic step
+(per-thread kernel; 2.5D marching kernel over tile rows x x-chunk x XCD
+placement) with HIP events on the launch stream, keeps the fastest, and caches
+the choice per (kernel, dtype, mesh extents) in a JSON file
+(``$CFDARCO_TUNE_CACHE``, default ``~/.cache/cfdarco/tune.json``).  Results are
+state-independent (the kernels do the same work for any data), so tuning runs
+on a scratch state and never touches the caller's fields.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+
+DEFAULT_CANDIDATES = [(0, 16, 32, 1)] + [
+    (1, ty, L, r) for ty, L, r in itertools.product((4, 8, 16, 32), (4, 8, 16, 32, 64), (0, 1))
+]
+
+
+def _cache_path():
+    return os.environ.get("CFDARCO_TUNE_CACHE", os.path.expanduser("~/.cache/cfdarco/tune.json"))
+
+
+def _load_cache():
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(cache):
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+    except OSError:
+        pass
+
+
+def wave_key(mesh) -> str:
+    s = mesh.shape
+    return f"wave/{'f32' if mesh.code == 0 else 'f64'}/{s.nx}x{s.ny}x{s.nz}"
+
+
+def tune_wave(mesh, steps: int = 20, candidates=None, use_cache: bool = True, verbose: bool = False) -> dict:
+    """Pick the fastest fused acoustic-step shape for `mesh` and apply it."""
+    import torch
+
+    key = wave_key(mesh)
+    cache = _load_cache() if use_cache else {}
+    if use_cache and key in cache:
+        best = cache[key]
+        mesh.set_wave_shape(best["variant"], best["tile_rows"], best["chunk"], best["xcd_remap"])
+        return best
+    cands = candidates or DEFAULT_CANDIDATES
+    N = mesh.N
+    tdt = mesh.torch_dtype
+    A = [torch.zeros(N, dtype=tdt, device="cuda") for _ in range(4)]
+    B = [torch.zeros(N, dtype=tdt, device="cuda") for _ in range(4)]
+    stream = torch.cuda.current_stream()
+    table = []
+    for shape in cands:
+        variant, ty, L, remap = shape
+        mesh.set_wave_shape(variant, ty, L, remap)
+        mesh.wave_run(A, B, -1.0, 1e-3, 2)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        mesh.wave_run(A, B, -1.0, 1e-3, steps)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        table.append({"variant": variant, "tile_rows": ty, "chunk": L, "xcd_remap": remap, "ms_per_step": ms})
+        if verbose:
+            print(f"  wave shape variant={variant} ty={ty} L={L} remap={remap}: {ms * 1e3:.1f} us/step", flush=True)
+    best = min(table, key=lambda r: r["ms_per_step"])
+    mesh.set_wave_shape(best["variant"], best["tile_rows"], best["chunk"], best["xcd_remap"])
+    if use_cache:
+        cache[key] = best
+        _store_cache(cache)
+    del A, B
+    return dict(best, table=table)

@@ -32,7 +32,7 @@ EXPORTS = [
     "cfd_memcpy", "cfd_stream_sync",
     "cfd_mesh_create_hex", "cfd_mesh_destroy", "cfd_mesh_get_info", "cfd_mesh_table", "cfd_hex_table",
     "cfd_grad", "cfd_lapl", "cfd_stab", "cfd_ewise", "cfd_cfl_min", "cfd_count_nonfinite",
-    "cfd_heat_run", "cfd_wave_run",
+    "cfd_heat_run", "cfd_wave_run", "cfd_mesh_set_wave_shape",
 ]
 
 
@@ -82,6 +82,7 @@ def load_library(path: str = LIB_PATH):
         "cfd_count_nonfinite": [vp, i, vp, i64, ctypes.POINTER(ctypes.c_int64), vp],
         "cfd_heat_run": [vp, vp, vp, d, vp, d, i64, vp],
         "cfd_wave_run": [vp, ctypes.c_void_p * 4, ctypes.c_void_p * 4, d, vp, d, i64, vp],
+        "cfd_mesh_set_wave_shape": [vp, i, i, i, i],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -285,6 +286,9 @@ class HexMesh:
         check(self.lib.cfd_heat_run(self.h, _dptr(T_a), _dptr(T_b), float(alpha), _dptr(alpha_field), float(dt),
                                     int(nsteps), _stream_ptr()))
         return T_a if nsteps % 2 == 0 else T_b
+
+    def set_wave_shape(self, variant=1, tile_rows=16, chunk=32, xcd_remap=True):
+        check(self.lib.cfd_mesh_set_wave_shape(self.h, int(variant), int(tile_rows), int(chunk), int(bool(xcd_remap))))
 
     def wave_run(self, state_a, state_b, K, dt, nsteps, K_field=None):
         """Fused acoustic steps on (p, u, v, w); returns the tuple holding the final state."""

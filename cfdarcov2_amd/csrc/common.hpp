@@ -113,4 +113,10 @@ struct cfd_mesh_s {
     // host copies of the reference-layout tables are rebuilt on demand
     std::vector<double> vertex_offsets;
     bool has_offsets = false;
+    // fused acoustic step on uniform meshes: variant 0 = per-thread kernel,
+    // 1 = 2.5D marching kernel with tile rows `wave_ty` and chunk `wave_L`
+    int wave_variant = 1;
+    int wave_ty = 16;
+    int wave_L = 32;
+    int wave_remap = 1;
 };

@@ -19,6 +19,10 @@ void launch_wave(const cfd_mesh_s* m, void* const a[4], void* const b[4], double
                  int64_t nsteps, hipStream_t s);
 
 template <class T>
+void launch_wave_march(const cfd_mesh_s* m, void* const a[4], void* const b[4], double K, const void* Kf, double dt,
+                       int64_t nsteps, int ty, int L, int remap, hipStream_t s);
+
+template <class T>
 void launch_ewise(int op, const void* a, const void* b, const void* aux, double s, void* out, int64_t n,
                   hipStream_t st);
 // min over cells of dl / (sqrt(gamma p / rho) + |u|) written to dev_out (one T)

@@ -391,6 +391,48 @@ __global__ void __launch_bounds__(256) wave_uniform_k(UniformGeo<T> g, const T* 
     po[c.id] = dt * (k * S) + p[c.id];
 }
 
+// General-mesh acoustic step.  Skewed face normals make every gradient
+// component depend on all six faces, so the step runs as two launches: the
+// three velocity equations (reading p), then the pressure equation (reading the
+// new velocities) - the same sequential semantics as the uniform kernel.
+template <class T>
+__global__ void __launch_bounds__(256) wave_vel_general_k(GeneralGeo<T> g, const T* __restrict__ p,
+                                                          const T* __restrict__ u, const T* __restrict__ v,
+                                                          const T* __restrict__ w, T* __restrict__ uo,
+                                                          T* __restrict__ vo, T* __restrict__ wo, T dt) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    const T pc = p[c.id];
+    T fv[6];
+    for (int fc = 0; fc < 6; ++fc) fv[fc] = lin(pc, p[nb_of(g, c, fc)]);
+    uo[c.id] = dt * (-gauss_general(g, c, fv, 0, false)) + u[c.id];
+    vo[c.id] = dt * (-gauss_general(g, c, fv, 1, false)) + v[c.id];
+    wo[c.id] = dt * (-gauss_general(g, c, fv, 2, false)) + w[c.id];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) wave_p_general_k(GeneralGeo<T> g, const T* __restrict__ uo,
+                                                        const T* __restrict__ vo, const T* __restrict__ wo,
+                                                        const T* __restrict__ p, T* __restrict__ po, T K,
+                                                        const T* __restrict__ Kf, T dt) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    const T* vel[3] = {uo, vo, wo};
+    T S = T(0);
+    for (int a = 0; a < 3; ++a) {
+        const T qc = vel[a][c.id];
+        T fv[6];
+        for (int fc = 0; fc < 6; ++fc) {
+            fv[fc] = lin(qc, vel[a][nb_of(g, c, fc)]);
+            if (fc_axis(fc) == a && is_bnd_face(g, c, fc)) fv[fc] = T(0);
+        }
+        const T G = gauss_general(g, c, fv, a, false);
+        S = (a == 0) ? G : S + G;
+    }
+    const T k = Kf ? Kf[c.id] : K;
+    po[c.id] = dt * (k * S) + p[c.id];
+}
+
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
@@ -469,8 +511,26 @@ void launch_heat(const cfd_mesh_s* m, void* Ta, void* Tb, double alpha, const vo
 template <class T>
 void launch_wave(const cfd_mesh_s* m, void* const a[4], void* const b[4], double K, const void* Kf, double dt,
                  int64_t nsteps, hipStream_t s) {
-    if (!m->uniform)
-        throw Error(CFD_E_UNSUPPORTED, "cfd_wave_run: the fused acoustic step needs a uniform mesh");
+    if (!m->uniform) {
+        const auto& g = ggeo<T>(m);
+        for (int64_t k = 0; k < nsteps; ++k) {
+            void* const* in = (k & 1) ? b : a;
+            void* const* out = (k & 1) ? a : b;
+            wave_vel_general_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(
+                g, (const T*)in[0], (const T*)in[1], (const T*)in[2], (const T*)in[3], (T*)out[1], (T*)out[2],
+                (T*)out[3], T(dt));
+            check_launch();
+            wave_p_general_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(
+                g, (const T*)out[1], (const T*)out[2], (const T*)out[3], (const T*)in[0], (T*)out[0], T(K),
+                (const T*)Kf, T(dt));
+            check_launch();
+        }
+        return;
+    }
+    if (m->wave_variant == 1) {
+        launch_wave_march<T>(m, a, b, K, Kf, dt, nsteps, m->wave_ty, m->wave_L, m->wave_remap, s);
+        return;
+    }
     const auto& g = ugeo<T>(m);
     for (int64_t k = 0; k < nsteps; ++k) {
         void* const* in = (k & 1) ? b : a;

@@ -182,6 +182,14 @@ int cfd_heat_run(cfd_mesh_t mesh, void* T_a, void* T_b, double alpha, const void
 int cfd_wave_run(cfd_mesh_t mesh, void* const state_a[4], void* const state_b[4], double K,
                  const void* K_field, double dt, int64_t nsteps, void* stream);
 
+/* Kernel shape of the fused acoustic step on uniform meshes (no counterpart
+ * in the reference, whose JIT kernels use fixed 64-thread blocks,
+ * src/fvm3d.cpp:1306-1308): variant 0 = one thread per cell, variant 1 = 2.5D
+ * marching kernel with `tile_rows` (4, 8, 16 or 32) y-rows of 64 z-cells per
+ * workgroup, `chunk` x-planes per workgroup and XCD-aware tile placement when
+ * `xcd_remap` = 1.  Set by the autotuner (cfdarcov2_amd/autotune.py). */
+int cfd_mesh_set_wave_shape(cfd_mesh_t mesh, int variant, int tile_rows, int chunk, int xcd_remap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -127,6 +127,36 @@ def test_wave_fused_matches_sequential_oracle(hip_ctx, oracle_lib, dtype, n):
         assert np.array_equal(g, r), f"field {name}: max diff {np.max(np.abs(g - r))}"
 
 
+SHAPES = [(0, 16, 32, 1), (1, 4, 1, 0), (1, 8, 3, 1), (1, 16, 32, 1), (1, 4, 7, 1), (1, 16, 5, 0),
+          (1, 32, 4, 1), (1, 32, 64, 0), (1, 8, 16, 0)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"v{a}-ty{b}-L{c}-r{d}" for a, b, c, d in SHAPES])
+def test_wave_kernel_shapes(hip_ctx, oracle_lib, dtype, shape):
+    """Every tile shape of the fused acoustic kernel (per-thread form and the
+    2.5D marching form with partial tiles in y and z and chunks that do not
+    divide x) gives the oracle's bits."""
+    from cfdarcov2_amd import capi
+    print(f"\n[case] wave shape {shape} {np.dtype(dtype).name}", flush=True)
+    n = (19, 21, 70)
+    d = (1.0 / 32, 1.0 / 32, 1.0 / 64)
+    X, Y, Z = workloads.cell_centres(*n, *d)
+    rng = np.random.default_rng(17)
+    p0 = np.exp(-((X - 0.3) ** 2 + (Y - 0.3) ** 2 + (Z - 0.6) ** 2) / (2 * 0.08 ** 2))
+    u0, v0, w0 = (0.1 * rng.standard_normal(p0.size) for _ in range(3))
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=dtype)
+    gm.set_wave_shape(*shape)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=dtype)
+    A = [_t(a, dtype) for a in (p0, u0, v0, w0)]
+    B = [torch.empty_like(A[0]) for _ in range(4)]
+    dt = 0.4 / 64
+    got = [_np(t) for t in gm.wave_run(A, B, -1.0, dt, 7)]
+    ref = om.wave_run(*(a.astype(dtype) for a in (p0, u0, v0, w0)), -1.0, dt, 7)
+    for name, g, r in zip("puvw", got, ref):
+        assert np.array_equal(g, r), f"field {name}"
+
+
 @pytest.mark.parametrize("kind", ["uniform", "non-dyadic", "perturbed"])
 def test_wave_K_field(hip_ctx, oracle_lib, kind):
     from cfdarcov2_amd import capi
