@@ -20,7 +20,7 @@ F32, F64 = 0, 1
 H2D, D2H, D2D = 1, 2, 3
 
 EW = {"add": 0, "sub": 1, "mul": 2, "div": 3, "neg": 4, "muls": 5, "divs": 6, "smul": 7,
-      "axpy": 8, "leap": 9, "fill": 10, "copy": 11}
+      "axpy": 8, "leap": 9, "fill": 10, "copy": 11, "adds": 12, "sadd": 13, "subs": 14, "ssub": 15, "sdiv": 16}
 
 TABLES = {"ids": 0, "ids_bf": 1, "nx": 2, "ny": 3, "nz": 4, "nax": 5, "nay": 6, "naz": 7,
           "area": 8, "alpha": 9, "len": 10, "vol": 11, "is_bnd": 12, "ccx": 13, "ccy": 14, "ccz": 15}
@@ -33,6 +33,8 @@ EXPORTS = [
     "cfd_mesh_create_hex", "cfd_mesh_destroy", "cfd_mesh_get_info", "cfd_mesh_table", "cfd_hex_table",
     "cfd_grad", "cfd_lapl", "cfd_stab", "cfd_ewise", "cfd_cfl_min", "cfd_count_nonfinite",
     "cfd_heat_run", "cfd_wave_run", "cfd_mesh_set_wave_shape",
+    "cfd_program_compile", "cfd_program_compile_only", "cfd_program_launch", "cfd_program_destroy",
+    "cfd_mesh_geometry",
 ]
 
 
@@ -78,11 +80,16 @@ def load_library(path: str = LIB_PATH):
         "cfd_hex_table": [ctypes.POINTER(HexDesc), i, vp],
         "cfd_grad": [vp, vp, vp, i, i, vp], "cfd_lapl": [vp, vp, vp, i, vp], "cfd_stab": [vp, vp, vp, i, vp],
         "cfd_ewise": [vp, i, i, vp, vp, vp, d, vp, i64, vp],
-        "cfd_cfl_min": [vp, vp, vp, vp, vp, vp, d, d, ctypes.POINTER(ctypes.c_double), vp],
+        "cfd_cfl_min": [vp, i, vp, vp, vp, vp, vp, i64, d, d, d, ctypes.POINTER(ctypes.c_double), vp],
+        "cfd_mesh_geometry": [vp, vp],
         "cfd_count_nonfinite": [vp, i, vp, i64, ctypes.POINTER(ctypes.c_int64), vp],
         "cfd_heat_run": [vp, vp, vp, d, vp, d, i64, vp],
         "cfd_wave_run": [vp, ctypes.c_void_p * 4, ctypes.c_void_p * 4, d, vp, d, i64, vp],
         "cfd_mesh_set_wave_shape": [vp, i, i, i, i],
+        "cfd_program_compile": [vp, ctypes.c_char_p, ctypes.c_char_p, pp],
+        "cfd_program_compile_only": [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)],
+        "cfd_program_launch": [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), pp, vp],
+        "cfd_program_destroy": [vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -123,6 +130,14 @@ def hex_table(nx, ny, nz, dx, dy, dz, dtype, name, vertex_offsets=None):
     out = np.empty(count, dtype=np_dt)
     check(L.cfd_hex_table(ctypes.byref(desc), tid, out.ctypes.data))
     return out.reshape(6, n) if count == 6 * n else out
+
+
+def compile_only(src: str, kernel_name: str) -> int:
+    """hipRTC-compile a source on the host (no device); returns the code-object size."""
+    L = load_library()
+    n = ctypes.c_int64()
+    check(L.cfd_program_compile_only(src.encode(), kernel_name.encode(), ctypes.byref(n)))
+    return n.value
 
 
 def _stream_ptr():
@@ -274,9 +289,13 @@ class HexMesh:
         return out
 
     def cfl_min(self, u, p, rho, v=None, w=None, cfl=1.0, gamma=5.0 / 3.0) -> float:
+        """cfl * min dl / (sqrt(gamma p / rho) + |u|), dl = min spacing (in the mesh dtype)."""
         dt = ctypes.c_double()
-        check(self.lib.cfd_cfl_min(self.h, _dptr(u), _dptr(v), _dptr(w), _dptr(p), _dptr(rho), float(cfl),
-                                   float(gamma), ctypes.byref(dt), _stream_ptr()))
+        dl = min(self.spacing)
+        if self.code == F32:
+            dl = float(np.float32(min(np.float32(s) for s in self.spacing)))
+        check(self.lib.cfd_cfl_min(self.ctx.h, self.code, _dptr(u), _dptr(v), _dptr(w), _dptr(p), _dptr(rho),
+                                   self.N, float(dl), float(cfl), float(gamma), ctypes.byref(dt), _stream_ptr()))
         return dt.value
 
     def heat_run(self, T_a, T_b, alpha, dt, nsteps, alpha_field=None):

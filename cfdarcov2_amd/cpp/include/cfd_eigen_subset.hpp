@@ -1,0 +1,277 @@
+// The subset of Eigen's dense API that cfdARCO drivers and the operator API
+// use: dynamic column vectors and column-major matrices of arithmetic types,
+// small fixed-size vectors, coefficient access, reductions, cwise products and
+// `.array()` arithmetic.  Everything evaluates eagerly (no expression
+// templates); scalars of another arithmetic type are converted to the
+// matrix's scalar type, as Eigen does for literals such as `0.5 * v.array()`.
+#pragma once
+
+#include <algorithm>
+#include <cassert>
+#include <cmath>
+#include <cstddef>
+#include <initializer_list>
+#include <limits>
+#include <numeric>
+#include <ostream>
+#include <stdexcept>
+#include <type_traits>
+#include <vector>
+
+namespace Eigen {
+
+using Index = std::ptrdiff_t;
+constexpr int Dynamic = -1;
+enum StorageOptions { ColMajor = 0, RowMajor = 1, AutoAlign = 0 };
+
+template <class S> class Array1;
+
+template <class S, int Rows, int Cols, int Options = ColMajor, int MaxRows = Rows, int MaxCols = Cols>
+class Matrix {
+    static_assert(std::is_arithmetic<S>::value, "arithmetic scalars only");
+
+public:
+    using Scalar = S;
+
+    Matrix() : rows_(Rows > 0 ? Rows : 0), cols_(Cols > 0 ? Cols : 0), v_(size_t(rows_ * cols_)) {}
+
+    // Matrix<S,-1,1>{n}: a vector of n entries; Matrix<S,-1,-1>{r, c}
+    template <class I, class = typename std::enable_if<std::is_integral<I>::value>::type>
+    explicit Matrix(I n) {
+        if (Cols == 1 || Rows == Dynamic) {
+            rows_ = Index(n);
+            cols_ = Cols > 0 ? Cols : 1;
+        } else {
+            rows_ = 1;
+            cols_ = Index(n);
+        }
+        v_.assign(size_t(rows_ * cols_), S(0));
+    }
+    template <class I, class J,
+              class = typename std::enable_if<std::is_integral<I>::value && std::is_integral<J>::value>::type>
+    Matrix(I r, J c) : rows_(Index(r)), cols_(Index(c)), v_(size_t(Index(r) * Index(c)), S(0)) {}
+
+    // fixed-size 3-vectors from coordinates
+    template <class A, class B, class C,
+              class = typename std::enable_if<std::is_floating_point<A>::value>::type>
+    Matrix(A x, B y, C z) : rows_(3), cols_(1), v_{S(x), S(y), S(z)} {}
+
+    // between vector/matrix shapes of the same data (N x 1 <-> dynamic N x M)
+    template <int R2, int C2, int O2, int MR2, int MC2>
+    Matrix(const Matrix<S, R2, C2, O2, MR2, MC2>& o) : rows_(o.rows()), cols_(o.cols()), v_(o.data(), o.data() + o.size()) {}
+
+    Matrix(const Array1<S>& a);
+
+    Index rows() const { return rows_; }
+    Index cols() const { return cols_; }
+    Index size() const { return rows_ * cols_; }
+    S* data() { return v_.data(); }
+    const S* data() const { return v_.data(); }
+
+    void resize(Index n) {
+        rows_ = n;
+        cols_ = 1;
+        v_.assign(size_t(n), S(0));
+    }
+    void resize(Index r, Index c) {
+        rows_ = r;
+        cols_ = c;
+        v_.assign(size_t(r * c), S(0));
+    }
+
+    S& operator()(Index i) { return v_[size_t(i)]; }
+    const S& operator()(Index i) const { return v_[size_t(i)]; }
+    S& operator()(Index r, Index c) { return v_[size_t(c * rows_ + r)]; }
+    const S& operator()(Index r, Index c) const { return v_[size_t(c * rows_ + r)]; }
+    S& operator[](Index i) { return v_[size_t(i)]; }
+    const S& operator[](Index i) const { return v_[size_t(i)]; }
+
+    template <class T> Matrix& setConstant(T c) {
+        std::fill(v_.begin(), v_.end(), S(c));
+        return *this;
+    }
+    Matrix& setZero() { return setConstant(0); }
+    static Matrix Zero(Index n) { return Matrix(n); }
+    static Matrix Zero(Index r, Index c) { return Matrix(r, c); }
+    template <class T> static Matrix Constant(Index n, T c) {
+        Matrix m(n);
+        m.setConstant(c);
+        return m;
+    }
+
+    // reductions run left to right
+    S sum() const { return std::accumulate(v_.begin(), v_.end(), S(0)); }
+    S maxCoeff() const { return *std::max_element(v_.begin(), v_.end()); }
+    S minCoeff() const { return *std::min_element(v_.begin(), v_.end()); }
+    S squaredNorm() const {
+        S s = 0;
+        for (const S& x : v_) s += x * x;
+        return s;
+    }
+    S norm() const { return std::sqrt(squaredNorm()); }
+    S dot(const Matrix& o) const {
+        S s = 0;
+        for (size_t i = 0; i < v_.size(); ++i) s += v_[i] * o.v_[i];
+        return s;
+    }
+
+    Matrix cwiseProduct(const Matrix& o) const { return zip(o, [](S a, S b) { return a * b; }); }
+    Matrix cwiseQuotient(const Matrix& o) const { return zip(o, [](S a, S b) { return a / b; }); }
+    Matrix cwiseSqrt() const { return map([](S a) { return std::sqrt(a); }); }
+    Matrix cwiseAbs() const { return map([](S a) { return std::abs(a); }); }
+    Matrix cwiseInverse() const { return map([](S a) { return S(1) / a; }); }
+
+    Matrix operator+(const Matrix& o) const { return zip(o, [](S a, S b) { return a + b; }); }
+    Matrix operator-(const Matrix& o) const { return zip(o, [](S a, S b) { return a - b; }); }
+    Matrix operator-() const { return map([](S a) { return -a; }); }
+    Matrix& operator+=(const Matrix& o) { return *this = *this + o; }
+    Matrix& operator-=(const Matrix& o) { return *this = *this - o; }
+    template <class T, class = typename std::enable_if<std::is_arithmetic<T>::value>::type>
+    Matrix operator*(T s) const {
+        const S c = S(s);
+        return map([c](S a) { return a * c; });
+    }
+    template <class T, class = typename std::enable_if<std::is_arithmetic<T>::value>::type>
+    Matrix operator/(T s) const {
+        const S c = S(s);
+        return map([c](S a) { return a / c; });
+    }
+    template <class T, class = typename std::enable_if<std::is_arithmetic<T>::value>::type>
+    Matrix& operator/=(T s) {
+        return *this = *this / s;
+    }
+    template <class T, class = typename std::enable_if<std::is_arithmetic<T>::value>::type>
+    Matrix& operator*=(T s) {
+        return *this = *this * s;
+    }
+
+    Array1<S> array() const;
+
+    std::vector<S>& storage() { return v_; }
+    const std::vector<S>& storage() const { return v_; }
+
+private:
+    template <class F> Matrix zip(const Matrix& o, F f) const {
+        if (o.size() != size()) throw std::invalid_argument("Eigen subset: size mismatch");
+        Matrix r = *this;
+        for (size_t i = 0; i < v_.size(); ++i) r.v_[i] = f(v_[i], o.v_[i]);
+        return r;
+    }
+    template <class F> Matrix map(F f) const {
+        Matrix r = *this;
+        for (auto& x : r.v_) x = f(x);
+        return r;
+    }
+
+    Index rows_ = 0, cols_ = 0;
+    std::vector<S> v_;
+};
+
+template <class T, class S, int R, int C, int O, int MR, int MC,
+          class = typename std::enable_if<std::is_arithmetic<T>::value>::type>
+Matrix<S, R, C, O, MR, MC> operator*(T s, const Matrix<S, R, C, O, MR, MC>& m) {
+    const S c = S(s);
+    Matrix<S, R, C, O, MR, MC> r = m;
+    for (auto& x : r.storage()) x = c * x;
+    return r;
+}
+
+// Owning one-dimensional array with coefficient-wise arithmetic (the value of
+// `.array()`), convertible back to a column vector.
+template <class S> class Array1 {
+public:
+    Array1() = default;
+    explicit Array1(std::vector<S> v) : v_(std::move(v)) {}
+    Index size() const { return Index(v_.size()); }
+    S& operator()(Index i) { return v_[size_t(i)]; }
+    const S& operator()(Index i) const { return v_[size_t(i)]; }
+    S& operator[](Index i) { return v_[size_t(i)]; }
+    const S& operator[](Index i) const { return v_[size_t(i)]; }
+    const std::vector<S>& storage() const { return v_; }
+    std::vector<S>& storage() { return v_; }
+    Matrix<S, Dynamic, 1> matrix() const { return Matrix<S, Dynamic, 1>(*this); }
+    S sum() const { return std::accumulate(v_.begin(), v_.end(), S(0)); }
+    S maxCoeff() const { return *std::max_element(v_.begin(), v_.end()); }
+    S minCoeff() const { return *std::min_element(v_.begin(), v_.end()); }
+
+    template <class F> Array1 map(F f) const {
+        Array1 r = *this;
+        for (auto& x : r.v_) x = f(x);
+        return r;
+    }
+    template <class F> Array1 zip(const Array1& o, F f) const {
+        if (o.size() != size()) throw std::invalid_argument("Eigen subset: size mismatch");
+        Array1 r = *this;
+        for (size_t i = 0; i < v_.size(); ++i) r.v_[i] = f(v_[i], o.v_[i]);
+        return r;
+    }
+    Array1 sqrt() const { return map([](S a) { return std::sqrt(a); }); }
+    Array1 cwiseSqrt() const { return sqrt(); }
+    Array1 exp() const { return map([](S a) { return std::exp(a); }); }
+    Array1 abs() const { return map([](S a) { return std::abs(a); }); }
+    Array1 inverse() const { return map([](S a) { return S(1) / a; }); }
+    Array1 cwiseInverse() const { return inverse(); }
+    Array1 cwiseQuotient(const Array1& o) const { return zip(o, [](S a, S b) { return a / b; }); }
+    Array1 cwiseProduct(const Array1& o) const { return zip(o, [](S a, S b) { return a * b; }); }
+    Array1 array() const { return *this; }
+
+private:
+    std::vector<S> v_;
+};
+
+template <class S, int R, int C, int O, int MR, int MC>
+Matrix<S, R, C, O, MR, MC>::Matrix(const Array1<S>& a) : rows_(a.size()), cols_(1), v_(a.storage()) {}
+
+template <class S, int R, int C, int O, int MR, int MC>
+Array1<S> Matrix<S, R, C, O, MR, MC>::array() const {
+    return Array1<S>(v_);
+}
+
+#define CFD_EIGEN_ARRAY_OP(OP)                                                                             \
+    template <class S> Array1<S> operator OP(const Array1<S>& a, const Array1<S>& b) {                     \
+        return a.zip(b, [](S x, S y) { return x OP y; });                                                  \
+    }                                                                                                      \
+    template <class S, class T, class = typename std::enable_if<std::is_arithmetic<T>::value>::type>       \
+    Array1<S> operator OP(const Array1<S>& a, T s) {                                                       \
+        const S c = S(s);                                                                                  \
+        return a.map([c](S x) { return x OP c; });                                                         \
+    }                                                                                                      \
+    template <class S, class T, class = typename std::enable_if<std::is_arithmetic<T>::value>::type>       \
+    Array1<S> operator OP(T s, const Array1<S>& a) {                                                       \
+        const S c = S(s);                                                                                  \
+        return a.map([c](S x) { return c OP x; });                                                         \
+    }
+CFD_EIGEN_ARRAY_OP(+)
+CFD_EIGEN_ARRAY_OP(-)
+CFD_EIGEN_ARRAY_OP(*)
+CFD_EIGEN_ARRAY_OP(/)
+#undef CFD_EIGEN_ARRAY_OP
+
+template <class S> Array1<S> operator-(const Array1<S>& a) {
+    return a.map([](S x) { return -x; });
+}
+template <class S> Array1<S> exp(const Array1<S>& a) { return a.exp(); }
+template <class S> Array1<S> abs(const Array1<S>& a) { return a.abs(); }
+template <class S> Array1<S> sqrt(const Array1<S>& a) { return a.sqrt(); }
+
+template <class S, int R, int C, int O, int MR, int MC>
+std::ostream& operator<<(std::ostream& os, const Matrix<S, R, C, O, MR, MC>& m) {
+    for (Index r = 0; r < m.rows(); ++r) {
+        for (Index c = 0; c < m.cols(); ++c) os << (c ? " " : "") << m(r, c);
+        os << "\n";
+    }
+    return os;
+}
+
+template <class S, int Size = Dynamic> using Vector = Matrix<S, Size, 1>;
+using VectorXf = Matrix<float, Dynamic, 1>;
+using VectorXd = Matrix<double, Dynamic, 1>;
+using MatrixXf = Matrix<float, Dynamic, Dynamic>;
+using MatrixXd = Matrix<double, Dynamic, Dynamic>;
+using Vector3f = Matrix<float, 3, 1>;
+
+// Placeholder for Eigen::Tensor, which only appears in a type alias of the
+// reference declarations.
+template <class S, int Rank, int Options = ColMajor> class Tensor {};
+
+}  // namespace Eigen

@@ -1,0 +1,598 @@
+// Variable / operator expression API and the time loop of the explicit
+// finite-volume path (src/fvm3d.cpp of the reference), evaluated on the MI355X.
+#include "fvm3d.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+
+#include "cfdarcho_main_3d.hpp"
+
+namespace cfdarco {
+void check(int rc, const char* what);
+}
+
+using cfdarco::Node;
+using cfdarco::NodeKind;
+
+namespace {
+
+std::shared_ptr<Node> make_node(NodeKind k) {
+    auto n = std::make_shared<Node>();
+    n->kind = k;
+    return n;
+}
+
+size_t mesh_cells(const Mesh3D* m) {
+    if (m == nullptr) throw std::runtime_error("expression has no mesh");
+    return m->_num_nodes;
+}
+
+Mesh3D* node_mesh(const Node& n) {
+    if (n.mesh) return n.mesh;
+    if (n.kind == NodeKind::Leaf && n.leaf) return n.leaf->mesh;
+    if (n.a) {
+        if (Mesh3D* m = node_mesh(*n.a)) return m;
+    }
+    if (n.b) return node_mesh(*n.b);
+    return nullptr;
+}
+
+CudaMesh3D* cuda_mesh(Mesh3D* m) {
+    auto* cm = dynamic_cast<CudaMesh3D*>(m);
+    if (cm == nullptr || cm->device_mesh() == nullptr)
+        throw std::runtime_error("operator needs a CudaMesh3D on the device (host_to_device)");
+    return cm;
+}
+
+std::shared_ptr<Node> binary(char op, const Variable& l, const Variable& r) {
+    auto n = make_node(NodeKind::Binary);
+    n->op = op;
+    n->a = l.node;
+    n->b = r.node;
+    n->mesh = l.mesh ? l.mesh : r.mesh;
+    return n;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// per-op evaluation (the reference's evaluate_cu tree walk)
+// ---------------------------------------------------------------------------
+namespace cfdarco {
+
+static CudaDataMatrixD scalar_op(int op, const CudaDataMatrixD& a, float s) {
+    CudaDataMatrixD res{a._size};
+    check(cfd_ewise(CFDArcoGlobalInit::context(), op, CFD_F32, a.data.get(), nullptr, nullptr, double(s),
+                    res.data.get(), int64_t(a._size), nullptr),
+          "elementwise op");
+    return res;
+}
+
+static bool scalar_value(const Node& n, float& v) {
+    switch (n.kind) {
+        case NodeKind::Const: v = n.value; return true;
+        case NodeKind::DtScalar: v = n.dt->_dt; return true;
+        case NodeKind::Pointer: v = *n.ptr; return true;
+        default: return false;
+    }
+}
+
+CudaDataMatrixD evaluate_node(const Node& n) {
+    switch (n.kind) {
+        case NodeKind::Leaf:
+            return n.leaf->current_cu;
+        case NodeKind::Array:
+            return n.array;
+        case NodeKind::Const:
+        case NodeKind::DtScalar:
+        case NodeKind::Pointer: {
+            float v = 0;
+            scalar_value(n, v);
+            return CudaDataMatrixD{mesh_cells(node_mesh(n)), v};
+        }
+        case NodeKind::Binary: {
+            // a scalar operand is applied as a scalar: the same IEEE operation
+            // as on a materialised constant vector, without the vector
+            float s = 0;
+            if (scalar_value(*n.a, s)) {
+                const CudaDataMatrixD r = evaluate_node(*n.b);
+                const int op = n.op == '+' ? CFD_EW_SADD : n.op == '-' ? CFD_EW_SSUB : n.op == '*' ? CFD_EW_SMUL
+                                                                                                  : CFD_EW_SDIV;
+                return scalar_op(op, r, s);
+            }
+            if (scalar_value(*n.b, s)) {
+                const CudaDataMatrixD l = evaluate_node(*n.a);
+                const int op = n.op == '+' ? CFD_EW_ADDS : n.op == '-' ? CFD_EW_SUBS : n.op == '*' ? CFD_EW_MULS
+                                                                                                  : CFD_EW_DIVS;
+                return scalar_op(op, l, s);
+            }
+            const CudaDataMatrixD l = evaluate_node(*n.a);
+            const CudaDataMatrixD r = evaluate_node(*n.b);
+            switch (n.op) {
+                case '+': return add_mtrx(l, r);
+                case '-': return sub_mtrx(l, r);
+                case '*': return mul_mtrx(l, r);
+                default: return div_mtrx(l, r);
+            }
+        }
+        case NodeKind::Neg:
+            return neg_mtrx(evaluate_node(*n.a));
+        case NodeKind::Exp:
+        case NodeKind::Abs:
+            // as the reference's device path (src/fvm3d.cpp:472-494)
+            throw std::runtime_error("Not implemented");
+        case NodeKind::Grad:
+            return eval_grad(cuda_mesh(node_mesh(n)), evaluate_node(*n.a), n.mask & 1, n.mask & 2, n.mask & 4);
+        case NodeKind::Grad2:
+            return eval_grad2(cuda_mesh(node_mesh(n)), evaluate_node(*n.a), n.mask & 1, n.mask & 2, n.mask & 4);
+        case NodeKind::Stab:
+            return eval_stab(cuda_mesh(node_mesh(n)), evaluate_node(*n.a), n.mask & 1, n.mask & 2, n.mask & 4);
+    }
+    throw std::logic_error("unknown node");
+}
+
+size_t stencil_depth(const Node& n) {
+    size_t d = 0;
+    if (n.a) d = std::max(d, stencil_depth(*n.a));
+    if (n.b) d = std::max(d, stencil_depth(*n.b));
+    if (n.kind == NodeKind::Grad || n.kind == NodeKind::Grad2 || n.kind == NodeKind::Stab) d += 1;
+    return d;
+}
+
+}  // namespace cfdarco
+
+// ---------------------------------------------------------------------------
+// Variable
+// ---------------------------------------------------------------------------
+Variable::Variable() : name{"uninitialized"} {
+    node = make_node(NodeKind::Const);
+}
+
+Variable::Variable(Mesh3D* mesh_, Eigen::Matrix<float, -1, 1>& initial_, BoundaryFN bc, std::string name_)
+    : name{std::move(name_)}, mesh{mesh_}, current{initial_}, boundary_conditions{std::move(bc)} {
+    num_nodes = mesh->_num_nodes;
+    is_basically_created = true;
+    has_boundary_conditions_cu = false;
+    if (CFDArcoGlobalInit::cuda_enabled || CFDArcoGlobalInit::hip_enabled)
+        current_cu = CudaDataMatrixD::from_eigen(current);
+    node = make_node(NodeKind::Leaf);
+    node->leaf = this;
+    node->mesh = mesh;
+}
+
+Variable::Variable(Mesh3D* mesh_, Eigen::Matrix<float, -1, 1>& initial_, BoundaryFN bc, BoundaryFNCU bc_cu,
+                   std::string name_)
+    : Variable(mesh_, initial_, std::move(bc), std::move(name_)) {
+    boundary_conditions_cu = std::move(bc_cu);
+    has_boundary_conditions_cu = true;
+}
+
+Variable::Variable(Mesh3D* mesh_, float value) : mesh{mesh_} {
+    num_nodes = mesh->_num_nodes;
+    is_constvar = true;
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "constval_%g", double(value));
+    name = buf;
+    node = make_node(NodeKind::Const);
+    node->value = value;
+    node->mesh = mesh;
+}
+
+Variable::Variable(Eigen::Matrix<float, -1, 1>& curr_) : current{curr_} {
+    num_nodes = size_t(curr_.rows());
+    name = "arr";
+    is_constvar = true;
+    node = make_node(NodeKind::Array);
+    if (CFDArcoGlobalInit::cuda_enabled || CFDArcoGlobalInit::hip_enabled) {
+        current_cu = CudaDataMatrixD::from_eigen(current);
+        node->array = current_cu;
+    }
+}
+
+Variable::Variable(std::shared_ptr<Node> n, std::string name_) : name{std::move(name_)}, node{std::move(n)} {
+    mesh = node_mesh(*node);
+    num_nodes = mesh ? mesh->_num_nodes : 0;
+    is_subvariable = node->kind != NodeKind::Leaf;
+}
+
+Variable Variable::from_node(std::shared_ptr<Node> n, std::string name_) { return Variable(std::move(n), std::move(name_)); }
+
+Variable::Variable(const Variable& o)
+    : name{o.name}, mesh{o.mesh}, current{o.current}, boundary_conditions{o.boundary_conditions},
+      boundary_conditions_cu{o.boundary_conditions_cu}, history{o.history}, num_nodes{o.num_nodes},
+      has_boundary_conditions_cu{o.has_boundary_conditions_cu}, is_subvariable{o.is_subvariable},
+      is_constvar{o.is_constvar}, is_basically_created{false}, is_dt2{o.is_dt2}, node{o.node} {
+    current_cu = o.current_cu;  // shared buffer, as the reference's copy
+}
+
+Variable& Variable::operator=(const Variable& o) {
+    if (this == &o) return *this;
+    name = o.name;
+    mesh = o.mesh;
+    current = o.current;
+    current_cu = o.current_cu;
+    boundary_conditions = o.boundary_conditions;
+    boundary_conditions_cu = o.boundary_conditions_cu;
+    history = o.history;
+    num_nodes = o.num_nodes;
+    has_boundary_conditions_cu = o.has_boundary_conditions_cu;
+    is_subvariable = o.is_subvariable;
+    is_constvar = o.is_constvar;
+    is_dt2 = o.is_dt2;
+    node = o.node;
+    return *this;
+}
+
+Variable::~Variable() = default;
+
+std::shared_ptr<Variable> Variable::clone() const {
+    if (is_basically_created) return std::shared_ptr<Variable>{const_cast<Variable*>(this), [](Variable*) {}};
+    return std::make_shared<Variable>(*this);
+}
+
+void Variable::set_bound(const DT* dt_) {
+    current = boundary_conditions(mesh, current, dt_);
+    if (CFDArcoGlobalInit::cuda_enabled || CFDArcoGlobalInit::hip_enabled) {
+        if (current_cu._size == size_t(current.size()) && current_cu.data.use_count() == 1)
+            cfdarco::copy_h2d(current_cu.data.get(), current.data(), current_cu._size * sizeof(float));
+        else
+            current_cu = CudaDataMatrixD::from_eigen(current);
+    }
+}
+
+void Variable::set_bound_cu(const DT* dt_) {
+    current_cu = boundary_conditions_cu(mesh, current_cu, dt_);
+    if (CFDArcoGlobalInit::get_size() > 1) current = current_cu.to_eigen();
+}
+
+void Variable::add_history() {
+    if (!CFDArcoGlobalInit::skip_history) {
+        if (CFDArcoGlobalInit::cuda_enabled || CFDArcoGlobalInit::hip_enabled) current = current_cu.to_eigen();
+        history.push_back(current);
+    }
+}
+
+Eigen::Matrix<float, -1, 1> Variable::extract(Eigen::Matrix<float, -1, 1>& left_part, float) { return left_part; }
+CudaDataMatrixD Variable::extract_cu(CudaDataMatrixD& left_part, float) { return left_part; }
+
+MatrixX4dRB Variable::evaluate() {
+    CudaDataMatrixD v = evaluate_cu();
+    return MatrixX4dRB(v.to_eigen());
+}
+
+CudaDataMatrixD Variable::evaluate_cu() { return cfdarco::evaluate_node(*node); }
+
+void Variable::set_current(Eigen::Matrix<float, -1, 1>& current_) {
+    current = current_;
+    if (CFDArcoGlobalInit::cuda_enabled || CFDArcoGlobalInit::hip_enabled) current_cu = CudaDataMatrixD::from_eigen(current);
+}
+
+void Variable::set_current(CudaDataMatrixD& current_, bool copy_to_host) {
+    current_cu = current_;
+    if (copy_to_host) current = current_.to_eigen();
+}
+
+std::vector<Eigen::Matrix<float, -1, 1>> Variable::get_history() const { return history; }
+
+void Variable::solve(Variable* equation, DT* dt) { EqSolver::solve_dt(equation, this, this, dt); }
+
+Variable Variable::operator+(const Variable& r) const { return from_node(binary('+', *this, r), name + "_plus_" + r.name); }
+Variable Variable::operator-(const Variable& r) const { return from_node(binary('-', *this, r), name + "_minus_" + r.name); }
+Variable Variable::operator*(const Variable& r) const { return from_node(binary('*', *this, r), name + "_mult_" + r.name); }
+Variable Variable::operator/(const Variable& r) const { return from_node(binary('/', *this, r), name + "_div_" + r.name); }
+
+Variable Variable::operator-() const {
+    auto n = make_node(NodeKind::Neg);
+    n->a = node;
+    n->mesh = mesh;
+    return from_node(n, "neg_" + name);
+}
+
+Variable Variable::exp() const {
+    auto n = make_node(NodeKind::Exp);
+    n->a = node;
+    n->mesh = mesh;
+    return from_node(n, "exp(" + name + ")");
+}
+
+Variable operator+(float l, const Variable& r) { return Variable(r.mesh, l) + r; }
+Variable operator-(float l, const Variable& r) { return Variable(r.mesh, l) - r; }
+Variable operator*(float l, const Variable& r) { return Variable(r.mesh, l) * r; }
+Variable operator/(float l, const Variable& r) { return Variable(r.mesh, l) / r; }
+Variable operator+(const Variable& l, float r) { return l + Variable(l.mesh, r); }
+Variable operator-(const Variable& l, float r) { return l - Variable(l.mesh, r); }
+Variable operator*(const Variable& l, float r) { return l * Variable(l.mesh, r); }
+Variable operator/(const Variable& l, float r) { return l / Variable(l.mesh, r); }
+
+Variable exp(const Variable& obj) { return obj.exp(); }
+
+Variable abs(const Variable& obj) {
+    auto n = make_node(NodeKind::Abs);
+    n->a = obj.node;
+    n->mesh = obj.mesh;
+    return Variable::from_node(n, "abs(" + obj.name + ")");
+}
+
+// ---------------------------------------------------------------------------
+// update policies (src/fvm3d.cpp:496-567): gamma = 5/3 and dl = min spacing
+// regardless of the driver (Q11)
+// ---------------------------------------------------------------------------
+float UpdatePolicies::CourantFriedrichsLewy(float CFL, std::vector<Eigen::Matrix<float, -1, 1>>& sv, Mesh3D* mesh) {
+    auto& u = sv.at(0);
+    auto& v = sv.at(1);
+    auto& p = sv.at(2);
+    auto& rho = sv.at(3);
+    const float gamma = float(5. / 3.);
+    const float dl = std::min({mesh->_dx, mesh->_dy, mesh->_dz});
+    float mn = std::numeric_limits<float>::infinity();
+    for (Eigen::Index i = 0; i < u.size(); ++i) {
+        const float a = std::sqrt((gamma * p(i)) / rho(i));
+        const float s = std::sqrt(u(i) * u(i) + v(i) * v(i));
+        mn = std::min(mn, dl * (1.0f / (a + s)));
+    }
+    return CFL * mn;
+}
+
+float UpdatePolicies::CourantFriedrichsLewy3D(float CFL, std::vector<Eigen::Matrix<float, -1, 1>>& sv, Mesh3D* mesh) {
+    auto& u = sv.at(0);
+    auto& v = sv.at(1);
+    auto& w = sv.at(2);
+    auto& p = sv.at(3);
+    auto& rho = sv.at(4);
+    const float gamma = float(5. / 3.);
+    const float dl = std::min({mesh->_dx, mesh->_dy, mesh->_dz});
+    float mn = std::numeric_limits<float>::infinity();
+    for (Eigen::Index i = 0; i < u.size(); ++i) {
+        const float a = std::sqrt((gamma * p(i)) / rho(i));
+        const float s = std::sqrt((u(i) * u(i) + v(i) * v(i)) + w(i) * w(i));
+        mn = std::min(mn, dl * (1.0f / (a + s)));
+    }
+    return CFL * mn;
+}
+
+float UpdatePolicies::CourantFriedrichsLewy1D(float CFL, std::vector<Eigen::Matrix<float, -1, 1>>& sv, Mesh3D* mesh) {
+    auto& u = sv.at(0);
+    auto& p = sv.at(1);
+    auto& rho = sv.at(2);
+    const float gamma = float(5. / 3.);
+    const float dl = std::min({mesh->_dx, mesh->_dy, mesh->_dz});
+    float mn = std::numeric_limits<float>::infinity();
+    for (Eigen::Index i = 0; i < u.size(); ++i) {
+        const float a = std::sqrt((gamma * p(i)) / rho(i));
+        const float s = std::sqrt(u(i) * u(i));
+        mn = std::min(mn, dl * (1.0f / (a + s)));
+    }
+    return CFL * mn;
+}
+
+float UpdatePolicies::CourantFriedrichsLewy3DCu(float CFL, std::vector<CudaDataMatrixD*>& sv, Mesh3D* mesh) {
+    const float gamma = float(5. / 3.);
+    const float dl = std::min(std::min(mesh->_dx, mesh->_dy), mesh->_dz);
+    return CFL * cfl_cu(dl, gamma, *sv.at(3), *sv.at(4), *sv.at(0), *sv.at(1), *sv.at(2));
+}
+
+float UpdatePolicies::CourantFriedrichsLewy1DCu(float CFL, std::vector<CudaDataMatrixD*>& sv, Mesh3D* mesh) {
+    const float gamma = float(5. / 3.);
+    const float dl = std::min(std::min(mesh->_dx, mesh->_dy), mesh->_dz);
+    return CFL * cfl_cu(dl, gamma, *sv.at(1), *sv.at(2), *sv.at(0));
+}
+
+float UpdatePolicies::CourantFriedrichsLewyCu(float CFL, std::vector<CudaDataMatrixD>& sv, Mesh3D* mesh) {
+    const float gamma = float(5. / 3.);
+    const float dl = std::min(std::min(mesh->_dx, mesh->_dy), mesh->_dz);
+    return CFL * cfl_cu(dl, gamma, sv.at(2), sv.at(3), sv.at(0), sv.at(1), sv.at(1));
+}
+
+// ---------------------------------------------------------------------------
+// PointerVariable, DT
+// ---------------------------------------------------------------------------
+PointerVariable::PointerVariable(Mesh3D* mesh_, float* ptr) : _ptr{ptr} {
+    name = "PointerVariable";
+    mesh = mesh_;
+    num_nodes = mesh ? mesh->_num_nodes : 0;
+    node = make_node(NodeKind::Pointer);
+    node->ptr = _ptr;
+    node->mesh = mesh;
+}
+
+std::shared_ptr<Variable> PointerVariable::clone() const {
+    return std::shared_ptr<Variable>{const_cast<PointerVariable*>(this), [](Variable*) {}};
+}
+
+DT::DT(Mesh3D* mesh_, UpdateFn update_fn_, float CFL_, std::vector<Variable*> space_vars_)
+    : update_fn{std::move(update_fn_)}, space_vars{std::move(space_vars_)}, CFL{CFL_} {
+    name = "dt";
+    mesh = mesh_;
+    num_nodes = mesh ? mesh->_num_nodes : 0;
+    node = make_node(NodeKind::DtScalar);
+    node->dt = this;
+    node->mesh = mesh;
+}
+
+DT::DT(Mesh3D* mesh_, UpdateFn update_fn_, UpdateFnCu update_fn_cu_, float CFL_, std::vector<Variable*> space_vars_)
+    : DT(mesh_, std::move(update_fn_), CFL_, std::move(space_vars_)) {
+    update_fn_cu = std::move(update_fn_cu_);
+    has_update_fn_cu = true;
+}
+
+std::shared_ptr<Variable> DT::clone() const { return std::shared_ptr<Variable>{const_cast<DT*>(this), [](Variable*) {}}; }
+
+void DT::update() {
+    float dt_c = 0.0;
+    if (has_update_fn_cu && CFDArcoGlobalInit::cuda_enabled) {
+        std::vector<CudaDataMatrixD*> redist{};
+        for (auto& var : space_vars) redist.push_back(&var->current_cu);
+        dt_c = update_fn_cu(CFL, redist, mesh);
+    } else {
+        std::vector<Eigen::Matrix<float, -1, 1>> redist{};
+        for (auto var : space_vars) redist.push_back(var->current);
+        dt_c = update_fn(CFL, redist, mesh);
+    }
+    _dt = dt_c;
+    _current_time_step_int++;
+    _current_time_dbl += _dt;
+}
+
+// ---------------------------------------------------------------------------
+// time operators
+// ---------------------------------------------------------------------------
+DtVar::DtVar(Variable* var_, int) {
+    var = std::shared_ptr<Variable>{var_, [](Variable*) {}};
+    mesh = var_->mesh;
+    num_nodes = var_->num_nodes;
+    node = var_->node;
+}
+
+Eigen::Matrix<float, -1, 1> DtVar::extract(Eigen::Matrix<float, -1, 1>& left_part, float dt) {
+    return dt * left_part + var->current;
+}
+
+// (rhs * dt) + current: the reference's mul_mtrx then add (src/fvm3d.cpp:694-697)
+CudaDataMatrixD DtVar::extract_cu(CudaDataMatrixD& left_part, float dt) {
+    CudaDataMatrixD res{left_part._size};
+    cfdarco::check(cfd_ewise(CFDArcoGlobalInit::context(), CFD_EW_AXPY, CFD_F32, left_part.data.get(),
+                             var->current_cu.data.get(), nullptr, double(dt), res.data.get(),
+                             int64_t(left_part._size), nullptr),
+                   "DtVar update");
+    return res;
+}
+
+void DtVar::solve(Variable* equation, DT* dt) { EqSolver::solve_dt(equation, this, var.get(), dt); }
+std::shared_ptr<Variable> DtVar::clone() const { return std::make_shared<DtVar>(*this); }
+
+D2tVar::D2tVar(Variable* var_, int) {
+    var = std::shared_ptr<Variable>{var_, [](Variable*) {}};
+    var->is_dt2 = true;
+    mesh = var_->mesh;
+    num_nodes = var_->num_nodes;
+    node = var_->node;
+    if (CFDArcoGlobalInit::cuda_enabled || CFDArcoGlobalInit::hip_enabled) {
+        pre_current_cu = CudaDataMatrixD{var->current_cu._size, 0.0f};
+        next_current_cu = CudaDataMatrixD{var->current_cu._size, 0.0f};
+    }
+}
+
+Eigen::Matrix<float, -1, 1> D2tVar::extract(Eigen::Matrix<float, -1, 1>& left_part, float dt) {
+    if (var->history.size() > 1)
+        return dt * dt * left_part + 2 * var->current - var->history.at(var->history.size() - 2);
+    return dt * left_part + var->current;
+}
+
+CudaDataMatrixD D2tVar::extract_cu(CudaDataMatrixD& left_part, float dt) {
+    CudaDataMatrixD res{left_part._size};
+    if (var->history.size() > 1) {
+        CudaDataMatrixD prev = CudaDataMatrixD::from_eigen(var->history.at(var->history.size() - 2));
+        cfdarco::check(cfd_ewise(CFDArcoGlobalInit::context(), CFD_EW_LEAP, CFD_F32, left_part.data.get(),
+                                 var->current_cu.data.get(), prev.data.get(), double(dt), res.data.get(),
+                                 int64_t(left_part._size), nullptr),
+                       "D2tVar update");
+        return res;
+    }
+    cfdarco::check(cfd_ewise(CFDArcoGlobalInit::context(), CFD_EW_AXPY, CFD_F32, left_part.data.get(),
+                             var->current_cu.data.get(), nullptr, double(dt), res.data.get(),
+                             int64_t(left_part._size), nullptr),
+                   "D2tVar update");
+    return res;
+}
+
+void D2tVar::solve(Variable* equation, DT* dt) { EqSolver::solve_dt(equation, this, var.get(), dt); }
+std::shared_ptr<Variable> D2tVar::clone() const { return std::make_shared<D2tVar>(*this); }
+
+// ---------------------------------------------------------------------------
+// spatial operators
+// ---------------------------------------------------------------------------
+namespace {
+std::shared_ptr<Node> stencil(NodeKind k, Variable* var_, bool x, bool y, bool z) {
+    auto n = make_node(k);
+    n->a = var_->node;
+    n->mask = (x ? 1 : 0) | (y ? 2 : 0) | (z ? 4 : 0);
+    n->mesh = var_->mesh;
+    return n;
+}
+std::string flags(bool x, bool y, bool z) { return std::to_string(x) + std::to_string(y) + std::to_string(z); }
+}  // namespace
+
+GradVar::GradVar(Variable* var_, bool x, bool y, bool z)
+    : Variable(stencil(NodeKind::Grad, var_, x, y, z), "Grad_" + flags(x, y, z) + var_->name), clc_x{x}, clc_y{y}, clc_z{z} {
+    var = var_->clone();
+}
+std::shared_ptr<Variable> GradVar::clone() const { return std::make_shared<GradVar>(*this); }
+
+Grad2Var::Grad2Var(Variable* var_, bool x, bool y, bool z)
+    : Variable(stencil(NodeKind::Grad2, var_, x, y, z), "Lapl_" + flags(x, y, z) + var_->name), clc_x{x}, clc_y{y}, clc_z{z} {
+    var = var_->clone();
+}
+std::shared_ptr<Variable> Grad2Var::clone() const { return std::make_shared<Grad2Var>(*this); }
+
+StabVar::StabVar(Variable* var_, bool x, bool y, bool z)
+    : Variable(stencil(NodeKind::Stab, var_, x, y, z), "Stab_" + flags(x, y, z) + var_->name), clc_x{x}, clc_y{y}, clc_z{z} {
+    var = var_->clone();
+}
+std::shared_ptr<Variable> StabVar::clone() const { return std::make_shared<StabVar>(*this); }
+
+// ---------------------------------------------------------------------------
+// solver and time loop (src/fvm3d.cpp:944-1069)
+// ---------------------------------------------------------------------------
+void EqSolver::solve_dt(Variable* equation, Variable* time_var, Variable* set_var, DT* dt) {
+    CudaDataMatrixD cur = equation->evaluate_cu();
+    CudaDataMatrixD extracted = time_var->extract_cu(cur, dt->_dt);
+    set_var->set_current(extracted, false);
+}
+
+Equation::Equation(size_t timesteps_) : timesteps{timesteps_} {}
+
+void Equation::evaluate(std::vector<Variable*>& all_vars, EquationTemplate& equation_system, DT* dt, bool visualize,
+                        std::vector<Variable*>) const {
+    std::vector<std::unique_ptr<KernelBuilder>> krns(equation_system.size());
+    for (size_t i = 0; i < equation_system.size(); ++i) {
+        if (std::get<3>(equation_system[i])) {
+            krns[i] = std::make_unique<KernelBuilder>(std::get<0>(equation_system[i]), &std::get<2>(equation_system[i]));
+            krns[i]->build();
+        }
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    int last_pct = -1;
+    for (size_t t = 0; t < timesteps; ++t) {
+        dt->update();
+        for (size_t i = 0; i < equation_system.size(); ++i) {
+            auto& eq = equation_system[i];
+            if (krns[i]) {
+                std::get<0>(eq)->set_dt(dt->_dt);
+                krns[i]->run();
+            } else {
+                std::get<0>(eq)->solve(&std::get<2>(eq), dt);
+            }
+        }
+        for (auto& var : all_vars) {
+            if (var->has_boundary_conditions_cu) {
+                var->set_bound_cu(dt);
+            } else {
+                var->current = var->current_cu.to_eigen();
+                var->set_bound(dt);
+            }
+            var->add_history();
+        }
+        if (visualize && CFDArcoGlobalInit::get_rank() == 0) {
+            const int pct = int(100.0 * double(t + 1) / double(timesteps));
+            if (pct != last_pct) {
+                const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                std::fprintf(stderr, "\r[%3d%%] step %zu/%zu  %.1f s", pct, t + 1, timesteps, el);
+                last_pct = pct;
+            }
+        }
+    }
+    if (visualize) std::fprintf(stderr, "\n");
+    sync_device();
+    for (auto var : all_vars) {
+        var->current = var->current_cu.to_eigen();
+        std::cout << "At " << var->name << " Current sum = " << var->current.sum()
+                  << " Current max = " << var->current.maxCoeff() << std::endl;
+    }
+    if (CFDArcoGlobalInit::get_rank() == 0) std::cout << "Time progress = " << dt->_current_time_dbl << std::endl;
+}

@@ -1,0 +1,201 @@
+// Process state and device helpers of the operator API: CFDArcoGlobalInit
+// (src/cfdarcho_main_3d.cpp:276-347 of the reference), device memory for
+// CudaDataMatrix, and the operator free functions of cuda_operators.hpp, all
+// on the HIP library's C-ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include <ctime>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+
+#include "cfdarcho_main_3d.hpp"
+#include "cuda_data_matrix.hpp"
+#include "cuda_operators.hpp"
+#include "mesh3d.hpp"
+#include "pool_allocator.hpp"
+
+int CFDArcoGlobalInit::world_size = 1;
+int CFDArcoGlobalInit::world_rank = 0;
+bool CFDArcoGlobalInit::skip_history = false;
+bool CFDArcoGlobalInit::cuda_enabled = false;
+bool CFDArcoGlobalInit::hip_enabled = false;
+bool CFDArcoGlobalInit::store_stepping = false;
+fs::path CFDArcoGlobalInit::store_dir = {};
+cfd_ctx_t CFDArcoGlobalInit::ctx_ = nullptr;
+bool Allocator::allocator_alive = false;
+
+namespace cfdarco {
+
+void check(int rc, const char* what) {
+    if (rc != CFD_OK) throw std::runtime_error(std::string(what) + ": " + cfd_last_error());
+}
+
+void* device_alloc(size_t bytes) {
+    void* p = nullptr;
+    check(cfd_alloc(CFDArcoGlobalInit::context(), int64_t(bytes), &p, nullptr), "device allocation");
+    return p;
+}
+
+void device_free(void* p) {
+    cfd_ctx_t ctx = CFDArcoGlobalInit::context_or_null();
+    if (ctx) (void)cfd_free(ctx, p, nullptr);  // a free after finalize() is a no-op
+}
+
+void copy_h2d(void* dst, const void* src, size_t bytes) {
+    check(cfd_memcpy(CFDArcoGlobalInit::context(), dst, src, int64_t(bytes), CFD_H2D, nullptr), "host-to-device copy");
+}
+void copy_d2h(void* dst, const void* src, size_t bytes) {
+    check(cfd_memcpy(CFDArcoGlobalInit::context(), dst, src, int64_t(bytes), CFD_D2H, nullptr), "device-to-host copy");
+}
+void copy_d2d(void* dst, const void* src, size_t bytes) {
+    check(cfd_memcpy(CFDArcoGlobalInit::context(), dst, src, int64_t(bytes), CFD_D2D, nullptr), "device copy");
+}
+
+void fill_device(void* dst, const void* value, size_t elem_bytes, size_t n) {
+    if (elem_bytes == 4) {
+        float v;
+        std::memcpy(&v, value, 4);
+        check(cfd_ewise(CFDArcoGlobalInit::context(), CFD_EW_FILL, CFD_F32, nullptr, nullptr, nullptr, double(v), dst,
+                        int64_t(n), nullptr),
+              "fill");
+    } else if (elem_bytes == 8) {
+        // 8-byte element types (double, size_t, ptrdiff_t) go through a host vector
+        std::vector<uint64_t> host(n);
+        uint64_t v;
+        std::memcpy(&v, value, 8);
+        std::fill(host.begin(), host.end(), v);
+        copy_h2d(dst, host.data(), n * 8);
+    } else {
+        throw std::runtime_error("fill_device: unsupported element size");
+    }
+}
+
+}  // namespace cfdarco
+
+void CFDArcoGlobalInit::initialize(int, char**, bool skip_history_, const fs::path& store_path) {
+    world_size = 1;
+    world_rank = 0;
+    skip_history = skip_history_;
+    std::time_t t = std::time(nullptr);
+    std::tm* now = std::localtime(&t);
+    const std::string stamp = std::to_string(now->tm_year + 1900) + "_" + std::to_string(now->tm_mon + 1) + "_" +
+                              std::to_string(now->tm_mday) + "_" + std::to_string(now->tm_hour) + "_" +
+                              std::to_string(now->tm_min) + "_" + std::to_string(now->tm_sec);
+    store_dir = store_path / ("run_" + stamp);
+}
+
+void CFDArcoGlobalInit::enable_cuda(int, bool use_cuda, bool use_hip) {
+    if (!use_cuda && !use_hip) throw std::runtime_error{"!use_cuda and !use_hip"};
+    if (ctx_ == nullptr) {
+        const char* env = std::getenv("CFDARCO_DEVICE");
+        const int dev = env ? std::atoi(env) : 0;
+        cfdarco::check(cfd_init(dev, &ctx_), "cfd_init");
+    }
+    cuda_enabled = use_cuda;
+    hip_enabled = use_hip;
+    Allocator::allocator_alive = true;
+}
+
+cfd_ctx_t CFDArcoGlobalInit::context() {
+    if (ctx_ == nullptr)
+        throw std::runtime_error("device path not enabled: call CFDArcoGlobalInit::enable_cuda (-c / -hip)");
+    return ctx_;
+}
+
+cfd_ctx_t CFDArcoGlobalInit::context_or_null() { return ctx_; }
+
+// Waits for the device.  The context and its pool stay alive until the
+// process ends, because Variables and meshes still own pool blocks (the
+// reference resets its pool here and leaks those handles).
+void CFDArcoGlobalInit::finalize() {
+    if (ctx_ != nullptr) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) std::cerr << "finalize: " << hipGetErrorString(e) << std::endl;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// cuda_operators.hpp
+// ---------------------------------------------------------------------------
+namespace {
+
+CudaDataMatrixD ewise2(int op, const CudaDataMatrixD& a, const CudaDataMatrixD& b) {
+    if (a._size != b._size) throw std::runtime_error("elementwise operands differ in size");
+    CudaDataMatrixD res{a._size};
+    cfdarco::check(cfd_ewise(CFDArcoGlobalInit::context(), op, CFD_F32, a.data.get(), b.data.get(), nullptr, 0.0,
+                             res.data.get(), int64_t(a._size), nullptr),
+                   "elementwise op");
+    return res;
+}
+
+CudaDataMatrixD ewise_s(int op, const CudaDataMatrixD& a, float s) {
+    CudaDataMatrixD res{a._size};
+    cfdarco::check(cfd_ewise(CFDArcoGlobalInit::context(), op, CFD_F32, a.data.get(), nullptr, nullptr, double(s),
+                             res.data.get(), int64_t(a._size), nullptr),
+                   "elementwise op");
+    return res;
+}
+
+int mask_of(bool x, bool y, bool z) { return (x ? 1 : 0) | (y ? 2 : 0) | (z ? 4 : 0); }
+
+cfd_mesh_t dev_mesh(CudaMesh3D* mesh) {
+    if (mesh == nullptr || mesh->device_mesh() == nullptr)
+        throw std::runtime_error("mesh is not on the device: call CudaMesh3D::host_to_device()");
+    return mesh->device_mesh();
+}
+
+}  // namespace
+
+CudaDataMatrixD add_mtrx(const CudaDataMatrixD& a, const CudaDataMatrixD& b) { return ewise2(CFD_EW_ADD, a, b); }
+CudaDataMatrixD sub_mtrx(const CudaDataMatrixD& a, const CudaDataMatrixD& b) { return ewise2(CFD_EW_SUB, a, b); }
+CudaDataMatrixD mul_mtrx(const CudaDataMatrixD& a, const CudaDataMatrixD& b) { return ewise2(CFD_EW_MUL, a, b); }
+CudaDataMatrixD mul_mtrx(const CudaDataMatrixD& a, float b) { return ewise_s(CFD_EW_MULS, a, b); }
+CudaDataMatrixD div_mtrx(const CudaDataMatrixD& a, const CudaDataMatrixD& b) { return ewise2(CFD_EW_DIV, a, b); }
+CudaDataMatrixD neg_mtrx(const CudaDataMatrixD& a) { return ewise_s(CFD_EW_NEG, a, 0.f); }
+CudaDataMatrixD div_const(const CudaDataMatrixD& a, float b) { return ewise_s(CFD_EW_DIVS, a, b); }
+
+CudaDataMatrixD eval_grad(CudaMesh3D* mesh, const CudaDataMatrixD& a, bool x, bool y, bool z) {
+    CudaDataMatrixD res{a._size};
+    cfdarco::check(cfd_grad(dev_mesh(mesh), a.data.get(), res.data.get(), mask_of(x, y, z), -1, nullptr), "eval_grad");
+    return res;
+}
+
+CudaDataMatrixD eval_grad2(CudaMesh3D* mesh, const CudaDataMatrixD& a, bool x, bool y, bool z) {
+    CudaDataMatrixD res{a._size};
+    cfdarco::check(cfd_lapl(dev_mesh(mesh), a.data.get(), res.data.get(), mask_of(x, y, z), nullptr), "eval_grad2");
+    return res;
+}
+
+CudaDataMatrixD eval_stab(CudaMesh3D* mesh, const CudaDataMatrixD& a, bool x, bool y, bool z) {
+    CudaDataMatrixD res{a._size};
+    cfdarco::check(cfd_stab(dev_mesh(mesh), a.data.get(), res.data.get(), mask_of(x, y, z), nullptr), "eval_stab");
+    return res;
+}
+
+static float cfl_common(float dl, float gamma, const CudaDataMatrixD& p, const CudaDataMatrixD& rho,
+                        const CudaDataMatrixD& u, const float* v, const float* w) {
+    double out = 0;
+    cfdarco::check(cfd_cfl_min(CFDArcoGlobalInit::context(), CFD_F32, u.data.get(), v, w, p.data.get(),
+                               rho.data.get(), int64_t(p._size), double(dl), 1.0, double(gamma), &out, nullptr),
+                   "cfl_cu");
+    return float(out);
+}
+
+float cfl_cu(float dl, float gamma, const CudaDataMatrixD& p, const CudaDataMatrixD& rho, const CudaDataMatrixD& u,
+             const CudaDataMatrixD& v, const CudaDataMatrixD& w) {
+    return cfl_common(dl, gamma, p, rho, u, v.data.get(), w.data.get());
+}
+
+float cfl_cu(float dl, float gamma, const CudaDataMatrixD& p, const CudaDataMatrixD& rho, const CudaDataMatrixD& u) {
+    return cfl_common(dl, gamma, p, rho, u, nullptr, nullptr);
+}
+
+void sync_device() {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) throw std::runtime_error(std::string("sync_device: ") + hipGetErrorString(e));
+}

@@ -29,6 +29,11 @@ __global__ void __launch_bounds__(256) ewise_k(int op, const T* __restrict__ a, 
             case CFD_EW_AXPY: r = s * a[i] + b[i]; break;
             case CFD_EW_LEAP: r = ((s * s) * a[i] + T(2) * b[i]) - aux[i]; break;
             case CFD_EW_FILL: r = s; break;
+            case CFD_EW_ADDS: r = a[i] + s; break;
+            case CFD_EW_SADD: r = s + a[i]; break;
+            case CFD_EW_SUBS: r = a[i] - s; break;
+            case CFD_EW_SSUB: r = s - a[i]; break;
+            case CFD_EW_SDIV: r = s / a[i]; break;
             default: r = a[i]; break;  // CFD_EW_COPY
         }
         out[i] = r;

@@ -454,7 +454,7 @@ int cfd_ewise(cfd_ctx_t ctx, int op, int dtype, const void* a, const void* b, co
     return guarded([&] {
         CFD_REQUIRE(ctx != nullptr && out != nullptr, "cfd_ewise: null argument");
         check_dtype(dtype);
-        CFD_REQUIRE(op >= CFD_EW_ADD && op <= CFD_EW_COPY, "cfd_ewise: unknown op");
+        CFD_REQUIRE(op >= CFD_EW_ADD && op <= CFD_EW_SDIV, "cfd_ewise: unknown op");
         CFD_REQUIRE(n >= 0, "cfd_ewise: negative length");
         const bool needs_a = op != CFD_EW_FILL;
         const bool needs_b = op == CFD_EW_ADD || op == CFD_EW_SUB || op == CFD_EW_MUL || op == CFD_EW_DIV ||
@@ -467,38 +467,67 @@ int cfd_ewise(cfd_ctx_t ctx, int op, int dtype, const void* a, const void* b, co
     });
 }
 
-int cfd_cfl_min(cfd_mesh_t m, const void* u, const void* v, const void* w, const void* p, const void* rho,
-                double cfl, double gamma, double* dt_out, void* stream) {
+int cfd_cfl_min(cfd_ctx_t ctx, int dtype, const void* u, const void* v, const void* w, const void* p,
+                const void* rho, int64_t n, double dl, double cfl, double gamma, double* dt_out, void* stream) {
     return guarded([&] {
-        CFD_REQUIRE(m && u && p && rho && dt_out, "cfd_cfl_min: null argument");
+        CFD_REQUIRE(ctx && u && p && rho && dt_out, "cfd_cfl_min: null argument");
+        check_dtype(dtype);
+        CFD_REQUIRE(n > 0, "cfd_cfl_min: empty field");
         hipStream_t s = as_stream(stream);
-        const int nb = int(std::min<int64_t>(std::max<int64_t>((m->N + 255) / 256, 1), 2048));
-        const int es = elem_size(m->dtype);
-        void* scratch = pool_alloc(m->ctx, int64_t(nb + 1) * es, s);
+        const int nb = int(std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 2048));
+        const int es = elem_size(dtype);
+        void* scratch = pool_alloc(ctx, int64_t(nb + 1) * es, s);
         char* base = (char*)scratch;
         double host = 0;
         try {
-            if (m->dtype == CFD_F32) {
-                const float dl = std::min(std::min(float(m->dx), float(m->dy)), float(m->dz));
-                launch_cfl<float>(u, v, w, p, rho, dl, float(gamma), m->N, base, nb, base + int64_t(nb) * es, s);
+            if (dtype == CFD_F32) {
+                launch_cfl<float>(u, v, w, p, rho, float(dl), float(gamma), n, base, nb, base + int64_t(nb) * es, s);
                 float r = 0;
                 CFD_HIP(hipMemcpyAsync(&r, base + int64_t(nb) * es, 4, hipMemcpyDeviceToHost, s));
                 CFD_HIP(hipStreamSynchronize(s));
                 host = double(float(cfl) * r);
             } else {
-                const double dl = std::min(std::min(m->dx, m->dy), m->dz);
-                launch_cfl<double>(u, v, w, p, rho, dl, gamma, m->N, base, nb, base + int64_t(nb) * es, s);
+                launch_cfl<double>(u, v, w, p, rho, dl, gamma, n, base, nb, base + int64_t(nb) * es, s);
                 double r = 0;
                 CFD_HIP(hipMemcpyAsync(&r, base + int64_t(nb) * es, 8, hipMemcpyDeviceToHost, s));
                 CFD_HIP(hipStreamSynchronize(s));
                 host = cfl * r;
             }
         } catch (...) {
-            pool_free(m->ctx, scratch, s);
+            pool_free(ctx, scratch, s);
             throw;
         }
-        pool_free(m->ctx, scratch, s);
+        pool_free(ctx, scratch, s);
         *dt_out = host;
+    });
+}
+
+int cfd_mesh_geometry(cfd_mesh_t m, cfd_geometry* out) {
+    return guarded([&] {
+        CFD_REQUIRE(m && out, "cfd_mesh_geometry: null argument");
+        cfd_geometry g{};
+        g.uniform = m->uniform ? 1 : 0;
+        g.dtype = m->dtype;
+        g.nx = m->X; g.ny = m->Y; g.nz = m->Z;
+        auto fill = [&](const auto& ug, const auto& gg) {
+            for (int a = 0; a < 3; ++a) {
+                g.area[a] = double(ug.area[a]);
+                g.alpha[a] = double(ug.alpha[a]);
+                g.alpha_b[a] = double(ug.alpha_b[a]);
+                g.len[a] = double(ug.len[a]);
+            }
+            g.vol = double(ug.vol);
+            for (int f = 0; f < 3; ++f) {
+                for (int d = 0; d < 3; ++d) g.face_normal[f][d] = gg.fn[f][d];
+                g.face_area[f] = gg.farea[f];
+                g.face_alpha[f] = gg.falpha[f];
+            }
+            g.cell_vol = gg.vol;
+            g.cell_len = gg.len;
+        };
+        if (m->dtype == CFD_F32) fill(m->ug32, m->gg32);
+        else fill(m->ug64, m->gg64);
+        *out = g;
     });
 }
 

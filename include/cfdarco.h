@@ -151,15 +151,21 @@ int cfd_stab(cfd_mesh_t mesh, const void* phi, void* out, int axis_mask, void* s
 #define CFD_EW_LEAP 9       /* out = (s*s) * a + 2*b - c (D2tVar extract), c passed as aux */
 #define CFD_EW_FILL 10      /* out = s */
 #define CFD_EW_COPY 11      /* out = a */
+#define CFD_EW_ADDS 12      /* out = a + s */
+#define CFD_EW_SADD 13      /* out = s + a */
+#define CFD_EW_SUBS 14      /* out = a - s */
+#define CFD_EW_SSUB 15      /* out = s - a */
+#define CFD_EW_SDIV 16      /* out = s / a */
 int cfd_ewise(cfd_ctx_t ctx, int op, int dtype, const void* a, const void* b, const void* aux, double s,
               void* out, int64_t n, void* stream);
 
-/* CFL time step, UpdatePolicies::CourantFriedrichsLewy3D/1D(Cu)
+/* CFL time step, UpdatePolicies::CourantFriedrichsLewy3D/1D(Cu) and cfl_cu
  * (src/fvm3d.cpp:497-567, src/cuda_operators.cu:138-235):
- * CFL * min_i dl / (sqrt(gamma p/rho) + |u|), dl = min(dx,dy,dz).  v and w may
- * be NULL (1D form).  Synchronises the stream to return the host value. */
-int cfd_cfl_min(cfd_mesh_t mesh, const void* u, const void* v, const void* w, const void* p,
-                const void* rho, double cfl, double gamma, double* dt_out, void* stream);
+ * cfl * min_i dl * (1 / (sqrt((gamma p)/rho) + |u|)) over n cells, evaluated
+ * in the dtype as the reference's CPU form.  v and w may be NULL (1D form).
+ * Synchronises the stream to return the host value. */
+int cfd_cfl_min(cfd_ctx_t ctx, int dtype, const void* u, const void* v, const void* w, const void* p,
+                const void* rho, int64_t n, double dl, double cfl, double gamma, double* dt_out, void* stream);
 
 /* Non-finite check: *bad = number of NaN/Inf entries in x (synchronises). */
 int cfd_count_nonfinite(cfd_ctx_t ctx, int dtype, const void* x, int64_t n, int64_t* bad, void* stream);
@@ -181,6 +187,41 @@ int cfd_heat_run(cfd_mesh_t mesh, void* T_a, void* T_b, double alpha, const void
  * Replaces four KernelBuilder launches per step (src/fvm3d.cpp:1019-1032). */
 int cfd_wave_run(cfd_mesh_t mesh, void* const state_a[4], void* const state_b[4], double K,
                  const void* K_field, double dt, int64_t nsteps, void* stream);
+
+/* ---- runtime-compiled kernels --------------------------------------------
+ * hipRTC compilation of HIP C++ source for gfx950 (-O3, no FMA contraction,
+ * no packed FP32), cached by a hash of (source, kernel name, options) in
+ * memory and on disk ($CFDARCO_JIT_CACHE, default ~/.cache/cfdarco/jit).
+ * Replaces KernelBuilder::build / OCCA buildKernelFromString
+ * (src/fvm3d.cpp:1219-1312).  Compile failures return CFD_E_COMPILE with the
+ * compiler log in cfd_last_error(). */
+int cfd_program_compile(cfd_ctx_t ctx, const char* src, const char* kernel_name, cfd_program_t* out);
+/* Host-only compile (no device, no cache): code-object size in *code_bytes. */
+int cfd_program_compile_only(const char* src, const char* kernel_name, int64_t* code_bytes);
+/* Launch with `args` = array of pointers to the argument values, in the
+ * kernel's parameter order.  Replaces KernelBuilder::run (src/fvm3d.cpp:1314-1394). */
+int cfd_program_launch(cfd_program_t prog, const uint32_t grid[3], const uint32_t block[3], void** args,
+                       void* stream);
+int cfd_program_destroy(cfd_program_t prog);
+
+/* Geometry as the kernels see it, for code generated at run time (the fused
+ * kernels of the operator API).  Uniform meshes: per-axis constants (index 0/1/2
+ * = x/y/z).  General meshes: device arrays per face family (0 = z-, 1 = y-,
+ * 2 = x-faces) in the face order documented in cfdarcov2_amd/csrc/common.hpp,
+ * holding the axis-direction normal, area and alpha_d, plus per-cell volume
+ * and the six centre-to-face lengths (6 x N). */
+typedef struct cfd_geometry {
+    int uniform;
+    int dtype;
+    int64_t nx, ny, nz;
+    double area[3], alpha[3], alpha_b[3], len[3], vol;
+    const void* face_normal[3][3];
+    const void* face_area[3];
+    const void* face_alpha[3];
+    const void* cell_vol;
+    const void* cell_len;
+} cfd_geometry;
+int cfd_mesh_geometry(cfd_mesh_t mesh, cfd_geometry* out);
 
 /* Kernel shape of the fused acoustic step on uniform meshes (no counterpart
  * in the reference, whose JIT kernels use fixed 64-thread blocks,

@@ -240,9 +240,13 @@ def test_ewise_and_reductions(hip_ctx, oracle_lib):
         s = dtype(0.37)
         expect = {"add": a + b, "sub": a - b, "mul": a * b, "div": a / b, "neg": -a, "muls": a * s,
                   "divs": a / s, "smul": s * a, "axpy": s * a + b, "leap": ((s * s) * a + dtype(2) * b) - c,
-                  "fill": np.full_like(a, s), "copy": a}
+                  "fill": np.full_like(a, s), "copy": a, "adds": a + s, "sadd": s + a, "subs": a - s,
+                  "ssub": s - a, "sdiv": s / b}
         for op, ref in expect.items():
-            hip_ctx.ewise(op, ta, tb, tc, float(s), out)
+            if op == "sdiv":
+                hip_ctx.ewise(op, tb, None, None, float(s), out)
+            else:
+                hip_ctx.ewise(op, ta, tb, tc, float(s), out)
             assert np.array_equal(_np(out), ref.astype(dtype)), op
     # CFL minimum against the oracle (3D and 1D forms)
     n, d = (8, 8, 8), (0.5, 0.25, 0.125)
