@@ -23,7 +23,9 @@ EW = {"add": 0, "sub": 1, "mul": 2, "div": 3, "neg": 4, "muls": 5, "divs": 6, "s
       "axpy": 8, "leap": 9, "fill": 10, "copy": 11, "adds": 12, "sadd": 13, "subs": 14, "ssub": 15, "sdiv": 16}
 
 TABLES = {"ids": 0, "ids_bf": 1, "nx": 2, "ny": 3, "nz": 4, "nax": 5, "nay": 6, "naz": 7,
-          "area": 8, "alpha": 9, "len": 10, "vol": 11, "is_bnd": 12, "ccx": 13, "ccy": 14, "ccz": 15}
+          "area": 8, "alpha": 9, "len": 10, "vol": 11, "is_bnd": 12, "ccx": 13, "ccy": 14, "ccz": 15,
+          "fv_nx": 16, "fv_ny": 17, "fv_nz": 18, "fv_area": 19, "fv_vol": 20}
+_N_TABLES = ("vol", "is_bnd", "ccx", "ccy", "ccz", "fv_vol")
 
 # Every symbol declared in include/cfdarco.h (checked by the CPU test suite).
 EXPORTS = [
@@ -32,7 +34,8 @@ EXPORTS = [
     "cfd_memcpy", "cfd_stream_sync",
     "cfd_mesh_create_hex", "cfd_mesh_destroy", "cfd_mesh_get_info", "cfd_mesh_table", "cfd_hex_table",
     "cfd_grad", "cfd_lapl", "cfd_stab", "cfd_ewise", "cfd_cfl_min", "cfd_count_nonfinite",
-    "cfd_heat_run", "cfd_wave_run", "cfd_mesh_set_wave_shape",
+    "cfd_heat_run", "cfd_wave_run", "cfd_euler_run", "cfd_mesh_set_wave_shape",
+    "cfd_wave_traces", "cfd_wave_adjoint",
     "cfd_program_compile", "cfd_program_compile_only", "cfd_program_launch", "cfd_program_destroy",
     "cfd_mesh_geometry",
 ]
@@ -85,6 +88,11 @@ def load_library(path: str = LIB_PATH):
         "cfd_count_nonfinite": [vp, i, vp, i64, ctypes.POINTER(ctypes.c_int64), vp],
         "cfd_heat_run": [vp, vp, vp, d, vp, d, i64, vp],
         "cfd_wave_run": [vp, ctypes.c_void_p * 4, ctypes.c_void_p * 4, d, vp, d, i64, vp],
+        "cfd_euler_run": [vp, ctypes.c_void_p * 5, ctypes.c_void_p * 5, d, d, i64, vp, vp],
+        "cfd_wave_traces": [vp, ctypes.c_void_p * 4, ctypes.c_void_p * 4, d, vp, d, i64,
+                            ctypes.POINTER(ctypes.c_int64), i, vp, vp, i64, vp],
+        "cfd_wave_adjoint": [vp, ctypes.c_void_p * 4, ctypes.c_void_p * 4, d, vp, d, i64,
+                             ctypes.POINTER(ctypes.c_int64), i, vp, vp, i64, vp, vp],
         "cfd_mesh_set_wave_shape": [vp, i, i, i, i],
         "cfd_program_compile": [vp, ctypes.c_char_p, ctypes.c_char_p, pp],
         "cfd_program_compile_only": [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)],
@@ -126,7 +134,7 @@ def hex_table(nx, ny, nz, dx, dy, dz, dtype, name, vertex_offsets=None):
     n = nx * ny * nz
     tid = TABLES[name]
     np_dt = np.int64 if name in ("ids", "ids_bf") else (np.float32 if code == F32 else np.float64)
-    count = n if name in ("vol", "is_bnd", "ccx", "ccy", "ccz") else 6 * n
+    count = n if name in _N_TABLES else 6 * n
     out = np.empty(count, dtype=np_dt)
     check(L.cfd_hex_table(ctypes.byref(desc), tid, out.ctypes.data))
     return out.reshape(6, n) if count == 6 * n else out
@@ -258,7 +266,7 @@ class HexMesh:
     def table(self, name):
         n = self.N
         np_dt = np.int64 if name in ("ids", "ids_bf") else (np.float32 if self.code == F32 else np.float64)
-        count = n if name in ("vol", "is_bnd", "ccx", "ccy", "ccz") else 6 * n
+        count = n if name in _N_TABLES else 6 * n
         out = np.empty(count, dtype=np_dt)
         check(self.lib.cfd_mesh_table(self.h, TABLES[name], out.ctypes.data))
         return out.reshape(6, n) if count == 6 * n else out
@@ -317,4 +325,18 @@ class HexMesh:
         B = (ctypes.c_void_p * 4)(*[t.data_ptr() for t in state_b])
         check(self.lib.cfd_wave_run(self.h, A, B, float(K), _dptr(K_field), float(dt), int(nsteps),
                                     _stream_ptr()))
+        return tuple(state_a) if nsteps % 2 == 0 else tuple(state_b)
+
+    def euler_run(self, state_a, state_b, gamma, cfl, nsteps, dt_hist=None):
+        """Rusanov steps on (rho, rho u, rho v, rho w, E); returns the tuple
+        holding the final state.  dt_hist (device tensor, nsteps long) gets
+        every step's dt."""
+        for t in list(state_a) + list(state_b):
+            self._check_field(t)
+        if dt_hist is not None:
+            assert dt_hist.is_cuda and dt_hist.dtype == self.torch_dtype and dt_hist.numel() >= nsteps
+        A = (ctypes.c_void_p * 5)(*[t.data_ptr() for t in state_a])
+        B = (ctypes.c_void_p * 5)(*[t.data_ptr() for t in state_b])
+        check(self.lib.cfd_euler_run(self.h, A, B, float(gamma), float(cfl), int(nsteps), _dptr(dt_hist),
+                                     _stream_ptr()))
         return tuple(state_a) if nsteps % 2 == 0 else tuple(state_b)

@@ -96,6 +96,16 @@ template <class T> struct GeneralGeo {
     T dl;
 };
 
+// Finite-volume geometry of the Euler stepper: per face family the +axis unit
+// normal and the area of the face's vector area, and the cell volumes built from
+// the vector areas (built on first use).
+template <class T> struct FvGeo {
+    int64_t X, Y, Z;
+    const T* n[3][3];    // [family z,y,x][component x,y,z]
+    const T* A[3];
+    const T* vol;
+};
+
 }  // namespace cfd
 
 struct cfd_mesh_s {
@@ -119,4 +129,9 @@ struct cfd_mesh_s {
     int wave_ty = 16;
     int wave_L = 32;
     int wave_remap = 1;
+    // Euler stepper geometry (device block allocated on first use)
+    cfd::FvGeo<float> fv32{};
+    cfd::FvGeo<double> fv64{};
+    void* fv_block = nullptr;
+    int64_t fv_bytes = 0;
 };

@@ -1,0 +1,270 @@
+// Compressible Euler with a Rusanov face flux on the hex mesh (no reference
+// counterpart; the reference's Sod driver, examples/main_3d_sod_shock.cpp:150-167,
+// runs a central predictor with stabilisation instead).
+//
+// State: five SoA fields (rho, rho u, rho v, rho w, E).  One thread per cell;
+// each thread rebuilds the primitive state of itself and its six neighbours,
+// evaluates the six face fluxes in the lower-cell -> upper-cell orientation (so
+// both cells of a face compute bit-identical fluxes and the scheme stays
+// conservative without scatter or atomics on the fields), and updates
+//   U <- U - (dt * net) / V,  net = ((Gx+ - Gx-) + (Gy+ - Gy-)) + (Gz+ - Gz-).
+//
+// Time step, all on the device: the step kernel reads dt from a rotating slot,
+// and from the cell's new state computes V / sum_f A_f (|u.n_f| + a), reduces
+// the block minimum and folds it into the next slot with one atomic minimum
+// per block (non-negative floats order like their bit patterns).  A third slot
+// is reset to +inf for the step after, so the loop needs no host round trip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <limits>
+#include <type_traits>
+#include <vector>
+
+#include "common.hpp"
+#include "hexmesh.hpp"
+#include "kernels.hpp"
+
+namespace cfd {
+
+template <class T> using Bits = std::conditional_t<sizeof(T) == 4, unsigned int, unsigned long long>;
+
+__device__ inline float from_bits(unsigned int b) { return __uint_as_float(b); }
+__device__ inline double from_bits(unsigned long long b) { return __longlong_as_double((long long)b); }
+__device__ inline unsigned int to_bits(float v) { return __float_as_uint(v); }
+__device__ inline unsigned long long to_bits(double v) { return (unsigned long long)__double_as_longlong(v); }
+template <class T> __device__ inline Bits<T> inf_bits() { return to_bits(std::numeric_limits<T>::infinity()); }
+
+template <class T> struct EIn { const T* u[5]; };
+template <class T> struct EOut { T* u[5]; };
+
+template <class T> struct Prim { T ux, uy, uz, p, a; };
+
+template <class T> __device__ inline Prim<T> prim(T gamma, const T U[5]) {
+    Prim<T> q;
+    q.ux = U[1] / U[0];
+    q.uy = U[2] / U[0];
+    q.uz = U[3] / U[0];
+    const T q2 = (q.ux * q.ux + q.uy * q.uy) + q.uz * q.uz;
+    q.p = (gamma - T(1)) * (U[4] - (T(0.5) * U[0]) * q2);
+    q.a = sqrt((gamma * q.p) / U[0]);
+    return q;
+}
+
+// Rusanov flux through a face of unit +axis normal n and area A, L = lower cell
+template <class T>
+__device__ inline void rusanov(const T UL[5], const Prim<T>& L, const T UR[5], const Prim<T>& R, T nx, T ny, T nz,
+                               T A, T G[5]) {
+    const T unL = (L.ux * nx + L.uy * ny) + L.uz * nz;
+    const T unR = (R.ux * nx + R.uy * ny) + R.uz * nz;
+    const T FL[5] = {UL[0] * unL, UL[1] * unL + L.p * nx, UL[2] * unL + L.p * ny, UL[3] * unL + L.p * nz,
+                     (UL[4] + L.p) * unL};
+    const T FR[5] = {UR[0] * unR, UR[1] * unR + R.p * nx, UR[2] * unR + R.p * ny, UR[3] * unR + R.p * nz,
+                     (UR[4] + R.p) * unR};
+    const T lam = fmax(fabs(unL) + L.a, fabs(unR) + R.a);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) G[k] = (T(0.5) * ((FL[k] + FR[k]) - lam * (UR[k] - UL[k]))) * A;
+}
+
+// family and family-local index of local face fc of cell (x, y, z)
+template <class T>
+__device__ inline int64_t fv_face(const FvGeo<T>& g, int64_t x, int64_t y, int64_t z, int fc, int& fam) {
+    switch (fc) {
+        case 0: fam = 0; return x * g.Y * (g.Z + 1) + y * (g.Z + 1) + z;
+        case 1: fam = 0; return x * g.Y * (g.Z + 1) + y * (g.Z + 1) + z + 1;
+        case 2: fam = 1; return x * (g.Y + 1) * g.Z + y * g.Z + z;
+        case 3: fam = 1; return x * (g.Y + 1) * g.Z + (y + 1) * g.Z + z;
+        case 4: fam = 2; return x * g.Y * g.Z + y * g.Z + z;
+        default: fam = 2; return (x + 1) * g.Y * g.Z + y * g.Z + z;
+    }
+}
+
+// V / sum over fc = 0..5 of A_f (|u.n_f| + a)
+template <class T>
+__device__ inline T local_dt(const FvGeo<T>& g, int64_t x, int64_t y, int64_t z, int64_t c, const Prim<T>& q) {
+    T s = T(0);
+#pragma unroll
+    for (int fc = 0; fc < 6; ++fc) {
+        int fam;
+        const int64_t f = fv_face(g, x, y, z, fc, fam);
+        const T un = (q.ux * g.n[fam][0][f] + q.uy * g.n[fam][1][f]) + q.uz * g.n[fam][2][f];
+        s = s + g.A[fam][f] * (fabs(un) + q.a);
+    }
+    return g.vol[c] / s;
+}
+
+// minimum over the 256-thread block, folded into *slot by thread (0, 0)
+template <class T> __device__ inline void block_min_into(T v, Bits<T>* slot) {
+    __shared__ T red[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    const int lane = threadIdx.x, wave = threadIdx.y;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    if (lane == 0 && wave == 0) {
+        const T m = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
+        if (m < std::numeric_limits<T>::infinity()) atomicMin(slot, to_bits(m));
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) euler_dt_k(FvGeo<T> g, EIn<T> in, T gamma, Bits<T>* slot) {
+    const int64_t z = int64_t(blockIdx.x) * 64 + threadIdx.x;
+    const int64_t y = int64_t(blockIdx.y) * 4 + threadIdx.y;
+    const int64_t x = blockIdx.z;
+    T loc = std::numeric_limits<T>::infinity();
+    if (z < g.Z && y < g.Y) {
+        const int64_t c = (x * g.Y + y) * g.Z + z;
+        T U[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) U[k] = in.u[k][c];
+        loc = local_dt(g, x, y, z, c, prim(gamma, U));
+    }
+    block_min_into(loc, slot);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) euler_step_k(FvGeo<T> g, EIn<T> in, EOut<T> out, T gamma, T cfl,
+                                                   const Bits<T>* dt_cur, Bits<T>* dt_next, Bits<T>* dt_reset,
+                                                   T* dt_hist, int64_t step) {
+    const int64_t z = int64_t(blockIdx.x) * 64 + threadIdx.x;
+    const int64_t y = int64_t(blockIdx.y) * 4 + threadIdx.y;
+    const int64_t x = blockIdx.z;
+    const T dt = cfl * from_bits(*dt_cur);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+        *dt_reset = inf_bits<T>();
+        if (dt_hist) dt_hist[step] = dt;
+    }
+    T loc = std::numeric_limits<T>::infinity();
+    if (z < g.Z && y < g.Y) {
+        const int64_t c = (x * g.Y + y) * g.Z + z;
+        T Ui[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) Ui[k] = in.u[k][c];
+        const Prim<T> qi = prim(gamma, Ui);
+        const int64_t crd[3] = {z, y, x};           // fc >> 1 = 0 z, 1 y, 2 x
+        const int64_t ext[3] = {g.Z, g.Y, g.X};
+        const int64_t str[3] = {1, g.Z, g.Y * g.Z};
+        T net[5] = {T(0), T(0), T(0), T(0), T(0)};
+        // axis pairs in the order x, y, z: net = ((x+ - x-) + (y+ - y-)) + (z+ - z-)
+#pragma unroll
+        for (int pair = 2; pair >= 0; --pair) {
+            T Gp[2][5];
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const int fc = 2 * pair + side;
+                const bool upper = side == 1;
+                const bool wall = upper ? crd[pair] == ext[pair] - 1 : crd[pair] == 0;
+                T Un[5];
+                if (wall) {
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) Un[k] = Ui[k];
+                    Un[3 - pair] = -Ui[3 - pair];
+                } else {
+                    const int64_t nb = upper ? c + str[pair] : c - str[pair];
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) Un[k] = in.u[k][nb];
+                }
+                const Prim<T> qn = prim(gamma, Un);
+                int fam;
+                const int64_t f = fv_face(g, x, y, z, fc, fam);
+                const T nx = g.n[fam][0][f], ny = g.n[fam][1][f], nz = g.n[fam][2][f], A = g.A[fam][f];
+                if (upper) rusanov(Ui, qi, Un, qn, nx, ny, nz, A, Gp[side]);
+                else rusanov(Un, qn, Ui, qi, nx, ny, nz, A, Gp[side]);
+            }
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const T d = Gp[1][k] - Gp[0][k];
+                net[k] = pair == 2 ? d : net[k] + d;
+            }
+        }
+        const T V = g.vol[c];
+        T Un[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            Un[k] = Ui[k] - (dt * net[k]) / V;
+            out.u[k][c] = Un[k];
+        }
+        loc = local_dt(g, x, y, z, c, prim(gamma, Un));
+    }
+    block_min_into(loc, dt_next);
+}
+
+template <class B> __global__ void fill_bits_k(B* p, int n, B v) {
+    if (int(threadIdx.x) < n) p[threadIdx.x] = v;
+}
+
+// Device geometry of the stepper, built on first use from the mesh's vertices.
+template <class T> static FvGeo<T>& fv_geometry(cfd_mesh_s* m, hipStream_t s) {
+    FvGeo<T>& fv = [&]() -> FvGeo<T>& {
+        if constexpr (sizeof(T) == 4) return m->fv32; else return m->fv64;
+    }();
+    if (m->fv_block) return fv;
+    HexGeometry<T> g;
+    build_hex_geometry<T>(g, m->X, m->Y, m->Z, T(m->dx), T(m->dy), T(m->dz),
+                          m->has_offsets ? m->vertex_offsets.data() : nullptr);
+    std::vector<T> host;
+    int64_t off_fam[3], off_vol;
+    build_fv_geometry<T>(g, host, off_fam, off_vol);
+    const int64_t bytes = int64_t(host.size() * sizeof(T));
+    void* dev = pool_alloc(m->ctx, bytes, s);
+    hipError_t e = hipMemcpyAsync(dev, host.data(), size_t(bytes), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        pool_free(m->ctx, dev, s);
+        CFD_HIP(e);
+    }
+    T* d = (T*)dev;
+    fv.X = m->X; fv.Y = m->Y; fv.Z = m->Z;
+    for (int fam = 0; fam < 3; ++fam) {
+        const int64_t nf = g.nf[fam];
+        for (int k = 0; k < 3; ++k) fv.n[fam][k] = d + off_fam[fam] + k * nf;
+        fv.A[fam] = d + off_fam[fam] + 3 * nf;
+    }
+    fv.vol = d + off_vol;
+    m->fv_block = dev;
+    m->fv_bytes = bytes;
+    return fv;
+}
+
+template <class T>
+void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamma, double cfl, int64_t nsteps,
+                  void* dt_hist, hipStream_t s) {
+    const FvGeo<T>& g = fv_geometry<T>(m, s);
+    Bits<T>* slots = (Bits<T>*)pool_alloc(m->ctx, 3 * sizeof(Bits<T>), s);
+    try {
+        const dim3 block(64, 4, 1);
+        const dim3 grid(unsigned((m->Z + 63) / 64), unsigned((m->Y + 3) / 4), unsigned(m->X));
+        const Bits<T> inf = sizeof(T) == 4 ? Bits<T>(0x7f800000u) : Bits<T>(0x7ff0000000000000ull);
+        fill_bits_k<Bits<T>><<<1, 64, 0, s>>>(slots, 3, inf);
+        EIn<T> in{};
+        EOut<T> out{};
+        for (int k = 0; k < 5; ++k) in.u[k] = (const T*)a[k];
+        euler_dt_k<T><<<grid, block, 0, s>>>(g, in, T(gamma), slots);
+        CFD_HIP(hipGetLastError());
+        for (int64_t st = 0; st < nsteps; ++st) {
+            void* const* src = (st & 1) ? b : a;
+            void* const* dst = (st & 1) ? a : b;
+            for (int k = 0; k < 5; ++k) {
+                in.u[k] = (const T*)src[k];
+                out.u[k] = (T*)dst[k];
+            }
+            Bits<T>* cur = slots + (st % 3);
+            Bits<T>* nxt = slots + ((st + 1) % 3);
+            Bits<T>* rst = slots + ((st + 2) % 3);
+            euler_step_k<T><<<grid, block, 0, s>>>(g, in, out, T(gamma), T(cfl), cur, nxt, rst, (T*)dt_hist, st);
+        }
+        CFD_HIP(hipGetLastError());
+    } catch (...) {
+        pool_free(m->ctx, slots, s);
+        throw;
+    }
+    pool_free(m->ctx, slots, s);
+}
+
+template void launch_euler<float>(cfd_mesh_s*, void* const[5], void* const[5], double, double, int64_t, void*,
+                                  hipStream_t);
+template void launch_euler<double>(cfd_mesh_s*, void* const[5], void* const[5], double, double, int64_t, void*,
+                                   hipStream_t);
+
+}  // namespace cfd

@@ -214,4 +214,73 @@ void build_hex_geometry(HexGeometry<T>& g, int64_t X, int64_t Y, int64_t Z, T dx
             }
 }
 
+// Geometry of the Euler stepper (no reference counterpart).  The reference's
+// sum-of-components face area and the volume built on it (src/mesh3d.cpp:61,
+// 101-104) are exact only for axis-aligned faces and turn negative on a mesh
+// whose interior vertices move by 20% of dx, so the stepper uses each quad's
+// vector area S = 0.5 (p2 - p0) x (p3 - p1) oriented along +axis, A = |S|,
+// n = S / A, and V = (1/3) sum over fc = 0..5 of (x_f - x_c) . S_out,f with
+// x_f the face centre and x_c the cell centre.  Output layout: per family
+// [nx, ny, nz, A] * nf, then V (N).
+template <class T> void build_fv_geometry(const HexGeometry<T>& g, std::vector<T>& out, int64_t off_fam[3],
+                                          int64_t& off_vol) {
+    int64_t total = 0;
+    for (int fam = 0; fam < 3; ++fam) {
+        off_fam[fam] = total;
+        total += 4 * g.nf[fam];
+    }
+    off_vol = total;
+    total += g.N;
+    out.assign(size_t(total), T(0));
+    const int64_t X = g.X, Y = g.Y, Z = g.Z;
+    auto face = [&](int fam, int64_t f, int64_t q0, int64_t q1, int64_t q2, int64_t q3) {
+        const P3<T> c = cross3(sub3(g.vert(q2), g.vert(q0)), sub3(g.vert(q3), g.vert(q1)));
+        P3<T> S{T(0.5) * c.x, T(0.5) * c.y, T(0.5) * c.z};
+        const T along = fam == 0 ? S.z : (fam == 1 ? S.y : S.x);
+        if (along < T(0)) S = {-S.x, -S.y, -S.z};
+        const T A = std::sqrt(dotf(S, S));
+        const int64_t nf = g.nf[fam], o = off_fam[fam];
+        out[o + f] = S.x / A;
+        out[o + nf + f] = S.y / A;
+        out[o + 2 * nf + f] = S.z / A;
+        out[o + 3 * nf + f] = A;
+    };
+#pragma omp parallel for schedule(static)
+    for (int64_t x = 0; x < X; ++x)
+        for (int64_t y = 0; y < Y; ++y)
+            for (int64_t zf = 0; zf <= Z; ++zf)
+                face(0, g.zface(x, y, zf), g.vid(x, y, zf), g.vid(x, y + 1, zf), g.vid(x + 1, y + 1, zf),
+                     g.vid(x + 1, y, zf));
+#pragma omp parallel for schedule(static)
+    for (int64_t x = 0; x < X; ++x)
+        for (int64_t yf = 0; yf <= Y; ++yf)
+            for (int64_t z = 0; z < Z; ++z)
+                face(1, g.yface(x, yf, z), g.vid(x, yf, z), g.vid(x, yf, z + 1), g.vid(x + 1, yf, z + 1),
+                     g.vid(x + 1, yf, z));
+#pragma omp parallel for schedule(static)
+    for (int64_t xf = 0; xf <= X; ++xf)
+        for (int64_t y = 0; y < Y; ++y)
+            for (int64_t z = 0; z < Z; ++z)
+                face(2, g.xface(xf, y, z), g.vid(xf, y, z + 1), g.vid(xf, y + 1, z + 1), g.vid(xf, y + 1, z),
+                     g.vid(xf, y, z));
+#pragma omp parallel for schedule(static)
+    for (int64_t x = 0; x < X; ++x)
+        for (int64_t y = 0; y < Y; ++y)
+            for (int64_t z = 0; z < Z; ++z) {
+                const int64_t cid = Z * Y * x + Z * y + z;
+                const P3<T> xc = g.ccen(cid);
+                T acc = T(0);
+                for (int fc = 0; fc < 6; ++fc) {
+                    int fam; int64_t f;
+                    g.cell_face(x, y, z, fc, fam, f);
+                    const int64_t nf = g.nf[fam], o = off_fam[fam];
+                    const T sgn = (fc & 1) ? T(1) : T(-1);
+                    const T A = out[o + 3 * nf + f];
+                    const P3<T> So{sgn * (out[o + f] * A), sgn * (out[o + nf + f] * A), sgn * (out[o + 2 * nf + f] * A)};
+                    acc = acc + dotf(sub3(g.fcen(fam, f), xc), So);
+                }
+                out[off_vol + cid] = acc / T(3);
+            }
+}
+
 }  // namespace cfd

@@ -7,6 +7,33 @@
 
 namespace cfd {
 
+// Sensor cells of the trace-recording forward step and the adjoint step,
+// passed by value as a kernel argument (row = cell / Z, z = cell % Z).
+constexpr int kMaxSensors = 64;
+struct SensorList {
+    int n;
+    int row[kMaxSensors];
+    int z[kMaxSensors];
+    int64_t cell[kMaxSensors];
+};
+SensorList make_sensors(const cfd_mesh_s* m, const int64_t* cells, int n);
+
+template <class T>
+void launch_wave_trace_step(const cfd_mesh_s* m, const void* const in[4], void* const out[4], double K,
+                            const void* Kf, double dt, const SensorList& sens, void* trace, hipStream_t s);
+template <class T>
+void launch_wave_adj_step(const cfd_mesh_s* m, const void* const in[4], void* const out[4], double K,
+                          const void* Kf, double dt, const SensorList& sens, const void* src,
+                          const void* const velt[3], void* gK, hipStream_t s);
+template <class T> void launch_scatter_add(void* lp, const SensorList& sens, const void* src, hipStream_t s);
+template <class T>
+void run_wave_traces(const cfd_mesh_s* m, void* const a[4], void* const b[4], double K, const void* Kf, double dt,
+                     int64_t nsteps, const SensorList& sens, void* traces, void* ckpt, int64_t every, hipStream_t s);
+template <class T>
+void run_wave_adjoint(const cfd_mesh_s* m, void* const la[4], void* const lb[4], double K, const void* Kf,
+                      double dt, int64_t nsteps, const SensorList& sens, const void* tgrad, const void* ckpt,
+                      int64_t every, void* gK, hipStream_t s);
+
 template <class T> void launch_grad(const cfd_mesh_s* m, const void* phi, void* out, int mask, int wall, hipStream_t s);
 template <class T> void launch_lapl(const cfd_mesh_s* m, const void* phi, void* out, int mask, hipStream_t s);
 template <class T>
@@ -21,6 +48,10 @@ void launch_wave(const cfd_mesh_s* m, void* const a[4], void* const b[4], double
 template <class T>
 void launch_wave_march(const cfd_mesh_s* m, void* const a[4], void* const b[4], double K, const void* Kf, double dt,
                        int64_t nsteps, int ty, int L, int remap, hipStream_t s);
+
+template <class T>
+void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamma, double cfl, int64_t nsteps,
+                  void* dt_hist, hipStream_t s);
 
 template <class T>
 void launch_ewise(int op, const void* a, const void* b, const void* aux, double s, void* out, int64_t n,

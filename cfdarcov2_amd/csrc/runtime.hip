@@ -202,6 +202,28 @@ static void hex_table(int64_t X_, int64_t Y_, int64_t Z_, double dx, double dy, 
     HexGeometry<T> g;
     build_hex_geometry<T>(g, X_, Y_, Z_, T(dx), T(dy), T(dz), offsets);
     const int64_t N = g.N, X = g.X, Y = g.Y, Z = g.Z;
+    if (which >= CFD_T_FV_NX && which <= CFD_T_FV_VOL) {
+        std::vector<T> fv;
+        int64_t off_fam[3], off_vol;
+        build_fv_geometry<T>(g, fv, off_fam, off_vol);
+        T* ot = (T*)out;
+        if (which == CFD_T_FV_VOL) {
+            std::memcpy(out, fv.data() + off_vol, size_t(N) * sizeof(T));
+            return;
+        }
+        const int comp = which - CFD_T_FV_NX;  // 0..2 normal components, 3 area
+        for (int64_t x = 0; x < X; ++x)
+            for (int64_t y = 0; y < Y; ++y)
+                for (int64_t z = 0; z < Z; ++z) {
+                    const int64_t i = Z * Y * x + Z * y + z;
+                    for (int fc = 0; fc < 6; ++fc) {
+                        int fam; int64_t f;
+                        g.cell_face(x, y, z, fc, fam, f);
+                        ot[int64_t(fc) * N + i] = fv[size_t(off_fam[fam] + comp * g.nf[fam] + f)];
+                    }
+                }
+        return;
+    }
     if (which == CFD_T_VOL || which == CFD_T_CCX || which == CFD_T_CCY || which == CFD_T_CCZ) {
         const std::vector<T>& src = which == CFD_T_VOL ? g.vol : which == CFD_T_CCX ? g.ccx
                                   : which == CFD_T_CCY ? g.ccy : g.ccz;
@@ -246,6 +268,8 @@ static void hex_table(int64_t X_, int64_t Y_, int64_t Z_, double dx, double dy, 
             }
     CFD_REQUIRE(which >= CFD_T_IDS && which <= CFD_T_CCZ, "unknown mesh table id");
 }
+
+static bool valid_table(int t) { return t >= CFD_T_IDS && t <= CFD_T_FV_VOL; }
 
 }  // namespace cfd
 
@@ -373,10 +397,9 @@ int cfd_mesh_create_hex(cfd_ctx_t ctx, const cfd_hex_desc* desc, cfd_mesh_t* out
 int cfd_mesh_destroy(cfd_mesh_t mesh) {
     return guarded([&] {
         if (!mesh) return;
-        if (mesh->dev_block) {
-            pool_free(mesh->ctx, mesh->dev_block, nullptr);
-            CFD_HIP(hipStreamSynchronize(nullptr));
-        }
+        if (mesh->dev_block) pool_free(mesh->ctx, mesh->dev_block, nullptr);
+        if (mesh->fv_block) pool_free(mesh->ctx, mesh->fv_block, nullptr);
+        if (mesh->dev_block || mesh->fv_block) CFD_HIP(hipStreamSynchronize(nullptr));
         delete mesh;
     });
 }
@@ -395,7 +418,7 @@ int cfd_mesh_get_info(cfd_mesh_t m, cfd_mesh_info* info) {
 int cfd_mesh_table(cfd_mesh_t m, int table, void* host_out) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr && host_out != nullptr, "null argument");
-        CFD_REQUIRE(table >= CFD_T_IDS && table <= CFD_T_CCZ, "unknown mesh table id");
+        CFD_REQUIRE(valid_table(table), "unknown mesh table id");
         const double* off = m->has_offsets ? m->vertex_offsets.data() : nullptr;
         if (m->dtype == CFD_F32) hex_table<float>(m->X, m->Y, m->Z, m->dx, m->dy, m->dz, off, table, host_out);
         else hex_table<double>(m->X, m->Y, m->Z, m->dx, m->dy, m->dz, off, table, host_out);
@@ -407,7 +430,7 @@ int cfd_hex_table(const cfd_hex_desc* d, int table, void* host_out) {
         CFD_REQUIRE(d != nullptr && host_out != nullptr, "null argument");
         check_dtype(d->dtype);
         CFD_REQUIRE(d->nx > 0 && d->ny > 0 && d->nz > 0, "mesh extents must be positive");
-        CFD_REQUIRE(table >= CFD_T_IDS && table <= CFD_T_CCZ, "unknown mesh table id");
+        CFD_REQUIRE(valid_table(table), "unknown mesh table id");
         if (d->dtype == CFD_F32)
             hex_table<float>(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->vertex_offsets, table, host_out);
         else
@@ -572,6 +595,60 @@ int cfd_wave_run(cfd_mesh_t m, void* const a[4], void* const b[4], double K, con
         }
         CFD_REQUIRE(nsteps >= 0, "cfd_wave_run: negative step count");
         DISPATCH(m, launch_wave, m, a, b, K, Kf, dt, nsteps, as_stream(stream));
+    });
+}
+
+int cfd_euler_run(cfd_mesh_t m, void* const a[5], void* const b[5], double gamma, double cfl, int64_t nsteps,
+                  void* dt_hist, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(m && a && b, "cfd_euler_run: null argument");
+        for (int i = 0; i < 5; ++i) {
+            CFD_REQUIRE(a[i] && b[i], "cfd_euler_run: null state pointer");
+            for (int j = 0; j < 5; ++j) CFD_REQUIRE(a[i] != b[j], "cfd_euler_run: state_a and state_b overlap");
+        }
+        CFD_REQUIRE(gamma > 1.0 && std::isfinite(gamma), "cfd_euler_run: gamma must be finite and > 1");
+        CFD_REQUIRE(cfl > 0.0 && std::isfinite(cfl), "cfd_euler_run: cfl must be finite and positive");
+        CFD_REQUIRE(nsteps >= 0, "cfd_euler_run: negative step count");
+        DISPATCH(m, launch_euler, m, a, b, gamma, cfl, nsteps, dt_hist, as_stream(stream));
+    });
+}
+
+static void check_state4(void* const a[4], void* const b[4], const char* what) {
+    CFD_REQUIRE(a && b, std::string(what) + ": null argument");
+    for (int i = 0; i < 4; ++i) {
+        CFD_REQUIRE(a[i] && b[i], std::string(what) + ": null state pointer");
+        for (int j = 0; j < 4; ++j) CFD_REQUIRE(a[i] != b[j], std::string(what) + ": state_a and state_b overlap");
+    }
+}
+
+int cfd_wave_traces(cfd_mesh_t m, void* const a[4], void* const b[4], double K, const void* Kf, double dt,
+                    int64_t nsteps, const int64_t* sensor_cells, int nsensors, void* traces, void* checkpoints,
+                    int64_t every, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(m != nullptr, "cfd_wave_traces: null mesh");
+        check_state4(a, b, "cfd_wave_traces");
+        if (!m->uniform) throw Error(CFD_E_UNSUPPORTED, "cfd_wave_traces: uniform meshes only");
+        CFD_REQUIRE(nsteps >= 0, "cfd_wave_traces: negative step count");
+        CFD_REQUIRE(!checkpoints || every >= 1, "cfd_wave_traces: checkpoint interval must be >= 1");
+        const SensorList sens = make_sensors(m, sensor_cells, nsensors);
+        DISPATCH(m, run_wave_traces, m, a, b, K, Kf, dt, nsteps, sens, traces, checkpoints, every,
+                 as_stream(stream));
+    });
+}
+
+int cfd_wave_adjoint(cfd_mesh_t m, void* const adj_a[4], void* const adj_b[4], double K, const void* Kf, double dt,
+                     int64_t nsteps, const int64_t* sensor_cells, int nsensors, const void* trace_grad,
+                     const void* checkpoints, int64_t every, void* grad_K, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(m != nullptr, "cfd_wave_adjoint: null mesh");
+        check_state4(adj_a, adj_b, "cfd_wave_adjoint");
+        if (!m->uniform) throw Error(CFD_E_UNSUPPORTED, "cfd_wave_adjoint: uniform meshes only");
+        CFD_REQUIRE(nsteps >= 0, "cfd_wave_adjoint: negative step count");
+        CFD_REQUIRE(!grad_K || (checkpoints && every >= 1),
+                    "cfd_wave_adjoint: grad_K needs the forward checkpoints and their interval");
+        const SensorList sens = make_sensors(m, sensor_cells, nsensors);
+        DISPATCH(m, run_wave_adjoint, m, adj_a, adj_b, K, Kf, dt, nsteps, sens, trace_grad, checkpoints, every,
+                 grad_K, as_stream(stream));
     });
 }
 

@@ -122,3 +122,74 @@ def heat_free_space(nx, ny, nz, dx, dy, dz, centre, sigma, alpha, t):
     s2 = sigma * sigma + 2.0 * alpha * t
     amp = (sigma * sigma / s2) ** 1.5
     return gaussian(nx, ny, nz, dx, dy, dz, centre, math.sqrt(s2), amp)
+
+
+def sod_state(ccx, gamma=1.4, x0=0.5, left=(1.0, 1.0), right=(0.125, 0.1), dtype=np.float64):
+    """Conserved Sod state (rho, rho u, rho v, rho w, E) at cell-centre x:
+    (rho, p) = left for x < x0, right otherwise, at rest; returns (5, N)."""
+    ccx = np.asarray(ccx, dtype=np.float64)
+    lt = ccx < x0
+    rho = np.where(lt, left[0], right[0])
+    p = np.where(lt, left[1], right[1])
+    U = np.zeros((5, ccx.size), dtype=np.float64)
+    U[0] = rho
+    U[4] = p / (gamma - 1.0)
+    return U.astype(dtype)
+
+
+@dataclass
+class SodCase:
+    n: tuple
+    spacing: tuple
+    gamma: float
+    cfl: float
+    steps: int
+    vertex_offsets: np.ndarray  # (V, 3) float64, None for the uniform mesh
+    seed: int
+    frac: float
+
+
+def sod_config(nx=512, ny=64, nz=64, steps=500, cfl=0.3, gamma=1.4, seed=0, frac=0.2):
+    """Sod shock tube on [0, 1] x [0, ny/nx] x [0, nz/nx] with dx = 1/nx;
+    interior vertices moved by seeded offsets up to frac * dx per component
+    (frac = 0 gives the uniform mesh).  The initial state needs the mesh's cell
+    centres: sod_state(mesh.table('ccx'))."""
+    d = 1.0 / nx
+    off = perturbed_vertex_offsets(nx, ny, nz, d, d, d, seed, frac) if frac > 0 else None
+    return SodCase((nx, ny, nz), (d, d, d), gamma, cfl, steps, off, seed, frac)
+
+
+@dataclass
+class AdjointCase:
+    n: tuple
+    spacing: tuple
+    dt: float
+    steps: int
+    every: int
+    sensors: np.ndarray      # int64 cell ids
+    p0_true: np.ndarray      # hidden initial pressure that makes the observations
+    p0_guess: np.ndarray     # initial pressure the gradient is taken at
+
+
+def adjoint_config(n=128, steps=2000, every=45, n_sensors=32, cfl=0.4, c=1.0, seed=0):
+    """Adjoint acoustic workload on n^3 over the unit cube: a hidden
+    4-component Gaussian mixture (sigma ~ U[0.03, 0.08], amplitude ~ U[0.5, 1],
+    centre ~ U[0.2, 0.8]^3) makes the observed traces at `n_sensors` distinct
+    seeded cells; the gradient is taken at a single centred Gaussian (sigma 0.1,
+    amplitude 0.5).  dt = cfl * dx / c."""
+    d = 1.0 / n
+    g = SplitMix64(seed)
+    p_true = np.zeros(n * n * n)
+    for _ in range(4):
+        sigma = 0.03 + 0.05 * g.uniform()
+        amp = 0.5 + 0.5 * g.uniform()
+        centre = tuple(0.2 + 0.6 * g.uniform() for _ in range(3))
+        p_true += gaussian(n, n, n, d, d, d, centre, sigma, amp)
+    cells = []
+    while len(cells) < n_sensors:
+        cid = int(g.next_u64() % (n * n * n))
+        if cid not in cells:
+            cells.append(cid)
+    guess = gaussian(n, n, n, d, d, d, (0.5, 0.5, 0.5), 0.1, 0.5)
+    return AdjointCase((n, n, n), (d, d, d), cfl * d / c, steps, every, np.array(cells, dtype=np.int64), p_true,
+                       guess)

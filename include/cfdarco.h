@@ -118,6 +118,15 @@ int cfd_mesh_get_info(cfd_mesh_t mesh, cfd_mesh_info* info);
 #define CFD_T_CCX 13
 #define CFD_T_CCY 14
 #define CFD_T_CCZ 15
+/* Geometry of the Euler stepper (cfd_euler_run; no reference counterpart): per
+ * (cell, fc) the +axis unit normal and the area of the face's vector area
+ * 0.5 (p2 - p0) x (p3 - p1), N x 6 like the tables above, and the cell volume
+ * built from those vector areas (N). */
+#define CFD_T_FV_NX 16
+#define CFD_T_FV_NY 17
+#define CFD_T_FV_NZ 18
+#define CFD_T_FV_AREA 19
+#define CFD_T_FV_VOL 20
 int cfd_mesh_table(cfd_mesh_t mesh, int table, void* host_out);
 /* The same tables computed on the host straight from a descriptor, without a
  * device or context (host-only; used for CPU-side parity checks). */
@@ -187,6 +196,38 @@ int cfd_heat_run(cfd_mesh_t mesh, void* T_a, void* T_b, double alpha, const void
  * Replaces four KernelBuilder launches per step (src/fvm3d.cpp:1019-1032). */
 int cfd_wave_run(cfd_mesh_t mesh, void* const state_a[4], void* const state_b[4], double K,
                  const void* K_field, double dt, int64_t nsteps, void* stream);
+
+/* Compressible Euler with a Rusanov face flux, one fused launch per step (no
+ * reference counterpart: the reference's Sod driver, examples/
+ * main_3d_sod_shock.cpp:150-167, composes a central predictor and StabVar).
+ * state_a/state_b each hold {rho, rho u, rho v, rho w, E}; after nsteps the
+ * state is in state_a when nsteps is even, state_b when odd.  The time step
+ * dt = cfl * min_i V_i / sum_f A_f (|u.n_f| + a_i) is recomputed on the device
+ * from the state at the start of every step; dt_hist (device, nullable, nsteps
+ * long, mesh dtype) receives each step's dt.  Walls are reflective. */
+int cfd_euler_run(cfd_mesh_t mesh, void* const state_a[5], void* const state_b[5], double gamma, double cfl,
+                  int64_t nsteps, void* dt_hist, void* stream);
+
+/* ---- adjoint of the fused acoustic step (uniform meshes; no reference
+ * counterpart) ---------------------------------------------------------------
+ * cfd_wave_traces: nsteps fused acoustic steps as cfd_wave_run, recording p
+ * after step t (1-based) at the sensor cells into traces[(t-1)*nsensors + j]
+ * (device, mesh dtype, nullable) and, when checkpoints (device, nullable) is
+ * given, copying the state {p,u,v,w} (4 N values each) before steps 0, every,
+ * 2*every, ... into checkpoint k = t / every.  sensor_cells is a host array of
+ * at most 64 distinct cell ids. */
+int cfd_wave_traces(cfd_mesh_t mesh, void* const state_a[4], void* const state_b[4], double K,
+                    const void* K_field, double dt, int64_t nsteps, const int64_t* sensor_cells, int nsensors,
+                    void* traces, void* checkpoints, int64_t every, void* stream);
+/* cfd_wave_adjoint: the transposed steps in reverse order.  adj_a holds the
+ * adjoint of the final state {p,u,v,w} (normally zero) and trace_grad (device,
+ * nsteps*nsensors, nullable) the gradient of the loss with respect to the
+ * traces; after the call adj_a (nsteps even) or adj_b (nsteps odd) holds the
+ * gradient with respect to the initial state.  grad_K (device, N, nullable)
+ * accumulates dL/dK per cell and needs the checkpoints of cfd_wave_traces. */
+int cfd_wave_adjoint(cfd_mesh_t mesh, void* const adj_a[4], void* const adj_b[4], double K,
+                     const void* K_field, double dt, int64_t nsteps, const int64_t* sensor_cells, int nsensors,
+                     const void* trace_grad, const void* checkpoints, int64_t every, void* grad_K, void* stream);
 
 /* ---- runtime-compiled kernels --------------------------------------------
  * hipRTC compilation of HIP C++ source for gfx950 (-O3, no FMA contraction,

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <stdexcept>
 #include <vector>
 
@@ -66,6 +67,10 @@ template <class T> struct HexMesh {
     std::vector<T> nax, nay, naz;           // _normals_alt (outward)
     std::vector<T> area6, alpha_d, len6;
     std::vector<T> vol, is_bnd, is_bnd_rev;
+
+    // finite-volume geometry of the Euler stepper (per global face / per cell)
+    std::vector<T> svx, svy, svz, sva;      // unit +axis normal and vector-area magnitude
+    std::vector<T> svol;                    // cell volume from the vector areas
 
     // Mesh3D::square_node_coord_to_idx, src/mesh3d.cpp:387-389
     int64_t cell_idx(int64_t x, int64_t y, int64_t z) const { return Z * Y * x + Z * y + z; }
@@ -456,6 +461,169 @@ void wave_run(const HexMesh<T>& m, T* p, T* u, T* v, T* w, T K, const T* K_field
             const T k = K_field ? K_field[i] : K;
             p[i] = euler_update(dt, k * div, p[i]);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Compressible Euler with a Rusanov face flux.  No reference counterpart: the
+// reference's Sod driver (examples/main_3d_sod_shock.cpp:150-167) uses a
+// central predictor plus stabilisation; this stepper is the finite-volume
+// scheme the Sod workload asks for, written on the reference mesh geometry
+// (Face3D normal from the first three vertices, sum-of-components area, cell
+// volume, src/mesh3d.cpp:43-105).  Walls are reflective: the state across a
+// wall face is the cell's own with the momentum component along the wall's
+// axis negated (the walls are axis planes: only interior vertices move), so no
+// mass or energy crosses a wall.
+//
+// State per cell U = (rho, rho u, rho v, rho w, E).  On the face between the
+// lower cell L and upper cell R along an axis, n is the unit axis-direction
+// normal (the reference's _normals, Q4) and A the face area:
+//   un = (u n_x + v n_y) + w n_z,  F(U) = (rho un, rho u un + p n_x,
+//        rho v un + p n_y, rho w un + p n_z, (E + p) un),
+//   lam = max(|un_L| + a_L, |un_R| + a_R),
+//   G_k = (0.5 ((F_L,k + F_R,k) - lam (U_R,k - U_L,k))) A.
+// Update: net_k = ((Gx+ - Gx-) + (Gy+ - Gy-)) + (Gz+ - Gz-),
+//         U_k <- U_k - (dt net_k) / V.
+// Time step, taken from the state at the start of each step:
+//   dt = CFL min_i V_i / s_i,  s_i = sum over fc = 0..5 of A_f (|un_f| + a_i).
+// ---------------------------------------------------------------------------
+
+// Geometry of the stepper.  The reference's sum-of-components area (Q2) and
+// the volume built on it (src/mesh3d.cpp:61,101-104) are exact only for
+// axis-aligned faces and give negative cell volumes on a mesh whose vertices
+// move by 20% of dx, so the stepper uses the vector area of each quad face,
+//   S = 0.5 (p2 - p0) x (p3 - p1),  oriented along +axis,  A = |S|, n = S / A,
+// (exact for a bilinear face; the six vector areas of a cell sum to zero, so a
+// uniform state is preserved) and the cell volume
+//   V = (1/3) sum over fc = 0..5 of (x_f - x_c) . S_out,f
+// with x_f the face centre and x_c the cell centre (mean of the vertices).
+template <class T> void compute_fv_geometry(HexMesh<T>& m) {
+    const int64_t F = m.F, N = m.N;
+    const int64_t nzf = m.X * m.Y * (m.Z + 1), nyf = m.X * (m.Y + 1) * m.Z;
+    m.svx.resize(F); m.svy.resize(F); m.svz.resize(F); m.sva.resize(F); m.svol.resize(N);
+#pragma omp parallel for schedule(static)
+    for (int64_t f = 0; f < F; ++f) {
+        V3<T> p[4];
+        for (int q = 0; q < 4; ++q) {
+            const int64_t v = m.face_verts[4 * f + q];
+            p[q] = {m.vx[v], m.vy[v], m.vz[v]};
+        }
+        const V3<T> c = vcross(vsub(p[2], p[0]), vsub(p[3], p[1]));
+        V3<T> S{T(0.5) * c.x, T(0.5) * c.y, T(0.5) * c.z};
+        const T along = f < nzf ? S.z : (f < nzf + nyf ? S.y : S.x);
+        if (along < T(0)) S = {-S.x, -S.y, -S.z};
+        const T A = std::sqrt(dot_fixed(S, S));
+        m.svx[f] = S.x / A; m.svy[f] = S.y / A; m.svz[f] = S.z / A; m.sva[f] = A;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        const V3<T> xc{m.ccx[i], m.ccy[i], m.ccz[i]};
+        T acc = T(0);
+        for (int fc = 0; fc < 6; ++fc) {
+            const int64_t f = m.cell_faces[6 * i + fc];
+            const T sgn = (fc & 1) ? T(1) : T(-1);
+            const T A = m.sva[f];
+            const V3<T> So{sgn * (m.svx[f] * A), sgn * (m.svy[f] * A), sgn * (m.svz[f] * A)};
+            acc = acc + dot_fixed(vsub(V3<T>{m.fcx[f], m.fcy[f], m.fcz[f]}, xc), So);
+        }
+        m.svol[i] = acc / T(3);
+    }
+}
+
+template <class T> struct EulerPrim { T ux, uy, uz, p, a; };
+
+template <class T> inline EulerPrim<T> euler_prim(T gamma, T r, T mx, T my, T mz, T E) {
+    EulerPrim<T> q;
+    q.ux = mx / r;
+    q.uy = my / r;
+    q.uz = mz / r;
+    const T q2 = (q.ux * q.ux + q.uy * q.uy) + q.uz * q.uz;
+    q.p = (gamma - T(1)) * (E - (T(0.5) * r) * q2);
+    q.a = std::sqrt((gamma * q.p) / r);
+    return q;
+}
+
+template <class T>
+inline void rusanov_flux(const T UL[5], const EulerPrim<T>& L, const T UR[5], const EulerPrim<T>& R, T nx, T ny,
+                         T nz, T A, T G[5]) {
+    const T unL = (L.ux * nx + L.uy * ny) + L.uz * nz;
+    const T unR = (R.ux * nx + R.uy * ny) + R.uz * nz;
+    const T FL[5] = {UL[0] * unL, UL[1] * unL + L.p * nx, UL[2] * unL + L.p * ny, UL[3] * unL + L.p * nz,
+                     (UL[4] + L.p) * unL};
+    const T FR[5] = {UR[0] * unR, UR[1] * unR + R.p * nx, UR[2] * unR + R.p * ny, UR[3] * unR + R.p * nz,
+                     (UR[4] + R.p) * unR};
+    const T lam = std::max(std::abs(unL) + L.a, std::abs(unR) + R.a);
+    for (int k = 0; k < 5; ++k) G[k] = (T(0.5) * ((FL[k] + FR[k]) - lam * (UR[k] - UL[k]))) * A;
+}
+
+// +axis unit normal and area of local face fc of cell i
+template <class T> inline void face_na(const HexMesh<T>& m, int64_t i, int fc, T& nx, T& ny, T& nz, T& A) {
+    const int64_t f = m.cell_faces[6 * i + fc];
+    nx = m.svx[f]; ny = m.svy[f]; nz = m.svz[f]; A = m.sva[f];
+}
+
+// CFL time step of the Rusanov scheme (formula above)
+template <class T> T euler_dt(const HexMesh<T>& m, const T* const U[5], T gamma, T CFL) {
+    T mn = std::numeric_limits<T>::infinity();
+#pragma omp parallel for schedule(static) reduction(min : mn)
+    for (int64_t i = 0; i < m.N; ++i) {
+        const EulerPrim<T> q = euler_prim(gamma, U[0][i], U[1][i], U[2][i], U[3][i], U[4][i]);
+        T s = T(0);
+        for (int fc = 0; fc < 6; ++fc) {
+            T nx, ny, nz, A;
+            face_na(m, i, fc, nx, ny, nz, A);
+            const T un = (q.ux * nx + q.uy * ny) + q.uz * nz;
+            s = s + A * (std::abs(un) + q.a);
+        }
+        mn = std::min(mn, m.svol[i] / s);
+    }
+    return CFL * mn;
+}
+
+// One Rusanov step with time step dt; U is updated in place (through a copy).
+template <class T> void euler_step(const HexMesh<T>& m, T* const U[5], T gamma, T dt) {
+    const int64_t N = m.N;
+    std::vector<T> old(5 * size_t(N));
+    for (int k = 0; k < 5; ++k) std::memcpy(old.data() + size_t(k) * N, U[k], sizeof(T) * size_t(N));
+    auto st = [&](int64_t c, T out[5]) {
+        for (int k = 0; k < 5; ++k) out[k] = old[size_t(k) * N + c];
+    };
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        T Ui[5];
+        st(i, Ui);
+        const EulerPrim<T> qi = euler_prim(gamma, Ui[0], Ui[1], Ui[2], Ui[3], Ui[4]);
+        T Gf[6][5];
+        for (int fc = 0; fc < 6; ++fc) {
+            const size_t kf = size_t(fc) * size_t(N) + size_t(i);
+            T Un[5];
+            if (m.ids_bf[kf] >= 0) {
+                st(m.ids_bf[kf], Un);
+            } else {  // wall: mirrored state (momentum index 1 = x, 2 = y, 3 = z)
+                for (int k = 0; k < 5; ++k) Un[k] = Ui[k];
+                Un[3 - (fc >> 1)] = -Ui[3 - (fc >> 1)];
+            }
+            const EulerPrim<T> qn = euler_prim(gamma, Un[0], Un[1], Un[2], Un[3], Un[4]);
+            T nx, ny, nz, A;
+            face_na(m, i, fc, nx, ny, nz, A);
+            if (fc & 1) rusanov_flux(Ui, qi, Un, qn, nx, ny, nz, A, Gf[fc]);   // i is the lower cell
+            else rusanov_flux(Un, qn, Ui, qi, nx, ny, nz, A, Gf[fc]);          // i is the upper cell
+        }
+        for (int k = 0; k < 5; ++k) {
+            // fc 4/5 = x-/x+, 2/3 = y-/y+, 0/1 = z-/z+
+            const T net = ((Gf[5][k] - Gf[4][k]) + (Gf[3][k] - Gf[2][k])) + (Gf[1][k] - Gf[0][k]);
+            U[k][i] = Ui[k] - (dt * net) / m.svol[i];
+        }
+    }
+}
+
+// `steps` Rusanov steps with the CFL time step recomputed from the state at the
+// start of each step; dt_hist (nullable) receives every step's dt.
+template <class T> void euler_run(const HexMesh<T>& m, T* const U[5], T gamma, T CFL, int64_t steps, T* dt_hist) {
+    for (int64_t s = 0; s < steps; ++s) {
+        const T dt = euler_dt(m, U, gamma, CFL);
+        if (dt_hist) dt_hist[s] = dt;
+        euler_step(m, U, gamma, dt);
     }
 }
 

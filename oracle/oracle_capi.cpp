@@ -31,6 +31,7 @@ void build(OMesh* om, int64_t X, int64_t Y, int64_t Z, double dx, double dy, dou
     }
     cfdo::build_topology<T>(m, X, Y, Z, T(dx), T(dy), T(dz), perturb ? pt.data() : nullptr);
     cfdo::compute_geometry<T>(m);
+    cfdo::compute_fv_geometry<T>(m);
 }
 
 template <class V> int copy_vec(const std::vector<V>& v, void* out) {
@@ -67,6 +68,11 @@ template <class T> int table(OMesh* om, const std::string& nm, void* out) {
     if (nm == "fny") return copy_vec(m.fny, out);
     if (nm == "fnz") return copy_vec(m.fnz, out);
     if (nm == "farea") return copy_vec(m.farea, out);
+    if (nm == "svx") return copy_vec(m.svx, out);
+    if (nm == "svy") return copy_vec(m.svy, out);
+    if (nm == "svz") return copy_vec(m.svz, out);
+    if (nm == "sva") return copy_vec(m.sva, out);
+    if (nm == "svol") return copy_vec(m.svol, out);
     return -1;
 }
 }  // namespace
@@ -148,6 +154,37 @@ double oracle_cfl1d(void* h, double CFL, const void* u, const void* p, const voi
     if (om->dtype == 0)
         return cfdo::cfl1d(float(CFL), om->f, (const float*)u, (const float*)p, (const float*)rho);
     return cfdo::cfl1d(CFL, om->d, (const double*)u, (const double*)p, (const double*)rho);
+}
+
+// `steps` Rusanov steps of the Euler system; U = 5 contiguous N-arrays (rho,
+// rho u, rho v, rho w, E), updated in place; dt_hist (nullable) gets each dt.
+void oracle_euler_run(void* h, void* U, double gamma, double CFL, int64_t steps, void* dt_hist) {
+    auto* om = static_cast<OMesh*>(h);
+    if (om->dtype == 0) {
+        float* b = (float*)U;
+        const int64_t N = om->f.N;
+        float* const S[5] = {b, b + N, b + 2 * N, b + 3 * N, b + 4 * N};
+        cfdo::euler_run(om->f, S, float(gamma), float(CFL), steps, (float*)dt_hist);
+    } else {
+        double* b = (double*)U;
+        const int64_t N = om->d.N;
+        double* const S[5] = {b, b + N, b + 2 * N, b + 3 * N, b + 4 * N};
+        cfdo::euler_run(om->d, S, gamma, CFL, steps, (double*)dt_hist);
+    }
+}
+
+double oracle_euler_dt(void* h, const void* U, double gamma, double CFL) {
+    auto* om = static_cast<OMesh*>(h);
+    if (om->dtype == 0) {
+        const float* b = (const float*)U;
+        const int64_t N = om->f.N;
+        const float* const S[5] = {b, b + N, b + 2 * N, b + 3 * N, b + 4 * N};
+        return cfdo::euler_dt(om->f, S, float(gamma), float(CFL));
+    }
+    const double* b = (const double*)U;
+    const int64_t N = om->d.N;
+    const double* const S[5] = {b, b + N, b + 2 * N, b + 3 * N, b + 4 * N};
+    return cfdo::euler_dt(om->d, S, gamma, CFL);
 }
 
 }  // extern "C"

@@ -49,6 +49,9 @@ def lib():
         L.oracle_cfl3d.argtypes = [vp, d, vp, vp, vp, vp, vp]
         L.oracle_cfl1d.restype = d
         L.oracle_cfl1d.argtypes = [vp, d, vp, vp, vp]
+        L.oracle_euler_run.argtypes = [vp, vp, d, d, i64, vp]
+        L.oracle_euler_dt.argtypes = [vp, vp, d, d]
+        L.oracle_euler_dt.restype = d
         _lib = L
     return _lib
 
@@ -87,6 +90,7 @@ class OracleMesh:
         sizes = {"face_verts": 4 * self.F, "face_cells": 2 * self.F, "cell_faces": 6 * self.N,
                  "cell_verts": 8 * self.N, "vx": self.V, "vy": self.V, "vz": self.V,
                  "fnx": self.F, "fny": self.F, "fnz": self.F, "farea": self.F,
+                 "svx": self.F, "svy": self.F, "svz": self.F, "sva": self.F, "svol": self.N,
                  "vol": self.N, "is_bnd": self.N, "ccx": self.N, "ccy": self.N, "ccz": self.N}
         n = sizes.get(name, n6)
         dt = np.int64 if name in _INT_TABLES else self.dtype
@@ -141,3 +145,17 @@ class OracleMesh:
     def cfl1d(self, CFL, u, p, rho):
         u, p, rho = (self._arr(a) for a in (u, p, rho))
         return lib().oracle_cfl1d(self._h, float(CFL), _ptr(u), _ptr(p), _ptr(rho))
+
+    def euler_run(self, U, gamma, CFL, steps):
+        """Rusanov steps of the Euler system; U is (5, N) = (rho, rho u, rho v,
+        rho w, E).  Returns (U after `steps` steps, the dt of every step)."""
+        U = np.ascontiguousarray(U, dtype=self.dtype).copy()
+        assert U.shape == (5, self.N), U.shape
+        hist = np.empty(int(steps), dtype=self.dtype)
+        lib().oracle_euler_run(self._h, _ptr(U), float(gamma), float(CFL), int(steps), _ptr(hist))
+        return U, hist
+
+    def euler_dt(self, U, gamma, CFL):
+        U = np.ascontiguousarray(U, dtype=self.dtype)
+        assert U.shape == (5, self.N), U.shape
+        return lib().oracle_euler_dt(self._h, _ptr(U), float(gamma), float(CFL))

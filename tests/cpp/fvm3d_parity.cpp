@@ -83,7 +83,7 @@ int main(int argc, char** argv) {
     auto mesh = std::make_shared<CudaMesh3D>(size_t(in.nx), size_t(in.ny), size_t(in.nz), float(in.dx), float(in.dy),
                                              float(in.dz));
     mesh->init_basic_internals();
-    if (kase.find("perturbed") != std::string::npos) mesh->perturb_interior_vertices(5, 0.2);
+    if (kase.find("perturbed") != std::string::npos) mesh->perturb_interior_vertices(5, 0.02);
     mesh->compute();
     CFDArcoGlobalInit::enable_cuda(1, true, false);
     mesh->host_to_device();

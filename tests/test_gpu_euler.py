@@ -1,0 +1,131 @@
+"""GPU parity of the Rusanov Euler stepper (cfd_euler_run) against the oracle,
+and the Sod workload's full-size properties.
+
+The kernel and the oracle evaluate the same operation sequence with no FMA
+contraction and correctly rounded division and square root, so the fields and
+every step's dt are compared for exact equality in FP64 and FP32.  The FP32
+run's error against the FP64 oracle is reported per field and must stay within
+twice the FP32 oracle's own error."""
+import numpy as np
+import pytest
+
+from cfdarcov2_amd import workloads
+from tests import riemann_exact
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _gpu_run(hip_ctx, n, d, off, dtype, U0, gamma, cfl, steps):
+    from cfdarcov2_amd import capi
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=dtype, vertex_offsets=off)
+    A = [torch.from_numpy(np.ascontiguousarray(U0[k], dtype=dtype)).cuda() for k in range(5)]
+    B = [torch.empty_like(a) for a in A]
+    hist = torch.empty(max(steps, 1), dtype=A[0].dtype, device="cuda")
+    res = gm.euler_run(A, B, gamma, cfl, steps, hist)
+    torch.cuda.synchronize()
+    out = np.stack([t.cpu().numpy() for t in res])
+    h = hist.cpu().numpy()[:steps]
+    gm.close()
+    return out, h
+
+
+CASES = [
+    # name, shape, perturbation seed (None = uniform), steps
+    ("uniform-1d", (40, 1, 1), None, 30),
+    ("uniform-3d", (20, 7, 9), None, 20),
+    ("perturbed", (24, 10, 12), 3, 25),
+    ("perturbed-wide", (70, 6, 5), 8, 15),
+]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_euler_matches_oracle(hip_ctx, oracle_lib, case, dtype):
+    name, n, seed, steps = case
+    print(f"\n[case] euler {name} {np.dtype(dtype).name} n={n} steps={steps}", flush=True)
+    d = (1.0 / n[0],) * 3
+    off = None if seed is None else workloads.perturbed_vertex_offsets(*n, *d, seed=seed, frac=0.2)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=dtype, perturb=off)
+    U0 = workloads.sod_state(om.table("ccx"), dtype=dtype)
+    if name == "uniform-3d":  # add transverse motion so every flux term is live
+        rng = np.random.default_rng(2)
+        U0[1:4] = (0.1 * rng.standard_normal((3, om.N)) * U0[0]).astype(dtype)
+    ref, ref_dt = om.euler_run(U0, 1.4, 0.3, steps)
+    got, got_dt = _gpu_run(hip_ctx, n, d, off, dtype, U0, 1.4, 0.3, steps)
+    assert np.all(np.isfinite(ref))
+    assert np.array_equal(got_dt, ref_dt), np.max(np.abs(got_dt - ref_dt))
+    for k in range(5):
+        assert np.array_equal(got[k], ref[k]), (k, np.max(np.abs(got[k] - ref[k])))
+
+
+def test_fp32_error_against_fp64_oracle(hip_ctx, oracle_lib):
+    n, steps = (48, 8, 8), 40
+    d = (1.0 / 48,) * 3
+    off = workloads.perturbed_vertex_offsets(*n, *d, seed=1, frac=0.2)
+    o64 = oracle_lib.OracleMesh(*n, *d, dtype=np.float64, perturb=off)
+    o32 = oracle_lib.OracleMesh(*n, *d, dtype=np.float32, perturb=off)
+    U64, _ = o64.euler_run(workloads.sod_state(o64.table("ccx")), 1.4, 0.3, steps)
+    U32, _ = o32.euler_run(workloads.sod_state(o32.table("ccx"), dtype=np.float32), 1.4, 0.3, steps)
+    got, _ = _gpu_run(hip_ctx, n, d, off, np.float32, workloads.sod_state(o32.table("ccx"), dtype=np.float32),
+                      1.4, 0.3, steps)
+    for k, name in zip((0, 1, 4), ("rho", "rho_u", "E")):
+        err_cpu = np.sum(np.abs(U32[k] - U64[k])) / np.sum(np.abs(U64[k]))
+        err_gpu = np.sum(np.abs(got[k] - U64[k])) / np.sum(np.abs(U64[k]))
+        print(f"[fp32 vs fp64] {name}: gpu rel L1 {err_gpu:.3e}, cpu fp32 oracle {err_cpu:.3e}")
+        assert err_gpu <= 2 * err_cpu + 1e-12
+
+
+def test_sod_against_exact_riemann(hip_ctx, oracle_lib):
+    """Uniform tube, 400 cells along x: the density profile is within the
+    scheme's first-order L1 error of the exact solution."""
+    n = (400, 2, 2)
+    d = (1.0 / 400,) * 3
+    om = oracle_lib.OracleMesh(*n, *d, dtype=np.float64)
+    x = om.table("ccx")
+    U0 = workloads.sod_state(x)
+    got, dts = _gpu_run(hip_ctx, n, d, None, np.float64, U0, 1.4, 0.3, 240)
+    t = float(np.sum(dts))
+    rho_ex, u_ex, p_ex = riemann_exact.sod(x, t)
+    assert np.mean(np.abs(got[0] - rho_ex)) < 0.005
+    assert np.mean(np.abs(got[1] / got[0] - u_ex)) < 0.02
+
+
+def test_sod_full_size_properties(hip_ctx):
+    """512 x 64 x 64, interior vertices moved by up to 20% of dx, FP32, 500
+    steps at CFL 0.3: finite, positive density and pressure, mass and energy
+    conserved (reflective walls), positive time steps, the right-moving shock
+    ahead of the contact."""
+    from cfdarcov2_amd import capi
+    cfg = workloads.sod_config()
+    n, d = cfg.n, cfg.spacing
+    print(f"\n[case] euler full size n={n} steps={cfg.steps}", flush=True)
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=np.float32, vertex_offsets=cfg.vertex_offsets)
+    ccx = gm.table("ccx")
+    vol = gm.table("fv_vol").astype(np.float64)
+    assert np.all(vol > 0)
+    U0 = workloads.sod_state(ccx, dtype=np.float32)
+    A = [torch.from_numpy(U0[k].copy()).cuda() for k in range(5)]
+    B = [torch.empty_like(a) for a in A]
+    hist = torch.empty(cfg.steps, dtype=torch.float32, device="cuda")
+    res = gm.euler_run(A, B, cfg.gamma, cfg.cfl, cfg.steps, hist)
+    torch.cuda.synchronize()
+    U = np.stack([t.cpu().numpy() for t in res]).astype(np.float64)
+    dts = hist.cpu().numpy()
+    assert np.all(np.isfinite(U)) and np.all(dts > 0)
+    for k in (0, 4):
+        before = np.sum(vol * U0[k].astype(np.float64))
+        after = np.sum(vol * U[k])
+        assert abs(after - before) <= 1e-5 * abs(before), (k, before, after)
+    rho = U[0]
+    p = 0.4 * (U[4] - 0.5 * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / rho)
+    assert rho.min() > 0.1 and p.min() > 0.09
+    t = float(np.sum(dts.astype(np.float64)))
+    # column-averaged density against the exact solution
+    X = n[0]
+    rho_x = rho.reshape(X, -1).mean(axis=1)
+    x_c = ccx.reshape(X, -1).mean(axis=1)
+    rho_ex, _, _ = riemann_exact.sod(x_c, t)
+    assert np.mean(np.abs(rho_x - rho_ex)) < 0.01
+    gm.close()

@@ -54,3 +54,18 @@ def test_header_symbols_exported():
     for name in sorted(declared):
         assert hasattr(L, name), f"{name} declared but not exported"
     assert declared == set(capi.EXPORTS), sorted(declared ^ set(capi.EXPORTS))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n,d,seed", CASES)
+def test_euler_geometry_matches_oracle(oracle_lib, dtype, n, d, seed):
+    """The Euler stepper's vector-area geometry, bit for bit."""
+    off = None if seed is None else workloads.perturbed_vertex_offsets(*n, *d, seed=seed, frac=0.2)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=dtype, perturb=off)
+    cf = om.table("cell_faces").reshape(-1, 6).T          # (6, N) global face ids
+    for mine_name, ref_name in (("fv_nx", "svx"), ("fv_ny", "svy"), ("fv_nz", "svz"), ("fv_area", "sva")):
+        mine = capi.hex_table(*n, *d, dtype, mine_name, vertex_offsets=off)
+        ref = om.table(ref_name)[cf]
+        assert mine.dtype == ref.dtype and mine.shape == ref.shape
+        assert np.array_equal(mine, ref), mine_name
+    assert np.array_equal(capi.hex_table(*n, *d, dtype, "fv_vol", vertex_offsets=off), om.table("svol"))
