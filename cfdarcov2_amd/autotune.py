@@ -16,6 +16,8 @@ import os
 
 DEFAULT_CANDIDATES = [(0, 16, 32, 1)] + [
     (1, ty, L, r) for ty, L, r in itertools.product((4, 8, 16, 32), (4, 8, 16, 32, 64), (0, 1))
+] + [
+    (2, ty, L, r) for ty, L, r in itertools.product((8, 16), (4, 8, 16, 32, 64), (0, 1))
 ]
 
 

@@ -36,6 +36,7 @@ EXPORTS = [
     "cfd_grad", "cfd_lapl", "cfd_stab", "cfd_ewise", "cfd_cfl_min", "cfd_count_nonfinite",
     "cfd_heat_run", "cfd_wave_run", "cfd_euler_run", "cfd_mesh_set_wave_shape",
     "cfd_wave_traces", "cfd_wave_adjoint",
+    "cfd_batch_create", "cfd_batch_set_shape", "cfd_batch_run", "cfd_batch_destroy",
     "cfd_program_compile", "cfd_program_compile_only", "cfd_program_launch", "cfd_program_destroy",
     "cfd_mesh_geometry",
 ]
@@ -45,6 +46,11 @@ class CfdError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[{code}] {msg}")
         self.code = code
+
+
+class BatchCase(ctypes.Structure):
+    _fields_ = [("mesh", ctypes.c_void_p), ("kind", ctypes.c_int), ("coef", ctypes.c_double),
+                ("dt", ctypes.c_double), ("state_a", ctypes.c_void_p * 4), ("state_b", ctypes.c_void_p * 4)]
 
 
 class HexDesc(ctypes.Structure):
@@ -98,6 +104,10 @@ def load_library(path: str = LIB_PATH):
         "cfd_program_compile_only": [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)],
         "cfd_program_launch": [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), pp, vp],
         "cfd_program_destroy": [vp],
+        "cfd_batch_create": [vp, ctypes.POINTER(BatchCase), i, pp],
+        "cfd_batch_set_shape": [vp, i, i],
+        "cfd_batch_run": [vp, i64, i, vp],
+        "cfd_batch_destroy": [vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -1,0 +1,226 @@
+// Batched ensemble of independent heat and acoustic cases (FP32, uniform
+// meshes; no reference counterpart).
+//
+// One launch advances every case by one step: the grid is the concatenation of
+// all cases' march tiles (tile rows x 64 columns x `chunk` planes), each
+// workgroup finds its case by a binary search over the cases' first-tile
+// offsets (scalar loads of a small descriptor table) and runs the same tile
+// code as the single-mesh kernels, so every case's fields equal a plain
+// single-case run bit for bit.  A run of n steps is captured once into a HIP
+// graph (n kernel nodes, no memset nodes) and replayed with one launch.
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "march.hpp"
+
+namespace cfd {
+
+struct BatchCaseDev {
+    UniformGeo<float> g;
+    float coef;          // heat: alpha; wave: K
+    float dt;
+    int kind;            // CFD_CASE_HEAT / CFD_CASE_WAVE
+    int nzt, nyt;
+    int64_t tile0;       // first tile of the case in the batched grid
+    float* a[4];
+    float* b[4];
+};
+
+template <int TYT, int R> union BatchLds {
+    MarchLds<float, TYT, R> w;
+    HeatLds<float, TYT, R> h;
+};
+
+template <int TYT, int R>
+__global__ void __launch_bounds__(64 * TYT) batch_step_k(const BatchCaseDev* __restrict__ cases, int ncases,
+                                                         int parity, int L) {
+    __shared__ BatchLds<TYT, R> lds;
+    const int64_t tile = blockIdx.x;
+    int lo = 0, hi = ncases - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cases[mid].tile0 <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchCaseDev& cs = cases[lo];
+    const int64_t t = tile - cs.tile0;
+    const int64_t zt = t % cs.nzt;
+    const int64_t yt = (t / cs.nzt) % cs.nyt;
+    const int64_t xc = t / (int64_t(cs.nzt) * cs.nyt);
+    float* const* in = parity ? cs.b : cs.a;
+    float* const* out = parity ? cs.a : cs.b;
+    if (cs.kind == CFD_CASE_HEAT)
+        heat_march_tile<float, TYT, R>(cs.g, in[0], out[0], cs.coef, cs.dt, L, zt, yt, xc, lds.h);
+    else
+        wave_march_tile<float, TYT, R>(cs.g, in[0], in[1], in[2], in[3], out[0], out[1], out[2], out[3], cs.coef,
+                                       nullptr, cs.dt, L, zt, yt, xc, lds.w);
+}
+
+}  // namespace cfd
+
+using namespace cfd;
+
+struct cfd_batch_s {
+    cfd_ctx_t ctx = nullptr;
+    std::vector<BatchCaseDev> host;
+    BatchCaseDev* dev = nullptr;
+    int tile_rows = 4;
+    int chunk = 8;
+    int64_t total_tiles = 0;
+    bool dirty = true;
+    hipStream_t capture = nullptr;
+    // instantiated graphs keyed by (steps, tile rows, chunk)
+    std::map<std::tuple<int64_t, int, int>, hipGraphExec_t> graphs;
+};
+
+namespace cfd {
+
+static void batch_layout(cfd_batch_s* b, hipStream_t s) {
+    const int TY = b->tile_rows;
+    int64_t off = 0;
+    for (auto& c : b->host) {
+        c.nzt = int((c.g.Z + 63) / 64);
+        c.nyt = int((c.g.Y + TY - 1) / TY);
+        const int64_t nxc = (c.g.X + b->chunk - 1) / b->chunk;
+        c.tile0 = off;
+        off += int64_t(c.nzt) * c.nyt * nxc;
+    }
+    CFD_REQUIRE(off < (int64_t(1) << 31), "batched grid too large");
+    b->total_tiles = off;
+    CFD_HIP(hipMemcpyAsync(b->dev, b->host.data(), b->host.size() * sizeof(BatchCaseDev), hipMemcpyHostToDevice, s));
+    CFD_HIP(hipStreamSynchronize(s));
+    b->dirty = false;
+}
+
+static void batch_launch(const cfd_batch_s* b, int parity, hipStream_t s) {
+    const int n = int(b->host.size());
+    const dim3 grid(unsigned(b->total_tiles));
+    switch (b->tile_rows) {
+        case 4: batch_step_k<4, 1><<<grid, dim3(64, 4), 0, s>>>(b->dev, n, parity, b->chunk); break;
+        case 8: batch_step_k<4, 2><<<grid, dim3(64, 4), 0, s>>>(b->dev, n, parity, b->chunk); break;
+        case 16: batch_step_k<4, 4><<<grid, dim3(64, 4), 0, s>>>(b->dev, n, parity, b->chunk); break;
+        default: throw Error(CFD_E_INVALID, "batch tile rows must be 4, 8 or 16");
+    }
+}
+
+static void drop_graphs(cfd_batch_s* b) {
+    for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+}
+
+}  // namespace cfd
+
+extern "C" {
+
+int cfd_batch_create(cfd_ctx_t ctx, const cfd_batch_case* cases, int ncases, cfd_batch_t* out) {
+    return guarded([&] {
+        CFD_REQUIRE(ctx && cases && out, "cfd_batch_create: null argument");
+        CFD_REQUIRE(ncases >= 1, "cfd_batch_create: need at least one case");
+        auto b = std::make_unique<cfd_batch_s>();
+        b->ctx = ctx;
+        for (int i = 0; i < ncases; ++i) {
+            const cfd_batch_case& c = cases[i];
+            CFD_REQUIRE(c.mesh != nullptr, "cfd_batch_create: null mesh");
+            if (c.mesh->dtype != CFD_F32 || !c.mesh->uniform)
+                throw Error(CFD_E_UNSUPPORTED, "cfd_batch_create: cases must be FP32 uniform meshes");
+            CFD_REQUIRE(c.kind == CFD_CASE_HEAT || c.kind == CFD_CASE_WAVE, "cfd_batch_create: unknown case kind");
+            const int nf = c.kind == CFD_CASE_HEAT ? 1 : 4;
+            for (int f = 0; f < nf; ++f) {
+                CFD_REQUIRE(c.state_a[f] && c.state_b[f], "cfd_batch_create: null state pointer");
+                for (int k = 0; k < nf; ++k)
+                    CFD_REQUIRE(c.state_a[f] != c.state_b[k], "cfd_batch_create: state_a and state_b overlap");
+            }
+            BatchCaseDev d{};
+            d.g = c.mesh->ug32;
+            d.coef = float(c.coef);
+            d.dt = float(c.dt);
+            d.kind = c.kind;
+            for (int f = 0; f < 4; ++f) {
+                d.a[f] = (float*)(f < nf ? c.state_a[f] : nullptr);
+                d.b[f] = (float*)(f < nf ? c.state_b[f] : nullptr);
+            }
+            b->host.push_back(d);
+        }
+        b->dev = (BatchCaseDev*)pool_alloc(ctx, int64_t(ncases) * int64_t(sizeof(BatchCaseDev)), nullptr);
+        try {
+            batch_layout(b.get(), nullptr);
+            CFD_HIP(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
+        } catch (...) {
+            pool_free(ctx, b->dev, nullptr);
+            throw;
+        }
+        *out = b.release();
+    });
+}
+
+int cfd_batch_set_shape(cfd_batch_t b, int tile_rows, int chunk) {
+    return guarded([&] {
+        CFD_REQUIRE(b != nullptr, "null batch");
+        CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || tile_rows == 16, "batch tile rows must be 4, 8 or 16");
+        CFD_REQUIRE(chunk >= 1 && chunk <= 4096, "batch chunk must be in [1, 4096]");
+        if (tile_rows != b->tile_rows || chunk != b->chunk) {
+            b->tile_rows = tile_rows;
+            b->chunk = chunk;
+            b->dirty = true;
+        }
+    });
+}
+
+int cfd_batch_run(cfd_batch_t b, int64_t nsteps, int use_graph, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(b != nullptr, "null batch");
+        CFD_REQUIRE(nsteps >= 0, "cfd_batch_run: negative step count");
+        hipStream_t s = as_stream(stream);
+        if (b->dirty) {  // new tile shape: new layout, and graphs of the old one are stale
+            CFD_HIP(hipStreamSynchronize(s));
+            drop_graphs(b);
+            batch_layout(b, s);
+        }
+        if (!use_graph) {
+            for (int64_t k = 0; k < nsteps; ++k) batch_launch(b, int(k & 1), s);
+            CFD_HIP(hipGetLastError());
+            return;
+        }
+        // a replay starts from state_a again, so the state must end there
+        CFD_REQUIRE(nsteps % 2 == 0, "cfd_batch_run: graph runs need an even step count");
+        if (nsteps == 0) return;
+        const auto key = std::make_tuple(nsteps, b->tile_rows, b->chunk);
+        auto it = b->graphs.find(key);
+        if (it == b->graphs.end()) {
+            hipGraph_t graph = nullptr;
+            CFD_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
+            try {
+                for (int64_t k = 0; k < nsteps; ++k) batch_launch(b, int(k & 1), b->capture);
+            } catch (...) {
+                (void)hipStreamEndCapture(b->capture, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            CFD_HIP(hipStreamEndCapture(b->capture, &graph));
+            hipGraphExec_t exec = nullptr;
+            hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            CFD_HIP(e);
+            it = b->graphs.emplace(key, exec).first;
+        }
+        CFD_HIP(hipGraphLaunch(it->second, s));
+    });
+}
+
+int cfd_batch_destroy(cfd_batch_t b) {
+    return guarded([&] {
+        if (!b) return;
+        drop_graphs(b);
+        if (b->capture) (void)hipStreamDestroy(b->capture);
+        pool_free(b->ctx, b->dev, nullptr);
+        CFD_HIP(hipStreamSynchronize(nullptr));
+        delete b;
+    });
+}
+
+}  // extern "C"

@@ -47,7 +47,7 @@ void launch_wave(const cfd_mesh_s* m, void* const a[4], void* const b[4], double
 
 template <class T>
 void launch_wave_march(const cfd_mesh_s* m, void* const a[4], void* const b[4], double K, const void* Kf, double dt,
-                       int64_t nsteps, int ty, int L, int remap, hipStream_t s);
+                       int64_t nsteps, int variant, int ty, int L, int remap, hipStream_t s);
 
 template <class T>
 void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamma, double cfl, int64_t nsteps,

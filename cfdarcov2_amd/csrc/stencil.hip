@@ -527,8 +527,8 @@ void launch_wave(const cfd_mesh_s* m, void* const a[4], void* const b[4], double
         }
         return;
     }
-    if (m->wave_variant == 1) {
-        launch_wave_march<T>(m, a, b, K, Kf, dt, nsteps, m->wave_ty, m->wave_L, m->wave_remap, s);
+    if (m->wave_variant >= 1) {
+        launch_wave_march<T>(m, a, b, K, Kf, dt, nsteps, m->wave_variant, m->wave_ty, m->wave_L, m->wave_remap, s);
         return;
     }
     const auto& g = ugeo<T>(m);

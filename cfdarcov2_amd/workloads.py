@@ -193,3 +193,41 @@ def adjoint_config(n=128, steps=2000, every=45, n_sensors=32, cfl=0.4, c=1.0, se
     guess = gaussian(n, n, n, d, d, d, (0.5, 0.5, 0.5), 0.1, 0.5)
     return AdjointCase((n, n, n), (d, d, d), cfl * d / c, steps, every, np.array(cells, dtype=np.int64), p_true,
                        guess)
+
+
+@dataclass
+class EnsembleCase:
+    kind: str                # "heat" or "wave"
+    n: int                   # cube side in cells
+    spacing: float           # cell size (dyadic, so the mesh is exactly uniform)
+    coef: float              # heat: alpha; wave: K = -c^2
+    dt: float
+    field0: np.ndarray       # heat: T0; wave: p0 (u = v = w = 0)
+
+
+ENSEMBLE_SIDES = (16, 24, 32, 48, 64, 96, 128)
+
+
+def ensemble_config(n_cases=64, steps=500, seed=0, spacing=1.0 / 128, sides=ENSEMBLE_SIDES,
+                    heat_dt_factor=0.15, wave_cfl=0.4):
+    """Batched ensemble: case i is heat for even i and acoustic for odd i; the
+    cube side is drawn uniformly from `sides`; alpha ~ log-uniform [1e-4, 1e-2],
+    c ~ U[0.5, 2]; a Gaussian pulse (sigma 0.1 of the side for heat, 0.05 for
+    waves, amplitude 1) at a seeded centre in [0.25, 0.75]^3 of the box.  Every
+    case uses the same dyadic cell size so the meshes are exactly uniform;
+    dt = 0.15 dx^2 / alpha (heat, stable) or 0.4 dx / c (wave)."""
+    g = SplitMix64(seed)
+    cases = []
+    for i in range(n_cases):
+        side = sides[int(g.uniform() * len(sides)) % len(sides)]
+        box = side * spacing
+        centre = tuple(box * (0.25 + 0.5 * g.uniform()) for _ in range(3))
+        if i % 2 == 0:
+            alpha = 10.0 ** (-4.0 + 2.0 * g.uniform())
+            f0 = gaussian(side, side, side, spacing, spacing, spacing, centre, 0.1 * box)
+            cases.append(EnsembleCase("heat", side, spacing, alpha, heat_dt_factor * spacing ** 2 / alpha, f0))
+        else:
+            c = 0.5 + 1.5 * g.uniform()
+            f0 = gaussian(side, side, side, spacing, spacing, spacing, centre, 0.05 * box)
+            cases.append(EnsembleCase("wave", side, spacing, -c * c, wave_cfl * spacing / c, f0))
+    return cases, steps

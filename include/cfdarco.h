@@ -229,6 +229,30 @@ int cfd_wave_adjoint(cfd_mesh_t mesh, void* const adj_a[4], void* const adj_b[4]
                      const void* K_field, double dt, int64_t nsteps, const int64_t* sensor_cells, int nsensors,
                      const void* trace_grad, const void* checkpoints, int64_t every, void* grad_K, void* stream);
 
+/* ---- batched ensemble (FP32 uniform meshes; no reference counterpart) -----
+ * Many independent heat / acoustic cases advance together: one launch per
+ * step covers every case's tiles, and cfd_batch_run(use_graph = 1) replays n
+ * steps from a captured HIP graph with a single launch.  Each case's fields
+ * equal a cfd_heat_run / cfd_wave_run of the same case bit for bit.  After n
+ * steps a case's state is in state_a when n is even, state_b when odd; graph
+ * runs need an even n.  The batch keeps pointers to the caller's meshes'
+ * geometry constants (copied at creation) and state buffers. */
+#define CFD_CASE_HEAT 0   /* d1t(T) = alpha * lapl(T), zero-gradient walls     */
+#define CFD_CASE_WAVE 1   /* fused acoustic step with K = -c^2, reflective walls */
+typedef struct cfd_batch_case {
+    cfd_mesh_t mesh;      /* FP32 mesh, exactly uniform */
+    int kind;             /* CFD_CASE_HEAT or CFD_CASE_WAVE */
+    double coef;          /* heat: alpha; wave: K */
+    double dt;
+    void* state_a[4];     /* heat: [0] = T; wave: p, u, v, w */
+    void* state_b[4];
+} cfd_batch_case;
+int cfd_batch_create(cfd_ctx_t ctx, const cfd_batch_case* cases, int ncases, cfd_batch_t* out);
+/* tile_rows 4, 8 or 16 (y-rows x 64 z-columns per workgroup), chunk = x-planes per tile */
+int cfd_batch_set_shape(cfd_batch_t batch, int tile_rows, int chunk);
+int cfd_batch_run(cfd_batch_t batch, int64_t nsteps, int use_graph, void* stream);
+int cfd_batch_destroy(cfd_batch_t batch);
+
 /* ---- runtime-compiled kernels --------------------------------------------
  * hipRTC compilation of HIP C++ source for gfx950 (-O3, no FMA contraction,
  * no packed FP32), cached by a hash of (source, kernel name, options) in
@@ -268,8 +292,11 @@ int cfd_mesh_geometry(cfd_mesh_t mesh, cfd_geometry* out);
  * in the reference, whose JIT kernels use fixed 64-thread blocks,
  * src/fvm3d.cpp:1306-1308): variant 0 = one thread per cell, variant 1 = 2.5D
  * marching kernel with `tile_rows` (4, 8, 16 or 32) y-rows of 64 z-cells per
- * workgroup, `chunk` x-planes per workgroup and XCD-aware tile placement when
- * `xcd_remap` = 1.  Set by the autotuner (cfdarcov2_amd/autotune.py). */
+ * workgroup carried by 4 (8 for 32) thread rows, variant 2 = the same kernel
+ * with one thread row per tile row (tile_rows 4, 8 or 16), `chunk` x-planes
+ * per workgroup and XCD-aware tile placement when `xcd_remap` = 1.  All
+ * variants give bit-identical results.  Set by the autotuner
+ * (cfdarcov2_amd/autotune.py). */
 int cfd_mesh_set_wave_shape(cfd_mesh_t mesh, int variant, int tile_rows, int chunk, int xcd_remap);
 
 #ifdef __cplusplus

@@ -76,7 +76,7 @@ def test_transpose_identity(hip_ctx):
 
 def test_gradients_match_autograd_of_plain_stepper(hip_ctx):
     from cfdarcov2_amd import adjoint
-    n, d = (10, 9, 12), (0.1, 0.1, 0.1)
+    n, d = (10, 9, 12), (0.125, 0.125, 0.0625)
     N = 10 * 9 * 12
     rng = np.random.default_rng(8)
     sensors = rng.choice(N, 7, replace=False)
@@ -95,7 +95,7 @@ def test_gradients_match_autograd_of_plain_stepper(hip_ctx):
     ck = torch.from_numpy(Kf).requires_grad_(True)
     Lr = ((torch_wave.traces(cp, ck, dt, steps, torch.from_numpy(sensors), n, d) - torch.from_numpy(obs)) ** 2).sum()
     Lr.backward()
-    assert float(L) == pytest.approx(float(Lr), rel=1e-14)
+    assert float(L.detach()) == pytest.approx(float(Lr.detach()), rel=1e-14)
     for got, ref in ((tp.grad, cp.grad), (tk.grad, ck.grad)):
         g = got.cpu().numpy()
         r = ref.numpy()
@@ -162,7 +162,9 @@ def test_K_gradient_with_recompute_against_finite_differences(hip_ctx):
     for k in range(4):
         e = torch.from_numpy(rng.standard_normal(N)).cuda()
         e = e / e.norm()
-        h = 1e-4 * float(K0.norm())
+        # L is not quadratic in K: a small step keeps the O(h^2) truncation
+        # below the tolerance (FP64 round-off in L stays far smaller)
+        h = 2e-6 * float(K0.norm())
         fd = (_loss(pr, p0, obs, K0 + h * e) - _loss(pr, p0, obs, K0 - h * e)) / (2 * h)
         dd = float(torch.dot(gK, e))
         assert fd == pytest.approx(dd, rel=1e-6), k

@@ -1,0 +1,94 @@
+"""GPU checks of the batched ensemble: every case's fields after a batched run
+(plain launches, one group or several, and HIP-graph replays) equal the same
+case advanced alone by its single-case fused kernel, bit for bit."""
+import numpy as np
+import pytest
+
+from cfdarcov2_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _specs():
+    # uneven sides, both kinds, sides that do not divide the tile shapes
+    return workloads.ensemble_config(n_cases=10, steps=0, seed=3, spacing=1.0 / 64,
+                                     sides=(5, 16, 23, 40, 67, 33))[0]
+
+
+def _snapshot(ens, nsteps):
+    torch.cuda.synchronize()
+    return [[t.clone() for t in ens.state(i, nsteps)] for i in range(len(ens.cases))]
+
+
+def _compare(got, ref):
+    for i, (g, r) in enumerate(zip(got, ref)):
+        for f, (a, b) in enumerate(zip(g, r)):
+            assert torch.equal(a, b), (i, f, float((a - b).abs().max()))
+
+
+@pytest.mark.parametrize("shape", [(4, 8, 0), (8, 5, 0), (16, 32, 0), (4, 16, 1 << 20)])
+def test_batched_equals_sequential(hip_ctx, shape):
+    from cfdarcov2_amd.ensemble import Ensemble
+    print(f"\n[case] ensemble batched shape={shape}", flush=True)
+    ens = Ensemble(hip_ctx, _specs())
+    try:
+        ens.run_sequential(7)
+        ref = _snapshot(ens, 7)
+        ens.reset()
+        ens.configure(*shape)
+        if shape[2]:
+            assert len(ens.groups) > 1
+        ens.run(7, use_graph=False)
+        _compare(_snapshot(ens, 7), ref)
+    finally:
+        ens.close()
+
+
+def test_graph_replays(hip_ctx):
+    """Three replays of a captured 6-step graph, with eager work and
+    synchronisations between them, each checked against the plain runs."""
+    from cfdarcov2_amd.ensemble import Ensemble
+    print("\n[case] ensemble graph replays", flush=True)
+    ens = Ensemble(hip_ctx, _specs())
+    try:
+        refs = []
+        for k in range(3):
+            ens.run_sequential(6)
+            refs.append(_snapshot(ens, 6))
+        ens.reset()
+        ens.configure(8, 8, 0)
+        for k in range(3):
+            ens.run(6, use_graph=True)
+            scratch = torch.randn(1 << 20, device="cuda")
+            scratch.mul_(2.0).sum().item()
+            _compare(_snapshot(ens, 6), refs[k])
+    finally:
+        ens.close()
+
+
+def test_errors(hip_ctx):
+    from cfdarcov2_amd import capi
+    from cfdarcov2_amd.ensemble import Ensemble
+    ens = Ensemble(hip_ctx, _specs()[:2])
+    try:
+        ens.configure()
+        with pytest.raises(capi.CfdError):
+            ens.run(3, use_graph=True)        # odd graph runs are refused
+        with pytest.raises(capi.CfdError):
+            capi.check(ens.lib.cfd_batch_set_shape(ens.groups[0], 12, 8))
+    finally:
+        ens.close()
+    # a non-uniform mesh cannot join a batch
+    m = capi.HexMesh(hip_ctx, 4, 4, 4, 0.1, 0.1, 0.1, dtype="f32")
+    assert m.info().uniform == 0
+    t = [torch.zeros(64, device="cuda") for _ in range(2)]
+    arr = (capi.BatchCase * 1)()
+    arr[0].mesh = m.h
+    arr[0].kind = 0
+    arr[0].state_a[0] = t[0].data_ptr()
+    arr[0].state_b[0] = t[1].data_ptr()
+    h = capi.ctypes.c_void_p()
+    assert ens.lib.cfd_batch_create(hip_ctx.h, arr, 1, capi.ctypes.byref(h)) == -4
+    m.close()
