@@ -26,16 +26,20 @@ if ROOT not in sys.path:
 
 METRIC = "MCUPS + % HBM roofline (1 GPU); adjoint/forward cost ratio; batched vs plain"
 HBM_PEAK_GBS = 8000.0
-KERNELS = {0: "wave_uniform_k", 1: "wave_march_k"}
+KERNELS = {0: "wave_uniform_k", 1: "wave_march_k", 2: "wave_march_k"}
 
 
-def _traffic(n, dtype):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+def _traffic(n, dtype, shape):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if one
+    exists for this workload and kernel shape."""
     path = os.path.join(ROOT, "profiles", f"wave{n}cube_{dtype}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         rec = json.load(f)
+    prof_shape = rec.get("kernel_shape")
+    if prof_shape is not None and any(prof_shape.get(k) != shape.get(k) for k in prof_shape):
+        return None
     return rec.get("hbm_bytes_per_launch")
 
 
@@ -57,6 +61,182 @@ def cpu_baseline(case, dtype, steps):
     return {"value": om.N * steps / t / 1e6, "unit": "MCUPS", "cores": cores, "kind": "port",
             "sample": f"oracle wave_run, {case.n[0]}^3 {dtype}, {steps} steps of the benchmark workload, "
                       f"{t:.2f} s"}
+
+
+def _events():
+    import torch
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def _timed(fn):
+    """(wall seconds, event milliseconds) of fn() on the current stream."""
+    import torch
+    stream = torch.cuda.current_stream()
+    e0, e1 = _events()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, e0.elapsed_time(e1)
+
+
+def suite_heat(ctx, cpu=True):
+    """Heat 64^3 FP64, 200 steps at dt = 0.2 dx^2/alpha (the reference CPU
+    configuration); 16 algorithmic bytes per cell-update; 4 MiB of state stays
+    in the last-level cache, so the fraction is of HBM peak but the data is
+    cache-resident."""
+    import numpy as np
+    import torch
+
+    from cfdarcov2_amd import capi, workloads
+    case = workloads.heat_config()
+    m = capi.HexMesh(ctx, *case.n, *case.spacing, dtype="f64")
+    a = torch.from_numpy(case.T0).cuda()
+    b = torch.empty_like(a)
+    m.heat_run(a, b, case.alpha, case.dt, 10)
+    a.copy_(torch.from_numpy(case.T0))
+    wall, ev = _timed(lambda: m.heat_run(a, b, case.alpha, case.dt, case.steps))
+    N = m.N
+    rec = {"workload": "heat 64^3 FP64, 200 steps, dt = 0.2 dx^2/alpha", "mcups": N * case.steps / wall / 1e6,
+           "kernel_us_per_step": ev * 1e3 / case.steps,
+           "gbs_algorithmic": 16 * N / (ev * 1e-3 / case.steps) / 1e9}
+    rec["roofline_frac"] = rec["gbs_algorithmic"] / HBM_PEAK_GBS
+    m.close()
+    if cpu:
+        from oracle import pyoracle
+        om = pyoracle.OracleMesh(*case.n, *case.spacing, dtype=np.float64)
+        t0 = time.perf_counter()
+        om.heat_run(case.T0, case.alpha, case.dt, 20)
+        t = time.perf_counter() - t0
+        rec["cpu_oracle_mcups"] = N * 20 / t / 1e6
+        rec["cpu_sample"] = f"oracle heat_run 64^3 FP64, 20 steps, {t:.2f} s"
+    return rec
+
+
+def suite_wave_f64(ctx, steps=200):
+    import torch
+
+    from cfdarcov2_amd import autotune, capi, workloads
+    case = workloads.wave_config(n=256, steps=steps)
+    m = capi.HexMesh(ctx, *case.n, *case.spacing, dtype="f64")
+    shape = autotune.tune_wave(m, steps=10, use_cache=False,
+                               candidates=[(1, ty, L, r) for ty in (4, 8) for L in (8, 16, 32) for r in (0, 1)]
+                               + [(2, 8, L, 0) for L in (8, 16)])
+    N = m.N
+    A = [torch.from_numpy(case.p0).cuda()] + [torch.zeros(N, dtype=torch.float64, device="cuda") for _ in range(3)]
+    B = [torch.empty_like(A[0]) for _ in range(4)]
+    wall, ev = _timed(lambda: m.wave_run(A, B, -1.0, case.dt, steps))
+    gbs = 64 * N / (ev * 1e-3 / steps) / 1e9
+    m.close()
+    return {"workload": "acoustic wave 256^3 FP64", "mcups": N * steps / wall / 1e6,
+            "kernel_us_per_step": ev * 1e3 / steps, "gbs_algorithmic": gbs, "roofline_frac": gbs / HBM_PEAK_GBS,
+            "kernel_shape": shape}
+
+
+def suite_sod(ctx, cpu=True):
+    """Sod 512x64x64, interior vertices moved by up to 20% of dx, FP32, 500
+    Rusanov steps at CFL 0.3.  92 algorithmic bytes per cell-update: 5 fields
+    read and written, 3 owned faces x (normal + area) and the volume."""
+    import numpy as np
+    import torch
+
+    from cfdarcov2_amd import capi, workloads
+    cfg = workloads.sod_config()
+    m = capi.HexMesh(ctx, *cfg.n, *cfg.spacing, dtype="f32", vertex_offsets=cfg.vertex_offsets)
+    U0 = workloads.sod_state(m.table("ccx"), dtype=np.float32)
+    A = [torch.from_numpy(U0[k].copy()).cuda() for k in range(5)]
+    B = [torch.empty_like(a) for a in A]
+    m.euler_run(A, B, cfg.gamma, cfg.cfl, 2)          # builds the stepper geometry
+    for k in range(5):
+        A[k].copy_(torch.from_numpy(U0[k]))
+    wall, ev = _timed(lambda: m.euler_run(A, B, cfg.gamma, cfg.cfl, cfg.steps))
+    res = A if cfg.steps % 2 == 0 else B
+    bad = sum(ctx.count_nonfinite(t) for t in res)
+    N = m.N
+    gbs = 92 * N / (ev * 1e-3 / cfg.steps) / 1e9
+    rec = {"workload": "Sod 512x64x64 perturbed (20% dx), FP32, Rusanov, CFL 0.3, 500 steps",
+           "mcups": N * cfg.steps / wall / 1e6, "kernel_us_per_step": ev * 1e3 / cfg.steps,
+           "gbs_algorithmic": gbs, "roofline_frac": gbs / HBM_PEAK_GBS, "nonfinite": int(bad),
+           "kernel": "euler_step_k"}
+    m.close()
+    if cpu:
+        from oracle import pyoracle
+        om = pyoracle.OracleMesh(*cfg.n, *cfg.spacing, dtype=np.float32, perturb=cfg.vertex_offsets)
+        U = workloads.sod_state(om.table("ccx"), dtype=np.float32)
+        t0 = time.perf_counter()
+        om.euler_run(U, cfg.gamma, cfg.cfl, 2)
+        t = time.perf_counter() - t0
+        rec["cpu_oracle_mcups"] = N * 2 / t / 1e6
+        rec["cpu_sample"] = f"oracle euler_run 512x64x64 FP32, 2 steps, {t:.2f} s"
+    return rec
+
+
+def suite_ensemble(ctx, steps=500):
+    """64 heat/acoustic cases of seeded uneven sides, FP32, 500 steps: the
+    autotuned batched graph run against 64 sequential single-case runs."""
+    from cfdarcov2_amd import autotune, workloads
+    from cfdarcov2_amd.ensemble import Ensemble, autotune as tune_batch, time_gpu
+    specs, steps = workloads.ensemble_config(steps=steps)
+    ens = Ensemble(ctx, specs)
+    # the plain runs use the fused acoustic kernel's tuned shape for the largest case
+    big = max((c for c in ens.cases if c.spec.kind == "wave"), key=lambda c: c.spec.n)
+    wshape = autotune.tune_wave(big.mesh, steps=10, use_cache=False,
+                                candidates=[(1, ty, L, r) for ty in (4, 8) for L in (8, 16, 32) for r in (0, 1)]
+                                + [(2, 8, L, 0) for L in (8, 16)])
+    for c in ens.cases:
+        if c.spec.kind == "wave":
+            c.mesh.set_wave_shape(wshape["variant"], wshape["tile_rows"], wshape["chunk"], wshape["xcd_remap"])
+    ens.reset()
+    ens.run_sequential(2)
+    ens.reset()
+    t_seq = time_gpu(lambda: ens.run_sequential(steps), reps=1)
+    best = tune_batch(ens, steps=20)
+    ens.run(steps)  # capture the full-length graph
+    ens.reset()
+    t_b = time_gpu(lambda: ens.run(steps), reps=1)
+    cu = ens.cell_updates(steps)
+    rec = {"workload": f"{len(specs)} cases (heat even, wave odd), seeded sides in {workloads.ENSEMBLE_SIDES}, "
+                       f"FP32, {steps} steps", "cells": ens.cell_updates(1),
+           "batched_mcups": cu / t_b / 1e6, "sequential_mcups": cu / t_seq / 1e6, "speedup": t_seq / t_b,
+           "batched_ms": t_b * 1e3, "sequential_ms": t_seq * 1e3, "batch_shape": best,
+           "plain_wave_shape": wshape,
+           "batched_gbs_algorithmic": ens.algorithmic_bytes(steps) / t_b / 1e9}
+    ens.close()
+    return rec
+
+
+def suite_adjoint(ctx):
+    """Adjoint 128^3 FP32, 2000 steps, checkpoints every 45, 32 sensors: mean
+    forward step vs mean adjoint step (dL/dp0 alone, and dL/dp0 + dL/dK with
+    checkpoint-and-recompute)."""
+    import numpy as np
+    import torch
+
+    from cfdarcov2_amd import adjoint, capi, workloads
+    cfg = workloads.adjoint_config()
+    m = capi.HexMesh(ctx, *cfg.n, *cfg.spacing, dtype="f32")
+    pr = adjoint.WaveProblem(m, cfg.sensors, cfg.dt, cfg.steps, K=-1.0, every=cfg.every)
+    p_true = torch.from_numpy(cfg.p0_true.astype(np.float32)).cuda()
+    obs, _, _ = adjoint.forward_traces(pr, p_true, checkpoints=False)
+    p0 = torch.from_numpy(cfg.p0_guess.astype(np.float32)).cuda()
+    Kf = torch.full((m.N,), -1.0, dtype=torch.float32, device="cuda")
+    out = {}
+    # forward with traces and checkpoints (as the autograd forward runs it)
+    wall_f, ev_f = _timed(lambda: out.__setitem__("f", adjoint.forward_traces(pr, p0, Kf, checkpoints=True)))
+    tr, _, ck = out["f"]
+    g = 2.0 * (tr - obs)
+    wall_a, ev_a = _timed(lambda: adjoint.adjoint(pr, g, Kf, ck, want_K=False))
+    wall_k, ev_k = _timed(lambda: adjoint.adjoint(pr, g, Kf, ck, want_K=True))
+    steps = cfg.steps
+    rec = {"workload": "adjoint acoustic 128^3 FP32, 2000 steps, checkpoints every 45, 32 sensors",
+           "forward_us_per_step": ev_f * 1e3 / steps,
+           "adjoint_us_per_step": ev_a * 1e3 / steps,
+           "adjoint_with_K_recompute_us_per_step": ev_k * 1e3 / steps,
+           "ratio": ev_a / ev_f, "ratio_with_K_recompute": ev_k / ev_f}
+    m.close()
+    return rec
 
 
 def calibrate_copy(args):
@@ -87,6 +267,8 @@ def main(argv=None):
     ap.add_argument("--shape", default="auto",
                     help="fused-kernel shape 'variant,tile_rows,chunk,xcd_remap' or 'auto' (autotuner)")
     ap.add_argument("--tune-verbose", action="store_true")
+    ap.add_argument("--no-suite", action="store_true",
+                    help="skip the secondary workloads (heat FP64, wave FP64, Sod, ensemble, adjoint)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of the benchmark's read volume (PMC calibration)")
     args = ap.parse_args(argv)
@@ -145,7 +327,7 @@ def main(argv=None):
     esize = 4 if args.dtype == "f32" else 8
     bytes_per_launch = 8 * esize * N          # 4 fields read + 4 written
     achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
-    traffic = _traffic(args.n, args.dtype)
+    traffic = _traffic(args.n, args.dtype, shape)
     out = {
         "metric": METRIC,
         "value": N * args.steps / wall / 1e6,
@@ -169,6 +351,21 @@ def main(argv=None):
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
     mesh.close()
+    del A, B, cur, nxt, res
+    torch.cuda.empty_cache()
+    if not args.no_suite:
+        suite = {}
+        for name, fn in (("heat64_f64", lambda: suite_heat(ctx, cpu=not args.no_cpu)),
+                         ("wave256_f64", lambda: suite_wave_f64(ctx)),
+                         ("sod512x64x64_f32", lambda: suite_sod(ctx, cpu=not args.no_cpu)),
+                         ("ensemble64_f32", lambda: suite_ensemble(ctx)),
+                         ("adjoint128_f32", lambda: suite_adjoint(ctx))):
+            print(f"[bench] {name}", file=sys.stderr, flush=True)
+            suite[name] = fn()
+            torch.cuda.empty_cache()
+        out["workloads"] = suite
+        out["adjoint_forward_ratio"] = suite["adjoint128_f32"]["ratio_with_K_recompute"]
+        out["batched_vs_plain"] = suite["ensemble64_f32"]["speedup"]
     ctx.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.dtype, args.cpu_steps)

@@ -321,7 +321,7 @@ Variable abs(const Variable& obj) {
 
 // ---------------------------------------------------------------------------
 // update policies (src/fvm3d.cpp:496-567): gamma = 5/3 and dl = min spacing
-// regardless of the driver (Q11)
+// regardless of the driver
 // ---------------------------------------------------------------------------
 float UpdatePolicies::CourantFriedrichsLewy(float CFL, std::vector<Eigen::Matrix<float, -1, 1>>& sv, Mesh3D* mesh) {
     auto& u = sv.at(0);

@@ -1,7 +1,7 @@
 // History output (include/io_operators_tmpl.hpp:44-212 of the reference):
 // VTK XML ImageData (.vti, ASCII) per stored step with one cell array per
 // variable, and mesh.json with the lattice extents and spacing.  Unlike the
-// reference's writer, every history entry is written (Q18).
+// reference's writer, every history entry is written.
 #include <fstream>
 #include <iomanip>
 #include <iostream>

@@ -31,7 +31,7 @@ bool Quadrangle3D::is_boundary() const {
            _coord_idx[1] + 1 == _mesh->_y || _coord_idx[2] == 0 || _coord_idx[2] + 1 == _mesh->_z;
 }
 // boundary tests along one axis of this cell (the reference's versions test
-// global faces 0-5 instead, Q19)
+// global faces 0-5 instead)
 bool Quadrangle3D::is_boundary_x() const { return _coord_idx[0] == 0 || _coord_idx[0] + 1 == _mesh->_x; }
 bool Quadrangle3D::is_boundary_y() const { return _coord_idx[1] == 0 || _coord_idx[1] + 1 == _mesh->_y; }
 bool Quadrangle3D::is_boundary_z() const { return _coord_idx[2] == 0 || _coord_idx[2] + 1 == _mesh->_z; }

@@ -84,7 +84,7 @@ template <class T> struct UniformGeo {
 // Face families in the reference's local face order: z-faces (fc 0,1), y-faces
 // (fc 2,3), x-faces (fc 4,5).  Family index of the face between cells along:
 //   z: x*Y*(Z+1) + y*(Z+1) + zf       y: x*(Y+1)*Z + yf*Z + z       x: xf*Y*Z + y*Z + z
-// Per face: the axis-direction normal (reference _normals, Q4), area, alpha_d.
+// Per face: the axis-direction normal (reference _normals), area, alpha_d.
 // Per cell: volume and the six centre-to-face distances (stab only).
 template <class T> struct GeneralGeo {
     int64_t X, Y, Z;

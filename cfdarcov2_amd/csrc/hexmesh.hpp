@@ -58,7 +58,7 @@ template <class T> struct HexGeometry {
     P3<T> fnorm(int fam, int64_t f) const { return {fn[fam][0][f], fn[fam][1][f], fn[fam][2][f]}; }
     P3<T> fcen(int fam, int64_t f) const { return {fcx[fam][f], fcy[fam][f], fcz[fam][f]}; }
     P3<T> ccen(int64_t c) const { return {ccx[c], ccy[c], ccz[c]}; }
-    // Reference _normals (axis direction, Q4): -n on fc 0,1,4,5 and +n on 2,3.
+    // Reference _normals (axis direction): -n on fc 0,1,4,5 and +n on 2,3.
     static P3<T> axis_normal(P3<T> n, int fc) {
         if (fc == 2 || fc == 3) return n;
         return {-n.x, -n.y, -n.z};

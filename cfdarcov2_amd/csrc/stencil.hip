@@ -35,7 +35,7 @@ template <class G> __device__ inline Cell cell_of_thread(const G& g) {
 }
 
 // Neighbour of cell c across its lower / upper face on axis a (0 = x, 1 = y,
-// 2 = z); the cell itself on a boundary face (reference rule, Q6).
+// 2 = z); the cell itself on a boundary face (reference rule).
 template <class G> __device__ inline int64_t nb_lo(const G& g, const Cell& c, int a) {
     if (a == 0) return c.x > 0 ? c.id - g.Y * g.Z : c.id;
     if (a == 1) return c.y > 0 ? c.id - g.Z : c.id;
@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(256) lapl_uniform_k(UniformGeo<T> g, const T* 
         const T al_hi = at_hi(g, c, a) ? g.alpha_b[a] : g.alpha[a];
         const T s_lo = (-pc + phi[nb_lo(g, c, a)]) * al_lo;
         const T s_hi = (-pc + phi[nb_hi(g, c, a)]) * al_hi;
-        // axis normals are +1 on both faces of the pair (Q4)
+        // axis normals are +1 on both faces of the pair
         res = res + (s_lo * g.area[a] + s_hi * g.area[a]) / g.vol;
     }
     out[c.id] = res;
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(256) grad3_uniform_k(UniformGeo<T> g, const T*
 
 // Second stage of StabVar on a uniform mesh (src/fvm3d.cpp:915-936): upwind
 // face values phi + (len * g_d) * n_out_d, jump against the neighbour's
-// opposite face (self's opposite face on the boundary, Q8), 0.5 * jump, Gauss
+// opposite face (self's opposite face on the boundary), 0.5 * jump, Gauss
 // with axis normals for component d.
 template <class T>
 __global__ void __launch_bounds__(256) stab_uniform_k(UniformGeo<T> g, const T* __restrict__ phi,

@@ -63,7 +63,7 @@ template <class T> struct HexMesh {
 
     // N x 6 column-major tables
     std::vector<int64_t> ids, ids_bf;
-    std::vector<T> nx, ny, nz;              // _normals (axis direction, Q4)
+    std::vector<T> nx, ny, nz;              // _normals (axis direction)
     std::vector<T> nax, nay, naz;           // _normals_alt (outward)
     std::vector<T> area6, alpha_d, len6;
     std::vector<T> vol, is_bnd, is_bnd_rev;
@@ -175,7 +175,7 @@ template <class T> void compute_geometry(HexMesh<T>& m) {
         const V3<T> c1 = vcross(AB, AD), c2 = vcross(CB, CD);
         const T s1 = std::abs(sum3_fixed(c1.x, c1.y, c1.z));
         const T s2 = std::abs(sum3_fixed(c2.x, c2.y, c2.z));
-        m.farea[f] = T(0.5 * (s1 + s2));  // Q2: |sum of components|, src/mesh3d.cpp:61
+        m.farea[f] = T(0.5 * (s1 + s2));  // |sum of components|, src/mesh3d.cpp:61
     }
 
     m.ccx.resize(N); m.ccy.resize(N); m.ccz.resize(N); m.vol.resize(N);
@@ -204,7 +204,7 @@ template <class T> void compute_geometry(HexMesh<T>& m) {
             V3<T> good = n;
             if (!(dot_fixed(n, vsub(fc, c)) >= T(0))) good = {-n.x, -n.y, -n.z};
             const size_t k = size_t(j) * size_t(N) + size_t(i);
-            // Q4: axis-direction normals, src/mesh3d.cpp:92-96
+            // axis-direction normals, src/mesh3d.cpp:92-96
             if (j == 4 || j == 5 || j == 0 || j == 1) {
                 m.nx[k] = -n.x; m.ny[k] = -n.y; m.nz[k] = -n.z;
             } else {
@@ -273,7 +273,7 @@ inline T face_linear(const HexMesh<T>& m, const T* phi, int64_t i, int fc, int w
 
 // Green-Gauss sum for component d over the six faces of cell i
 // (grad_utils.hpp:82-99).  `axis_normals` = the reference's use_alt_normals=true
-// (Q5: that flag selects the axis normals).  The row sum runs fc = 0..5 from 0.
+// (that flag selects the axis normals).  The row sum runs fc = 0..5 from 0.
 template <class T>
 inline T gauss_component(const HexMesh<T>& m, const T* face_val6_i /*6 values*/, int64_t i, int d, bool axis_normals) {
     const std::vector<T>* nrm;
@@ -327,7 +327,7 @@ void lapl(const HexMesh<T>& m, const T* phi, int axis_mask, T* out) {
 // gradient, upwind face extrapolation phi + len * g_d * n_out_d
 // (grad_utils.hpp:38-59), the neighbour's opposite-face value
 // (collect_vals_neigh_faces<.., false>, grad_utils.hpp:61-80; boundary reads
-// self's opposite face, Q8), 0.5 * (own - collected), Gauss with axis normals,
+// self's opposite face), 0.5 * (own - collected), Gauss with axis normals,
 // component d, summed over the selected axes from 0.
 template <class T>
 void stab(const HexMesh<T>& m, const T* phi, int axis_mask, T* out) {
@@ -378,7 +378,7 @@ template <class T> inline T leapfrog_update(T dt, T rhs, T cur, T prev) {
 }
 
 // UpdatePolicies::CourantFriedrichsLewy3D (src/fvm3d.cpp:512-527): gamma = 5/3
-// regardless of the driver (Q11), dl = min(dx, dy, dz).
+// regardless of the driver, dl = min(dx, dy, dz).
 template <class T>
 T cfl3d(T CFL, const HexMesh<T>& m, const T* u, const T* v, const T* w, const T* p, const T* rho) {
     const T gamma = T(5. / 3.);
@@ -417,7 +417,7 @@ T cfl1d(T CFL, const HexMesh<T>& m, const T* u, const T* p, const T* rho) {
 // ---------------------------------------------------------------------------
 
 // Heat: {d1t(T), '=', alpha * lapl(T)} with no boundary callback; zero-gradient
-// walls come from neighbour = self (Q6).  alpha_field (nullable) replaces the
+// walls come from neighbour = self.  alpha_field (nullable) replaces the
 // constant by a per-cell coefficient.  Operators are materialised per node, as the
 // reference CPU path does.
 template <class T>
@@ -477,7 +477,7 @@ void wave_run(const HexMesh<T>& m, T* p, T* u, T* v, T* w, T K, const T* K_field
 //
 // State per cell U = (rho, rho u, rho v, rho w, E).  On the face between the
 // lower cell L and upper cell R along an axis, n is the unit axis-direction
-// normal (the reference's _normals, Q4) and A the face area:
+// normal (the reference's _normals) and A the face area:
 //   un = (u n_x + v n_y) + w n_z,  F(U) = (rho un, rho u un + p n_x,
 //        rho v un + p n_y, rho w un + p n_z, (E + p) un),
 //   lam = max(|un_L| + a_L, |un_R| + a_R),
@@ -488,7 +488,7 @@ void wave_run(const HexMesh<T>& m, T* p, T* u, T* v, T* w, T K, const T* K_field
 //   dt = CFL min_i V_i / s_i,  s_i = sum over fc = 0..5 of A_f (|un_f| + a_i).
 // ---------------------------------------------------------------------------
 
-// Geometry of the stepper.  The reference's sum-of-components area (Q2) and
+// Geometry of the stepper.  The reference's sum-of-components area and
 // the volume built on it (src/mesh3d.cpp:61,101-104) are exact only for
 // axis-aligned faces and give negative cell volumes on a mesh whose vertices
 // move by 20% of dx, so the stepper uses the vector area of each quad face,

@@ -87,7 +87,7 @@ def test_uniform_geometry_closed_form(oracle_lib, dtype, d):
     for fc, h in zip(range(6), [dz, dz, dy, dy, dx, dx]):
         np.testing.assert_allclose(ln[fc], h / 2, rtol=rtol)
     np.testing.assert_allclose(m.table("vol"), dx * dy * dz, rtol=rtol)
-    # axis normals (Q4) are +e on both faces of a pair; outward normals flip on
+    # axis normals are +e on both faces of a pair; outward normals flip on
     # the lower face
     ax = {0: "nz", 1: "nz", 2: "ny", 3: "ny", 4: "nx", 5: "nx"}
     out = {0: "naz", 1: "naz", 2: "nay", 3: "nay", 4: "nax", 5: "nax"}
@@ -102,7 +102,7 @@ def test_uniform_geometry_closed_form(oracle_lib, dtype, d):
 
 
 def test_area_sum_of_components_quirk(oracle_lib):
-    """Q2: face area = |A_vec . (1,1,1)|, not |A_vec|.  Tilting the top face of a
+    """Face area = |A_vec . (1,1,1)|, not |A_vec|.  Tilting the top face of a
     unit cube so its normal is along (-s, 0, 1) makes the reference area
     dx*dy*|1 - s| (the true area is dx*dy*sqrt(1 + s^2))."""
     s = 0.5
@@ -205,7 +205,7 @@ def test_constants_are_annihilated(oracle_lib):
     assert np.max(np.abs(m.lapl(c))) == 0
     assert np.max(np.abs(m.stab(c))) == 0
     # On a perturbed mesh only the Laplacian (differences of cell values)
-    # still annihilates constants: the Q2 face areas do not close the cell
+    # still annihilates constants: the sum-of-components face areas do not close the cell
     # surface, so sum_f n_out A != 0 and the Green-Gauss gradient of a
     # constant is not zero there - a property of the reference geometry.
     off = workloads.perturbed_vertex_offsets(*N3, *D3, seed=3, frac=0.2)
