@@ -135,7 +135,7 @@ def suite_wave_f64(ctx, steps=200):
             "kernel_shape": shape}
 
 
-def suite_sod(ctx, cpu=True):
+def suite_sod(ctx, cpu=True, verbose=False):
     """Sod 512x64x64, interior vertices moved by up to 20% of dx, FP32, 500
     Rusanov steps at CFL 0.3.  92 algorithmic bytes per cell-update: 5 fields
     read and written, 3 owned faces x (normal + area) and the volume."""
@@ -148,9 +148,8 @@ def suite_sod(ctx, cpu=True):
     U0 = workloads.sod_state(m.table("ccx"), dtype=np.float32)
     A = [torch.from_numpy(U0[k].copy()).cuda() for k in range(5)]
     B = [torch.empty_like(a) for a in A]
-    m.euler_run(A, B, cfg.gamma, cfg.cfl, 2)          # builds the stepper geometry
-    for k in range(5):
-        A[k].copy_(torch.from_numpy(U0[k]))
+    from cfdarcov2_amd import autotune
+    shape = autotune.tune_euler(m, A, cfg.gamma, cfg.cfl, steps=10, verbose=verbose)  # also builds the geometry
     wall, ev = _timed(lambda: m.euler_run(A, B, cfg.gamma, cfg.cfl, cfg.steps))
     res = A if cfg.steps % 2 == 0 else B
     bad = sum(ctx.count_nonfinite(t) for t in res)
@@ -159,7 +158,8 @@ def suite_sod(ctx, cpu=True):
     rec = {"workload": "Sod 512x64x64 perturbed (20% dx), FP32, Rusanov, CFL 0.3, 500 steps",
            "mcups": N * cfg.steps / wall / 1e6, "kernel_us_per_step": ev * 1e3 / cfg.steps,
            "gbs_algorithmic": gbs, "roofline_frac": gbs / HBM_PEAK_GBS, "nonfinite": int(bad),
-           "kernel": "euler_step_k"}
+           "kernel": "euler_march_k" if shape["variant"] == 1 else "euler_step_k",
+           "kernel_shape": {k: shape[k] for k in ("variant", "tile_rows", "chunk")}}
     m.close()
     if cpu:
         from oracle import pyoracle
@@ -357,7 +357,8 @@ def main(argv=None):
         suite = {}
         for name, fn in (("heat64_f64", lambda: suite_heat(ctx, cpu=not args.no_cpu)),
                          ("wave256_f64", lambda: suite_wave_f64(ctx)),
-                         ("sod512x64x64_f32", lambda: suite_sod(ctx, cpu=not args.no_cpu)),
+                         ("sod512x64x64_f32", lambda: suite_sod(ctx, cpu=not args.no_cpu,
+                                                                verbose=args.tune_verbose)),
                          ("ensemble64_f32", lambda: suite_ensemble(ctx)),
                          ("adjoint128_f32", lambda: suite_adjoint(ctx))):
             print(f"[bench] {name}", file=sys.stderr, flush=True)

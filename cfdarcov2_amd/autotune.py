@@ -18,6 +18,8 @@ DEFAULT_CANDIDATES = [(0, 16, 32, 1)] + [
     (1, ty, L, r) for ty, L, r in itertools.product((4, 8, 16, 32), (4, 8, 16, 32, 64), (0, 1))
 ] + [
     (2, ty, L, r) for ty, L, r in itertools.product((8, 16), (4, 8, 16, 32, 64), (0, 1))
+] + [
+    (3, ty, L, r) for ty, L, r in itertools.product((2, 4, 8), (4, 8, 16, 32, 64), (0, 1))
 ]
 
 
@@ -83,5 +85,41 @@ def tune_wave(mesh, steps: int = 20, candidates=None, use_cache: bool = True, ve
     if use_cache:
         cache[key] = best
         _store_cache(cache)
+    del A, B
+    return dict(best, table=table)
+
+
+EULER_CANDIDATES = [(0, 4, 16)] + [(1, ty, L) for ty in (2, 4, 8) for L in (8, 16, 32, 64)]
+
+
+def tune_euler(mesh, U, gamma=1.4, cfl=0.3, steps=10, candidates=None, verbose=False) -> dict:
+    """Pick the fastest Rusanov-step shape for `mesh` from runs on a copy of
+    the state U (list of five device fields); the caller's fields are not
+    touched.  Returns the best shape and the table."""
+    import torch
+
+    cands = candidates or EULER_CANDIDATES
+    A = [u.clone() for u in U]
+    B = [torch.empty_like(u) for u in U]
+    stream = torch.cuda.current_stream()
+    table = []
+    for variant, ty, L in cands:
+        mesh.set_euler_shape(variant, ty, L)
+        for a, u in zip(A, U):
+            a.copy_(u)
+        mesh.euler_run(A, B, gamma, cfl, 2)
+        for a, u in zip(A, U):
+            a.copy_(u)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        mesh.euler_run(A, B, gamma, cfl, steps)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        table.append({"variant": variant, "tile_rows": ty, "chunk": L, "ms_per_step": ms})
+        if verbose:
+            print(f"  euler shape variant={variant} ty={ty} L={L}: {ms * 1e3:.1f} us/step", flush=True)
+    best = min(table, key=lambda r: r["ms_per_step"])
+    mesh.set_euler_shape(best["variant"], best["tile_rows"], best["chunk"])
     del A, B
     return dict(best, table=table)

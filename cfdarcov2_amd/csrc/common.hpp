@@ -129,6 +129,11 @@ struct cfd_mesh_s {
     int wave_ty = 16;
     int wave_L = 32;
     int wave_remap = 1;
+    // Euler stepper: variant 0 = one thread per cell, 1 = 2.5D march with
+    // `euler_ty` rows x 64 columns x `euler_L` planes per workgroup
+    int euler_variant = 1;
+    int euler_ty = 4;
+    int euler_L = 16;
     // Euler stepper geometry (device block allocated on first use)
     cfd::FvGeo<float> fv32{};
     cfd::FvGeo<double> fv64{};

@@ -93,16 +93,18 @@ __device__ inline T local_dt(const FvGeo<T>& g, int64_t x, int64_t y, int64_t z,
     return g.vol[c] / s;
 }
 
-// minimum over the 256-thread block, folded into *slot by thread (0, 0)
+// minimum over the block (64 x blockDim.y threads, one wave per y row), folded
+// into *slot by thread (0, 0) with one atomic per block
 template <class T> __device__ inline void block_min_into(T v, Bits<T>* slot) {
-    __shared__ T red[4];
+    __shared__ T red[16];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
     const int lane = threadIdx.x, wave = threadIdx.y;
     if (lane == 0) red[wave] = v;
     __syncthreads();
     if (lane == 0 && wave == 0) {
-        const T m = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
+        T m = red[0];
+        for (int w = 1; w < int(blockDim.y); ++w) m = fmin(m, red[w]);
         if (m < std::numeric_limits<T>::infinity()) atomicMin(slot, to_bits(m));
     }
 }
@@ -190,6 +192,181 @@ __global__ void __launch_bounds__(256) euler_step_k(FvGeo<T> g, EIn<T> in, EOut<
     block_min_into(loc, dt_next);
 }
 
+// The same step as a 2.5D march: a workgroup owns TY rows x 64 z-columns and
+// walks `L` x-planes.  Each cell's state and primitives go to LDS once per
+// plane (with a one-cell y/z halo loaded by edge threads), the y/z face fluxes
+// read the neighbours from there, and the x-face flux between planes x and x+1
+// is carried in registers to the next plane, so each cell converts its state
+// to primitives once instead of seven times.  Every face flux is evaluated with
+// the lower cell as L, exactly as in euler_step_k, so both give the same bits.
+template <class T> __device__ inline void mirror(const T U[5], const Prim<T>& q, int axis, T Um[5], Prim<T>& qm) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Um[k] = U[k];
+    Um[1 + axis] = -U[1 + axis];  // axis 0 x, 1 y, 2 z
+    qm = q;
+    if (axis == 0) qm.ux = -q.ux;
+    else if (axis == 1) qm.uy = -q.uy;
+    else qm.uz = -q.uz;
+}
+
+template <class T, int TY>
+__global__ void __launch_bounds__(64 * TY) euler_march_k(FvGeo<T> g, EIn<T> in, EOut<T> out, T gamma, T cfl,
+                                                        const Bits<T>* dt_cur, Bits<T>* dt_next, Bits<T>* dt_reset,
+                                                        T* dt_hist, int64_t step, int L, int nzt, int nyt) {
+    constexpr int TZ = 64;
+    constexpr int PZ = TZ + 2;
+    constexpr int PS = (TY + 2) * PZ;
+    __shared__ T sh[2][10][PS];  // U0..U4, ux, uy, uz, p, a per slot, double-buffered
+
+    const int64_t X = g.X, Y = g.Y, Z = g.Z, SX = Y * Z;
+    const int64_t lin = blockIdx.x;
+    const int64_t zt = lin % nzt;
+    const int64_t yt = (lin / nzt) % nyt;
+    const int64_t xc = lin / (int64_t(nzt) * nyt);
+    const int lz = threadIdx.x, ly = threadIdx.y;
+    const int64_t z = zt * TZ + lz, y = yt * TY + ly;
+    const bool valid = z < Z && y < Y;
+    const int64_t col = valid ? y * Z + z : 0;
+    const int64_t x0 = xc * L;
+    const int64_t x1 = (x0 + L < X) ? x0 + L : X;
+    const T dt = cfl * from_bits(*dt_cur);
+    if (lin == 0 && lz == 0 && ly == 0) {
+        *dt_reset = inf_bits<T>();
+        if (dt_hist) dt_hist[step] = dt;
+    }
+    const int own = (ly + 1) * PZ + (lz + 1);
+    // halo cells: row -1 (ly 0) / row TY (ly 1) at this column, column -1
+    // (lane 0) / column 64 (lane 63) at this row; -1 where outside the domain
+    int64_t hy_col = -1, hz_col = -1;
+    int hy_slot = 0, hz_slot = 0;
+    if (ly < 2 && z < Z) {
+        const int64_t yy = ly == 0 ? yt * TY - 1 : yt * TY + TY;
+        if (yy >= 0 && yy < Y) {
+            hy_col = yy * Z + z;
+            hy_slot = (ly == 0 ? 0 : TY + 1) * PZ + (lz + 1);
+        }
+    }
+    if ((lz == 0 || lz == TZ - 1) && y < Y) {
+        const int64_t zz = lz == 0 ? zt * TZ - 1 : zt * TZ + TZ;
+        if (zz >= 0 && zz < Z) {
+            hz_col = y * Z + zz;
+            hz_slot = (ly + 1) * PZ + (lz == 0 ? 0 : TZ + 1);
+        }
+    }
+    auto load = [&](int64_t o, T U[5]) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) U[k] = in.u[k][o];
+    };
+    auto put = [&](T* const* S, int slot, const T U[5], const Prim<T>& q) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) S[k][slot] = U[k];
+        S[5][slot] = q.ux; S[6][slot] = q.uy; S[7][slot] = q.uz; S[8][slot] = q.p; S[9][slot] = q.a;
+    };
+    auto get = [&](T* const* S, int slot, T U[5], Prim<T>& q) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) U[k] = S[k][slot];
+        q.ux = S[5][slot]; q.uy = S[6][slot]; q.uz = S[7][slot]; q.p = S[8][slot]; q.a = S[9][slot];
+    };
+    auto face = [&](int fam, int64_t f, T& nx, T& ny, T& nz, T& A) {
+        nx = g.n[fam][0][f]; ny = g.n[fam][1][f]; nz = g.n[fam][2][f]; A = g.A[fam][f];
+    };
+
+    T Uc[5] = {}, Un[5] = {}, Gxl[5] = {};
+    Prim<T> qc{}, qn{};
+    if (valid) {  // warm-up: state of plane x0 and the flux through its lower x-face
+        load(x0 * SX + col, Uc);
+        qc = prim(gamma, Uc);
+        T nx, ny, nz, A;
+        face(2, x0 * SX + col, nx, ny, nz, A);
+        T Ul[5];
+        Prim<T> ql;
+        if (x0 == 0) mirror(Uc, qc, 0, Ul, ql);
+        else {
+            load((x0 - 1) * SX + col, Ul);
+            ql = prim(gamma, Ul);
+        }
+        rusanov(Ul, ql, Uc, qc, nx, ny, nz, A, Gxl);
+    }
+    T loc = std::numeric_limits<T>::infinity();
+    int b = 0;
+#pragma unroll 1
+    for (int64_t x = x0; x < x1; ++x, b ^= 1) {
+        T* S[10];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) S[k] = sh[b][k];
+        if (valid && x + 1 < X) {
+            load((x + 1) * SX + col, Un);
+            qn = prim(gamma, Un);
+        }
+        if (hy_col >= 0) {
+            T Uh[5];
+            load(x * SX + hy_col, Uh);
+            put(S, hy_slot, Uh, prim(gamma, Uh));
+        }
+        if (hz_col >= 0) {
+            T Uh[5];
+            load(x * SX + hz_col, Uh);
+            put(S, hz_slot, Uh, prim(gamma, Uh));
+        }
+        if (valid) put(S, own, Uc, qc);
+        __syncthreads();
+        if (valid) {
+            T nx, ny, nz, A, Um[5];
+            Prim<T> qm;
+            // net = ((Gx+ - Gx-) + (Gy+ - Gy-)) + (Gz+ - Gz-), accumulated one
+            // axis at a time; the upper x flux becomes the next plane's lower
+            T acc[5], G1[5], G2[5];
+            face(2, (x + 1) * SX + col, nx, ny, nz, A);
+            if (x == X - 1) {
+                mirror(Uc, qc, 0, Um, qm);
+                rusanov(Uc, qc, Um, qm, nx, ny, nz, A, G1);
+            } else {
+                rusanov(Uc, qc, Un, qn, nx, ny, nz, A, G1);
+            }
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                acc[k] = G1[k] - Gxl[k];
+                Gxl[k] = G1[k];
+            }
+            // y faces
+            face(1, x * (Y + 1) * Z + y * Z + z, nx, ny, nz, A);
+            if (y == 0) mirror(Uc, qc, 1, Um, qm);
+            else get(S, own - PZ, Um, qm);
+            rusanov(Um, qm, Uc, qc, nx, ny, nz, A, G1);
+            face(1, x * (Y + 1) * Z + (y + 1) * Z + z, nx, ny, nz, A);
+            if (y == Y - 1) mirror(Uc, qc, 1, Um, qm);
+            else get(S, own + PZ, Um, qm);
+            rusanov(Uc, qc, Um, qm, nx, ny, nz, A, G2);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) acc[k] = acc[k] + (G2[k] - G1[k]);
+            // z faces
+            const int64_t fz = x * Y * (Z + 1) + y * (Z + 1) + z;
+            face(0, fz, nx, ny, nz, A);
+            if (z == 0) mirror(Uc, qc, 2, Um, qm);
+            else get(S, own - 1, Um, qm);
+            rusanov(Um, qm, Uc, qc, nx, ny, nz, A, G1);
+            face(0, fz + 1, nx, ny, nz, A);
+            if (z == Z - 1) mirror(Uc, qc, 2, Um, qm);
+            else get(S, own + 1, Um, qm);
+            rusanov(Uc, qc, Um, qm, nx, ny, nz, A, G2);
+            const int64_t c = x * SX + col;
+            const T V = g.vol[c];
+            T Unew[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const T net = acc[k] + (G2[k] - G1[k]);
+                Unew[k] = Uc[k] - (dt * net) / V;
+                out.u[k][c] = Unew[k];
+            }
+            loc = fmin(loc, local_dt(g, x, y, z, c, prim(gamma, Unew)));
+#pragma unroll
+            for (int k = 0; k < 5; ++k) Uc[k] = Un[k];
+            qc = qn;
+        }
+    }
+    block_min_into(loc, dt_next);
+}
+
 template <class B> __global__ void fill_bits_k(B* p, int n, B v) {
     if (int(threadIdx.x) < n) p[threadIdx.x] = v;
 }
@@ -242,6 +419,10 @@ void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamm
         for (int k = 0; k < 5; ++k) in.u[k] = (const T*)a[k];
         euler_dt_k<T><<<grid, block, 0, s>>>(g, in, T(gamma), slots);
         CFD_HIP(hipGetLastError());
+        const int ty = m->euler_ty, L = m->euler_L;
+        const int nzt = int((m->Z + 63) / 64), nyt = int((m->Y + ty - 1) / ty);
+        const int64_t nwg = int64_t(nzt) * nyt * ((m->X + L - 1) / L);
+        CFD_REQUIRE(nwg < (int64_t(1) << 31), "euler march grid too large");
         for (int64_t st = 0; st < nsteps; ++st) {
             void* const* src = (st & 1) ? b : a;
             void* const* dst = (st & 1) ? a : b;
@@ -252,7 +433,27 @@ void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamm
             Bits<T>* cur = slots + (st % 3);
             Bits<T>* nxt = slots + ((st + 1) % 3);
             Bits<T>* rst = slots + ((st + 2) % 3);
-            euler_step_k<T><<<grid, block, 0, s>>>(g, in, out, T(gamma), T(cfl), cur, nxt, rst, (T*)dt_hist, st);
+            if (m->euler_variant == 0) {
+                euler_step_k<T><<<grid, block, 0, s>>>(g, in, out, T(gamma), T(cfl), cur, nxt, rst, (T*)dt_hist,
+                                                       st);
+                continue;
+            }
+            const dim3 mgrid{unsigned(nwg), 1u, 1u};
+            switch (ty) {
+                case 2:
+                    euler_march_k<T, 2><<<mgrid, dim3(64, 2), 0, s>>>(g, in, out, T(gamma), T(cfl), cur, nxt, rst,
+                                                                      (T*)dt_hist, st, L, nzt, nyt);
+                    break;
+                case 4:
+                    euler_march_k<T, 4><<<mgrid, dim3(64, 4), 0, s>>>(g, in, out, T(gamma), T(cfl), cur, nxt, rst,
+                                                                      (T*)dt_hist, st, L, nzt, nyt);
+                    break;
+                case 8:
+                    euler_march_k<T, 8><<<mgrid, dim3(64, 8), 0, s>>>(g, in, out, T(gamma), T(cfl), cur, nxt, rst,
+                                                                      (T*)dt_hist, st, L, nzt, nyt);
+                    break;
+                default: throw Error(CFD_E_INVALID, "euler march tile rows must be 2, 4 or 8");
+            }
         }
         CFD_HIP(hipGetLastError());
     } catch (...) {

@@ -652,12 +652,27 @@ int cfd_wave_adjoint(cfd_mesh_t m, void* const adj_a[4], void* const adj_b[4], d
     });
 }
 
+int cfd_mesh_set_euler_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk) {
+    return guarded([&] {
+        CFD_REQUIRE(m != nullptr, "null mesh");
+        CFD_REQUIRE(variant == 0 || variant == 1, "euler variant must be 0 or 1");
+        CFD_REQUIRE(tile_rows == 2 || tile_rows == 4 || tile_rows == 8, "euler tile_rows must be 2, 4 or 8");
+        CFD_REQUIRE(chunk >= 1 && chunk <= 65535, "euler chunk must be in [1, 65535]");
+        m->euler_variant = variant;
+        m->euler_ty = tile_rows;
+        m->euler_L = chunk;
+    });
+}
+
 int cfd_mesh_set_wave_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk, int xcd_remap) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");
-        CFD_REQUIRE(variant >= 0 && variant <= 2, "wave variant must be 0, 1 or 2");
-        CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || tile_rows == 16 || (tile_rows == 32 && variant != 2),
-                    "tile_rows must be 4, 8, 16 or 32 (4, 8 or 16 for variant 2)");
+        CFD_REQUIRE(variant >= 0 && variant <= 3, "wave variant must be 0, 1, 2 or 3");
+        if (variant == 3)
+            CFD_REQUIRE(tile_rows == 2 || tile_rows == 4 || tile_rows == 8, "variant 3 tile_rows must be 2, 4 or 8");
+        else
+            CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || tile_rows == 16 || (tile_rows == 32 && variant != 2),
+                        "tile_rows must be 4, 8, 16 or 32 (4, 8 or 16 for variant 2)");
         CFD_REQUIRE(chunk >= 1 && chunk <= 65535, "chunk must be in [1, 65535]");
         m->wave_variant = variant;
         m->wave_ty = tile_rows;

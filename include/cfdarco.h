@@ -207,6 +207,12 @@ int cfd_wave_run(cfd_mesh_t mesh, void* const state_a[4], void* const state_b[4]
  * long, mesh dtype) receives each step's dt.  Walls are reflective. */
 int cfd_euler_run(cfd_mesh_t mesh, void* const state_a[5], void* const state_b[5], double gamma, double cfl,
                   int64_t nsteps, void* dt_hist, void* stream);
+/* Kernel shape of cfd_euler_run: variant 0 = one thread per cell (every face
+ * flux evaluated by both of its cells), 1 = 2.5D march (default) with
+ * tile_rows (2, 4 or 8) y-rows x 64 z-cells per workgroup and `chunk`
+ * x-planes, each cell's primitives computed once per plane and shared through
+ * LDS.  Both give bit-identical results. */
+int cfd_mesh_set_euler_shape(cfd_mesh_t mesh, int variant, int tile_rows, int chunk);
 
 /* ---- adjoint of the fused acoustic step (uniform meshes; no reference
  * counterpart) ---------------------------------------------------------------
@@ -293,8 +299,10 @@ int cfd_mesh_geometry(cfd_mesh_t mesh, cfd_geometry* out);
  * src/fvm3d.cpp:1306-1308): variant 0 = one thread per cell, variant 1 = 2.5D
  * marching kernel with `tile_rows` (4, 8, 16 or 32) y-rows of 64 z-cells per
  * workgroup carried by 4 (8 for 32) thread rows, variant 2 = the same kernel
- * with one thread row per tile row (tile_rows 4, 8 or 16), `chunk` x-planes
- * per workgroup and XCD-aware tile placement when `xcd_remap` = 1.  All
+ * with one thread row per tile row (tile_rows 4, 8 or 16), variant 3 = a
+ * register form without LDS or barriers in which each wave owns tile_rows (2,
+ * 4 or 8) consecutive rows x 64 z-cells, `chunk` x-planes per workgroup (per
+ * wave for variant 3) and XCD-aware tile placement when `xcd_remap` = 1.  All
  * variants give bit-identical results.  Set by the autotuner
  * (cfdarcov2_amd/autotune.py). */
 int cfd_mesh_set_wave_shape(cfd_mesh_t mesh, int variant, int tile_rows, int chunk, int xcd_remap);

@@ -17,9 +17,11 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-def _gpu_run(hip_ctx, n, d, off, dtype, U0, gamma, cfl, steps):
+def _gpu_run(hip_ctx, n, d, off, dtype, U0, gamma, cfl, steps, shape=None):
     from cfdarcov2_amd import capi
     gm = capi.HexMesh(hip_ctx, *n, *d, dtype=dtype, vertex_offsets=off)
+    if shape is not None:
+        gm.set_euler_shape(*shape)
     A = [torch.from_numpy(np.ascontiguousarray(U0[k], dtype=dtype)).cuda() for k in range(5)]
     B = [torch.empty_like(a) for a in A]
     hist = torch.empty(max(steps, 1), dtype=A[0].dtype, device="cuda")
@@ -58,6 +60,29 @@ def test_euler_matches_oracle(hip_ctx, oracle_lib, case, dtype):
     assert np.array_equal(got_dt, ref_dt), np.max(np.abs(got_dt - ref_dt))
     for k in range(5):
         assert np.array_equal(got[k], ref[k]), (k, np.max(np.abs(got[k] - ref[k])))
+
+
+SHAPES = [(0, 4, 16), (1, 2, 5), (1, 4, 16), (1, 8, 3), (1, 4, 64)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"v{a}-ty{b}-L{c}" for a, b, c in SHAPES])
+def test_euler_kernel_shapes(hip_ctx, oracle_lib, shape, dtype):
+    """Per-thread and 2.5D-march forms, partial tiles in y and z and chunks
+    that do not divide x: the oracle's bits, fields and time steps."""
+    n, steps = (23, 9, 70), 9
+    print(f"\n[case] euler shape {shape} {np.dtype(dtype).name}", flush=True)
+    d = (1.0 / 23,) * 3
+    off = workloads.perturbed_vertex_offsets(*n, *d, seed=4, frac=0.2)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=dtype, perturb=off)
+    U0 = workloads.sod_state(om.table("ccx"), dtype=dtype)
+    rng = np.random.default_rng(6)
+    U0[1:4] = (0.05 * rng.standard_normal((3, om.N)) * U0[0]).astype(dtype)
+    ref, ref_dt = om.euler_run(U0, 1.4, 0.3, steps)
+    got, got_dt = _gpu_run(hip_ctx, n, d, off, dtype, U0, 1.4, 0.3, steps, shape)
+    assert np.array_equal(got_dt, ref_dt)
+    for k in range(5):
+        assert np.array_equal(got[k], ref[k]), k
 
 
 def test_fp32_error_against_fp64_oracle(hip_ctx, oracle_lib):
