@@ -82,6 +82,27 @@ def _timed(fn):
     return time.perf_counter() - t0, e0.elapsed_time(e1)
 
 
+def copy_ceiling(nbytes=256 << 20, reps=20):
+    """Device-to-device copy rate (GB/s, read + write bytes) of one buffer of
+    `nbytes`: the practical streaming ceiling the stencil is compared with."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    stream = torch.cuda.current_stream()
+    e0, e1 = _events()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3 / reps
+    del a, b
+    return 2 * nbytes / t / 1e9
+
+
 def suite_heat(ctx, cpu=True):
     """Heat 64^3 FP64, 200 steps at dt = 0.2 dx^2/alpha (the reference CPU
     configuration); 16 algorithmic bytes per cell-update; 4 MiB of state stays
@@ -267,6 +288,9 @@ def main(argv=None):
     ap.add_argument("--shape", default="auto",
                     help="fused-kernel shape 'variant,tile_rows,chunk,xcd_remap' or 'auto' (autotuner)")
     ap.add_argument("--tune-verbose", action="store_true")
+    ap.add_argument("--only", choices=["heat64_f64", "wave256_f64", "sod512x64x64_f32", "ensemble64_f32",
+                                       "adjoint128_f32"],
+                    help="run just one secondary workload and print its record (for profiling)")
     ap.add_argument("--no-suite", action="store_true",
                     help="skip the secondary workloads (heat FP64, wave FP64, Sod, ensemble, adjoint)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -284,6 +308,16 @@ def main(argv=None):
 
     if args.calibrate_copy:
         calibrate_copy(args)
+        return
+
+    if args.only:
+        ctx = capi.Context(0)
+        fns = {"heat64_f64": lambda: suite_heat(ctx, cpu=False), "wave256_f64": lambda: suite_wave_f64(ctx),
+               "sod512x64x64_f32": lambda: suite_sod(ctx, cpu=False, verbose=args.tune_verbose),
+               "ensemble64_f32": lambda: suite_ensemble(ctx), "adjoint128_f32": lambda: suite_adjoint(ctx)}
+        rec = fns[args.only]()
+        ctx.close()
+        print(json.dumps({"workload_only": args.only, **rec}), flush=True)
         return
 
     case = workloads.wave_config(n=args.n, steps=args.steps)
@@ -353,6 +387,9 @@ def main(argv=None):
     mesh.close()
     del A, B, cur, nxt, res
     torch.cuda.empty_cache()
+    ceil = copy_ceiling()
+    out["roofline"]["copy_ceiling_gbs"] = ceil
+    out["roofline"]["frac_of_copy_ceiling"] = achieved / ceil
     if not args.no_suite:
         suite = {}
         for name, fn in (("heat64_f64", lambda: suite_heat(ctx, cpu=not args.no_cpu)),

@@ -20,6 +20,8 @@ DEFAULT_CANDIDATES = [(0, 16, 32, 1)] + [
     (2, ty, L, r) for ty, L, r in itertools.product((8, 16), (4, 8, 16, 32, 64), (0, 1))
 ] + [
     (3, ty, L, r) for ty, L, r in itertools.product((2, 4, 8), (4, 8, 16, 32, 64), (0, 1))
+] + [
+    (4, ty, L, r) for ty, L, r in itertools.product((4, 8), (4, 8, 16, 32), (0, 1))
 ]
 
 

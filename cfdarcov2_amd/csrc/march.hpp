@@ -52,7 +52,9 @@ template <class T, int TYT, int R> struct MarchLds {
 // One (zt, yt, xc) tile of the fused step: columns [yt*TY, +TY) x [zt*64, +64),
 // planes [xc*L, +L).  Called by the single-mesh kernel below and by the
 // batched ensemble kernel.
-template <class T, int TYT, int R>
+// NT: stream the outputs past the L2 (non-temporal stores), leaving its
+// capacity to the halo rows that neighbouring tiles read again.
+template <class T, int TYT, int R, bool NT = false>
 __device__ inline void wave_march_tile(const UniformGeo<T>& g, const T* __restrict__ p, const T* __restrict__ u,
                                        const T* __restrict__ v, const T* __restrict__ w, T* __restrict__ po,
                                        T* __restrict__ uo, T* __restrict__ vo, T* __restrict__ wo, T K,
@@ -226,10 +228,18 @@ __device__ inline void wave_march_tile(const UniformGeo<T>& g, const T* __restri
                 S = S + mggp(fz_lo, fz_hi, Az, V);
                 const int64_t o = x * SX + col[r];
                 const T k = Kf ? Kf[o] : K;
-                po[o] = dt * (k * S) + P0[r];
-                uo[o] = U0[r];
-                vo[o] = vnew[r];
-                wo[o] = wnew[r];
+                const T pn = dt * (k * S) + P0[r];
+                if constexpr (NT) {
+                    __builtin_nontemporal_store(pn, &po[o]);
+                    __builtin_nontemporal_store(U0[r], &uo[o]);
+                    __builtin_nontemporal_store(vnew[r], &vo[o]);
+                    __builtin_nontemporal_store(wnew[r], &wo[o]);
+                } else {
+                    po[o] = pn;
+                    uo[o] = U0[r];
+                    vo[o] = vnew[r];
+                    wo[o] = wnew[r];
+                }
             }
             Pm1[r] = P0[r];
             P0[r] = Pp1[r];

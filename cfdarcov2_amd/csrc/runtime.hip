@@ -667,8 +667,10 @@ int cfd_mesh_set_euler_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk
 int cfd_mesh_set_wave_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk, int xcd_remap) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");
-        CFD_REQUIRE(variant >= 0 && variant <= 3, "wave variant must be 0, 1, 2 or 3");
-        if (variant == 3)
+        CFD_REQUIRE(variant >= 0 && variant <= 4, "wave variant must be 0, 1, 2, 3 or 4");
+        if (variant == 4)
+            CFD_REQUIRE(tile_rows == 4 || tile_rows == 8, "variant 4 tile_rows must be 4 or 8");
+        else if (variant == 3)
             CFD_REQUIRE(tile_rows == 2 || tile_rows == 4 || tile_rows == 8, "variant 3 tile_rows must be 2, 4 or 8");
         else
             CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || tile_rows == 16 || (tile_rows == 32 && variant != 2),

@@ -23,7 +23,7 @@
 
 namespace cfd {
 
-template <class T, int TYT, int R>
+template <class T, int TYT, int R, bool NT = false>
 __global__ void __launch_bounds__(64 * TYT) wave_march_k(UniformGeo<T> g, const T* __restrict__ p,
                                                          const T* __restrict__ u, const T* __restrict__ v,
                                                          const T* __restrict__ w, T* __restrict__ po,
@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(64 * TYT) wave_march_k(UniformGeo<T> g, const 
     const int64_t zt = lin_id % nzt;
     const int64_t yt = (lin_id / nzt) % nyt;
     const int64_t xc = lin_id / (int64_t(nzt) * nyt);
-    wave_march_tile<T, TYT, R>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, L, zt, yt, xc, lds);
+    wave_march_tile<T, TYT, R, NT>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, L, zt, yt, xc, lds);
 }
 
 // Register form (variant 3): every wave is independent and owns R consecutive
@@ -57,92 +57,110 @@ __global__ void __launch_bounds__(256) wave_reg_k(UniformGeo<T> g, const T* __re
     const int lane = threadIdx.x;
     int64_t blk = blockIdx.x;
     if (remap) blk = (blk % 8) * (int64_t(gridDim.x) / 8) + blk / 8;
-    const int64_t gw = blk * 4 + threadIdx.y;
+    // the wave index is uniform across the wave: keep it (and everything derived
+    // from it: tile coordinates, plane base pointers) in scalar registers
+    const int64_t gw = __builtin_amdgcn_readfirstlane(int(blk * 4 + threadIdx.y));
     if (gw >= nwaves) return;  // whole wave; waves never synchronise
-    const int64_t yb = gw % nyb;
-    const int64_t zt = (gw / nyb) % nzt;
+    const int yb = int(gw % nyb);
+    const int zt = int((gw / nyb) % nzt);
     const int64_t xc = gw / (int64_t(nyb) * nzt);
-    const int64_t X = g.X, Y = g.Y, Z = g.Z, SX = Y * Z;
+    const int64_t X = g.X, SX = g.Y * g.Z;
+    const int Y = int(g.Y), Z = int(g.Z);  // a plane holds fewer than 2^31 cells
     const T Ax = g.area[0], Ay = g.area[1], Az = g.area[2], V = g.vol;
-    const int64_t z = zt * 64 + lane;
+    const int z = zt * 64 + lane;
     const bool zval = z < Z;
-    const int64_t zc = zval ? z : Z - 1;  // in-bounds column for the idle lanes' loads
-    const int64_t y0 = yb * R;
+    const int zc = zval ? z : Z - 1;  // in-bounds column for the idle lanes' loads
+    const int y0 = yb * R;
     const int64_t x0 = xc * L;
     const int64_t x1 = (x0 + L < X) ? x0 + L : X;
     // z-halo lanes: lane 0 reads column z0-1 (and z0-2), lane 63 column z0+64 (and z0+65)
     const bool hlo = lane == 0 && zt * 64 - 1 >= 0;
     const bool hhi = lane == 63 && zt * 64 + 64 < Z;
-    const int64_t hz1 = hlo ? zt * 64 - 1 : (hhi ? zt * 64 + 64 : zc);              // adjacent halo column
-    const int64_t hz2 = hlo ? zt * 64 - 2 : (hhi ? (zt * 64 + 65 < Z ? zt * 64 + 65 : zt * 64 + 64) : zc);
-    auto rowy = [&](int64_t yy) { return yy < 0 ? int64_t(0) : (yy >= Y ? Y - 1 : yy); };
-
-    T Pm1[R], P0[R], Pp1[R], Um1[R], U0[R];
-    // warm-up: p(x0-1..x0+1), u'(x0-1), u'(x0)
+    const int hz1 = hlo ? zt * 64 - 1 : (hhi ? zt * 64 + 64 : zc);
+    const int hz2 = hlo ? zt * 64 - 2 : (hhi ? (zt * 64 + 65 < Z ? zt * 64 + 65 : zt * 64 + 64) : zc);
+    auto rowy = [&](int yy) { return yy < 0 ? 0 : (yy >= Y ? Y - 1 : yy); };
+    // in-plane offsets: own rows, halo p rows (-2, -1, R, R+1), v rows (-1..R)
+    unsigned col[R], hcol1[R], hcol2[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int64_t col = rowy(y0 + r) * Z + zc;
-        P0[r] = p[x0 * SX + col];
-        Pm1[r] = x0 >= 1 ? p[(x0 - 1) * SX + col] : P0[r];
-        Pp1[r] = x0 + 1 < X ? p[(x0 + 1) * SX + col] : P0[r];
-        Um1[r] = T(0);
-        if (x0 >= 1) {
-            const T Pm2 = x0 >= 2 ? p[(x0 - 2) * SX + col] : Pm1[r];
-            Um1[r] = mvel(Pm2, Pm1[r], P0[r], u[(x0 - 1) * SX + col], Ax, V, dt);
+        col[r] = unsigned(rowy(y0 + r) * Z + zc);
+        hcol1[r] = unsigned(rowy(y0 + r) * Z + hz1);
+        hcol2[r] = unsigned(rowy(y0 + r) * Z + hz2);
+    }
+    unsigned hrow[4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) hrow[h] = unsigned(rowy(h < 2 ? y0 - 2 + h : y0 + R + (h - 2)) * Z + zc);
+    const unsigned vlo = unsigned(rowy(y0 - 1) * Z + zc), vhi = unsigned(rowy(y0 + R) * Z + zc);
+
+    T P0[R], Pp1[R], Um1[R], U0[R];
+    {  // warm-up: p(x0-1..x0+1), u'(x0-1), u'(x0)
+        const T* p0 = p + x0 * SX;
+        const T* pm1 = x0 >= 1 ? p0 - SX : p0;
+        const T* pm2 = x0 >= 2 ? p0 - 2 * SX : pm1;
+        const T* pp1 = x0 + 1 < X ? p0 + SX : p0;
+        const T* u0 = u + x0 * SX;
+        const T* um1 = x0 >= 1 ? u0 - SX : u0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            P0[r] = p0[col[r]];
+            const T Pm1 = pm1[col[r]];
+            Pp1[r] = pp1[col[r]];
+            Um1[r] = T(0);
+            if (x0 >= 1) Um1[r] = mvel(pm2[col[r]], Pm1, P0[r], um1[col[r]], Ax, V, dt);
+            U0[r] = mvel(Pm1, P0[r], Pp1[r], u0[col[r]], Ax, V, dt);
         }
-        U0[r] = mvel(x0 >= 1 ? Pm1[r] : P0[r], P0[r], x0 + 1 < X ? Pp1[r] : P0[r], u[x0 * SX + col], Ax, V, dt);
     }
 #pragma unroll 1
     for (int64_t x = x0; x < x1; ++x) {
         const int64_t base = x * SX;
+        const T* px = p + base;
+        const T* vx = v + base;
+        const T* wx = w + base;
+        const T* ux1 = u + (x + 1 < X ? base + SX : base);
+        const T* px2 = p + (x + 2 < X ? base + 2 * SX : base);
         // p rows y0-2, y0-1 and y0+R, y0+R+1 of plane x (walls: the row itself)
-        T ph[4];
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            const int64_t yy = h < 2 ? y0 - 2 + h : y0 + R + (h - 2);
-            ph[h] = p[base + rowy(yy) * Z + zc];
-        }
-        // v rows y0-1 .. y0+R
-        T vr[R + 2];
-#pragma unroll
-        for (int r = 0; r < R + 2; ++r) vr[r] = v[base + rowy(y0 - 1 + r) * Z + zc];
-        T Up1[R], wv[R], whl[R], whh[R], pz1[R], pz2[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int64_t col = rowy(y0 + r) * Z + zc;
-            const T Pp2 = x + 2 < X ? p[base + 2 * SX + col] : Pp1[r];
-            Up1[r] = x + 1 < X ? mvel(P0[r], Pp1[r], Pp2, u[base + SX + col], Ax, V, dt) : T(0);
-            wv[r] = w[base + col];
-            pz1[r] = pz2[r] = whl[r] = T(0);
-            if (hlo || hhi) {
-                const int64_t hcol1 = rowy(y0 + r) * Z + hz1, hcol2 = rowy(y0 + r) * Z + hz2;
-                pz1[r] = p[base + hcol1];
-                pz2[r] = p[base + hcol2];
-                whl[r] = w[base + hcol1];
-            }
-            Pm1[r] = Pp2;  // p(x+2), shifted into Pp1 after this plane (Pm1 is not read in the loop)
-        }
-        // pressures of rows -2..R+1 at plane x
         T pr[R + 4];
-        pr[0] = ph[0];
-        pr[1] = ph[1];
+        pr[0] = px[hrow[0]];
+        pr[1] = px[hrow[1]];
+        pr[R + 2] = px[hrow[2]];
+        pr[R + 3] = px[hrow[3]];
 #pragma unroll
         for (int r = 0; r < R; ++r) pr[2 + r] = P0[r];
-        pr[R + 2] = ph[2];
-        pr[R + 3] = ph[3];
-        // v' on rows -1..R (only where the face to the tile is not a wall)
+        // v rows y0-1 .. y0+R
+        T vr[R + 2];
+        vr[0] = vx[vlo];
+        vr[R + 1] = vx[vhi];
+#pragma unroll
+        for (int r = 0; r < R; ++r) vr[r + 1] = vx[col[r]];
+        T Up1[R], Pp2[R], wv[R], whl[R], pz1[R], pz2[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            Pp2[r] = x + 2 < X ? px2[col[r]] : Pp1[r];
+            Up1[r] = x + 1 < X ? mvel(P0[r], Pp1[r], Pp2[r], ux1[col[r]], Ax, V, dt) : T(0);
+            wv[r] = wx[col[r]];
+            pz1[r] = pz2[r] = whl[r] = T(0);
+            if (hlo || hhi) {
+                pz1[r] = px[hcol1[r]];
+                pz2[r] = px[hcol2[r]];
+                whl[r] = wx[hcol1[r]];
+            }
+        }
+        // v' on rows -1..R (only read where the face to the tile is not a wall)
         T vn[R + 2];
 #pragma unroll
         for (int r = 0; r < R + 2; ++r) {
-            const int64_t yy = y0 - 1 + r;
+            const int yy = y0 - 1 + r;
             const T pl = (yy - 1 >= 0) ? pr[r] : pr[r + 1];
             const T ph_ = (yy + 1 < Y) ? pr[r + 2] : pr[r + 1];
             vn[r] = mvel(pl, pr[r + 1], ph_, vr[r], Ay, V, dt);
         }
-        // w' on the own rows, neighbours in z from the adjacent lanes
-        T wn[R];
+        T* pox = po + base;
+        T* uox = uo + base;
+        T* vox = vo + base;
+        T* wox = wo + base;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
+            // w' on the row, neighbours in z from the adjacent lanes
             const T pc = P0[r];
             T pzl = __shfl_up(pc, 1, 64);
             T pzh = __shfl_down(pc, 1, 64);
@@ -150,43 +168,38 @@ __global__ void __launch_bounds__(256) wave_reg_k(UniformGeo<T> g, const T* __re
             if (lane == 63) pzh = pz1[r];
             if (z == 0) pzl = pc;
             if (z == Z - 1) pzh = pc;
-            wn[r] = mvel(pzl, pc, pzh, wv[r], Az, V, dt);
+            const T wn = mvel(pzl, pc, pzh, wv[r], Az, V, dt);
             // halo w' at z0-1 (lane 0) / z0+64 (lane 63)
-            whh[r] = T(0);
-            if (hlo) whh[r] = mvel(pz2[r], pz1[r], pc, whl[r], Az, V, dt);
-            if (hhi) whh[r] = mvel(pc, pz1[r], pz2[r], whl[r], Az, V, dt);
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int64_t y = y0 + r;
-            T wl = __shfl_up(wn[r], 1, 64);
-            T wh = __shfl_down(wn[r], 1, 64);
-            if (lane == 0) wl = whh[r];
-            if (lane == 63) wh = whh[r];
+            T whh = T(0);
+            if (hlo) whh = mvel(pz2[r], pz1[r], pc, whl[r], Az, V, dt);
+            if (hhi) whh = mvel(pc, pz1[r], pz2[r], whl[r], Az, V, dt);
+            T wl = __shfl_up(wn, 1, 64);
+            T wh = __shfl_down(wn, 1, 64);
+            if (lane == 0) wl = whh;
+            if (lane == 63) wh = whh;
+            const int y = y0 + r;
             if (zval && y < Y) {
                 const T fx_lo = (x > 0) ? mlin(U0[r], Um1[r]) : T(0);
                 const T fx_hi = (x < X - 1) ? mlin(U0[r], Up1[r]) : T(0);
                 const T fy_lo = (y == 0) ? T(0) : mlin(vn[r + 1], vn[r]);
                 const T fy_hi = (y == Y - 1) ? T(0) : mlin(vn[r + 1], vn[r + 2]);
-                const T fz_lo = (z > 0) ? mlin(wn[r], wl) : T(0);
-                const T fz_hi = (z < Z - 1) ? mlin(wn[r], wh) : T(0);
+                const T fz_lo = (z > 0) ? mlin(wn, wl) : T(0);
+                const T fz_hi = (z < Z - 1) ? mlin(wn, wh) : T(0);
                 T S = mggp(fx_lo, fx_hi, Ax, V);
                 S = S + mggp(fy_lo, fy_hi, Ay, V);
                 S = S + mggp(fz_lo, fz_hi, Az, V);
-                const int64_t o = base + y * Z + z;
-                const T k = Kf ? Kf[o] : K;
-                po[o] = dt * (k * S) + P0[r];
-                uo[o] = U0[r];
-                vo[o] = vn[r + 1];
-                wo[o] = wn[r];
+                const unsigned o = unsigned(y * Z + z);
+                const T k = Kf ? Kf[base + o] : K;
+                pox[o] = dt * (k * S) + pc;
+                uox[o] = U0[r];
+                vox[o] = vn[r + 1];
+                wox[o] = wn;
             }
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const T Pp2 = Pm1[r];
-            Pm1[r] = P0[r];
             P0[r] = Pp1[r];
-            Pp1[r] = Pp2;
+            Pp1[r] = Pp2[r];
             Um1[r] = U0[r];
             U0[r] = Up1[r];
         }
@@ -201,7 +214,7 @@ static void launch_reg(const UniformGeo<T>& g, void* const* in, void* const* out
     const int64_t nxc = (g.X + L - 1) / L;
     const int64_t nwaves = int64_t(nzt) * nyb * nxc;
     const int64_t nblk = (nwaves + 3) / 4;
-    CFD_REQUIRE(nblk < (int64_t(1) << 31), "register march grid too large");
+    CFD_REQUIRE(nwaves < (int64_t(1) << 31), "register march grid too large");
     const int rm = (remap && nblk % 8 == 0) ? 1 : 0;
     wave_reg_k<T, R><<<dim3(unsigned(nblk)), dim3(64, 4), 0, s>>>(
         g, (const T*)in[0], (const T*)in[1], (const T*)in[2], (const T*)in[3], (T*)out[0], (T*)out[1], (T*)out[2],
@@ -209,7 +222,7 @@ static void launch_reg(const UniformGeo<T>& g, void* const* in, void* const* out
     CFD_HIP(hipGetLastError());
 }
 
-template <class T, int TYT, int R>
+template <class T, int TYT, int R, bool NT = false>
 static void launch_one(const UniformGeo<T>& g, void* const* in, void* const* out, double K, const void* Kf,
                        double dt, int L, int remap, hipStream_t s) {
     constexpr int TY = TYT * R;
@@ -219,7 +232,7 @@ static void launch_one(const UniformGeo<T>& g, void* const* in, void* const* out
     const int64_t nwg = int64_t(nzt) * nyt * nxc;
     CFD_REQUIRE(nwg < (int64_t(1) << 31), "march grid too large");
     const int rm = (remap && nwg % 8 == 0) ? 1 : 0;
-    wave_march_k<T, TYT, R><<<dim3(unsigned(nwg)), dim3(64, TYT), 0, s>>>(
+    wave_march_k<T, TYT, R, NT><<<dim3(unsigned(nwg)), dim3(64, TYT), 0, s>>>(
         g, (const T*)in[0], (const T*)in[1], (const T*)in[2], (const T*)in[3], (T*)out[0], (T*)out[1], (T*)out[2],
         (T*)out[3], T(K), (const T*)Kf, T(dt), L, nzt, nyt, rm);
     CFD_HIP(hipGetLastError());
@@ -236,6 +249,14 @@ void launch_wave_march(const cfd_mesh_s* m, void* const a[4], void* const b[4], 
     for (int64_t k = 0; k < nsteps; ++k) {
         void* const* in = (k & 1) ? b : a;
         void* const* out = (k & 1) ? a : b;
+        if (variant == 4) {  // variant 1 tiles with non-temporal output stores
+            switch (ty) {
+                case 4: launch_one<T, 4, 1, true>(g, in, out, K, Kf, dt, L, remap, s); break;
+                case 8: launch_one<T, 4, 2, true>(g, in, out, K, Kf, dt, L, remap, s); break;
+                default: throw Error(CFD_E_INVALID, "variant 4 tiles have 4 or 8 rows");
+            }
+            continue;
+        }
         if (variant == 3) {
             switch (ty) {
                 case 2: launch_reg<T, 2>(g, in, out, K, Kf, dt, L, remap, s); break;

@@ -301,7 +301,8 @@ int cfd_mesh_geometry(cfd_mesh_t mesh, cfd_geometry* out);
  * workgroup carried by 4 (8 for 32) thread rows, variant 2 = the same kernel
  * with one thread row per tile row (tile_rows 4, 8 or 16), variant 3 = a
  * register form without LDS or barriers in which each wave owns tile_rows (2,
- * 4 or 8) consecutive rows x 64 z-cells, `chunk` x-planes per workgroup (per
+ * 4 or 8) consecutive rows x 64 z-cells, variant 4 = variant 1 (tile_rows 4
+ * or 8) with non-temporal output stores, `chunk` x-planes per workgroup (per
  * wave for variant 3) and XCD-aware tile placement when `xcd_remap` = 1.  All
  * variants give bit-identical results.  Set by the autotuner
  * (cfdarcov2_amd/autotune.py). */

@@ -129,7 +129,7 @@ def test_wave_fused_matches_sequential_oracle(hip_ctx, oracle_lib, dtype, n):
 
 SHAPES = [(0, 16, 32, 1), (1, 4, 1, 0), (1, 8, 3, 1), (1, 16, 32, 1), (1, 4, 7, 1), (1, 16, 5, 0),
           (1, 32, 4, 1), (1, 32, 64, 0), (1, 8, 16, 0), (2, 8, 5, 0), (2, 16, 3, 1), (2, 4, 9, 0),
-          (3, 2, 5, 0), (3, 4, 1, 1), (3, 8, 7, 0), (3, 4, 64, 1)]
+          (3, 2, 5, 0), (3, 4, 1, 1), (3, 8, 7, 0), (3, 4, 64, 1), (4, 4, 8, 0), (4, 8, 5, 1)]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
