@@ -26,7 +26,8 @@ if ROOT not in sys.path:
 
 METRIC = "MCUPS + % HBM roofline (1 GPU); adjoint/forward cost ratio; batched vs plain"
 HBM_PEAK_GBS = 8000.0
-KERNELS = {0: "wave_uniform_k", 1: "wave_march_k", 2: "wave_march_k"}
+KERNELS = {0: "wave_uniform_k", 1: "wave_march_k", 2: "wave_march_k", 3: "wave_reg_k", 4: "wave_march_k",
+           5: "wave_march_occ_k", 6: "wave_march_pf2_k"}
 
 
 def _traffic(n, dtype, shape):
@@ -194,21 +195,57 @@ def suite_sod(ctx, cpu=True, verbose=False):
     return rec
 
 
+PLAIN_WAVE_CANDIDATES = ([(0, 4, 8, 0)] + [(1, 4, L, r) for L in (4, 8, 16) for r in (0, 1)]
+                         + [(2, 8, L, 0) for L in (8, 16)] + [(3, 2, L, 0) for L in (8, 16, 32)]
+                         + [(4, 4, L, 0) for L in (8, 16)])
+
+
+def ensemble_api_seconds(specs, steps, timeout=900):
+    """Time of the same cases solved one after another through the C++
+    operator API (Variable / Equation::evaluate with hipRTC kernels), by the
+    helper program cfdarcov2_amd/_build/bench/ensemble_api."""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+    prog = os.path.join(ROOT, "cfdarcov2_amd", "_build", "bench", "ensemble_api")
+    if not os.path.exists(prog):
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "cases.bin")
+        with open(path, "wb") as f:
+            np.array([len(specs), steps], dtype=np.int64).tofile(f)
+            for sp in specs:
+                kind = 0 if sp.kind == "heat" else 1
+                coef = sp.coef if kind == 0 else -sp.coef      # c^2 for the acoustic system
+                np.array([kind, sp.n], dtype=np.int64).tofile(f)
+                np.array([sp.spacing, coef, sp.dt], dtype=np.float64).tofile(f)
+                np.ascontiguousarray(sp.field0, dtype=np.float32).tofile(f)
+        res = subprocess.run([prog, path], capture_output=True, text=True, timeout=timeout, cwd=tmp)
+    if res.returncode != 0:
+        raise RuntimeError("ensemble_api failed: " + res.stdout[-2000:] + res.stderr[-2000:])
+    line = [l for l in res.stdout.splitlines() if l.startswith("{")][-1]
+    return json.loads(line)["ensemble_api_seconds"]
+
+
 def suite_ensemble(ctx, steps=500):
     """64 heat/acoustic cases of seeded uneven sides, FP32, 500 steps: the
-    autotuned batched graph run against 64 sequential single-case runs."""
+    autotuned batched graph run against 64 sequential single-case runs (each
+    case with the fused kernel shape tuned for its own size), and against the
+    same cases solved one by one through the operator API."""
     from cfdarcov2_amd import autotune, workloads
     from cfdarcov2_amd.ensemble import Ensemble, autotune as tune_batch, time_gpu
     specs, steps = workloads.ensemble_config(steps=steps)
     ens = Ensemble(ctx, specs)
-    # the plain runs use the fused acoustic kernel's tuned shape for the largest case
-    big = max((c for c in ens.cases if c.spec.kind == "wave"), key=lambda c: c.spec.n)
-    wshape = autotune.tune_wave(big.mesh, steps=10, use_cache=False,
-                                candidates=[(1, ty, L, r) for ty in (4, 8) for L in (8, 16, 32) for r in (0, 1)]
-                                + [(2, 8, L, 0) for L in (8, 16)])
+    plain_shapes = {}
     for c in ens.cases:
-        if c.spec.kind == "wave":
-            c.mesh.set_wave_shape(wshape["variant"], wshape["tile_rows"], wshape["chunk"], wshape["xcd_remap"])
+        if c.spec.kind != "wave":
+            continue
+        if c.spec.n not in plain_shapes:
+            plain_shapes[c.spec.n] = autotune.tune_wave(c.mesh, steps=10, use_cache=False,
+                                                        candidates=PLAIN_WAVE_CANDIDATES)
+        b = plain_shapes[c.spec.n]
+        c.mesh.set_wave_shape(b["variant"], b["tile_rows"], b["chunk"], b["xcd_remap"])
     ens.reset()
     ens.run_sequential(2)
     ens.reset()
@@ -222,9 +259,14 @@ def suite_ensemble(ctx, steps=500):
                        f"FP32, {steps} steps", "cells": ens.cell_updates(1),
            "batched_mcups": cu / t_b / 1e6, "sequential_mcups": cu / t_seq / 1e6, "speedup": t_seq / t_b,
            "batched_ms": t_b * 1e3, "sequential_ms": t_seq * 1e3, "batch_shape": best,
-           "plain_wave_shape": wshape,
+           "plain_wave_shapes": {n: {k: v[k] for k in ("variant", "tile_rows", "chunk", "xcd_remap")}
+                                 for n, v in plain_shapes.items()},
            "batched_gbs_algorithmic": ens.algorithmic_bytes(steps) / t_b / 1e9}
     ens.close()
+    t_api = ensemble_api_seconds(specs, steps)
+    if t_api is not None:
+        rec["operator_api_sequential_ms"] = t_api * 1e3
+        rec["speedup_vs_operator_api"] = t_api / t_b
     return rec
 
 

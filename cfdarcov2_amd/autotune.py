@@ -22,6 +22,8 @@ DEFAULT_CANDIDATES = [(0, 16, 32, 1)] + [
     (3, ty, L, r) for ty, L, r in itertools.product((2, 4, 8), (4, 8, 16, 32, 64), (0, 1))
 ] + [
     (4, ty, L, r) for ty, L, r in itertools.product((4, 8), (4, 8, 16, 32), (0, 1))
+] + [
+    (v, 4, L, r) for v, L, r in itertools.product((5, 6), (4, 8, 16), (0, 1))
 ]
 
 

@@ -126,8 +126,18 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_bench_programs(verbose: bool = False) -> list:
+    """C++ benchmark helpers under cpp/bench (written against the operator API)."""
+    built = []
+    for src in sorted(glob.glob(os.path.join(CPP, "bench", "*.cpp"))):
+        name = os.path.splitext(os.path.basename(src))[0]
+        built.append(build_driver(src, os.path.join(BUILD_DIR, "bench", name), verbose))
+    return built
+
+
 def build_all(verbose: bool = False):
     out = [build(verbose), build_fvm3d(verbose)]
+    out += build_bench_programs(verbose)
     out += build_test_programs(verbose)
     out += build_reference_drivers(verbose)
     return out

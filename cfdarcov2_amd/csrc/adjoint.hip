@@ -36,10 +36,13 @@ namespace cfd {
 namespace {
 
 template <class T> __device__ inline T a_lin(T c, T nb) { return (c + nb) * T(0.5); }
-template <class T> __device__ inline T a_pair(T f_lo, T f_hi, T A, T V) {
+// Green-Gauss face pair ((-f_lo) A + f_hi A) / V; with PW the volume is an
+// exact power of two and the division is the (bit-identical) product with 1/V.
+template <bool PW, class T> __device__ inline T a_pair(T f_lo, T f_hi, T A, const UniformGeo<T>& g) {
     const T t_lo = (-f_lo) * A;
     const T t_hi = f_hi * A;
-    return (t_lo + t_hi) / V;
+    if constexpr (PW) return (t_lo + t_hi) * g.inv_vol;
+    else return (t_lo + t_hi) / g.vol;
 }
 
 struct TCell {
@@ -74,13 +77,13 @@ __device__ inline int wave_row(int64_t x, int64_t Y) {
 }
 
 // velocity update of cell j along axis a from p (reference boundary rule)
-template <class T>
+template <bool PW, class T>
 __device__ inline T fwd_vel(const UniformGeo<T>& g, const T* __restrict__ p, const T* __restrict__ q, int64_t j,
                             int64_t stride, bool lo, bool hi, int a, T dt) {
     const T pc = p[j];
     const T pl = lo ? pc : p[j - stride];
     const T ph = hi ? pc : p[j + stride];
-    const T G = a_pair(a_lin(pc, pl), a_lin(pc, ph), g.area[a], g.vol);
+    const T G = a_pair<PW>(a_lin(pc, pl), a_lin(pc, ph), g.area[a], g);
     return dt * (-G) + q[j];
 }
 
@@ -88,7 +91,7 @@ __device__ inline T fwd_vel(const UniformGeo<T>& g, const T* __restrict__ p, con
 
 // Forward step with optional sensor recording: bitwise the same arithmetic as
 // the per-thread fused acoustic kernel.
-template <class T>
+template <class T, bool PW>
 __global__ void __launch_bounds__(256) wave_trace_k(UniformGeo<T> g, const T* __restrict__ p,
                                                     const T* __restrict__ u, const T* __restrict__ v,
                                                     const T* __restrict__ w, T* __restrict__ po,
@@ -106,12 +109,12 @@ __global__ void __launch_bounds__(256) wave_trace_k(UniformGeo<T> g, const T* __
     T S = T(0);
     for (int a = 0; a < 3; ++a) {
         const bool lo = crd[a] == 0, hi = crd[a] == ext[a] - 1;
-        const T qc = fwd_vel(g, p, vel[a], c.id, str[a], lo, hi, a, dt);
+        const T qc = fwd_vel<PW>(g, p, vel[a], c.id, str[a], lo, hi, a, dt);
         velo[a][c.id] = qc;
         T f_lo = T(0), f_hi = T(0);
-        if (!lo) f_lo = a_lin(qc, fwd_vel(g, p, vel[a], c.id - str[a], str[a], crd[a] - 1 == 0, false, a, dt));
-        if (!hi) f_hi = a_lin(qc, fwd_vel(g, p, vel[a], c.id + str[a], str[a], false, crd[a] + 1 == ext[a] - 1, a, dt));
-        const T G = a_pair(f_lo, f_hi, g.area[a], g.vol);
+        if (!lo) f_lo = a_lin(qc, fwd_vel<PW>(g, p, vel[a], c.id - str[a], str[a], crd[a] - 1 == 0, false, a, dt));
+        if (!hi) f_hi = a_lin(qc, fwd_vel<PW>(g, p, vel[a], c.id + str[a], str[a], false, crd[a] + 1 == ext[a] - 1, a, dt));
+        const T G = a_pair<PW>(f_lo, f_hi, g.area[a], g);
         S = (a == 0) ? G : S + G;
     }
     const T k = Kf ? Kf[c.id] : K;
@@ -124,21 +127,21 @@ __global__ void __launch_bounds__(256) wave_trace_k(UniformGeo<T> g, const T* __
 }
 
 // adjoint velocity of cell j along axis a: mu = lam_a - G_a(q), q = (dt k) lam_p
-template <class T>
+template <bool PW, class T>
 __device__ inline T adj_vel(const UniformGeo<T>& g, const T* __restrict__ lp, const T* __restrict__ la, T K,
                             const T* __restrict__ Kf, int64_t j, int64_t stride, bool lo, bool hi, int a, T dt) {
     auto q = [&](int64_t i) { return (dt * (Kf ? Kf[i] : K)) * lp[i]; };
     const T qc = q(j);
     const T ql = lo ? qc : q(j - stride);
     const T qh = hi ? qc : q(j + stride);
-    const T G = a_pair(a_lin(qc, ql), a_lin(qc, qh), g.area[a], g.vol);
+    const T G = a_pair<PW>(a_lin(qc, ql), a_lin(qc, qh), g.area[a], g);
     return la[j] - G;
 }
 
 // One transposed step lam_t -> lam_{t-1}.  src (nullable): trace gradients of
 // step t-1, added at the sensors.  vel_t/gK (nullable): the forward
 // velocities of step t and the K-gradient accumulator.
-template <class T>
+template <class T, bool PW>
 __global__ void __launch_bounds__(256) wave_adj_k(UniformGeo<T> g, const T* __restrict__ lp,
                                                   const T* __restrict__ lu, const T* __restrict__ lv,
                                                   const T* __restrict__ lw, T* __restrict__ op,
@@ -159,19 +162,19 @@ __global__ void __launch_bounds__(256) wave_adj_k(UniformGeo<T> g, const T* __re
     T S = T(0), St = T(0);
     for (int a = 0; a < 3; ++a) {
         const bool lo = crd[a] == 0, hi = crd[a] == ext[a] - 1;
-        const T mc = adj_vel(g, lp, lam[a], K, Kf, c.id, str[a], lo, hi, a, dt);
+        const T mc = adj_vel<PW>(g, lp, lam[a], K, Kf, c.id, str[a], lo, hi, a, dt);
         outv[a][c.id] = mc;
         T f_lo = T(0), f_hi = T(0);
-        if (!lo) f_lo = a_lin(mc, adj_vel(g, lp, lam[a], K, Kf, c.id - str[a], str[a], crd[a] - 1 == 0, false, a, dt));
+        if (!lo) f_lo = a_lin(mc, adj_vel<PW>(g, lp, lam[a], K, Kf, c.id - str[a], str[a], crd[a] - 1 == 0, false, a, dt));
         if (!hi)
-            f_hi = a_lin(mc, adj_vel(g, lp, lam[a], K, Kf, c.id + str[a], str[a], false, crd[a] + 1 == ext[a] - 1, a, dt));
-        const T G = a_pair(f_lo, f_hi, g.area[a], g.vol);
+            f_hi = a_lin(mc, adj_vel<PW>(g, lp, lam[a], K, Kf, c.id + str[a], str[a], false, crd[a] + 1 == ext[a] - 1, a, dt));
+        const T G = a_pair<PW>(f_lo, f_hi, g.area[a], g);
         S = (a == 0) ? G : S + G;
         if (gK) {  // forward divergence of step t, as the forward computes it
             const T vc = velt[a][c.id];
             const T tl = lo ? T(0) : a_lin(vc, velt[a][c.id - str[a]]);
             const T th = hi ? T(0) : a_lin(vc, velt[a][c.id + str[a]]);
-            const T Gt = a_pair(tl, th, g.area[a], g.vol);
+            const T Gt = a_pair<PW>(tl, th, g.area[a], g);
             St = (a == 0) ? Gt : St + Gt;
         }
     }
@@ -224,9 +227,10 @@ template <class T>
 void launch_wave_trace_step(const cfd_mesh_s* m, const void* const in[4], void* const out[4], double K,
                             const void* Kf, double dt, const SensorList& sens, void* trace, hipStream_t s) {
     const UniformGeo<T>& g = ug<T>(m);
-    wave_trace_k<T><<<tgrid(g), kBlk, 0, s>>>(g, (const T*)in[0], (const T*)in[1], (const T*)in[2],
-                                               (const T*)in[3], (T*)out[0], (T*)out[1], (T*)out[2], (T*)out[3],
-                                               T(K), (const T*)Kf, T(dt), sens, (T*)trace);
+    auto* kern = g.vol_pow2 ? wave_trace_k<T, true> : wave_trace_k<T, false>;
+    kern<<<tgrid(g), kBlk, 0, s>>>(g, (const T*)in[0], (const T*)in[1], (const T*)in[2], (const T*)in[3],
+                                   (T*)out[0], (T*)out[1], (T*)out[2], (T*)out[3], T(K), (const T*)Kf, T(dt), sens,
+                                   (T*)trace);
 }
 
 template <class T>
@@ -237,9 +241,10 @@ void launch_wave_adj_step(const cfd_mesh_s* m, const void* const in[4], void* co
     const T* vt[3] = {nullptr, nullptr, nullptr};
     if (gK)
         for (int a = 0; a < 3; ++a) vt[a] = (const T*)velt[a];
-    wave_adj_k<T><<<tgrid(g), kBlk, 0, s>>>(g, (const T*)in[0], (const T*)in[1], (const T*)in[2],
-                                             (const T*)in[3], (T*)out[0], (T*)out[1], (T*)out[2], (T*)out[3], T(K),
-                                             (const T*)Kf, T(dt), sens, (const T*)src, vt[0], vt[1], vt[2], (T*)gK);
+    auto* kern = g.vol_pow2 ? wave_adj_k<T, true> : wave_adj_k<T, false>;
+    kern<<<tgrid(g), kBlk, 0, s>>>(g, (const T*)in[0], (const T*)in[1], (const T*)in[2], (const T*)in[3],
+                                   (T*)out[0], (T*)out[1], (T*)out[2], (T*)out[3], T(K), (const T*)Kf, T(dt), sens,
+                                   (const T*)src, vt[0], vt[1], vt[2], (T*)gK);
 }
 
 template <class T> void launch_scatter_add(void* lp, const SensorList& sens, const void* src, hipStream_t s) {

@@ -78,6 +78,8 @@ template <class T> struct UniformGeo {
     T len[3];        // cell centre to a-face centre distance
     T vol;           // cell volume
     T dl;            // min(dx, dy, dz)
+    T inv_vol;       // 1 / vol
+    int vol_pow2;    // vol is an exact power of two: x / vol == x * inv_vol bit for bit
 };
 
 // Face-ordered SoA geometry for a general (non-uniform) structured hex mesh.

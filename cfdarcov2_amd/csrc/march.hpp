@@ -32,6 +32,26 @@ template <class T> __device__ inline T mvel(T pl, T pc, T ph, T q, T A, T V, T d
     return dt * (-mggp(mlin(pc, pl), mlin(pc, ph), A, V)) + q;
 }
 
+// Division by the cell volume.  When the volume is an exact power of two the
+// quotient equals the product with its reciprocal bit for bit (both are the
+// correctly rounded value of the same real number), and the product is one
+// instruction instead of a division sequence.
+template <class T, bool PW> struct VolDiv {
+    T V, inv;
+    __device__ T operator()(T x) const {
+        if constexpr (PW) return x * inv;
+        else return x / V;
+    }
+};
+template <class T, bool PW> __device__ inline T mggp(T f_lo, T f_hi, T A, const VolDiv<T, PW>& d) {
+    const T t_lo = (-f_lo) * A;
+    const T t_hi = f_hi * A;
+    return d(t_lo + t_hi);
+}
+template <class T, bool PW> __device__ inline T mvel(T pl, T pc, T ph, T q, T A, const VolDiv<T, PW>& d, T dt) {
+    return dt * (-mggp(mlin(pc, pl), mlin(pc, ph), A, d)) + q;
+}
+
 }  // namespace march
 using namespace march;
 
@@ -54,7 +74,7 @@ template <class T, int TYT, int R> struct MarchLds {
 // batched ensemble kernel.
 // NT: stream the outputs past the L2 (non-temporal stores), leaving its
 // capacity to the halo rows that neighbouring tiles read again.
-template <class T, int TYT, int R, bool NT = false>
+template <class T, int TYT, int R, bool NT = false, int PF = 1, bool PW = false>
 __device__ inline void wave_march_tile(const UniformGeo<T>& g, const T* __restrict__ p, const T* __restrict__ u,
                                        const T* __restrict__ v, const T* __restrict__ w, T* __restrict__ po,
                                        T* __restrict__ uo, T* __restrict__ vo, T* __restrict__ wo, T K,
@@ -71,7 +91,8 @@ __device__ inline void wave_march_tile(const UniformGeo<T>& g, const T* __restri
     auto& sw = lds.sw;
 
     const int64_t X = g.X, Y = g.Y, Z = g.Z, SX = Y * Z;
-    const T Ax = g.area[0], Ay = g.area[1], Az = g.area[2], V = g.vol;
+    const T Ax = g.area[0], Ay = g.area[1], Az = g.area[2];
+    const VolDiv<T, PW> V{g.vol, g.inv_vol};
 
     const int lz = threadIdx.x, ly = threadIdx.y;
     const int64_t z = zt * TZ + lz;
@@ -146,40 +167,54 @@ __device__ inline void wave_march_tile(const UniformGeo<T>& g, const T* __restri
         }
     }
 
-    // ---- prefetch registers (loads of plane x, issued one plane ahead) ------
-    T f_pp2[R], f_un[R], f_vn[R], f_wn[R], f_ch[R], f_wlo[R], f_whi[R];
-    T f_rh = 0, f_vh = 0;
-    auto fetch = [&](int64_t x) {
+    // ---- prefetch registers: the loads of plane x are issued PF planes ahead --
+    struct Fetch {
+        T pp2[R], un[R], vn[R], wn[R], ch[R], wlo[R], whi[R];
+        T rh, vh;
+    };
+    auto fetch = [&](int64_t x, Fetch& f) {
         const int64_t base = x * SX;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            if (slot[r] && x + 2 < X) f_pp2[r] = p[base + 2 * SX + col[r]];
-            if (valid[r] && x + 1 < X) f_un[r] = u[base + SX + col[r]];
+            if (slot[r] && x + 2 < X) f.pp2[r] = p[base + 2 * SX + col[r]];
+            if (valid[r] && x + 1 < X) f.un[r] = u[base + SX + col[r]];
             if (valid[r]) {
-                f_vn[r] = v[base + col[r]];
-                f_wn[r] = w[base + col[r]];
+                f.vn[r] = v[base + col[r]];
+                f.wn[r] = w[base + col[r]];
             }
-            if (ch_col[r] >= 0) f_ch[r] = p[base + ch_col[r]];
-            if (valid[r] && lz == 0 && z >= 1) f_wlo[r] = w[base + col[r] - 1];
-            if (valid[r] && lz == TZ - 1 && z + 1 < Z) f_whi[r] = w[base + col[r] + 1];
+            if (ch_col[r] >= 0) f.ch[r] = p[base + ch_col[r]];
+            if (valid[r] && lz == 0 && z >= 1) f.wlo[r] = w[base + col[r] - 1];
+            if (valid[r] && lz == TZ - 1 && z + 1 < Z) f.whi[r] = w[base + col[r] + 1];
         }
-        if (rh_col >= 0) f_rh = p[base + rh_col];
-        if (vh_lo || vh_hi) f_vh = v[base + vh_col];
+        if (rh_col >= 0) f.rh = p[base + rh_col];
+        if (vh_lo || vh_hi) f.vh = v[base + vh_col];
     };
+    auto clear = [&](Fetch& f) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) f_pp2[r] = f_un[r] = f_vn[r] = f_wn[r] = f_ch[r] = f_wlo[r] = f_whi[r] = T(0);
-    fetch(x0);
+        for (int r = 0; r < R; ++r) f.pp2[r] = f.un[r] = f.vn[r] = f.wn[r] = f.ch[r] = f.wlo[r] = f.whi[r] = T(0);
+        f.rh = f.vh = T(0);
+    };
+    Fetch fa, fb;
+    clear(fa);
+    clear(fb);
+    fetch(x0, fa);
+    if (PF == 2 && x0 + 1 < x1) fetch(x0 + 1, fb);
 
     int b = 0;
     for (int64_t x = x0; x < x1; ++x, b ^= 1) {
         T Pp2[R], un[R], vold[R], wold[R], ch[R], wlo[R], whi[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            Pp2[r] = f_pp2[r]; un[r] = f_un[r]; vold[r] = f_vn[r]; wold[r] = f_wn[r];
-            ch[r] = f_ch[r]; wlo[r] = f_wlo[r]; whi[r] = f_whi[r];
+            Pp2[r] = fa.pp2[r]; un[r] = fa.un[r]; vold[r] = fa.vn[r]; wold[r] = fa.wn[r];
+            ch[r] = fa.ch[r]; wlo[r] = fa.wlo[r]; whi[r] = fa.whi[r];
         }
-        const T rh = f_rh, vh = f_vh;
-        if (x + 1 < x1) fetch(x + 1);
+        const T rh = fa.rh, vh = fa.vh;
+        if (PF == 2) {
+            fa = fb;
+            if (x + 2 < x1) fetch(x + 2, fb);
+        } else if (x + 1 < x1) {
+            fetch(x + 1, fa);
+        }
 
         T* P = sp[b];
         T* SV = sv[b];
@@ -263,7 +298,7 @@ template <class T, int TYT, int R> struct HeatLds {
 // uniform mesh, marching along x with T(x-1), T(x), T(x+1) in registers and the
 // plane's y/z neighbours in LDS.  The arithmetic is heat_uniform_k's
 // (Grad2Var with neighbour = self on the walls), so the two agree bit for bit.
-template <class T, int TYT, int R>
+template <class T, int TYT, int R, bool PW = false>
 __device__ inline void heat_march_tile(const UniformGeo<T>& g, const T* __restrict__ Tin, T* __restrict__ Tout,
                                        T alpha, T dt, int L, int64_t zt, int64_t yt, int64_t xc,
                                        HeatLds<T, TYT, R>& lds) {
@@ -273,6 +308,7 @@ __device__ inline void heat_march_tile(const UniformGeo<T>& g, const T* __restri
     constexpr int TZ = H::TZ;
     constexpr int PZ = H::PZ;
     const int64_t X = g.X, Y = g.Y, Z = g.Z, SX = Y * Z;
+    const VolDiv<T, PW> Vd{g.vol, g.inv_vol};
     const int lz = threadIdx.x, ly = threadIdx.y;
     const int64_t z = zt * TZ + lz;
     const int64_t y0 = yt * TY;
@@ -358,7 +394,7 @@ __device__ inline void heat_march_tile(const UniformGeo<T>& g, const T* __restri
                     const T al_hi = hi_w[a] ? g.alpha_b[a] : g.alpha[a];
                     const T s_lo = (-pc + nb_lo[a]) * al_lo;
                     const T s_hi = (-pc + nb_hi[a]) * al_hi;
-                    Lp = Lp + (s_lo * g.area[a] + s_hi * g.area[a]) / g.vol;
+                    Lp = Lp + Vd(s_lo * g.area[a] + s_hi * g.area[a]);
                 }
                 Tout[base + col[r]] = dt * (alpha * Lp) + pc;
             }

@@ -184,6 +184,10 @@ template <class T> static void create_mesh(cfd_mesh_s* m, const cfd_hex_desc& d)
     const T dl = min3(T(d.dx), T(d.dy), T(d.dz));
     if (uni) {
         u.dl = dl;
+        // dividing by a power of two is exact, so it may become a multiplication
+        int ex = 0;
+        u.vol_pow2 = (u.vol > T(0) && std::frexp(u.vol, &ex) == T(0.5)) ? 1 : 0;
+        u.inv_vol = T(1) / u.vol;
         m->uniform = true;
         if constexpr (sizeof(T) == 4) m->ug32 = u; else m->ug64 = u;
     } else {
@@ -667,8 +671,10 @@ int cfd_mesh_set_euler_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk
 int cfd_mesh_set_wave_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk, int xcd_remap) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");
-        CFD_REQUIRE(variant >= 0 && variant <= 4, "wave variant must be 0, 1, 2, 3 or 4");
-        if (variant == 4)
+        CFD_REQUIRE(variant >= 0 && variant <= 6, "wave variant must be 0 to 6");
+        if (variant == 5 || variant == 6)
+            CFD_REQUIRE(tile_rows == 4, "variant 5 and 6 tile_rows must be 4");
+        else if (variant == 4)
             CFD_REQUIRE(tile_rows == 4 || tile_rows == 8, "variant 4 tile_rows must be 4 or 8");
         else if (variant == 3)
             CFD_REQUIRE(tile_rows == 2 || tile_rows == 4 || tile_rows == 8, "variant 3 tile_rows must be 2, 4 or 8");

@@ -296,10 +296,9 @@ __global__ void __launch_bounds__(256) stab_general_k(GeneralGeo<T> g, const T* 
 // Fused heat step: T' = dt * (alpha * lapl(T)) + T   (DtVar::extract order,
 // src/fvm3d.cpp:690-692, with lapl = ((0 + L_x) + L_y) + L_z).
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(256) heat_uniform_k(UniformGeo<T> g, const T* __restrict__ Tin,
-                                                      T* __restrict__ Tout, T alpha, const T* __restrict__ alpha_f,
-                                                      T dt) {
+template <class T, bool PW>
+__device__ inline void heat_uniform_cell(const UniformGeo<T>& g, const T* __restrict__ Tin, T* __restrict__ Tout,
+                                         T alpha, const T* __restrict__ alpha_f, T dt) {
     const Cell c = cell_of_thread(g);
     if (!c.valid) return;
     const T pc = Tin[c.id];
@@ -309,10 +308,21 @@ __global__ void __launch_bounds__(256) heat_uniform_k(UniformGeo<T> g, const T* 
         const T al_hi = at_hi(g, c, a) ? g.alpha_b[a] : g.alpha[a];
         const T s_lo = (-pc + Tin[nb_lo(g, c, a)]) * al_lo;
         const T s_hi = (-pc + Tin[nb_hi(g, c, a)]) * al_hi;
-        L = L + (s_lo * g.area[a] + s_hi * g.area[a]) / g.vol;
+        const T num = s_lo * g.area[a] + s_hi * g.area[a];
+        // a power-of-two volume divides exactly, so the product is the same number
+        if constexpr (PW) L = L + num * g.inv_vol;
+        else L = L + num / g.vol;
     }
     const T k = alpha_f ? alpha_f[c.id] : alpha;
     Tout[c.id] = dt * (k * L) + pc;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) heat_uniform_k(UniformGeo<T> g, const T* __restrict__ Tin,
+                                                      T* __restrict__ Tout, T alpha, const T* __restrict__ alpha_f,
+                                                      T dt) {
+    if (g.vol_pow2) heat_uniform_cell<T, true>(g, Tin, Tout, alpha, alpha_f, dt);
+    else heat_uniform_cell<T, false>(g, Tin, Tout, alpha, alpha_f, dt);
 }
 
 template <class T>

@@ -39,7 +39,8 @@ __global__ void __launch_bounds__(64 * TYT) wave_march_k(UniformGeo<T> g, const 
     const int64_t zt = lin_id % nzt;
     const int64_t yt = (lin_id / nzt) % nyt;
     const int64_t xc = lin_id / (int64_t(nzt) * nyt);
-    wave_march_tile<T, TYT, R, NT>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, L, zt, yt, xc, lds);
+    if (g.vol_pow2) wave_march_tile<T, TYT, R, NT, 1, true>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, L, zt, yt, xc, lds);
+    else wave_march_tile<T, TYT, R, NT, 1, false>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, L, zt, yt, xc, lds);
 }
 
 // Register form (variant 3): every wave is independent and owns R consecutive
@@ -222,6 +223,63 @@ static void launch_reg(const UniformGeo<T>& g, void* const* in, void* const* out
     CFD_HIP(hipGetLastError());
 }
 
+// The variant-4 tile at full occupancy: at most 64 VGPRs, so eight waves fit
+// each SIMD (variant 5).
+template <class T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+wave_march_occ_k(UniformGeo<T> g, const T* __restrict__ p, const T* __restrict__ u, const T* __restrict__ v,
+                 const T* __restrict__ w, T* __restrict__ po, T* __restrict__ uo, T* __restrict__ vo,
+                 T* __restrict__ wo, T K, const T* __restrict__ Kf, T dt, int L, int nzt, int nyt, int remap) {
+    __shared__ MarchLds<T, 4, 1> lds;
+    const int64_t nwg = gridDim.x;
+    int64_t lin_id = blockIdx.x;
+    if (remap) lin_id = (lin_id % 8) * (nwg / 8) + lin_id / 8;
+    const int64_t zt = lin_id % nzt;
+    const int64_t yt = (lin_id / nzt) % nyt;
+    const int64_t xc = lin_id / (int64_t(nzt) * nyt);
+    if (g.vol_pow2) wave_march_tile<T, 4, 1, true, 1, true>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, L, zt, yt, xc, lds);
+    else wave_march_tile<T, 4, 1, true, 1, false>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, L, zt, yt, xc, lds);
+}
+
+// Variant 6: variant-4 tiles (4 rows) with the loads of two planes in flight.
+template <class T>
+__global__ void __launch_bounds__(256) wave_march_pf2_k(UniformGeo<T> g, const T* __restrict__ p,
+                                                        const T* __restrict__ u, const T* __restrict__ v,
+                                                        const T* __restrict__ w, T* __restrict__ po,
+                                                        T* __restrict__ uo, T* __restrict__ vo, T* __restrict__ wo,
+                                                        T K, const T* __restrict__ Kf, T dt, int L, int nzt, int nyt,
+                                                        int remap) {
+    __shared__ MarchLds<T, 4, 1> lds;
+    const int64_t nwg = gridDim.x;
+    int64_t lin_id = blockIdx.x;
+    if (remap) lin_id = (lin_id % 8) * (nwg / 8) + lin_id / 8;
+    const int64_t zt = lin_id % nzt;
+    const int64_t yt = (lin_id / nzt) % nyt;
+    const int64_t xc = lin_id / (int64_t(nzt) * nyt);
+    if (g.vol_pow2) wave_march_tile<T, 4, 1, true, 2, true>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, L, zt, yt, xc, lds);
+    else wave_march_tile<T, 4, 1, true, 2, false>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, L, zt, yt, xc, lds);
+}
+
+template <class T>
+static void launch_occ(const UniformGeo<T>& g, void* const* in, void* const* out, double K, const void* Kf, double dt,
+                       int L, int remap, hipStream_t s, bool pf2 = false) {
+    const int nzt = int((g.Z + 63) / 64);
+    const int nyt = int((g.Y + 3) / 4);
+    const int nxc = int((g.X + L - 1) / L);
+    const int64_t nwg = int64_t(nzt) * nyt * nxc;
+    CFD_REQUIRE(nwg < (int64_t(1) << 31), "march grid too large");
+    const int rm = (remap && nwg % 8 == 0) ? 1 : 0;
+    if (pf2)
+        wave_march_pf2_k<T><<<dim3(unsigned(nwg)), dim3(64, 4), 0, s>>>(
+            g, (const T*)in[0], (const T*)in[1], (const T*)in[2], (const T*)in[3], (T*)out[0], (T*)out[1],
+            (T*)out[2], (T*)out[3], T(K), (const T*)Kf, T(dt), L, nzt, nyt, rm);
+    else
+        wave_march_occ_k<T><<<dim3(unsigned(nwg)), dim3(64, 4), 0, s>>>(
+            g, (const T*)in[0], (const T*)in[1], (const T*)in[2], (const T*)in[3], (T*)out[0], (T*)out[1],
+            (T*)out[2], (T*)out[3], T(K), (const T*)Kf, T(dt), L, nzt, nyt, rm);
+    CFD_HIP(hipGetLastError());
+}
+
 template <class T, int TYT, int R, bool NT = false>
 static void launch_one(const UniformGeo<T>& g, void* const* in, void* const* out, double K, const void* Kf,
                        double dt, int L, int remap, hipStream_t s) {
@@ -249,6 +307,11 @@ void launch_wave_march(const cfd_mesh_s* m, void* const a[4], void* const b[4], 
     for (int64_t k = 0; k < nsteps; ++k) {
         void* const* in = (k & 1) ? b : a;
         void* const* out = (k & 1) ? a : b;
+        if (variant == 5 || variant == 6) {
+            CFD_REQUIRE(ty == 4, "variant 5 and 6 tiles have 4 rows");
+            launch_occ<T>(g, in, out, K, Kf, dt, L, remap, s, variant == 6);
+            continue;
+        }
         if (variant == 4) {  // variant 1 tiles with non-temporal output stores
             switch (ty) {
                 case 4: launch_one<T, 4, 1, true>(g, in, out, K, Kf, dt, L, remap, s); break;

@@ -302,7 +302,9 @@ int cfd_mesh_geometry(cfd_mesh_t mesh, cfd_geometry* out);
  * with one thread row per tile row (tile_rows 4, 8 or 16), variant 3 = a
  * register form without LDS or barriers in which each wave owns tile_rows (2,
  * 4 or 8) consecutive rows x 64 z-cells, variant 4 = variant 1 (tile_rows 4
- * or 8) with non-temporal output stores, `chunk` x-planes per workgroup (per
+ * or 8) with non-temporal output stores, variant 5 = variant 4 with 4 rows
+ * limited to 64 VGPRs (eight waves per SIMD), variant 6 = variant 4 with 4
+ * rows and the loads of two planes in flight, `chunk` x-planes per workgroup (per
  * wave for variant 3) and XCD-aware tile placement when `xcd_remap` = 1.  All
  * variants give bit-identical results.  Set by the autotuner
  * (cfdarcov2_amd/autotune.py). */

@@ -129,7 +129,7 @@ def test_wave_fused_matches_sequential_oracle(hip_ctx, oracle_lib, dtype, n):
 
 SHAPES = [(0, 16, 32, 1), (1, 4, 1, 0), (1, 8, 3, 1), (1, 16, 32, 1), (1, 4, 7, 1), (1, 16, 5, 0),
           (1, 32, 4, 1), (1, 32, 64, 0), (1, 8, 16, 0), (2, 8, 5, 0), (2, 16, 3, 1), (2, 4, 9, 0),
-          (3, 2, 5, 0), (3, 4, 1, 1), (3, 8, 7, 0), (3, 4, 64, 1), (4, 4, 8, 0), (4, 8, 5, 1)]
+          (3, 2, 5, 0), (3, 4, 1, 1), (3, 8, 7, 0), (3, 4, 64, 1), (4, 4, 8, 0), (4, 8, 5, 1), (5, 4, 8, 0), (5, 4, 3, 1), (6, 4, 8, 0), (6, 4, 2, 1), (6, 4, 1, 0)]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -287,3 +287,26 @@ def test_error_paths(hip_ctx):
         hip_ctx.pool_set_limit(0)
     p = hip_ctx.alloc(1 << 20)
     hip_ctx.free(p)
+
+
+@pytest.mark.parametrize("shape", [(1, 4, 8, 0), (4, 4, 8, 1), (6, 4, 3, 0)])
+def test_wave_uniform_volume_not_power_of_two(hip_ctx, oracle_lib, shape):
+    """A uniform mesh whose cell volume is not a power of two (the kernels
+    then divide by the volume; powers of two multiply by the exact reciprocal):
+    the oracle's bits in both cases, heat included."""
+    from cfdarcov2_amd import capi
+    n, d = (13, 11, 70), (0.375, 0.25, 0.125)      # V = 3/256
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=np.float32)
+    assert gm.info().uniform == 1
+    gm.set_wave_shape(*shape)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=np.float32)
+    rng = np.random.default_rng(23)
+    f = [rng.standard_normal(om.N).astype(np.float32) for _ in range(4)]
+    A = [_t(a, np.float32) for a in f]
+    B = [torch.empty_like(A[0]) for _ in range(4)]
+    got = [_np(t) for t in gm.wave_run(A, B, -1.0, 0.01, 5)]
+    ref = om.wave_run(*f, -1.0, 0.01, 5)
+    for name, g, r in zip("puvw", got, ref):
+        assert np.array_equal(g, r), name
+    a, b = _t(f[0], np.float32), torch.empty_like(_t(f[0], np.float32))
+    assert np.array_equal(_np(gm.heat_run(a, b, 0.01, 0.5, 4)), om.heat_run(f[0], 0.01, 0.5, 4))
