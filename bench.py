@@ -27,7 +27,7 @@ if ROOT not in sys.path:
 METRIC = "MCUPS + % HBM roofline (1 GPU); adjoint/forward cost ratio; batched vs plain"
 HBM_PEAK_GBS = 8000.0
 KERNELS = {0: "wave_uniform_k", 1: "wave_march_k", 2: "wave_march_k", 3: "wave_reg_k", 4: "wave_march_k",
-           5: "wave_march_occ_k", 6: "wave_march_pf2_k"}
+           5: "wave_march_occ_k", 6: "wave_march_pf2_k", 7: "wave_row_k"}
 
 
 def _traffic(n, dtype, shape):
@@ -145,6 +145,7 @@ def suite_wave_f64(ctx, steps=200):
     m = capi.HexMesh(ctx, *case.n, *case.spacing, dtype="f64")
     shape = autotune.tune_wave(m, steps=10, use_cache=False,
                                candidates=[(1, ty, L, r) for ty in (4, 8) for L in (8, 16, 32) for r in (0, 1)]
+                               + [(7, ty, L, r) for ty in (4, 8) for L in (8, 16, 32) for r in (0, 1)]
                                + [(2, 8, L, 0) for L in (8, 16)])
     N = m.N
     A = [torch.from_numpy(case.p0).cuda()] + [torch.zeros(N, dtype=torch.float64, device="cuda") for _ in range(3)]
@@ -197,7 +198,7 @@ def suite_sod(ctx, cpu=True, verbose=False):
 
 PLAIN_WAVE_CANDIDATES = ([(0, 4, 8, 0)] + [(1, 4, L, r) for L in (4, 8, 16) for r in (0, 1)]
                          + [(2, 8, L, 0) for L in (8, 16)] + [(3, 2, L, 0) for L in (8, 16, 32)]
-                         + [(4, 4, L, 0) for L in (8, 16)])
+                         + [(4, 4, L, 0) for L in (8, 16)] + [(7, ty, L, 0) for ty in (4, 8) for L in (8, 16)])
 
 
 def ensemble_api_seconds(specs, steps, timeout=900):

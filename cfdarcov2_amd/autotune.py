@@ -24,6 +24,8 @@ DEFAULT_CANDIDATES = [(0, 16, 32, 1)] + [
     (4, ty, L, r) for ty, L, r in itertools.product((4, 8), (4, 8, 16, 32), (0, 1))
 ] + [
     (v, 4, L, r) for v, L, r in itertools.product((5, 6), (4, 8, 16), (0, 1))
+] + [
+    (7, ty, L, r) for ty, L, r in itertools.product((4, 8, 14), (4, 8, 16, 32), (0, 1))
 ]
 
 
@@ -71,9 +73,13 @@ def tune_wave(mesh, steps: int = 20, candidates=None, use_cache: bool = True, ve
     B = [torch.zeros(N, dtype=tdt, device="cuda") for _ in range(4)]
     stream = torch.cuda.current_stream()
     table = []
+    from .capi import CfdError
     for shape in cands:
         variant, ty, L, remap = shape
-        mesh.set_wave_shape(variant, ty, L, remap)
+        try:
+            mesh.set_wave_shape(variant, ty, L, remap)
+        except CfdError:  # a form this mesh's extents do not allow
+            continue
         mesh.wave_run(A, B, -1.0, 1e-3, 2)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)

@@ -49,6 +49,14 @@ template <class T>
 void launch_wave_march(const cfd_mesh_s* m, void* const a[4], void* const b[4], double K, const void* Kf, double dt,
                        int64_t nsteps, int variant, int ty, int L, int remap, hipStream_t s);
 
+// Row form of the acoustic step (wave_row.hip): z extent a multiple of 4, at
+// most 256.  Returns false (nothing launched) for meshes or field pointers it
+// cannot take.
+bool wave_row_supported(int64_t Z);
+template <class T>
+bool launch_wave_row(const UniformGeo<T>& g, void* const* in, void* const* out, double K, const void* Kf, double dt,
+                     int ty, int L, int remap, hipStream_t s);
+
 template <class T>
 void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamma, double cfl, int64_t nsteps,
                   void* dt_hist, hipStream_t s);

@@ -671,8 +671,13 @@ int cfd_mesh_set_euler_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk
 int cfd_mesh_set_wave_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk, int xcd_remap) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");
-        CFD_REQUIRE(variant >= 0 && variant <= 6, "wave variant must be 0 to 6");
-        if (variant == 5 || variant == 6)
+        CFD_REQUIRE(variant >= 0 && variant <= 7, "wave variant must be 0 to 7");
+        if (variant == 7) {
+            CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || (tile_rows == 14 && m->dtype == CFD_F32),
+                        "variant 7 tile_rows must be 4 or 8 (or 14 in FP32)");
+            if (!wave_row_supported(m->Z))
+                throw Error(CFD_E_UNSUPPORTED, "variant 7 needs a z extent that is a multiple of 4 and at most 256");
+        } else if (variant == 5 || variant == 6)
             CFD_REQUIRE(tile_rows == 4, "variant 5 and 6 tile_rows must be 4");
         else if (variant == 4)
             CFD_REQUIRE(tile_rows == 4 || tile_rows == 8, "variant 4 tile_rows must be 4 or 8");

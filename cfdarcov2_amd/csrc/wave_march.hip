@@ -307,6 +307,11 @@ void launch_wave_march(const cfd_mesh_s* m, void* const a[4], void* const b[4], 
     for (int64_t k = 0; k < nsteps; ++k) {
         void* const* in = (k & 1) ? b : a;
         void* const* out = (k & 1) ? a : b;
+        if (variant == 7) {  // row form; misaligned fields take the variant-4 tile
+            if (!launch_wave_row<T>(g, in, out, K, Kf, dt, ty, L, remap, s))
+                launch_one<T, 4, 1, true>(g, in, out, K, Kf, dt, L, remap, s);
+            continue;
+        }
         if (variant == 5 || variant == 6) {
             CFD_REQUIRE(ty == 4, "variant 5 and 6 tiles have 4 rows");
             launch_occ<T>(g, in, out, K, Kf, dt, L, remap, s, variant == 6);

@@ -310,3 +310,86 @@ def test_wave_uniform_volume_not_power_of_two(hip_ctx, oracle_lib, shape):
         assert np.array_equal(g, r), name
     a, b = _t(f[0], np.float32), torch.empty_like(_t(f[0], np.float32))
     assert np.array_equal(_np(gm.heat_run(a, b, 0.01, 0.5, 4)), om.heat_run(f[0], 0.01, 0.5, 4))
+
+
+ROW_CASES = [
+    # (extents, spacing, shape): partial y tiles, chunks that do not divide x,
+    # idle lanes (z < 256), one-lane rows, full 256-cell rows, a volume that is
+    # not a power of two
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (7, 4, 5, 0)),
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (7, 8, 3, 1)),
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (7, 14, 1, 0)),
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (7, 14, 3, 1)),
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (7, 8, 1, 0)),
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (7, 4, 1, 1)),
+    ((9, 30, 256), (1.0 / 16, 1.0 / 32, 1.0 / 256), (7, 8, 4, 1)),
+    ((6, 5, 4), (0.25, 0.25, 0.25), (7, 4, 2, 0)),
+    ((1, 3, 8), (0.25, 0.25, 0.25), (7, 4, 16, 0)),
+    ((13, 11, 60), (0.375, 0.25, 0.125), (7, 8, 16, 0)),
+]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("case", ROW_CASES, ids=[f"{c[0][0]}x{c[0][1]}x{c[0][2]}-ty{c[2][1]}-L{c[2][2]}" for c in ROW_CASES])
+def test_wave_row_form(hip_ctx, oracle_lib, dtype, case):
+    """Row form of the acoustic step (one wave per z row, four cells per lane,
+    z-neighbours by lane shifts): the oracle's bits."""
+    from cfdarcov2_amd import capi
+    n, d, shape = case
+    if dtype == np.float64 and shape[1] == 14:
+        pytest.skip("14-row tiles are FP32 only")
+    print(f"\n[case] wave row form n={n} shape={shape} {np.dtype(dtype).name}", flush=True)
+    rng = np.random.default_rng(41)
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=dtype)
+    gm.set_wave_shape(*shape)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=dtype)
+    f = [rng.standard_normal(om.N).astype(dtype) for _ in range(4)]
+    A = [_t(a, dtype) for a in f]
+    B = [torch.empty_like(A[0]) for _ in range(4)]
+    dt = 0.3 * min(d)
+    got = [_np(t) for t in gm.wave_run(A, B, -1.0, dt, 7)]
+    ref = om.wave_run(*f, -1.0, dt, 7)
+    for name, g, r in zip("puvw", got, ref):
+        assert np.array_equal(g, r), f"field {name}: max diff {np.max(np.abs(g - r))}"
+
+
+def test_wave_row_form_K_field_and_misaligned(hip_ctx, oracle_lib):
+    """Row form with a per-cell K field, then with field pointers that are not
+    16-byte aligned (the launch takes the 2.5D tile instead): the oracle's bits
+    in both."""
+    from cfdarcov2_amd import capi
+    n, d = (10, 9, 36), (0.125, 0.125, 0.0625)
+    rng = np.random.default_rng(7)
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=np.float32)
+    gm.set_wave_shape(7, 4, 3, 0)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=np.float32)
+    f = [rng.standard_normal(om.N).astype(np.float32) for _ in range(4)]
+    K = (-(0.5 + rng.random(om.N)) ** 2).astype(np.float32)
+    A = [_t(a, np.float32) for a in f]
+    B = [torch.empty_like(A[0]) for _ in range(4)]
+    got = [_np(t) for t in gm.wave_run(A, B, 0.0, 0.02, 5, K_field=_t(K, np.float32))]
+    ref = om.wave_run(*f, 0.0, 0.02, 5, K_field=K)
+    for name, g, r in zip("puvw", got, ref):
+        assert np.array_equal(g, r), f"K field, field {name}"
+    # views starting one element into a larger buffer
+    big = [torch.zeros(om.N + 1, dtype=torch.float32, device="cuda") for _ in range(8)]
+    A = [b[1:] for b in big[:4]]
+    B = [b[1:] for b in big[4:]]
+    for a, x in zip(A, f):
+        a.copy_(_t(x, np.float32))
+    got = [_np(t) for t in gm.wave_run(A, B, -1.0, 0.02, 5)]
+    ref = om.wave_run(*f, -1.0, 0.02, 5)
+    for name, g, r in zip("puvw", got, ref):
+        assert np.array_equal(g, r), f"misaligned, field {name}"
+
+
+def test_wave_row_form_rejects_unsupported_rows(hip_ctx):
+    from cfdarcov2_amd import capi
+    gm = capi.HexMesh(hip_ctx, 8, 8, 70, 0.125, 0.125, 0.125, dtype=np.float32)
+    with pytest.raises(capi.CfdError) as e:
+        gm.set_wave_shape(7, 4, 8, 0)
+    assert e.value.code == -4
+    with pytest.raises(capi.CfdError):
+        capi.HexMesh(hip_ctx, 8, 8, 64, 0.125, 0.125, 0.125, dtype=np.float32).set_wave_shape(7, 6, 8, 0)
+    with pytest.raises(capi.CfdError):
+        capi.HexMesh(hip_ctx, 8, 8, 64, 0.125, 0.125, 0.125, dtype=np.float64).set_wave_shape(7, 14, 8, 0)
