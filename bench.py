@@ -44,7 +44,7 @@ def _traffic(n, dtype, shape):
     return rec.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(case, dtype, steps):
+def cpu_baseline(case, dtype, steps, seconds=12.0):
     """The oracle (reference-faithful CPU evaluation: every operator materialised
     over the whole mesh, equations in order) on `steps` steps of the same mesh."""
     import numpy as np
@@ -55,13 +55,18 @@ def cpu_baseline(case, dtype, steps):
     om = pyoracle.OracleMesh(*case.n, *case.spacing, dtype=npdt)
     z = np.zeros(om.N, dtype=npdt)
     p0 = case.p0.astype(npdt)
+    # a short probe sizes the sample to about `seconds` of CPU work
+    t0 = time.perf_counter()
+    om.wave_run(p0, z, z, z, -case.c ** 2, case.dt, steps)
+    probe = time.perf_counter() - t0
+    steps = max(steps, min(500, int(seconds * steps / max(probe, 1e-9))))
     t0 = time.perf_counter()
     om.wave_run(p0, z, z, z, -case.c ** 2, case.dt, steps)
     t = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": om.N * steps / t / 1e6, "unit": "MCUPS", "cores": cores, "kind": "port",
-            "sample": f"oracle wave_run, {case.n[0]}^3 {dtype}, {steps} steps of the benchmark workload, "
-                      f"{t:.2f} s"}
+            "sample": f"oracle wave_run, {case.n[0]}^3 {dtype}, first {steps} steps of the benchmark workload, "
+                      f"{t:.1f} s"}
 
 
 def _events():
