@@ -27,7 +27,7 @@ if ROOT not in sys.path:
 METRIC = "MCUPS + % HBM roofline (1 GPU); adjoint/forward cost ratio; batched vs plain"
 HBM_PEAK_GBS = 8000.0
 KERNELS = {0: "wave_uniform_k", 1: "wave_march_k", 2: "wave_march_k", 3: "wave_reg_k", 4: "wave_march_k",
-           5: "wave_march_occ_k", 6: "wave_march_pf2_k", 7: "wave_row_k"}
+           5: "wave_march_occ_k", 6: "wave_march_pf2_k", 7: "wave_row_k", 8: "wave_row_k", 9: "wave_row_k", 10: "wave_row_k"}
 
 
 def _traffic(n, dtype, shape):
@@ -109,6 +109,32 @@ def copy_ceiling(nbytes=256 << 20, reps=20):
     return 2 * nbytes / t / 1e9
 
 
+def stream_ceiling(nfloat=256 ** 3, reps=30):
+    """Best rate (GB/s, 8 fields x 4 B per cell) of the acoustic step's access
+    pattern without its arithmetic: 4 fields read and 4 written per pass,
+    passes ping-ponging between two sets, over the cache policies and grid
+    sizes of the stream probe.  Returns (GB/s, description)."""
+    import ctypes
+
+    from cfdarcov2_amd import build as hipbuild
+    lib = ctypes.CDLL(hipbuild.STREAM_PROBE)
+    lib.stream_probe_pingpong.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_double)]
+    names = {0: "plain loads and stores", 1: "non-temporal stores", 2: "non-temporal loads",
+             3: "non-temporal loads and stores"}
+    best = (0.0, "")
+    for policy in range(4):
+        for grid in (2048, 8192, 16384):
+            us = ctypes.c_double()
+            if lib.stream_probe_pingpong(nfloat, reps, grid, policy, ctypes.byref(us)) != 0:
+                raise RuntimeError("stream probe failed")
+            gbs = 8 * 4 * nfloat / (us.value * 1e-6) / 1e9
+            if gbs > best[0]:
+                best = (gbs, f"ping-pong 4 reads + 4 writes of {nfloat * 4 >> 20} MiB fields, {names[policy]}, "
+                              f"grid {grid}")
+    return best
+
+
 def suite_heat(ctx, cpu=True):
     """Heat 64^3 FP64, 200 steps at dt = 0.2 dx^2/alpha (the reference CPU
     configuration); 16 algorithmic bytes per cell-update; 4 MiB of state stays
@@ -150,7 +176,7 @@ def suite_wave_f64(ctx, steps=200):
     m = capi.HexMesh(ctx, *case.n, *case.spacing, dtype="f64")
     shape = autotune.tune_wave(m, steps=10, use_cache=False,
                                candidates=[(1, ty, L, r) for ty in (4, 8) for L in (8, 16, 32) for r in (0, 1)]
-                               + [(7, ty, L, r) for ty in (4, 8) for L in (8, 16, 32) for r in (0, 1)]
+                               + [(v, ty, L, r) for v in (7, 9) for ty in (4, 8) for L in (8, 16, 32) for r in (1, 3)]
                                + [(2, 8, L, 0) for L in (8, 16)])
     N = m.N
     A = [torch.from_numpy(case.p0).cuda()] + [torch.zeros(N, dtype=torch.float64, device="cuda") for _ in range(3)]
@@ -203,7 +229,8 @@ def suite_sod(ctx, cpu=True, verbose=False):
 
 PLAIN_WAVE_CANDIDATES = ([(0, 4, 8, 0)] + [(1, 4, L, r) for L in (4, 8, 16) for r in (0, 1)]
                          + [(2, 8, L, 0) for L in (8, 16)] + [(3, 2, L, 0) for L in (8, 16, 32)]
-                         + [(4, 4, L, 0) for L in (8, 16)] + [(7, ty, L, 0) for ty in (4, 8) for L in (8, 16)])
+                         + [(4, 4, L, 0) for L in (8, 16)]
+                         + [(v, ty, L, 3) for v in (7, 9) for ty in (4, 8) for L in (8, 16)])
 
 
 def ensemble_api_seconds(specs, steps, timeout=900):
@@ -438,6 +465,10 @@ def main(argv=None):
     ceil = copy_ceiling()
     out["roofline"]["copy_ceiling_gbs"] = ceil
     out["roofline"]["frac_of_copy_ceiling"] = achieved / ceil
+    sc, sdesc = stream_ceiling()
+    out["roofline"]["stream_ceiling_gbs"] = sc
+    out["roofline"]["stream_ceiling"] = sdesc
+    out["roofline"]["frac_of_stream_ceiling"] = achieved / sc
     if not args.no_suite:
         suite = {}
         for name, fn in (("heat64_f64", lambda: suite_heat(ctx, cpu=not args.no_cpu)),

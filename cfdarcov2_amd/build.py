@@ -135,9 +135,23 @@ def build_bench_programs(verbose: bool = False) -> list:
     return built
 
 
+STREAM_PROBE = os.path.join(BUILD_DIR, "bench", "libstream_probe.so")
+
+
+def build_stream_probe(verbose: bool = False) -> str:
+    """The streaming-bandwidth probe the benchmark reports its ceiling from."""
+    os.makedirs(os.path.dirname(STREAM_PROBE), exist_ok=True)
+    src = os.path.join(CPP, "bench", "stream_probe.hip")
+    cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-o", STREAM_PROBE, src]
+    _run(cmd, "the stream probe", verbose)
+    return STREAM_PROBE
+
+
 def build_all(verbose: bool = False):
     out = [build(verbose), build_fvm3d(verbose)]
     out += build_bench_programs(verbose)
+    out.append(build_stream_probe(verbose))
     out += build_test_programs(verbose)
     out += build_reference_drivers(verbose)
     return out

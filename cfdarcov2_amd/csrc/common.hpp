@@ -126,7 +126,9 @@ struct cfd_mesh_s {
     std::vector<double> vertex_offsets;
     bool has_offsets = false;
     // fused acoustic step on uniform meshes: variant 0 = per-thread kernel,
-    // 1 = 2.5D marching kernel with tile rows `wave_ty` and chunk `wave_L`
+    // 1-6 = 2.5D marching tiles, 7-10 = row form (the variants of
+    // cfd_mesh_set_wave_shape); uniform meshes whose z extent suits the row
+    // form start on it
     int wave_variant = 1;
     int wave_ty = 16;
     int wave_L = 32;

@@ -50,12 +50,13 @@ void launch_wave_march(const cfd_mesh_s* m, void* const a[4], void* const b[4], 
                        int64_t nsteps, int variant, int ty, int L, int remap, hipStream_t s);
 
 // Row form of the acoustic step (wave_row.hip): z extent a multiple of 4, at
-// most 256.  Returns false (nothing launched) for meshes or field pointers it
-// cannot take.
+// most 256.  order: bit 0 XCD-contiguous tiles, bit 1 tiles last to first;
+// cache: bit 0 non-temporal stores, bit 1 non-temporal loads.  Returns false
+// (nothing launched) for meshes or field pointers it cannot take.
 bool wave_row_supported(int64_t Z);
 template <class T>
 bool launch_wave_row(const UniformGeo<T>& g, void* const* in, void* const* out, double K, const void* Kf, double dt,
-                     int ty, int L, int remap, hipStream_t s);
+                     int ty, int L, int order, int cache, hipStream_t s);
 
 template <class T>
 void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamma, double cfl, int64_t nsteps,

@@ -189,6 +189,12 @@ template <class T> static void create_mesh(cfd_mesh_s* m, const cfd_hex_desc& d)
         u.vol_pow2 = (u.vol > T(0) && std::frexp(u.vol, &ex) == T(0.5)) ? 1 : 0;
         u.inv_vol = T(1) / u.vol;
         m->uniform = true;
+        if (wave_row_supported(d.nz)) {  // row form, non-temporal loads (fastest measured at 256^3)
+            m->wave_variant = 9;
+            m->wave_ty = sizeof(T) == 4 ? 8 : 4;
+            m->wave_L = 16;
+            m->wave_remap = 3;
+        }
         if constexpr (sizeof(T) == 4) m->ug32 = u; else m->ug64 = u;
     } else {
         m->uniform = false;
@@ -671,12 +677,13 @@ int cfd_mesh_set_euler_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk
 int cfd_mesh_set_wave_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk, int xcd_remap) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");
-        CFD_REQUIRE(variant >= 0 && variant <= 7, "wave variant must be 0 to 7");
-        if (variant == 7) {
+        CFD_REQUIRE(variant >= 0 && variant <= 10, "wave variant must be 0 to 10");
+        const bool row_form = variant >= 7;
+        if (row_form) {
             CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || (tile_rows == 14 && m->dtype == CFD_F32),
-                        "variant 7 tile_rows must be 4 or 8 (or 14 in FP32)");
+                        "row-form (7 to 10) tile_rows must be 4 or 8 (or 14 in FP32)");
             if (!wave_row_supported(m->Z))
-                throw Error(CFD_E_UNSUPPORTED, "variant 7 needs a z extent that is a multiple of 4 and at most 256");
+                throw Error(CFD_E_UNSUPPORTED, "the row form (variants 7 to 10) needs a z extent that is a multiple of 4 and at most 256");
         } else if (variant == 5 || variant == 6)
             CFD_REQUIRE(tile_rows == 4, "variant 5 and 6 tile_rows must be 4");
         else if (variant == 4)
@@ -690,7 +697,9 @@ int cfd_mesh_set_wave_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk,
         m->wave_variant = variant;
         m->wave_ty = tile_rows;
         m->wave_L = chunk;
-        m->wave_remap = xcd_remap ? 1 : 0;
+        // bit 0: XCD-contiguous tiles; bit 1 (row form only): odd steps walk
+        // the tiles in reverse
+        m->wave_remap = row_form ? (xcd_remap & 3) : (xcd_remap ? 1 : 0);
     });
 }
 

@@ -307,9 +307,13 @@ void launch_wave_march(const cfd_mesh_s* m, void* const a[4], void* const b[4], 
     for (int64_t k = 0; k < nsteps; ++k) {
         void* const* in = (k & 1) ? b : a;
         void* const* out = (k & 1) ? a : b;
-        if (variant == 7) {  // row form; misaligned fields take the variant-4 tile
-            if (!launch_wave_row<T>(g, in, out, K, Kf, dt, ty, L, remap, s))
-                launch_one<T, 4, 1, true>(g, in, out, K, Kf, dt, L, remap, s);
+        if (variant >= 7) {  // row form; misaligned fields take the variant-4 tile
+            // remap bit 1: odd steps walk the tiles in reverse.  Cache policy:
+            // 7 non-temporal stores, 8 plain, 9 non-temporal loads, 10 both
+            const int order = (remap & 1) | ((remap & 2) && (k & 1) ? 2 : 0);
+            const int cache = variant == 7 ? 1 : variant == 8 ? 0 : variant == 9 ? 2 : 3;
+            if (!launch_wave_row<T>(g, in, out, K, Kf, dt, ty, L, order, cache, s))
+                launch_one<T, 4, 1, true>(g, in, out, K, Kf, dt, L, remap & 1, s);
             continue;
         }
         if (variant == 5 || variant == 6) {

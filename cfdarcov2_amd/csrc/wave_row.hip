@@ -67,11 +67,11 @@ template <class V4> __device__ inline void put(V4* dst, const V4& v, bool nt) {
     else *dst = v;
 }
 
-template <class T, int TY, bool PW, bool KF, bool XB>
+template <class T, int TY, bool PW, bool KF, bool XB, bool NTL>
 __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p, const T* __restrict__ u,
                                 const T* __restrict__ v, const T* __restrict__ w, T* __restrict__ po,
                                 T* __restrict__ uo, T* __restrict__ vo, T* __restrict__ wo, T K,
-                                const T* __restrict__ Kf, T dt, int64_t x0, int64_t x1, int y0, int wv,
+                                const T* __restrict__ Kf, T dt, int64_t x0, int64_t x1, int y0, int wv, bool nt,
                                 typename Row4<T>::type (*sp)[64], typename Row4<T>::type (*sv)[64]) {
     using V4 = typename Row4<T>::type;
     const int64_t X = g.X;
@@ -116,7 +116,13 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
         V4 a, b, c, d, k;
     };
     const V4 zero4 = {T(0), T(0), T(0), T(0)};
-    auto ld = [&](const T* base, int64_t off) -> V4 { return reinterpret_cast<const V4*>(base + off)[lane]; };
+    // NTL: non-temporal loads (streamed past the caches' retention, which then
+    // keep what this step writes for the next step to read)
+    auto ld = [&](const T* base, int64_t off) -> V4 {
+        const V4* a = reinterpret_cast<const V4*>(base + off) + lane;
+        if constexpr (NTL) return __builtin_nontemporal_load(a);
+        else return *a;
+    };
     auto fetch = [&](int64_t xx, Fetch& f) {
         if (!lv) return;
         const int64_t pb = xx * SX;
@@ -232,10 +238,10 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
             }
             if (lv) {
                 const int64_t o = x * SX + rb;
-                put(reinterpret_cast<V4*>(po + o) + lane, pn, true);
-                put(reinterpret_cast<V4*>(uo + o) + lane, U0, true);
-                put(reinterpret_cast<V4*>(vo + o) + lane, vn, true);
-                put(reinterpret_cast<V4*>(wo + o) + lane, wn, true);
+                put(reinterpret_cast<V4*>(po + o) + lane, pn, nt);
+                put(reinterpret_cast<V4*>(uo + o) + lane, U0, nt);
+                put(reinterpret_cast<V4*>(vo + o) + lane, vn, nt);
+                put(reinterpret_cast<V4*>(wo + o) + lane, wn, nt);
             }
             P0 = Pp1;
             Pp1 = cur.a;
@@ -247,19 +253,23 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
 
 }  // namespace row
 
-template <class T, int TY, bool PW, bool KF>
+template <class T, int TY, bool PW, bool KF, bool NTL>
 __global__ void __launch_bounds__(64 * (TY + 2))
 wave_row_k(UniformGeo<T> g, const T* __restrict__ p, const T* __restrict__ u, const T* __restrict__ v,
            const T* __restrict__ w, T* __restrict__ po, T* __restrict__ uo, T* __restrict__ vo, T* __restrict__ wo,
-           T K, const T* __restrict__ Kf, T dt, int L, int nyt, int remap) {
+           T K, const T* __restrict__ Kf, T dt, int L, int nyt, int order, int nt) {
     using V4 = typename row::Row4<T>::type;
     __shared__ V4 sp[TY + 2][64];
     __shared__ V4 sv[TY + 2][64];
-    // workgroup -> (y tile, x chunk); with remap each XCD (round-robin dispatch)
-    // gets a contiguous run of y tiles, so tiles sharing halo rows share an L2
+    // workgroup -> (y tile, x chunk).  order bit 1: walk the tiles last to
+    // first (alternate steps run in opposite directions, so a step starts on
+    // the planes the previous one wrote last, which the caches still hold);
+    // order bit 0: each XCD (round-robin dispatch) gets a contiguous run of y
+    // tiles, so tiles sharing halo rows share an L2
     const int64_t nwg = gridDim.x;
     int64_t lin = blockIdx.x;
-    if (remap) lin = (lin % 8) * (nwg / 8) + lin / 8;
+    if (order & 2) lin = nwg - 1 - lin;
+    if (order & 1) lin = (lin % 8) * (nwg / 8) + lin / 8;
     const int yt = int(lin % nyt);
     const int64_t xc = lin / nyt;
     const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.y));
@@ -269,42 +279,49 @@ wave_row_k(UniformGeo<T> g, const T* __restrict__ p, const T* __restrict__ u, co
     const int y0 = yt * TY;
     // chunks away from the x walls need no x-boundary tests
     if (x0 >= 1 && x1 + 2 <= X)
-        row::row_tile<T, TY, PW, KF, false>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, x0, x1, y0, wv, sp, sv);
+        row::row_tile<T, TY, PW, KF, false, NTL>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, x0, x1, y0, wv, nt != 0, sp, sv);
     else
-        row::row_tile<T, TY, PW, KF, true>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, x0, x1, y0, wv, sp, sv);
+        row::row_tile<T, TY, PW, KF, true, NTL>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, x0, x1, y0, wv, nt != 0, sp, sv);
 }
 
-template <class T, int TY, bool PW, bool KF>
+template <class T, int TY, bool PW, bool KF, bool NTL>
 static void launch_row_k(const UniformGeo<T>& g, void* const* in, void* const* out, double K, const void* Kf,
-                         double dt, int L, int remap, hipStream_t s) {
+                         double dt, int L, int order, bool nt, hipStream_t s) {
     const int nyt = int((g.Y + TY - 1) / TY);
     const int64_t nxc = (g.X + L - 1) / L;
     const int64_t nwg = int64_t(nyt) * nxc;
     CFD_REQUIRE(nwg < (int64_t(1) << 31), "row march grid too large");
-    const int rm = (remap && nwg % 8 == 0) ? 1 : 0;
-    wave_row_k<T, TY, PW, KF><<<dim3(unsigned(nwg)), dim3(64, TY + 2), 0, s>>>(
+    const int ord = ((order & 1) && nwg % 8 == 0 ? 1 : 0) | (order & 2);
+    wave_row_k<T, TY, PW, KF, NTL><<<dim3(unsigned(nwg)), dim3(64, TY + 2), 0, s>>>(
         g, (const T*)in[0], (const T*)in[1], (const T*)in[2], (const T*)in[3], (T*)out[0], (T*)out[1], (T*)out[2],
-        (T*)out[3], T(K), (const T*)Kf, T(dt), L, nyt, rm);
+        (T*)out[3], T(K), (const T*)Kf, T(dt), L, nyt, ord, nt ? 1 : 0);
     CFD_HIP(hipGetLastError());
 }
 
-template <class T, int TY>
+template <class T, int TY, bool NTL>
 static void launch_row_ty(const UniformGeo<T>& g, void* const* in, void* const* out, double K, const void* Kf,
-                          double dt, int L, int remap, hipStream_t s) {
+                          double dt, int L, int order, bool nt, hipStream_t s) {
     if (g.vol_pow2) {
-        if (Kf) launch_row_k<T, TY, true, true>(g, in, out, K, Kf, dt, L, remap, s);
-        else launch_row_k<T, TY, true, false>(g, in, out, K, Kf, dt, L, remap, s);
+        if (Kf) launch_row_k<T, TY, true, true, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
+        else launch_row_k<T, TY, true, false, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
     } else {
-        if (Kf) launch_row_k<T, TY, false, true>(g, in, out, K, Kf, dt, L, remap, s);
-        else launch_row_k<T, TY, false, false>(g, in, out, K, Kf, dt, L, remap, s);
+        if (Kf) launch_row_k<T, TY, false, true, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
+        else launch_row_k<T, TY, false, false, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
     }
+}
+
+template <class T, int TY>
+static void launch_row_c(const UniformGeo<T>& g, void* const* in, void* const* out, double K, const void* Kf,
+                         double dt, int L, int order, int cache, hipStream_t s) {
+    if (cache & 2) launch_row_ty<T, TY, true>(g, in, out, K, Kf, dt, L, order, (cache & 1) != 0, s);
+    else launch_row_ty<T, TY, false>(g, in, out, K, Kf, dt, L, order, (cache & 1) != 0, s);
 }
 
 bool wave_row_supported(int64_t Z) { return Z >= 4 && Z <= 256 && Z % 4 == 0; }
 
 template <class T>
 bool launch_wave_row(const UniformGeo<T>& g, void* const* in, void* const* out, double K, const void* Kf, double dt,
-                     int ty, int L, int remap, hipStream_t s) {
+                     int ty, int L, int order, int cache, hipStream_t s) {
     // rows start on 4-cell boundaries; the vector accesses need the field
     // pointers aligned to four cells as well
     if (!wave_row_supported(g.Z)) return false;
@@ -313,11 +330,11 @@ bool launch_wave_row(const UniformGeo<T>& g, void* const* in, void* const* out, 
         if ((reinterpret_cast<uintptr_t>(in[i]) & al) || (reinterpret_cast<uintptr_t>(out[i]) & al)) return false;
     if (Kf && (reinterpret_cast<uintptr_t>(Kf) & al)) return false;
     switch (ty) {
-        case 4: launch_row_ty<T, 4>(g, in, out, K, Kf, dt, L, remap, s); break;
-        case 8: launch_row_ty<T, 8>(g, in, out, K, Kf, dt, L, remap, s); break;
+        case 4: launch_row_c<T, 4>(g, in, out, K, Kf, dt, L, order, cache, s); break;
+        case 8: launch_row_c<T, 8>(g, in, out, K, Kf, dt, L, order, cache, s); break;
         case 14:  // FP32 only: the FP64 form of this tile spills registers
             if constexpr (sizeof(T) == 4) {
-                launch_row_ty<T, 14>(g, in, out, K, Kf, dt, L, remap, s);
+                launch_row_c<T, 14>(g, in, out, K, Kf, dt, L, order, cache, s);
                 break;
             }
             [[fallthrough]];
@@ -327,8 +344,8 @@ bool launch_wave_row(const UniformGeo<T>& g, void* const* in, void* const* out, 
 }
 
 template bool launch_wave_row<float>(const UniformGeo<float>&, void* const*, void* const*, double, const void*, double,
-                                     int, int, int, hipStream_t);
+                                     int, int, int, int, hipStream_t);
 template bool launch_wave_row<double>(const UniformGeo<double>&, void* const*, void* const*, double, const void*,
-                                      double, int, int, int, hipStream_t);
+                                      double, int, int, int, int, hipStream_t);
 
 }  // namespace cfd

@@ -304,9 +304,17 @@ int cfd_mesh_geometry(cfd_mesh_t mesh, cfd_geometry* out);
  * 4 or 8) consecutive rows x 64 z-cells, variant 4 = variant 1 (tile_rows 4
  * or 8) with non-temporal output stores, variant 5 = variant 4 with 4 rows
  * limited to 64 VGPRs (eight waves per SIMD), variant 6 = variant 4 with 4
- * rows and the loads of two planes in flight, `chunk` x-planes per workgroup (per
- * wave for variant 3) and XCD-aware tile placement when `xcd_remap` = 1.  All
- * variants give bit-identical results.  Set by the autotuner
+ * rows and the loads of two planes in flight, variant 7 = row form (z extent a
+ * multiple of 4 and at most 256): one wave per z row with four cells per lane,
+ * `tile_rows` (4 or 8; 14 in FP32) output rows plus two halo rows per
+ * workgroup, non-temporal output stores, variant 8 = variant 7 with ordinary
+ * stores, variant 9 = variant 7 with non-temporal loads and ordinary stores,
+ * variant 10 = non-temporal loads and stores; `chunk` x-planes per workgroup
+ * (per wave for variant 3) and XCD-aware tile placement when bit 0 of
+ * `xcd_remap` is set; for the row form (7 to 10), bit 1 of `xcd_remap` makes
+ * every other step walk its tiles in reverse.  The row form returns
+ * CFD_E_UNSUPPORTED for other z extents.
+ * All variants give bit-identical results.  Set by the autotuner
  * (cfdarcov2_amd/autotune.py). */
 int cfd_mesh_set_wave_shape(cfd_mesh_t mesh, int variant, int tile_rows, int chunk, int xcd_remap);
 

@@ -205,6 +205,12 @@ def test_wave_full_size_properties(hip_ctx, oracle_lib):
     got2 = [_np(t) for t in run(g32, p0f, torch.float32, 2)]
     for name, g, r in zip("puvw", got2, ref2):
         assert np.array_equal(g, r), f"256^3 field {name}"
+    for shape in ((7, 4, 4, 0), (8, 8, 16, 3), (9, 4, 4, 2)):  # row forms, forward and reversed tile order
+        g32.set_wave_shape(*shape)
+        got2 = [_np(t) for t in run(g32, p0f, torch.float32, 2)]
+        for name, g, r in zip("puvw", got2, ref2):
+            assert np.array_equal(g, r), f"256^3 field {name}, shape {shape}"
+    g32.set_wave_shape(1, 16, 32, 1)
     del got2, ref2
     # (b) linearity
     g64 = capi.HexMesh(hip_ctx, *case.n, *case.spacing, dtype=np.float64)
@@ -326,11 +332,16 @@ ROW_CASES = [
     ((6, 5, 4), (0.25, 0.25, 0.25), (7, 4, 2, 0)),
     ((1, 3, 8), (0.25, 0.25, 0.25), (7, 4, 16, 0)),
     ((13, 11, 60), (0.375, 0.25, 0.125), (7, 8, 16, 0)),
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (8, 4, 5, 2)),
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (8, 8, 3, 3)),
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (7, 14, 4, 2)),
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (9, 4, 6, 2)),
+    ((9, 30, 256), (1.0 / 16, 1.0 / 32, 1.0 / 256), (10, 8, 4, 3)),
 ]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("case", ROW_CASES, ids=[f"{c[0][0]}x{c[0][1]}x{c[0][2]}-ty{c[2][1]}-L{c[2][2]}" for c in ROW_CASES])
+@pytest.mark.parametrize("case", ROW_CASES, ids=[f"{c[0][0]}x{c[0][1]}x{c[0][2]}-v{c[2][0]}-ty{c[2][1]}-L{c[2][2]}-r{c[2][3]}" for c in ROW_CASES])
 def test_wave_row_form(hip_ctx, oracle_lib, dtype, case):
     """Row form of the acoustic step (one wave per z row, four cells per lane,
     z-neighbours by lane shifts): the oracle's bits."""
