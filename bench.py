@@ -27,7 +27,7 @@ if ROOT not in sys.path:
 METRIC = "MCUPS + % HBM roofline (1 GPU); adjoint/forward cost ratio; batched vs plain"
 HBM_PEAK_GBS = 8000.0
 KERNELS = {0: "wave_uniform_k", 1: "wave_march_k", 2: "wave_march_k", 3: "wave_reg_k", 4: "wave_march_k",
-           5: "wave_march_occ_k", 6: "wave_march_pf2_k", 7: "wave_row_k", 8: "wave_row_k", 9: "wave_row_k", 10: "wave_row_k"}
+           5: "wave_march_occ_k", 6: "wave_march_pf2_k", 7: "wave_row_k", 8: "wave_row_k", 9: "wave_row_k", 10: "wave_row_k", 11: "wave_row2_k", 12: "wave_row2_k"}
 
 
 def _traffic(n, dtype, shape):
@@ -230,7 +230,8 @@ def suite_sod(ctx, cpu=True, verbose=False):
 PLAIN_WAVE_CANDIDATES = ([(0, 4, 8, 0)] + [(1, 4, L, r) for L in (4, 8, 16) for r in (0, 1)]
                          + [(2, 8, L, 0) for L in (8, 16)] + [(3, 2, L, 0) for L in (8, 16, 32)]
                          + [(4, 4, L, 0) for L in (8, 16)]
-                         + [(v, ty, L, 3) for v in (7, 9) for ty in (4, 8) for L in (8, 16)])
+                         + [(v, ty, L, 3) for v in (7, 9) for ty in (4, 8) for L in (8, 16)]
+                         + [(11, 8, L, 3) for L in (8, 16, 32)])
 
 
 def ensemble_api_seconds(specs, steps, timeout=900):
@@ -434,8 +435,15 @@ def main(argv=None):
         raise RuntimeError(f"non-finite state after the timed run ({bad} entries)")
 
     esize = 4 if args.dtype == "f32" else 8
-    bytes_per_launch = 8 * esize * N          # 4 fields read + 4 written
-    achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+    # the two-step kernels advance two steps per launch (in pairs of launches;
+    # a K field or a step count that is not a multiple of 4 leaves single steps)
+    two_step = shape["variant"] >= 11 and args.dtype == "f32" and args.steps % 4 == 0
+    steps_per_launch = 2 if two_step else 1
+    launch_ms = kernel_ms * steps_per_launch
+    # algorithmic bytes: 32 B (FP32) per cell-update, so per launch 8 fields x N
+    # x the steps one launch advances
+    bytes_per_launch = 8 * esize * N * steps_per_launch
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     traffic = _traffic(args.n, args.dtype, shape)
     out = {
         "metric": METRIC,
@@ -456,7 +464,11 @@ def main(argv=None):
                    "kernel_shape": {k: shape[k] for k in ("variant", "tile_rows", "chunk", "xcd_remap")}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": KERNELS[shape["variant"]], "kernel_ms": kernel_ms,
+                     "kernel": KERNELS[shape["variant"]], "kernel_ms": launch_ms,
+                     "steps_per_launch": steps_per_launch,
+                     **({"note": "two steps per launch: every field crosses HBM once per two steps, so the rate at "
+                                 "32 B per cell-update can pass what a one-step kernel streams"}
+                        if steps_per_launch == 2 else {}),
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
     mesh.close()

@@ -58,6 +58,11 @@ template <class T>
 bool launch_wave_row(const UniformGeo<T>& g, void* const* in, void* const* out, double K, const void* Kf, double dt,
                      int ty, int L, int order, int cache, hipStream_t s);
 
+// Two steps per pass of the row form (wave_row2.hip), FP32 and constant K.
+// Returns false (nothing launched) for meshes or field pointers it cannot take.
+bool launch_wave_row2(const UniformGeo<float>& g, void* const* in, void* const* out, double K, double dt, int ty,
+                      int L, int order, bool nt, hipStream_t s);
+
 template <class T>
 void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamma, double cfl, int64_t nsteps,
                   void* dt_hist, hipStream_t s);

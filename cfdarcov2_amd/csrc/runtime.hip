@@ -189,11 +189,18 @@ template <class T> static void create_mesh(cfd_mesh_s* m, const cfd_hex_desc& d)
         u.vol_pow2 = (u.vol > T(0) && std::frexp(u.vol, &ex) == T(0.5)) ? 1 : 0;
         u.inv_vol = T(1) / u.vol;
         m->uniform = true;
-        if (wave_row_supported(d.nz)) {  // row form, non-temporal loads (fastest measured at 256^3)
-            m->wave_variant = 9;
-            m->wave_ty = sizeof(T) == 4 ? 8 : 4;
-            m->wave_L = 16;
-            m->wave_remap = 3;
+        if (wave_row_supported(d.nz)) {  // row forms, the fastest measured at 256^3
+            if constexpr (sizeof(T) == 4) {  // two steps per pass
+                m->wave_variant = 11;
+                m->wave_ty = 8;
+                m->wave_L = 32;
+                m->wave_remap = 3;
+            } else {
+                m->wave_variant = 7;
+                m->wave_ty = 4;
+                m->wave_L = 32;
+                m->wave_remap = 1;
+            }
         }
         if constexpr (sizeof(T) == 4) m->ug32 = u; else m->ug64 = u;
     } else {
@@ -677,9 +684,15 @@ int cfd_mesh_set_euler_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk
 int cfd_mesh_set_wave_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk, int xcd_remap) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");
-        CFD_REQUIRE(variant >= 0 && variant <= 10, "wave variant must be 0 to 10");
+        CFD_REQUIRE(variant >= 0 && variant <= 12, "wave variant must be 0 to 12");
         const bool row_form = variant >= 7;
-        if (row_form) {
+        if (variant >= 11) {
+            CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || tile_rows == 10,
+                        "two-step row-form (11, 12) tile_rows must be 4, 8 or 10");
+            if (m->dtype != CFD_F32) throw Error(CFD_E_UNSUPPORTED, "the two-step row form is FP32 only");
+            if (!wave_row_supported(m->Z))
+                throw Error(CFD_E_UNSUPPORTED, "the row form needs a z extent that is a multiple of 4 and at most 256");
+        } else if (row_form) {
             CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || (tile_rows == 14 && m->dtype == CFD_F32),
                         "row-form (7 to 10) tile_rows must be 4 or 8 (or 14 in FP32)");
             if (!wave_row_supported(m->Z))

@@ -1,0 +1,342 @@
+// Two time steps per pass of the row form (variants 11 and 12): FP32, constant
+// K, z extent a multiple of 4 and at most 256.
+//
+// The single-step row form (wave_row.hip) reads and writes every field once per
+// step and runs at the memory's streaming rate.  This kernel advances the
+// state two steps per pass, so each field crosses HBM once per two steps; the
+// price is recomputing the first step on a wider band of rows.
+//
+// A workgroup owns output rows y0 .. y0+TY-1 and holds one wave per row of
+// y0-3 .. y0+TY+2 (TY + 6 waves).  With d the distance of a wave's row from the
+// output band:
+//   * d <= 2: the first step in full (u', v', w', p'), as the single-step kernel;
+//   * d == 3: the first step's v' only (the y-halo of the d == 2 rows' p');
+//   * d <= 1: the second step's v'' and w'';
+//   * d == 0: the second step in full, and the outputs.
+// Marching along x, iteration x computes the first step at plane x and the
+// second step at plane x-2, with every quantity the second step needs kept in
+// per-lane register queues (p' of three planes, u', v', w' of two, u'' of three).
+// Four LDS planes carry the y-neighbours: p and v' of the first step, p' and v''
+// of the second.  Per iteration:
+//   A. u'(x+1); p(x) and p'(x-2) into LDS;
+//   B. (barrier) v'(x), w'(x) and v''(x-2), w''(x-2); v'(x), v''(x-2) into LDS;
+//   C. (barrier) p'(x), then u''(x-1), then p''(x-2) and the stores.
+// A plane is rewritten one iteration later, after a barrier every wave passes
+// only once its reads of the plane are done, so single buffers suffice.
+//
+// Each value is computed by the operation sequence of the single-step kernels,
+// so two passes of this kernel equal four single steps bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "march.hpp"
+#include "row.hpp"
+
+namespace cfd {
+
+namespace row2 {
+
+using V4 = row::Row4<float>::type;
+using row::kWaveShl1;
+using row::kWaveShr1;
+using row::lane_shift;
+
+// z-neighbours of a row vector, neighbour = self at the row's two ends
+__device__ inline void z_nbrs(const V4& c, int lane, int nl, V4& lo, V4& hi) {
+    lo[0] = lane_shift<kWaveShr1>(c[0], c[3]);
+    lo[1] = c[0];
+    lo[2] = c[1];
+    lo[3] = c[2];
+    hi[0] = c[1];
+    hi[1] = c[2];
+    hi[2] = c[3];
+    hi[3] = lane_shift<kWaveShl1>(c[3], c[0]);
+    if (lane == 0) lo[0] = c[0];
+    if (lane == nl - 1) hi[3] = c[3];
+}
+
+template <bool PW>
+__device__ inline V4 vel4(const V4& lo, const V4& c, const V4& hi, const V4& q, float A,
+                          const VolDiv<float, PW>& Vd, float dt) {
+    V4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = mvel(lo[i], c[i], hi[i], q[i], A, Vd, dt);
+    return r;
+}
+
+// pressure update of a row vector: the single-step kernels' flux sums
+template <bool PW>
+__device__ inline V4 pres4(const V4& P, const V4& Um, const V4& U, const V4& Up, bool xlo, bool xhi, const V4& vn,
+                           const V4& vyl, const V4& vyh, bool first, bool last, const V4& wn, int lane, int nl,
+                           float Ax, float Ay, float Az, const VolDiv<float, PW>& Vd, float K, float dt) {
+    V4 wzl, wzh;
+    z_nbrs(wn, lane, nl, wzl, wzh);
+    V4 pn;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float fx_lo = xlo ? mlin(U[i], Um[i]) : 0.f;
+        const float fx_hi = xhi ? mlin(U[i], Up[i]) : 0.f;
+        const float fy_lo = first ? 0.f : mlin(vn[i], vyl[i]);
+        const float fy_hi = last ? 0.f : mlin(vn[i], vyh[i]);
+        const float fz_lo = (i == 0 && lane == 0) ? 0.f : mlin(wn[i], wzl[i]);
+        const float fz_hi = (i == 3 && lane == nl - 1) ? 0.f : mlin(wn[i], wzh[i]);
+        float S = mggp(fx_lo, fx_hi, Ax, Vd);
+        S = S + mggp(fy_lo, fy_hi, Ay, Vd);
+        S = S + mggp(fz_lo, fz_hi, Az, Vd);
+        pn[i] = dt * (K * S) + P[i];
+    }
+    return pn;
+}
+
+template <int TY, bool PW, bool XB, bool NTL>
+__device__ inline void tile2(const UniformGeo<float>& g, const float* __restrict__ p, const float* __restrict__ u,
+                             const float* __restrict__ v, const float* __restrict__ w, float* __restrict__ po,
+                             float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo, float K,
+                             float dt, int64_t x0, int64_t x1, int y0, int j, bool nt, V4 (*P1)[64], V4 (*V1)[64],
+                             V4 (*P2)[64], V4 (*V2)[64]) {
+    const int64_t X = g.X;
+    const int Y = int(g.Y), Z = int(g.Z);
+    const int64_t SX = int64_t(Y) * Z;
+    const float Ax = g.area[0], Ay = g.area[1], Az = g.area[2];
+    const VolDiv<float, PW> Vd{g.vol, g.inv_vol};
+    const int lane = threadIdx.x;
+    const int nl = Z / 4;
+    const bool lv = lane < nl;
+
+    // ---- the wave's roles (uniform across the wave) --------------------------
+    const int y = y0 - 3 + j;
+    const bool real = y >= 0 && y < Y;
+    const int d = y < y0 ? y0 - y : (y > y0 + TY - 1 ? y - (y0 + TY - 1) : 0);
+    const bool rA = real && d <= 2;  // first step in full
+    const bool rH = real && d == 3;  // first step's v' only
+    const bool rB = real && d <= 1;  // second step's v'', w''
+    const bool rO = real && d == 0;  // second step in full, outputs
+    const bool first = y == 0, last = y == Y - 1;
+    const int jl = first ? j : j - 1, jh = last ? j : j + 1;  // LDS rows of the y-neighbours (self at walls)
+    const bool lowerH = y < y0;
+    const int orow = lowerH ? (y >= 1 ? y - 1 : y) : (y + 1 < Y ? y + 1 : y);  // halo wave's outer row
+    const int jin = lowerH ? j + 1 : j - 1;                                    // and its inner LDS row
+    const int64_t rb = int64_t(real ? y : 0) * Z;
+    const int64_t ob = int64_t(real ? orow : 0) * Z;
+
+    // per-plane loads as (field, plane offset, row) slots:
+    //   d <= 2: a = p(xx+2), b = u(xx+1), c = v(xx), d = w(xx)
+    //   d == 3: a = p(xx), b = p(xx) of the outer row, c = v(xx)
+    const float* const fbp = rA ? u : p;
+    const int da = rA ? 2 : 0, db = rA ? 1 : 0;
+    const int64_t rbb = rA ? rb : ob;
+    const bool ea = rA || rH;
+    struct Fetch {
+        V4 a, b, c, d;
+    };
+    const V4 zero4 = {0.f, 0.f, 0.f, 0.f};
+    auto ld = [&](const float* base, int64_t off) -> V4 {
+        const V4* q = reinterpret_cast<const V4*>(base + off) + lane;
+        if constexpr (NTL) return __builtin_nontemporal_load(q);
+        else return *q;
+    };
+    auto fetch = [&](int64_t xx, Fetch& f) {
+        if (!lv || !ea) return;
+        const int64_t pb = xx * SX;
+        if (!XB || xx + da < X) f.a = ld(p, pb + da * SX + rb);
+        if (!XB || xx + db < X) f.b = ld(fbp, pb + db * SX + rbb);
+        f.c = ld(v, pb + rb);
+        if (rA) f.d = ld(w, pb + rb);
+    };
+
+    const int64_t xs = x0 >= 2 ? x0 - 2 : 0;  // first plane of the first step
+    const int64_t xe = x1 + 2;                // end of the iterations
+    const int64_t xs1 = xe < X ? xe : X;      // end of the first step's planes
+
+    // ---- warm-up of the first step's x queue (the single-step kernel's) ------
+    V4 P0 = zero4, Pp1 = zero4, Um1 = zero4, U0 = zero4;
+    if (rA && lv) {
+        V4 Pm1 = zero4;
+        P0 = ld(p, xs * SX + rb);
+        if (xs >= 1) Pm1 = ld(p, (xs - 1) * SX + rb);
+        if (xs + 1 < X) Pp1 = ld(p, (xs + 1) * SX + rb);
+        if (xs >= 1) {
+            const V4 Pm2 = (xs >= 2) ? ld(p, (xs - 2) * SX + rb) : Pm1;
+            const V4 um = ld(u, (xs - 1) * SX + rb);
+            Um1 = vel4(Pm2, Pm1, P0, um, Ax, Vd, dt);
+        }
+        const V4 uc = ld(u, xs * SX + rb);
+        V4 lo = P0, hi = P0;
+        if (xs >= 1) lo = Pm1;
+        if (xs + 1 < X) hi = Pp1;
+        U0 = vel4(lo, P0, hi, uc, Ax, Vd, dt);
+    }
+    // second-step queues: p'(x-2), p'(x-1); v', w' of x-2, x-1; u''(x-3), u''(x-2)
+    V4 Q2 = zero4, Q1 = zero4, Va2 = zero4, Va1 = zero4, Wa2 = zero4, Wa1 = zero4, R3 = zero4, R2 = zero4;
+
+    Fetch nx{zero4, zero4, zero4, zero4};
+    fetch(xs, nx);
+
+    for (int64_t x = xs; x < xe; ++x) {
+        const bool s1 = !XB || x < X;  // first step at plane x
+        const int64_t xp = x - 2;      // second step's plane
+        const bool s2 = xp >= x0 && xp < x1;
+        const int64_t m = x - 1;       // u'' computed at plane m
+        const bool su = m >= x0 - 1 && m >= 0 && m < X;
+        const Fetch cur = nx;
+        if (x + 1 < xs1) fetch(x + 1, nx);
+
+        // A. u'(x+1); p(x) and p'(x-2) into LDS
+        V4 Up1 = zero4;
+        if (rA) {
+            if (s1) {
+                if (!XB || x + 1 < X) {
+                    const V4 hi = (!XB || x + 2 < X) ? cur.a : Pp1;
+                    Up1 = vel4(P0, Pp1, hi, cur.b, Ax, Vd, dt);
+                }
+                P1[j][lane] = P0;
+            }
+            if (s2) P2[j][lane] = Q2;
+        } else if (rH && s1) {
+            P1[j][lane] = cur.a;
+        }
+        __syncthreads();
+
+        // B. v'(x), w'(x); v''(x-2), w''(x-2)
+        V4 vn = zero4, wn = zero4, v2 = zero4, w2 = zero4;
+        if (rA) {
+            if (s1) {
+                const V4 pyl = P1[jl][lane];
+                const V4 pyh = P1[jh][lane];
+                V4 pzl, pzh;
+                z_nbrs(P0, lane, nl, pzl, pzh);
+                vn = vel4(pyl, P0, pyh, cur.c, Ay, Vd, dt);
+                wn = vel4(pzl, P0, pzh, cur.d, Az, Vd, dt);
+                V1[j][lane] = vn;
+            }
+            if (rB && s2) {
+                const V4 qyl = P2[jl][lane];
+                const V4 qyh = P2[jh][lane];
+                V4 qzl, qzh;
+                z_nbrs(Q2, lane, nl, qzl, qzh);
+                v2 = vel4(qyl, Q2, qyh, Va2, Ay, Vd, dt);
+                if (rO) w2 = vel4(qzl, Q2, qzh, Wa2, Az, Vd, dt);
+                V2[j][lane] = v2;
+            }
+        } else if (rH && s1) {
+            const V4 in = P1[jin][lane];
+            const V4 pl = lowerH ? cur.b : in;
+            const V4 ph = lowerH ? in : cur.b;
+            V1[j][lane] = vel4(pl, cur.a, ph, cur.c, Ay, Vd, dt);
+        }
+        __syncthreads();
+
+        // C. p'(x); u''(x-1); p''(x-2) and the outputs
+        if (rA) {
+            V4 Q0 = zero4;
+            if (s1) {
+                const V4 vyl = first ? vn : V1[j - 1][lane];
+                const V4 vyh = last ? vn : V1[j + 1][lane];
+                Q0 = pres4(P0, Um1, U0, Up1, !XB || x > 0, !XB || x < X - 1, vn, vyl, vyh, first, last, wn, lane,
+                           nl, Ax, Ay, Az, Vd, K, dt);
+            }
+            V4 R1 = zero4;
+            if (rO) {
+                if (su) {
+                    const V4 lo = (!XB || m >= 1) ? Q2 : Q1;
+                    const V4 hi = (!XB || x < X) ? Q0 : Q1;
+                    R1 = vel4(lo, Q1, hi, Um1, Ax, Vd, dt);
+                }
+                if (s2) {
+                    const V4 vyl = first ? v2 : V2[j - 1][lane];
+                    const V4 vyh = last ? v2 : V2[j + 1][lane];
+                    const V4 pn = pres4(Q2, R3, R2, R1, !XB || xp > 0, !XB || xp < X - 1, v2, vyl, vyh, first, last,
+                                        w2, lane, nl, Ax, Ay, Az, Vd, K, dt);
+                    if (lv) {
+                        const int64_t o = xp * SX + rb;
+                        row::put(reinterpret_cast<V4*>(po + o) + lane, pn, nt);
+                        row::put(reinterpret_cast<V4*>(uo + o) + lane, R2, nt);
+                        row::put(reinterpret_cast<V4*>(vo + o) + lane, v2, nt);
+                        row::put(reinterpret_cast<V4*>(wo + o) + lane, w2, nt);
+                    }
+                }
+            }
+            P0 = Pp1;
+            Pp1 = cur.a;
+            Um1 = U0;
+            U0 = Up1;
+            Q2 = Q1;
+            Q1 = Q0;
+            Va2 = Va1;
+            Va1 = vn;
+            Wa2 = Wa1;
+            Wa1 = wn;
+            R3 = R2;
+            R2 = R1;
+        }
+    }
+}
+
+}  // namespace row2
+
+template <int TY, bool PW, bool NTL>
+__global__ void __launch_bounds__(64 * (TY + 6))
+wave_row2_k(UniformGeo<float> g, const float* __restrict__ p, const float* __restrict__ u,
+            const float* __restrict__ v, const float* __restrict__ w, float* __restrict__ po, float* __restrict__ uo,
+            float* __restrict__ vo, float* __restrict__ wo, float K, float dt, int L, int nyt, int order, int nt) {
+    using row2::V4;
+    __shared__ V4 P1[TY + 6][64];
+    __shared__ V4 V1[TY + 6][64];
+    __shared__ V4 P2[TY + 6][64];
+    __shared__ V4 V2[TY + 6][64];
+    const int64_t nwg = gridDim.x;
+    int64_t lin = blockIdx.x;
+    if (order & 2) lin = nwg - 1 - lin;
+    if (order & 1) lin = (lin % 8) * (nwg / 8) + lin / 8;
+    const int yt = int(lin % nyt);
+    const int64_t xc = lin / nyt;
+    const int j = __builtin_amdgcn_readfirstlane(int(threadIdx.y));
+    const int64_t X = g.X;
+    const int64_t x0 = xc * L;
+    const int64_t x1 = (x0 + L < X) ? x0 + L : X;
+    const int y0 = yt * TY;
+    // chunks four planes away from the x walls need no x-boundary tests
+    if (x0 >= 4 && x1 + 4 <= X)
+        row2::tile2<TY, PW, false, NTL>(g, p, u, v, w, po, uo, vo, wo, K, dt, x0, x1, y0, j, nt != 0, P1, V1, P2, V2);
+    else
+        row2::tile2<TY, PW, true, NTL>(g, p, u, v, w, po, uo, vo, wo, K, dt, x0, x1, y0, j, nt != 0, P1, V1, P2, V2);
+}
+
+template <int TY, bool PW>
+static void launch_row2_k(const UniformGeo<float>& g, void* const* in, void* const* out, double K, double dt, int L,
+                          int order, bool nt, hipStream_t s) {
+    const int nyt = int((g.Y + TY - 1) / TY);
+    const int64_t nxc = (g.X + L - 1) / L;
+    const int64_t nwg = int64_t(nyt) * nxc;
+    CFD_REQUIRE(nwg < (int64_t(1) << 31), "row march grid too large");
+    const int ord = ((order & 1) && nwg % 8 == 0 ? 1 : 0) | (order & 2);
+    wave_row2_k<TY, PW, true><<<dim3(unsigned(nwg)), dim3(64, TY + 6), 0, s>>>(
+        g, (const float*)in[0], (const float*)in[1], (const float*)in[2], (const float*)in[3], (float*)out[0],
+        (float*)out[1], (float*)out[2], (float*)out[3], float(K), float(dt), L, nyt, ord, nt ? 1 : 0);
+    CFD_HIP(hipGetLastError());
+}
+
+template <int TY>
+static void launch_row2_ty(const UniformGeo<float>& g, void* const* in, void* const* out, double K, double dt, int L,
+                           int order, bool nt, hipStream_t s) {
+    if (g.vol_pow2) launch_row2_k<TY, true>(g, in, out, K, dt, L, order, nt, s);
+    else launch_row2_k<TY, false>(g, in, out, K, dt, L, order, nt, s);
+}
+
+bool launch_wave_row2(const UniformGeo<float>& g, void* const* in, void* const* out, double K, double dt, int ty,
+                      int L, int order, bool nt, hipStream_t s) {
+    if (!wave_row_supported(g.Z)) return false;
+    const uintptr_t al = 4 * sizeof(float) - 1;
+    for (int i = 0; i < 4; ++i)
+        if ((reinterpret_cast<uintptr_t>(in[i]) & al) || (reinterpret_cast<uintptr_t>(out[i]) & al)) return false;
+    switch (ty) {
+        case 4: launch_row2_ty<4>(g, in, out, K, dt, L, order, nt, s); break;
+        case 8: launch_row2_ty<8>(g, in, out, K, dt, L, order, nt, s); break;
+        case 10: launch_row2_ty<10>(g, in, out, K, dt, L, order, nt, s); break;
+        default: throw Error(CFD_E_INVALID, "two-step row-form tiles have 4, 8 or 10 rows");
+    }
+    return true;
+}
+
+}  // namespace cfd

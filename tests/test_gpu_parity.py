@@ -210,8 +210,15 @@ def test_wave_full_size_properties(hip_ctx, oracle_lib):
         got2 = [_np(t) for t in run(g32, p0f, torch.float32, 2)]
         for name, g, r in zip("puvw", got2, ref2):
             assert np.array_equal(g, r), f"256^3 field {name}, shape {shape}"
+    om = oracle_lib.OracleMesh(*case.n, *case.spacing, dtype=np.float32)
+    ref4 = om.wave_run(p0f, zf, zf, zf, -1.0, case.dt, 4)
+    del om
+    g32.set_wave_shape(11, 8, 16, 3)  # two passes of two steps each
+    got4 = [_np(t) for t in run(g32, p0f, torch.float32, 4)]
+    for name, g, r in zip("puvw", got4, ref4):
+        assert np.array_equal(g, r), f"256^3 field {name}, two steps per pass"
     g32.set_wave_shape(1, 16, 32, 1)
-    del got2, ref2
+    del got2, ref2, got4, ref4
     # (b) linearity
     g64 = capi.HexMesh(hip_ctx, *case.n, *case.spacing, dtype=np.float64)
     d1 = run(g64, case.p0, torch.float64, case.steps)
@@ -337,6 +344,13 @@ ROW_CASES = [
     ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (7, 14, 4, 2)),
     ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (9, 4, 6, 2)),
     ((9, 30, 256), (1.0 / 16, 1.0 / 32, 1.0 / 256), (10, 8, 4, 3)),
+    # two steps per pass (FP32): 7 steps = one pair of two-step passes + 3 single steps
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (11, 4, 5, 0)),
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (11, 8, 3, 3)),
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (12, 10, 16, 2)),
+    ((30, 27, 256), (1.0 / 32, 1.0 / 32, 1.0 / 256), (11, 8, 4, 1)),
+    ((6, 5, 4), (0.25, 0.25, 0.25), (11, 4, 1, 0)),
+    ((13, 11, 60), (0.375, 0.25, 0.125), (12, 8, 2, 3)),
 ]
 
 
@@ -347,8 +361,8 @@ def test_wave_row_form(hip_ctx, oracle_lib, dtype, case):
     z-neighbours by lane shifts): the oracle's bits."""
     from cfdarcov2_amd import capi
     n, d, shape = case
-    if dtype == np.float64 and shape[1] == 14:
-        pytest.skip("14-row tiles are FP32 only")
+    if dtype == np.float64 and (shape[1] == 14 or shape[0] >= 11):
+        pytest.skip("14-row tiles and the two-step form are FP32 only")
     print(f"\n[case] wave row form n={n} shape={shape} {np.dtype(dtype).name}", flush=True)
     rng = np.random.default_rng(41)
     gm = capi.HexMesh(hip_ctx, *n, *d, dtype=dtype)
@@ -364,7 +378,8 @@ def test_wave_row_form(hip_ctx, oracle_lib, dtype, case):
         assert np.array_equal(g, r), f"field {name}: max diff {np.max(np.abs(g - r))}"
 
 
-def test_wave_row_form_K_field_and_misaligned(hip_ctx, oracle_lib):
+@pytest.mark.parametrize("shape", [(7, 4, 3, 0), (11, 4, 3, 0)])
+def test_wave_row_form_K_field_and_misaligned(hip_ctx, oracle_lib, shape):
     """Row form with a per-cell K field, then with field pointers that are not
     16-byte aligned (the launch takes the 2.5D tile instead): the oracle's bits
     in both."""
@@ -372,14 +387,14 @@ def test_wave_row_form_K_field_and_misaligned(hip_ctx, oracle_lib):
     n, d = (10, 9, 36), (0.125, 0.125, 0.0625)
     rng = np.random.default_rng(7)
     gm = capi.HexMesh(hip_ctx, *n, *d, dtype=np.float32)
-    gm.set_wave_shape(7, 4, 3, 0)
+    gm.set_wave_shape(*shape)
     om = oracle_lib.OracleMesh(*n, *d, dtype=np.float32)
     f = [rng.standard_normal(om.N).astype(np.float32) for _ in range(4)]
     K = (-(0.5 + rng.random(om.N)) ** 2).astype(np.float32)
     A = [_t(a, np.float32) for a in f]
     B = [torch.empty_like(A[0]) for _ in range(4)]
-    got = [_np(t) for t in gm.wave_run(A, B, 0.0, 0.02, 5, K_field=_t(K, np.float32))]
-    ref = om.wave_run(*f, 0.0, 0.02, 5, K_field=K)
+    got = [_np(t) for t in gm.wave_run(A, B, 0.0, 0.02, 9, K_field=_t(K, np.float32))]
+    ref = om.wave_run(*f, 0.0, 0.02, 9, K_field=K)
     for name, g, r in zip("puvw", got, ref):
         assert np.array_equal(g, r), f"K field, field {name}"
     # views starting one element into a larger buffer
@@ -388,8 +403,8 @@ def test_wave_row_form_K_field_and_misaligned(hip_ctx, oracle_lib):
     B = [b[1:] for b in big[4:]]
     for a, x in zip(A, f):
         a.copy_(_t(x, np.float32))
-    got = [_np(t) for t in gm.wave_run(A, B, -1.0, 0.02, 5)]
-    ref = om.wave_run(*f, -1.0, 0.02, 5)
+    got = [_np(t) for t in gm.wave_run(A, B, -1.0, 0.02, 9)]
+    ref = om.wave_run(*f, -1.0, 0.02, 9)
     for name, g, r in zip("puvw", got, ref):
         assert np.array_equal(g, r), f"misaligned, field {name}"
 
@@ -404,3 +419,6 @@ def test_wave_row_form_rejects_unsupported_rows(hip_ctx):
         capi.HexMesh(hip_ctx, 8, 8, 64, 0.125, 0.125, 0.125, dtype=np.float32).set_wave_shape(7, 6, 8, 0)
     with pytest.raises(capi.CfdError):
         capi.HexMesh(hip_ctx, 8, 8, 64, 0.125, 0.125, 0.125, dtype=np.float64).set_wave_shape(7, 14, 8, 0)
+    with pytest.raises(capi.CfdError) as e:
+        capi.HexMesh(hip_ctx, 8, 8, 64, 0.125, 0.125, 0.125, dtype=np.float64).set_wave_shape(11, 8, 8, 0)
+    assert e.value.code == -4
