@@ -262,7 +262,7 @@ def ensemble_api_seconds(specs, steps, timeout=900):
     return json.loads(line)["ensemble_api_seconds"]
 
 
-def suite_ensemble(ctx, steps=500):
+def suite_ensemble(ctx, steps=500, verbose=False):
     """64 heat/acoustic cases of seeded uneven sides, FP32, 500 steps: the
     autotuned batched graph run against 64 sequential single-case runs (each
     case with the fused kernel shape tuned for its own size), and against the
@@ -284,7 +284,7 @@ def suite_ensemble(ctx, steps=500):
     ens.run_sequential(2)
     ens.reset()
     t_seq = time_gpu(lambda: ens.run_sequential(steps), reps=1)
-    best = tune_batch(ens, steps=20)
+    best = tune_batch(ens, steps=20, verbose=verbose)
     ens.run(steps)  # capture the full-length graph
     ens.reset()
     t_b = time_gpu(lambda: ens.run(steps), reps=1)
@@ -390,7 +390,7 @@ def main(argv=None):
         ctx = capi.Context(0)
         fns = {"heat64_f64": lambda: suite_heat(ctx, cpu=False), "wave256_f64": lambda: suite_wave_f64(ctx),
                "sod512x64x64_f32": lambda: suite_sod(ctx, cpu=False, verbose=args.tune_verbose),
-               "ensemble64_f32": lambda: suite_ensemble(ctx), "adjoint128_f32": lambda: suite_adjoint(ctx)}
+               "ensemble64_f32": lambda: suite_ensemble(ctx, verbose=args.tune_verbose), "adjoint128_f32": lambda: suite_adjoint(ctx)}
         rec = fns[args.only]()
         ctx.close()
         print(json.dumps({"workload_only": args.only, **rec}), flush=True)

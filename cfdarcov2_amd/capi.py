@@ -36,7 +36,7 @@ EXPORTS = [
     "cfd_grad", "cfd_lapl", "cfd_stab", "cfd_ewise", "cfd_cfl_min", "cfd_count_nonfinite",
     "cfd_heat_run", "cfd_wave_run", "cfd_euler_run", "cfd_mesh_set_wave_shape", "cfd_mesh_set_euler_shape",
     "cfd_wave_traces", "cfd_wave_adjoint",
-    "cfd_batch_create", "cfd_batch_set_shape", "cfd_batch_run", "cfd_batch_destroy",
+    "cfd_batch_create", "cfd_batch_set_form", "cfd_batch_set_shape", "cfd_batch_run", "cfd_batch_destroy",
     "cfd_program_compile", "cfd_program_compile_only", "cfd_program_launch", "cfd_program_destroy",
     "cfd_mesh_geometry",
 ]
@@ -106,6 +106,7 @@ def load_library(path: str = LIB_PATH):
         "cfd_program_launch": [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), pp, vp],
         "cfd_program_destroy": [vp],
         "cfd_batch_create": [vp, ctypes.POINTER(BatchCase), i, pp],
+        "cfd_batch_set_form": [vp, i],
         "cfd_batch_set_shape": [vp, i, i],
         "cfd_batch_run": [vp, i64, i, vp],
         "cfd_batch_destroy": [vp],

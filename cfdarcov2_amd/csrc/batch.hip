@@ -17,6 +17,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "march.hpp"
+#include "row.hpp"
 
 namespace cfd {
 
@@ -66,6 +67,59 @@ __global__ void __launch_bounds__(64 * TYT) batch_step_k(const BatchCaseDev* __r
     }
 }
 
+// Row form of the batched step (form 1): every case's z extent is a multiple
+// of 4 and at most 256, so a case's tiles are (y tile, x chunk) pairs of the
+// row-form kernels (one wave per z row, TY output waves and two halo waves).
+template <int TY>
+__global__ void __launch_bounds__(64 * (TY + 2)) batch_row_k(const BatchCaseDev* __restrict__ cases, int ncases,
+                                                             int parity, int L) {
+    using V4 = row::Row4<float>::type;
+    __shared__ V4 sp[TY + 2][64];
+    __shared__ V4 sv[TY + 2][64];
+    const int64_t tile = blockIdx.x;
+    int lo = 0, hi = ncases - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cases[mid].tile0 <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchCaseDev& cs = cases[lo];
+    const int64_t t = tile - cs.tile0;
+    const int yt = int(t % cs.nyt);
+    const int64_t xc = t / cs.nyt;
+    const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.y));
+    const int64_t X = cs.g.X;
+    const int64_t x0 = xc * L;
+    const int64_t x1 = (x0 + L < X) ? x0 + L : X;
+    const int y0 = yt * TY * row::rows_per_wave(cs.g.Z);
+    float* const* in = parity ? cs.b : cs.a;
+    float* const* out = parity ? cs.a : cs.b;
+    const bool pw = cs.g.vol_pow2 != 0;
+    if (cs.kind == CFD_CASE_HEAT) {
+        const bool inner = x0 >= 1 && x1 + 1 <= X;
+        if (pw) {
+            if (inner) row::heat_row_tile<TY, true, false, true>(cs.g, in[0], out[0], cs.coef, cs.dt, x0, x1, y0, wv, sp);
+            else row::heat_row_tile<TY, true, true, true>(cs.g, in[0], out[0], cs.coef, cs.dt, x0, x1, y0, wv, sp);
+        } else {
+            if (inner) row::heat_row_tile<TY, false, false, true>(cs.g, in[0], out[0], cs.coef, cs.dt, x0, x1, y0, wv, sp);
+            else row::heat_row_tile<TY, false, true, true>(cs.g, in[0], out[0], cs.coef, cs.dt, x0, x1, y0, wv, sp);
+        }
+    } else {
+        const bool inner = x0 >= 1 && x1 + 2 <= X;
+#define CFD_ROW_TILE(PW, XB)                                                                                     \
+    row::row_tile<float, TY, PW, false, XB, true>(cs.g, in[0], in[1], in[2], in[3], out[0], out[1], out[2], out[3], \
+                                                  cs.coef, nullptr, cs.dt, x0, x1, y0, wv, false, sp, sv)
+        if (pw) {
+            if (inner) CFD_ROW_TILE(true, false);
+            else CFD_ROW_TILE(true, true);
+        } else {
+            if (inner) CFD_ROW_TILE(false, false);
+            else CFD_ROW_TILE(false, true);
+        }
+#undef CFD_ROW_TILE
+    }
+}
+
 }  // namespace cfd
 
 using namespace cfd;
@@ -76,6 +130,7 @@ struct cfd_batch_s {
     BatchCaseDev* dev = nullptr;
     int tile_rows = 4;
     int chunk = 8;
+    int form = 0;  // 0: 2.5D march tiles, 1: row form
     int64_t total_tiles = 0;
     bool dirty = true;
     hipStream_t capture = nullptr;
@@ -89,8 +144,9 @@ static void batch_layout(cfd_batch_s* b, hipStream_t s) {
     const int TY = b->tile_rows;
     int64_t off = 0;
     for (auto& c : b->host) {
-        c.nzt = int((c.g.Z + 63) / 64);
-        c.nyt = int((c.g.Y + TY - 1) / TY);
+        c.nzt = b->form == 1 ? 1 : int((c.g.Z + 63) / 64);  // a row-form tile spans the z row
+        const int64_t rows = b->form == 1 ? int64_t(TY) * row::rows_per_wave(c.g.Z) : TY;  // rows per tile
+        c.nyt = int((c.g.Y + rows - 1) / rows);
         const int64_t nxc = (c.g.X + b->chunk - 1) / b->chunk;
         c.tile0 = off;
         off += int64_t(c.nzt) * c.nyt * nxc;
@@ -105,6 +161,14 @@ static void batch_layout(cfd_batch_s* b, hipStream_t s) {
 static void batch_launch(const cfd_batch_s* b, int parity, hipStream_t s) {
     const int n = int(b->host.size());
     const dim3 grid(unsigned(b->total_tiles));
+    if (b->form == 1) {
+        switch (b->tile_rows) {
+            case 4: batch_row_k<4><<<grid, dim3(64, 6), 0, s>>>(b->dev, n, parity, b->chunk); break;
+            case 8: batch_row_k<8><<<grid, dim3(64, 10), 0, s>>>(b->dev, n, parity, b->chunk); break;
+            default: throw Error(CFD_E_INVALID, "row-form batch tile rows must be 4 or 8");
+        }
+        return;
+    }
     switch (b->tile_rows) {
         case 4: batch_step_k<4, 1><<<grid, dim3(64, 4), 0, s>>>(b->dev, n, parity, b->chunk); break;
         case 8: batch_step_k<4, 2><<<grid, dim3(64, 4), 0, s>>>(b->dev, n, parity, b->chunk); break;
@@ -163,10 +227,33 @@ int cfd_batch_create(cfd_ctx_t ctx, const cfd_batch_case* cases, int ncases, cfd
     });
 }
 
+int cfd_batch_set_form(cfd_batch_t b, int form) {
+    return guarded([&] {
+        CFD_REQUIRE(b != nullptr, "null batch");
+        CFD_REQUIRE(form == 0 || form == 1, "batch form must be 0 (march tiles) or 1 (row form)");
+        if (form == 1) {
+            for (const auto& c : b->host) {
+                if (!wave_row_supported(c.g.Z))
+                    throw Error(CFD_E_UNSUPPORTED,
+                                "the row form needs every case's z extent to be a multiple of 4 and at most 256");
+                for (int f = 0; f < 4; ++f)
+                    if ((reinterpret_cast<uintptr_t>(c.a[f]) & 15) || (reinterpret_cast<uintptr_t>(c.b[f]) & 15))
+                        throw Error(CFD_E_UNSUPPORTED, "the row form needs 16-byte aligned state fields");
+            }
+            if (b->tile_rows != 4 && b->tile_rows != 8) b->tile_rows = 4;
+        }
+        if (form != b->form) {
+            b->form = form;
+            b->dirty = true;
+        }
+    });
+}
+
 int cfd_batch_set_shape(cfd_batch_t b, int tile_rows, int chunk) {
     return guarded([&] {
         CFD_REQUIRE(b != nullptr, "null batch");
-        CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || tile_rows == 16, "batch tile rows must be 4, 8 or 16");
+        CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || (tile_rows == 16 && b->form == 0),
+                    "batch tile rows must be 4, 8 or 16 (4 or 8 in the row form)");
         CFD_REQUIRE(chunk >= 1 && chunk <= 4096, "batch chunk must be in [1, 4096]");
         if (tile_rows != b->tile_rows || chunk != b->chunk) {
             b->tile_rows = tile_rows;
@@ -194,7 +281,7 @@ int cfd_batch_run(cfd_batch_t b, int64_t nsteps, int use_graph, void* stream) {
         // a replay starts from state_a again, so the state must end there
         CFD_REQUIRE(nsteps % 2 == 0, "cfd_batch_run: graph runs need an even step count");
         if (nsteps == 0) return;
-        const auto key = std::make_tuple(nsteps, b->tile_rows, b->chunk);
+        const auto key = std::make_tuple(nsteps, b->tile_rows * 2 + b->form, b->chunk);
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {
             hipGraph_t graph = nullptr;

@@ -4,6 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include "common.hpp"
+#include "march.hpp"
+
 namespace cfd {
 
 namespace row {
@@ -39,6 +42,306 @@ template <int CTRL> __device__ inline double lane_shift(double own, double v) {
 template <class V4> __device__ inline void put(V4* dst, const V4& v, bool nt) {
     if (nt) __builtin_nontemporal_store(v, dst);
     else *dst = v;
+}
+
+// Rows per wave: a wave holds RW = 256 / Z consecutive z rows (one contiguous
+// span of at most 256 cells, four per lane), so narrow meshes keep their lanes
+// busy.
+__host__ __device__ inline int rows_per_wave(int64_t Z) { return Z >= 256 ? 1 : int(256 / Z); }
+
+// Per-lane place of a wave's cells: the lane's first cell is (row r, column
+// zc) of the wave's span of RW rows; `span` says the lane holds cells at all.
+struct LanePlace {
+    int r, zc;
+    bool span;
+};
+__device__ inline LanePlace lane_place(int lane, int Z, int RW) {
+    const int c = 4 * lane;
+    return {c / Z, c % Z, c < RW * Z};
+}
+
+// z-neighbours of a lane's four cells, neighbour = self at the z walls
+template <class V4>
+__device__ inline void z_nbrs_row(const V4& c, bool zlo_wall, bool zhi_wall, V4& lo, V4& hi) {
+    lo[0] = lane_shift<kWaveShr1>(c[0], c[3]);
+    lo[1] = c[0];
+    lo[2] = c[1];
+    lo[3] = c[2];
+    hi[0] = c[1];
+    hi[1] = c[2];
+    hi[2] = c[3];
+    hi[3] = lane_shift<kWaveShl1>(c[3], c[0]);
+    if (zlo_wall) lo[0] = c[0];
+    if (zhi_wall) hi[3] = c[3];
+}
+
+// One (y tile, x chunk) of the single-step acoustic row form: TY output waves
+// of RW rows each (rows y0 .. y0+TY*RW-1) and two halo waves (rows y0-1 and
+// y0+TY*RW, one row each), marching x over [x0, x1).  The LDS planes hold the
+// tile's rows with the halo, row j at element j*Z.  Used by the single-mesh
+// kernel (wave_row.hip) and the batched ensemble kernel.
+template <class T, int TY, bool PW, bool KF, bool XB, bool NTL>
+__device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p, const T* __restrict__ u,
+                                const T* __restrict__ v, const T* __restrict__ w, T* __restrict__ po,
+                                T* __restrict__ uo, T* __restrict__ vo, T* __restrict__ wo, T K,
+                                const T* __restrict__ Kf, T dt, int64_t x0, int64_t x1, int y0, int wv, bool nt,
+                                typename Row4<T>::type (*sp)[64], typename Row4<T>::type (*sv)[64]) {
+    using V4 = typename Row4<T>::type;
+    const int64_t X = g.X;
+    const int Y = int(g.Y), Z = int(g.Z);
+    const int64_t SX = int64_t(Y) * Z;
+    const T Ax = g.area[0], Ay = g.area[1], Az = g.area[2];
+    const VolDiv<T, PW> Vd{g.vol, g.inv_vol};
+    const int lane = threadIdx.x;
+    const int RW = rows_per_wave(Z);
+    const int zq = Z / 4;                 // V4 entries per row
+    V4* const SP = &sp[0][0];
+    V4* const SV = &sv[0][0];
+
+    // ---- the wave's role (uniform) and the lane's cells ------------------------
+    const bool is_out = wv < TY;
+    const bool lower = wv == TY;
+    const LanePlace lp = is_out ? lane_place(lane, Z, RW) : LanePlace{0, 4 * lane, lane < zq};
+    const int y = is_out ? y0 + wv * RW + lp.r : (lower ? y0 - 1 : y0 + TY * RW);
+    const bool real = lp.span && y >= 0 && y < Y;
+    const bool out = is_out && real;             // a cell of an output row
+    const bool halo_v = !is_out && real;         // a cell of a halo row: v' of the row
+    const int own = is_out ? (1 + wv * RW) * zq + lane : (lower ? lane : (1 + TY * RW) * zq + lane);
+    const int ilo = y == 0 ? own : own - zq;     // LDS neighbours, self at the y walls
+    const int ihi = y == Y - 1 ? own : own + zq;
+    const bool first = y == 0, last = y == Y - 1;
+    const bool zlo = lp.zc == 0, zhi = lp.zc + 4 == Z;
+    // halo rows: the outer p row (clamped) and the LDS entry of the inner one
+    const int orow = lower ? (y >= 1 ? y - 1 : 0) : (y + 1 < Y ? y + 1 : Y - 1);
+    const int iin = lower ? own + zq : own - zq;
+    const int64_t rb = int64_t(real ? y : 0) * Z + lp.zc;   // first cell of the lane's run
+    const int64_t ob = int64_t(real ? orow : 0) * Z + lp.zc;
+
+    // Per-plane loads as (field, plane offset, row) slots:
+    //   output cells: a = p(xx+2), b = u(xx+1), c = v(xx), d = w(xx), k = K(xx);
+    //   halo cells:   a = p(xx), b = p(xx) of the outer row, c = v(xx).
+    const T* const fb = out ? u : p;
+    const int da = out ? 2 : 0, db = out ? 1 : 0;
+    const int64_t rbb = out ? rb : ob;
+    struct Fetch {
+        V4 a, b, c, d, k;
+    };
+    const V4 zero4 = {T(0), T(0), T(0), T(0)};
+    // NTL: non-temporal loads (streamed past the caches' retention, which then
+    // keep what this step writes for the next step to read)
+    auto ld = [&](const T* base, int64_t off) -> V4 {
+        const V4* a = reinterpret_cast<const V4*>(base + off);
+        if constexpr (NTL) return __builtin_nontemporal_load(a);
+        else return *a;
+    };
+    auto fetch = [&](int64_t xx, Fetch& f) {
+        if (!real) return;
+        const int64_t pb = xx * SX;
+        if (!XB || xx + da < X) f.a = ld(p, pb + da * SX + rb);
+        if (!XB || xx + db < X) f.b = ld(fb, pb + db * SX + rbb);
+        f.c = ld(v, pb + rb);
+        if (out) f.d = ld(w, pb + rb);
+        if constexpr (KF) {
+            if (out) f.k = ld(Kf, pb + rb);
+        }
+    };
+
+    // ---- warm-up of the output cells' x queue ---------------------------------
+    V4 P0 = zero4, Pp1 = zero4, Um1 = zero4, U0 = zero4;
+    if (out) {
+        V4 Pm1 = zero4;
+        P0 = ld(p, x0 * SX + rb);
+        if (x0 >= 1) Pm1 = ld(p, (x0 - 1) * SX + rb);
+        if (x0 + 1 < X) Pp1 = ld(p, (x0 + 1) * SX + rb);
+        if (x0 >= 1) {
+            const V4 Pm2 = (x0 >= 2) ? ld(p, (x0 - 2) * SX + rb) : Pm1;
+            const V4 um = ld(u, (x0 - 1) * SX + rb);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) Um1[i] = mvel(Pm2[i], Pm1[i], P0[i], um[i], Ax, Vd, dt);
+        }
+        const V4 uc = ld(u, x0 * SX + rb);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            U0[i] = mvel(x0 >= 1 ? Pm1[i] : P0[i], P0[i], x0 + 1 < X ? Pp1[i] : P0[i], uc[i], Ax, Vd, dt);
+    }
+
+    Fetch nx{zero4, zero4, zero4, zero4, zero4};
+    fetch(x0, nx);
+
+    for (int64_t x = x0; x < x1; ++x) {
+        const Fetch cur = nx;
+        if (x + 1 < x1) fetch(x + 1, nx);
+
+        // 1. u'(x+1); p into the LDS plane
+        V4 Up1 = zero4;
+        if (is_out) {
+            if (!XB || x + 1 < X) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    Up1[i] = mvel(P0[i], Pp1[i], (!XB || x + 2 < X) ? cur.a[i] : Pp1[i], cur.b[i], Ax, Vd, dt);
+            }
+            if (lp.span) SP[own] = P0;
+        } else if (halo_v) {
+            SP[own] = cur.a;
+        }
+        __syncthreads();
+
+        // 2. v' and w' of the plane; the halo rows' v'
+        V4 vn = zero4, wn = zero4;
+        if (is_out) {
+            const V4 pyl = SP[ilo];
+            const V4 pyh = SP[ihi];
+            V4 pzl, pzh;
+            z_nbrs_row(P0, zlo, zhi, pzl, pzh);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                vn[i] = mvel(pyl[i], P0[i], pyh[i], cur.c[i], Ay, Vd, dt);
+                wn[i] = mvel(pzl[i], P0[i], pzh[i], cur.d[i], Az, Vd, dt);
+            }
+            if (lp.span) SV[own] = vn;
+        } else if (halo_v) {
+            const V4 in = SP[iin];
+            const V4 pl = lower ? cur.b : in;
+            const V4 ph = lower ? in : cur.b;
+            V4 vh;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) vh[i] = mvel(pl[i], cur.a[i], ph[i], cur.c[i], Ay, Vd, dt);
+            SV[own] = vh;
+        }
+        __syncthreads();
+
+        // 3. pressure update and the four outputs
+        if (is_out) {
+            const V4 vyl = SV[ilo];
+            const V4 vyh = SV[ihi];
+            V4 wzl, wzh;
+            z_nbrs_row(wn, zlo, zhi, wzl, wzh);
+            V4 pn;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const T fx_lo = (XB && x == 0) ? T(0) : mlin(U0[i], Um1[i]);
+                const T fx_hi = (XB && x == X - 1) ? T(0) : mlin(U0[i], Up1[i]);
+                const T fy_lo = first ? T(0) : mlin(vn[i], vyl[i]);
+                const T fy_hi = last ? T(0) : mlin(vn[i], vyh[i]);
+                const T fz_lo = (i == 0 && zlo) ? T(0) : mlin(wn[i], wzl[i]);
+                const T fz_hi = (i == 3 && zhi) ? T(0) : mlin(wn[i], wzh[i]);
+                T S = mggp(fx_lo, fx_hi, Ax, Vd);
+                S = S + mggp(fy_lo, fy_hi, Ay, Vd);
+                S = S + mggp(fz_lo, fz_hi, Az, Vd);
+                const T k = KF ? cur.k[i] : K;
+                pn[i] = dt * (k * S) + P0[i];
+            }
+            if (out) {
+                const int64_t o = x * SX + rb;
+                put(reinterpret_cast<V4*>(po + o), pn, nt);
+                put(reinterpret_cast<V4*>(uo + o), U0, nt);
+                put(reinterpret_cast<V4*>(vo + o), vn, nt);
+                put(reinterpret_cast<V4*>(wo + o), wn, nt);
+            }
+            P0 = Pp1;
+            Pp1 = cur.a;
+            Um1 = U0;
+            U0 = Up1;
+        }
+    }
+}
+
+// One (y tile, x chunk) of the heat step dt * (alpha * lapl(T)) + T in row
+// form: TY output waves of RW rows each and two halo waves (rows y0-1 and
+// y0+TY*RW) that only put their T row into the LDS plane; x neighbours in a
+// per-lane register queue, z neighbours in the lane's registers or the adjacent
+// lane's.  The arithmetic is heat_uniform_k's (Grad2Var with neighbour = self
+// on the walls).
+template <int TY, bool PW, bool XB, bool NTL>
+__device__ inline void heat_row_tile(const UniformGeo<float>& g, const float* __restrict__ Tin,
+                                     float* __restrict__ Tout, float alpha, float dt, int64_t x0, int64_t x1, int y0,
+                                     int wv, typename Row4<float>::type (*sp)[64]) {
+    using V4 = typename Row4<float>::type;
+    const int64_t X = g.X;
+    const int Y = int(g.Y), Z = int(g.Z);
+    const int64_t SX = int64_t(Y) * Z;
+    const VolDiv<float, PW> Vd{g.vol, g.inv_vol};
+    const int lane = threadIdx.x;
+    const int RW = rows_per_wave(Z);
+    const int zq = Z / 4;
+    V4* const SP = &sp[0][0];
+    const bool is_out = wv < TY;
+    const bool lower = wv == TY;
+    const LanePlace lp = is_out ? lane_place(lane, Z, RW) : LanePlace{0, 4 * lane, lane < zq};
+    const int y = is_out ? y0 + wv * RW + lp.r : (lower ? y0 - 1 : y0 + TY * RW);
+    const bool real = lp.span && y >= 0 && y < Y;
+    const bool out = is_out && real;
+    const int own = is_out ? (1 + wv * RW) * zq + lane : (lower ? lane : (1 + TY * RW) * zq + lane);
+    const bool first = y == 0, last = y == Y - 1;
+    const bool zlo = lp.zc == 0, zhi = lp.zc + 4 == Z;
+    const int ilo = first ? own : own - zq, ihi = last ? own : own + zq;
+    const int64_t rb = int64_t(real ? y : 0) * Z + lp.zc;
+    const V4 zero4 = {0.f, 0.f, 0.f, 0.f};
+    auto ld = [&](int64_t off) -> V4 {
+        const V4* q = reinterpret_cast<const V4*>(Tin + off);
+        if constexpr (NTL) return __builtin_nontemporal_load(q);
+        else return *q;
+    };
+    // x queue of the output cells: T(x-1), T(x), T(x+1); the halo cells keep
+    // T(x) in T0 only.  nx: the next plane's load (T(x+2), or T(x+1) for halo
+    // cells)
+    V4 Tm1 = zero4, T0 = zero4, Tp1 = zero4, nx = zero4;
+    if (real) {
+        T0 = ld(x0 * SX + rb);
+        if (out) {
+            if (x0 >= 1) Tm1 = ld((x0 - 1) * SX + rb);
+            if (x0 + 1 < X) Tp1 = ld((x0 + 1) * SX + rb);
+        }
+    }
+    auto fetch = [&](int64_t xx) {  // the loads for plane xx
+        if (!real) return;
+        if (out) {
+            if (!XB || xx + 1 < X) nx = ld((xx + 1) * SX + rb);
+        } else {
+            nx = ld(xx * SX + rb);
+        }
+    };
+    if (x0 + 1 < x1) fetch(x0 + 1);
+    for (int64_t x = x0; x < x1; ++x) {
+        const V4 cur = nx;
+        if (x + 2 < x1) fetch(x + 2);
+        if (real || (is_out && lp.span)) SP[own] = T0;
+        __syncthreads();
+        if (out) {
+            const V4 ylo = SP[ilo];
+            const V4 yhi = SP[ihi];
+            V4 zl, zh;
+            z_nbrs_row(T0, zlo, zhi, zl, zh);
+            V4 r;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float pc = T0[i];
+                const bool lo_w[3] = {XB && x == 0, first, i == 0 && zlo};
+                const bool hi_w[3] = {XB && x == X - 1, last, i == 3 && zhi};
+                const float nb_lo[3] = {lo_w[0] ? pc : Tm1[i], lo_w[1] ? pc : ylo[i], lo_w[2] ? pc : zl[i]};
+                const float nb_hi[3] = {hi_w[0] ? pc : Tp1[i], hi_w[1] ? pc : yhi[i], hi_w[2] ? pc : zh[i]};
+                float Lp = 0.f;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float al_lo = lo_w[a] ? g.alpha_b[a] : g.alpha[a];
+                    const float al_hi = hi_w[a] ? g.alpha_b[a] : g.alpha[a];
+                    const float s_lo = (-pc + nb_lo[a]) * al_lo;
+                    const float s_hi = (-pc + nb_hi[a]) * al_hi;
+                    Lp = Lp + Vd(s_lo * g.area[a] + s_hi * g.area[a]);
+                }
+                r[i] = dt * (alpha * Lp) + pc;
+            }
+            *reinterpret_cast<V4*>(Tout + x * SX + rb) = r;
+        }
+        __syncthreads();  // the plane is rewritten next iteration
+        if (out) {
+            Tm1 = T0;
+            T0 = Tp1;
+            Tp1 = cur;
+        } else {
+            T0 = cur;
+        }
+    }
 }
 
 }  // namespace row

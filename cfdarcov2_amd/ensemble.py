@@ -70,9 +70,10 @@ class Ensemble:
             capi.check(self.lib.cfd_batch_destroy(h))
         self.groups = []
 
-    def configure(self, tile_rows=4, chunk=8, group_bytes=0):
+    def configure(self, tile_rows=4, chunk=8, group_bytes=0, form=0):
         """Pack cases into groups of at most `group_bytes` of fields (0: one
-        group), largest first, and set the tile shape."""
+        group), largest first, and set the tile form (0: 2.5D march tiles, 1:
+        row form) and shape."""
         self._destroy_groups()
         order = sorted(range(len(self.cases)), key=lambda i: -self.cases[i].nbytes)
         packs, cur, cur_b = [], [], 0
@@ -98,9 +99,11 @@ class Ensemble:
                     arr[k].state_b[f] = c.b[f].data_ptr() if f < len(c.b) else None
             h = ctypes.c_void_p()
             capi.check(self.lib.cfd_batch_create(self.ctx.h, arr, len(pack), ctypes.byref(h)))
-            capi.check(self.lib.cfd_batch_set_shape(h, int(tile_rows), int(chunk)))
             self.groups.append(h)
+            capi.check(self.lib.cfd_batch_set_form(h, int(form)))
+            capi.check(self.lib.cfd_batch_set_shape(h, int(tile_rows), int(chunk)))
         self.shape = (tile_rows, chunk)
+        self.form = form
         self.group_bytes = group_bytes
         self.packs = packs
 
@@ -149,19 +152,27 @@ def autotune(ens: Ensemble, steps=20, candidates=None, verbose=False):
     """Pick (tile_rows, chunk, group_bytes) by timing graph runs of `steps`
     steps; leaves the ensemble configured with the winner and its state reset."""
     if candidates is None:
-        candidates = [(ty, L, gb) for ty in (4, 8, 16) for L in (8, 16, 32)
+        candidates = [(ty, L, gb, 0) for ty in (4, 8, 16) for L in (8, 16, 32)
                       for gb in (0, 96 << 20, 192 << 20)]
+        candidates += [(ty, L, gb, 1) for ty in (4, 8) for L in (4, 8, 16, 32)
+                       for gb in (0, 96 << 20, 192 << 20)]
     results = []
-    for ty, L, gb in candidates:
-        ens.configure(ty, L, gb)
+    for cand in candidates:
+        ty, L, gb = cand[:3]
+        form = cand[3] if len(cand) > 3 else 0
+        try:
+            ens.configure(ty, L, gb, form)
+        except capi.CfdError:  # a form these cases do not allow
+            continue
         ens.run(steps)  # capture + warm
         t = time_gpu(lambda: ens.run(steps), reps=2)
-        results.append((t, ty, L, gb))
+        results.append((t, ty, L, gb, form))
         if verbose:
-            print(f"  tile_rows={ty:2d} chunk={L:2d} group={gb >> 20:4d} MiB: {t / steps * 1e6:8.1f} us/step",
-                  flush=True)
+            print(f"  form={form} tile_rows={ty:2d} chunk={L:2d} group={gb >> 20:4d} MiB: "
+                  f"{t / steps * 1e6:8.1f} us/step", flush=True)
     results.sort()
-    _, ty, L, gb = results[0]
-    ens.configure(ty, L, gb)
+    _, ty, L, gb, form = results[0]
+    ens.configure(ty, L, gb, form)
     ens.reset()
-    return {"tile_rows": ty, "chunk": L, "group_mib": gb >> 20, "us_per_step": results[0][0] / steps * 1e6}
+    return {"form": form, "tile_rows": ty, "chunk": L, "group_mib": gb >> 20,
+            "us_per_step": results[0][0] / steps * 1e6}

@@ -254,7 +254,12 @@ typedef struct cfd_batch_case {
     void* state_b[4];
 } cfd_batch_case;
 int cfd_batch_create(cfd_ctx_t ctx, const cfd_batch_case* cases, int ncases, cfd_batch_t* out);
-/* tile_rows 4, 8 or 16 (y-rows x 64 z-columns per workgroup), chunk = x-planes per tile */
+/* Tile form: 0 = 2.5D march tiles (any extents; tile_rows 4, 8 or 16 y-rows x
+ * 64 z-columns per workgroup), 1 = row form (one wave per z row, tile_rows 4
+ * or 8 output rows + 2 halo rows per workgroup; every case's z extent a
+ * multiple of 4 and at most 256 and its fields 16-byte aligned, else
+ * CFD_E_UNSUPPORTED).  chunk = x-planes per tile. */
+int cfd_batch_set_form(cfd_batch_t batch, int form);
 int cfd_batch_set_shape(cfd_batch_t batch, int tile_rows, int chunk);
 int cfd_batch_run(cfd_batch_t batch, int64_t nsteps, int use_graph, void* stream);
 int cfd_batch_destroy(cfd_batch_t batch);

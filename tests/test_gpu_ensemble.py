@@ -92,3 +92,47 @@ def test_errors(hip_ctx):
     h = capi.ctypes.c_void_p()
     assert ens.lib.cfd_batch_create(hip_ctx.h, arr, 1, capi.ctypes.byref(h)) == -4
     m.close()
+
+
+def _row_specs():
+    # z extents that suit the row form (multiples of 4 up to 256), uneven
+    # sides that do not divide the tile shapes, both kinds
+    return workloads.ensemble_config(n_cases=10, steps=0, seed=5, spacing=1.0 / 64,
+                                     sides=(4, 16, 24, 36, 68, 40, 8))[0]
+
+
+@pytest.mark.parametrize("shape", [(4, 8, 0), (8, 5, 0), (4, 1, 0), (8, 16, 1 << 20)])
+def test_batched_row_form_equals_sequential(hip_ctx, shape):
+    from cfdarcov2_amd.ensemble import Ensemble
+    print(f"\n[case] ensemble batched row form shape={shape}", flush=True)
+    ens = Ensemble(hip_ctx, _row_specs())
+    try:
+        ens.run_sequential(7)
+        ref = _snapshot(ens, 7)
+        ens.reset()
+        ens.configure(*shape, form=1)
+        ens.run(7, use_graph=False)
+        _compare(_snapshot(ens, 7), ref)
+        # graph replays
+        ens.reset()
+        ens.run_sequential(6)
+        ref6 = _snapshot(ens, 6)
+        for _ in range(2):
+            ens.reset()
+            ens.run(6, use_graph=True)
+            torch.randn(1 << 16, device="cuda").sum().item()
+            _compare(_snapshot(ens, 6), ref6)
+    finally:
+        ens.close()
+
+
+def test_batched_row_form_refuses_unsuited_cases(hip_ctx):
+    from cfdarcov2_amd import capi
+    from cfdarcov2_amd.ensemble import Ensemble
+    ens = Ensemble(hip_ctx, _specs())  # z extents 5, 23, ... are not multiples of 4
+    try:
+        with pytest.raises(capi.CfdError) as e:
+            ens.configure(4, 8, 0, form=1)
+        assert e.value.code == -4
+    finally:
+        ens.close()
