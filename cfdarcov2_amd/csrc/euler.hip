@@ -93,6 +93,18 @@ __device__ inline T local_dt(const FvGeo<T>& g, int64_t x, int64_t y, int64_t z,
     return g.vol[c] / s;
 }
 
+// The same value from the six faces' geometry already at hand, in the local
+// face order z-, z+, y-, y+, x-, x+ (f[fc] = {nx, ny, nz, A})
+template <class T> __device__ inline T local_dt_faces(T V, const Prim<T>& q, const T (&f)[6][4]) {
+    T s = T(0);
+#pragma unroll
+    for (int fc = 0; fc < 6; ++fc) {
+        const T un = (q.ux * f[fc][0] + q.uy * f[fc][1]) + q.uz * f[fc][2];
+        s = s + f[fc][3] * (fabs(un) + q.a);
+    }
+    return V / s;
+}
+
 // minimum over the block (64 x blockDim.y threads, one wave per y row), folded
 // into *slot by thread (0, 0) with one atomic per block
 template <class T> __device__ inline void block_min_into(T v, Bits<T>* slot) {
@@ -270,14 +282,15 @@ __global__ void __launch_bounds__(64 * TY) euler_march_k(FvGeo<T> g, EIn<T> in, 
     auto face = [&](int fam, int64_t f, T& nx, T& ny, T& nz, T& A) {
         nx = g.n[fam][0][f]; ny = g.n[fam][1][f]; nz = g.n[fam][2][f]; A = g.A[fam][f];
     };
+    auto face4 = [&](int fam, int64_t f, T (&o)[4]) { face(fam, f, o[0], o[1], o[2], o[3]); };
 
     T Uc[5] = {}, Un[5] = {}, Gxl[5] = {};
+    T fg[6][4] = {};  // this cell's face geometry in local face order; fg[4] carried along x
     Prim<T> qc{}, qn{};
     if (valid) {  // warm-up: state of plane x0 and the flux through its lower x-face
         load(x0 * SX + col, Uc);
         qc = prim(gamma, Uc);
-        T nx, ny, nz, A;
-        face(2, x0 * SX + col, nx, ny, nz, A);
+        face4(2, x0 * SX + col, fg[4]);
         T Ul[5];
         Prim<T> ql;
         if (x0 == 0) mirror(Uc, qc, 0, Ul, ql);
@@ -285,7 +298,7 @@ __global__ void __launch_bounds__(64 * TY) euler_march_k(FvGeo<T> g, EIn<T> in, 
             load((x0 - 1) * SX + col, Ul);
             ql = prim(gamma, Ul);
         }
-        rusanov(Ul, ql, Uc, qc, nx, ny, nz, A, Gxl);
+        rusanov(Ul, ql, Uc, qc, fg[4][0], fg[4][1], fg[4][2], fg[4][3], Gxl);
     }
     T loc = std::numeric_limits<T>::infinity();
     int b = 0;
@@ -311,17 +324,17 @@ __global__ void __launch_bounds__(64 * TY) euler_march_k(FvGeo<T> g, EIn<T> in, 
         if (valid) put(S, own, Uc, qc);
         __syncthreads();
         if (valid) {
-            T nx, ny, nz, A, Um[5];
+            T Um[5], G1[5], G2[5], acc[5];
             Prim<T> qm;
             // net = ((Gx+ - Gx-) + (Gy+ - Gy-)) + (Gz+ - Gz-), accumulated one
-            // axis at a time; the upper x flux becomes the next plane's lower
-            T acc[5], G1[5], G2[5];
-            face(2, (x + 1) * SX + col, nx, ny, nz, A);
+            // axis at a time; the upper x flux becomes the next plane's lower.
+            // The six faces' geometry stays in registers for the next dt.
+            face4(2, (x + 1) * SX + col, fg[5]);
             if (x == X - 1) {
                 mirror(Uc, qc, 0, Um, qm);
-                rusanov(Uc, qc, Um, qm, nx, ny, nz, A, G1);
+                rusanov(Uc, qc, Um, qm, fg[5][0], fg[5][1], fg[5][2], fg[5][3], G1);
             } else {
-                rusanov(Uc, qc, Un, qn, nx, ny, nz, A, G1);
+                rusanov(Uc, qc, Un, qn, fg[5][0], fg[5][1], fg[5][2], fg[5][3], G1);
             }
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
@@ -329,26 +342,26 @@ __global__ void __launch_bounds__(64 * TY) euler_march_k(FvGeo<T> g, EIn<T> in, 
                 Gxl[k] = G1[k];
             }
             // y faces
-            face(1, x * (Y + 1) * Z + y * Z + z, nx, ny, nz, A);
+            face4(1, x * (Y + 1) * Z + y * Z + z, fg[2]);
             if (y == 0) mirror(Uc, qc, 1, Um, qm);
             else get(S, own - PZ, Um, qm);
-            rusanov(Um, qm, Uc, qc, nx, ny, nz, A, G1);
-            face(1, x * (Y + 1) * Z + (y + 1) * Z + z, nx, ny, nz, A);
+            rusanov(Um, qm, Uc, qc, fg[2][0], fg[2][1], fg[2][2], fg[2][3], G1);
+            face4(1, x * (Y + 1) * Z + (y + 1) * Z + z, fg[3]);
             if (y == Y - 1) mirror(Uc, qc, 1, Um, qm);
             else get(S, own + PZ, Um, qm);
-            rusanov(Uc, qc, Um, qm, nx, ny, nz, A, G2);
+            rusanov(Uc, qc, Um, qm, fg[3][0], fg[3][1], fg[3][2], fg[3][3], G2);
 #pragma unroll
             for (int k = 0; k < 5; ++k) acc[k] = acc[k] + (G2[k] - G1[k]);
             // z faces
             const int64_t fz = x * Y * (Z + 1) + y * (Z + 1) + z;
-            face(0, fz, nx, ny, nz, A);
+            face4(0, fz, fg[0]);
             if (z == 0) mirror(Uc, qc, 2, Um, qm);
             else get(S, own - 1, Um, qm);
-            rusanov(Um, qm, Uc, qc, nx, ny, nz, A, G1);
-            face(0, fz + 1, nx, ny, nz, A);
+            rusanov(Um, qm, Uc, qc, fg[0][0], fg[0][1], fg[0][2], fg[0][3], G1);
+            face4(0, fz + 1, fg[1]);
             if (z == Z - 1) mirror(Uc, qc, 2, Um, qm);
             else get(S, own + 1, Um, qm);
-            rusanov(Uc, qc, Um, qm, nx, ny, nz, A, G2);
+            rusanov(Uc, qc, Um, qm, fg[1][0], fg[1][1], fg[1][2], fg[1][3], G2);
             const int64_t c = x * SX + col;
             const T V = g.vol[c];
             T Unew[5];
@@ -358,10 +371,12 @@ __global__ void __launch_bounds__(64 * TY) euler_march_k(FvGeo<T> g, EIn<T> in, 
                 Unew[k] = Uc[k] - (dt * net) / V;
                 out.u[k][c] = Unew[k];
             }
-            loc = fmin(loc, local_dt(g, x, y, z, c, prim(gamma, Unew)));
+            loc = fmin(loc, local_dt_faces(V, prim(gamma, Unew), fg));
 #pragma unroll
             for (int k = 0; k < 5; ++k) Uc[k] = Un[k];
             qc = qn;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) fg[4][k] = fg[5][k];
         }
     }
     block_min_into(loc, dt_next);
