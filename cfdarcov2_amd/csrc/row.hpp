@@ -105,6 +105,10 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
     const int y = is_out ? y0 + wv * RW + lp.r : (lower ? y0 - 1 : y0 + TY * RW);
     const bool real = lp.span && y >= 0 && y < Y;
     const bool out = is_out && real;             // a cell of an output row
+    // an output wave with no cell inside the mesh (a tile past the last row)
+    // skips the arithmetic; no real cell reads its LDS slots (the last real
+    // row takes itself as its wall neighbour)
+    const bool live = is_out && __any(out);
     const bool halo_v = !is_out && real;         // a cell of a halo row: v' of the row
     const int own = is_out ? (1 + wv * RW) * zq + lane : (lower ? lane : (1 + TY * RW) * zq + lane);
     const int ilo = y == 0 ? own : own - zq;     // LDS neighbours, self at the y walls
@@ -174,7 +178,7 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
 
         // 1. u'(x+1); p into the LDS plane
         V4 Up1 = zero4;
-        if (is_out) {
+        if (live) {
             if (!XB || x + 1 < X) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
@@ -188,7 +192,7 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
 
         // 2. v' and w' of the plane; the halo rows' v'
         V4 vn = zero4, wn = zero4;
-        if (is_out) {
+        if (live) {
             const V4 pyl = SP[ilo];
             const V4 pyh = SP[ihi];
             V4 pzl, pzh;
@@ -211,7 +215,7 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
         __syncthreads();
 
         // 3. pressure update and the four outputs
-        if (is_out) {
+        if (live) {
             const V4 vyl = SV[ilo];
             const V4 vyh = SV[ihi];
             V4 wzl, wzh;
