@@ -135,7 +135,7 @@ def stream_ceiling(nfloat=256 ** 3, reps=30):
     return best
 
 
-def suite_heat(ctx, cpu=True):
+def suite_heat(ctx, cpu=True, verbose=False):
     """Heat 64^3 FP64, 200 steps at dt = 0.2 dx^2/alpha (the reference CPU
     configuration); 16 algorithmic bytes per cell-update; 4 MiB of state stays
     in the last-level cache, so the fraction is of HBM peak but the data is
@@ -143,9 +143,10 @@ def suite_heat(ctx, cpu=True):
     import numpy as np
     import torch
 
-    from cfdarcov2_amd import capi, workloads
+    from cfdarcov2_amd import autotune, capi, workloads
     case = workloads.heat_config()
     m = capi.HexMesh(ctx, *case.n, *case.spacing, dtype="f64")
+    shape = autotune.tune_heat(m, case.alpha, case.dt, verbose=verbose)
     a = torch.from_numpy(case.T0).cuda()
     b = torch.empty_like(a)
     m.heat_run(a, b, case.alpha, case.dt, 10)
@@ -154,7 +155,8 @@ def suite_heat(ctx, cpu=True):
     N = m.N
     rec = {"workload": "heat 64^3 FP64, 200 steps, dt = 0.2 dx^2/alpha", "mcups": N * case.steps / wall / 1e6,
            "kernel_us_per_step": ev * 1e3 / case.steps,
-           "gbs_algorithmic": 16 * N / (ev * 1e-3 / case.steps) / 1e9}
+           "gbs_algorithmic": 16 * N / (ev * 1e-3 / case.steps) / 1e9,
+           "kernel_shape": {k: shape[k] for k in ("variant", "waves", "chunk")}}
     rec["roofline_frac"] = rec["gbs_algorithmic"] / HBM_PEAK_GBS
     m.close()
     if cpu:
@@ -388,7 +390,7 @@ def main(argv=None):
 
     if args.only:
         ctx = capi.Context(0)
-        fns = {"heat64_f64": lambda: suite_heat(ctx, cpu=False), "wave256_f64": lambda: suite_wave_f64(ctx),
+        fns = {"heat64_f64": lambda: suite_heat(ctx, cpu=False, verbose=args.tune_verbose), "wave256_f64": lambda: suite_wave_f64(ctx),
                "sod512x64x64_f32": lambda: suite_sod(ctx, cpu=False, verbose=args.tune_verbose),
                "ensemble64_f32": lambda: suite_ensemble(ctx, verbose=args.tune_verbose), "adjoint128_f32": lambda: suite_adjoint(ctx)}
         rec = fns[args.only]()

@@ -135,3 +135,41 @@ def tune_euler(mesh, U, gamma=1.4, cfl=0.3, steps=10, candidates=None, verbose=F
     mesh.set_euler_shape(best["variant"], best["tile_rows"], best["chunk"])
     del A, B
     return dict(best, table=table)
+
+
+HEAT_CANDIDATES = [(0, 8, 16)] + [(1, w, L) for w in (8, 12, 16) for L in (1, 2, 4, 8, 16)]
+
+
+def tune_heat(mesh, alpha, dt, steps=20, candidates=None, verbose=False) -> dict:
+    """Pick the fastest heat-step form for `mesh` (per-thread, or two steps per
+    launch in row form with its waves and chunk) from runs on scratch fields;
+    forms the mesh does not allow are skipped."""
+    import torch
+
+    from .capi import CfdError
+    cands = candidates or HEAT_CANDIDATES
+    N = mesh.N
+    tdt = mesh.torch_dtype
+    a = torch.zeros(N, dtype=tdt, device="cuda")
+    b = torch.zeros(N, dtype=tdt, device="cuda")
+    stream = torch.cuda.current_stream()
+    table = []
+    for variant, nw, L in cands:
+        try:
+            mesh.set_heat_shape(variant, nw, L)
+        except CfdError:
+            continue
+        mesh.heat_run(a, b, alpha, dt, 4)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        mesh.heat_run(a, b, alpha, dt, steps)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        table.append({"variant": variant, "waves": nw, "chunk": L, "ms_per_step": ms})
+        if verbose:
+            print(f"  heat shape variant={variant} waves={nw} chunk={L}: {ms * 1e3:.2f} us/step", flush=True)
+    best = min(table, key=lambda r: r["ms_per_step"])
+    mesh.set_heat_shape(best["variant"], best["waves"], best["chunk"])
+    del a, b
+    return dict(best, table=table)

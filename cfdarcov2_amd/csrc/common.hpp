@@ -133,6 +133,12 @@ struct cfd_mesh_s {
     int wave_ty = 16;
     int wave_L = 32;
     int wave_remap = 1;
+    // heat step on uniform meshes: variant 0 = one thread per cell, one step
+    // per launch; 1 = two steps per launch in row form (`heat_nw` waves per
+    // workgroup, `heat_L` planes per chunk)
+    int heat_variant = 0;
+    int heat_nw = 8;
+    int heat_L = 16;
     // Euler stepper: variant 0 = one thread per cell, 1 = 2.5D march with
     // `euler_ty` rows x 64 columns x `euler_L` planes per workgroup
     int euler_variant = 1;

@@ -58,6 +58,14 @@ template <class T>
 bool launch_wave_row(const UniformGeo<T>& g, void* const* in, void* const* out, double K, const void* Kf, double dt,
                      int ty, int L, int order, int cache, hipStream_t s);
 
+// Two heat steps per launch in row form (heat_row.hip), constant alpha; nw
+// waves per workgroup (8, 12 or 16).  Returns false (nothing launched) for
+// meshes or field pointers it cannot take.
+bool heat_row_supported(int64_t Z, int nw);
+template <class T>
+bool launch_heat_row2(const UniformGeo<T>& g, const void* in, void* out, double alpha, double dt, int nw, int L,
+                      hipStream_t s);
+
 // Two steps per pass of the row form (wave_row2.hip), FP32 and constant K.
 // Returns false (nothing launched) for meshes or field pointers it cannot take.
 bool launch_wave_row2(const UniformGeo<float>& g, void* const* in, void* const* out, double K, double dt, int ty,

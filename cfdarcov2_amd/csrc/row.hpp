@@ -348,6 +348,120 @@ __device__ inline void heat_row_tile(const UniformGeo<float>& g, const float* __
     }
 }
 
+
+// Two heat steps per pass in row form.  A workgroup of NW waves holds a band
+// of NW*RW consecutive rows; the tile's output rows are the band without its
+// two outer rows on each side (rows y0 .. y0+NW*RW-5).  Every lane evaluates
+// both steps; the first step is correct on all but the band's outer rows and
+// the second on the output rows, which alone are stored.  Iteration x takes
+// the first step at plane x and the second at plane x-1, the second reading
+// the first's planes x-2 .. x from a register queue and its y-neighbours from
+// an LDS plane.  The arithmetic per cell is heat_uniform_k's.
+template <class T, int NW, bool PW, bool XB, bool NTL>
+__device__ inline void heat_row2_tile(const UniformGeo<T>& g, const T* __restrict__ Tin, T* __restrict__ Tout,
+                                      T alpha, T dt, int64_t x0, int64_t x1, int y0, int wv,
+                                      typename Row4<T>::type (*sp1)[64], typename Row4<T>::type (*sp2)[64]) {
+    using V4 = typename Row4<T>::type;
+    const int64_t X = g.X;
+    const int Y = int(g.Y), Z = int(g.Z);
+    const int64_t SX = int64_t(Y) * Z;
+    const VolDiv<T, PW> Vd{g.vol, g.inv_vol};
+    const int lane = threadIdx.x;
+    const int RW = rows_per_wave(Z);
+    const int zq = Z / 4;
+    const int band = NW * RW;
+    const int TR = band - 4;  // output rows
+    V4* const S1 = &sp1[0][0];
+    V4* const S2 = &sp2[0][0];
+    const LanePlace lp = lane_place(lane, Z, RW);
+    const int bi = wv * RW + lp.r;  // row within the band
+    const int y = y0 - 2 + bi;
+    const bool real = lp.span && y >= 0 && y < Y;
+    const bool out = real && y >= y0 && y < y0 + TR;
+    const bool live = __any(real);
+    const int own = wv * RW * zq + lane;
+    const bool first = y == 0, last = y == Y - 1;
+    // LDS neighbours: self at the y walls and at the band's ends; lanes past
+    // the wave's span read their own slot (their results are never used)
+    const int ilo = (!lp.span || first || bi == 0) ? own : own - zq;
+    const int ihi = (!lp.span || last || bi == band - 1) ? own : own + zq;
+    const bool zlo = lp.zc == 0, zhi = lp.zc + 4 == Z;
+    const int64_t rb = int64_t(real ? y : 0) * Z + lp.zc;
+    const V4 zero4 = {T(0), T(0), T(0), T(0)};
+    auto ld = [&](int64_t off) -> V4 {
+        const V4* q = reinterpret_cast<const V4*>(Tin + off);
+        if constexpr (NTL) return __builtin_nontemporal_load(q);
+        else return *q;
+    };
+    // one step of a lane's four cells: centre c, x-neighbours (xm, xp), y
+    // neighbours (ym, yp) and the walls of the plane
+    auto cell4 = [&](const V4& c, const V4& xm, const V4& xpn, const V4& ym, const V4& yp, bool xlo_w, bool xhi_w) {
+        V4 zl, zh;
+        z_nbrs_row(c, zlo, zhi, zl, zh);
+        V4 r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const T pc = c[i];
+            const bool lo_w[3] = {xlo_w, first, i == 0 && zlo};
+            const bool hi_w[3] = {xhi_w, last, i == 3 && zhi};
+            const T nb_lo[3] = {lo_w[0] ? pc : xm[i], lo_w[1] ? pc : ym[i], lo_w[2] ? pc : zl[i]};
+            const T nb_hi[3] = {hi_w[0] ? pc : xpn[i], hi_w[1] ? pc : yp[i], hi_w[2] ? pc : zh[i]};
+            T Lp = T(0);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const T al_lo = lo_w[a] ? g.alpha_b[a] : g.alpha[a];
+                const T al_hi = hi_w[a] ? g.alpha_b[a] : g.alpha[a];
+                const T s_lo = (-pc + nb_lo[a]) * al_lo;
+                const T s_hi = (-pc + nb_hi[a]) * al_hi;
+                Lp = Lp + Vd(s_lo * g.area[a] + s_hi * g.area[a]);
+            }
+            r[i] = dt * (alpha * Lp) + pc;
+        }
+        return r;
+    };
+
+    const int64_t xs = x0 >= 1 ? x0 - 1 : 0;  // first plane of the first step
+    const int64_t xe = x1 + 1;                // end of the iterations
+    const int64_t xf = xe < X ? xe : X;       // end of the first step's planes
+    // first step's x queue T(x-1), T(x), T(x+1) and the next load T(x+2)
+    V4 Tm1 = zero4, T0 = zero4, Tp1 = zero4, nx = zero4;
+    if (real) {
+        T0 = ld(xs * SX + rb);
+        if (xs >= 1) Tm1 = ld((xs - 1) * SX + rb);
+        if (xs + 1 < X) Tp1 = ld((xs + 1) * SX + rb);
+        if (xs + 1 < xf && (!XB || xs + 2 < X)) nx = ld((xs + 2) * SX + rb);
+    }
+    // second step's queue: T'(x-2), T'(x-1)
+    V4 Q2 = zero4, Q1 = zero4;
+    for (int64_t x = xs; x < xe; ++x) {
+        const bool s1 = !XB || x < X;
+        const int64_t xq = x - 1;  // second step's plane
+        const bool s2 = xq >= x0 && xq < x1;
+        const V4 cur = nx;
+        if (real && x + 2 < xf && (!XB || x + 3 < X)) nx = ld((x + 3) * SX + rb);
+        if (live && lp.span) {
+            if (s1) S1[own] = T0;
+            if (s2) S2[own] = Q1;
+        }
+        __syncthreads();
+        V4 Q0 = zero4;
+        if (live) {
+            if (s1) Q0 = cell4(T0, Tm1, Tp1, S1[ilo], S1[ihi], XB && x == 0, XB && x == X - 1);
+            if (s2) {
+                const V4 hi = (!XB || x < X) ? Q0 : Q1;  // T'(xq + 1), self at the wall
+                const V4 r = cell4(Q1, Q2, hi, S2[ilo], S2[ihi], XB && xq == 0, XB && xq == X - 1);
+                if (out) *reinterpret_cast<V4*>(Tout + xq * SX + rb) = r;
+            }
+        }
+        __syncthreads();  // the planes are rewritten next iteration
+        Tm1 = T0;
+        T0 = Tp1;
+        Tp1 = cur;
+        Q2 = Q1;
+        Q1 = Q0;
+    }
+}
+
 }  // namespace row
 
 }  // namespace cfd

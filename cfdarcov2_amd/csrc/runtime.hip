@@ -681,6 +681,22 @@ int cfd_mesh_set_euler_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk
     });
 }
 
+int cfd_mesh_set_heat_shape(cfd_mesh_t m, int variant, int waves, int chunk) {
+    return guarded([&] {
+        CFD_REQUIRE(m != nullptr, "null mesh");
+        CFD_REQUIRE(variant == 0 || variant == 1, "heat variant must be 0 or 1");
+        CFD_REQUIRE(waves == 8 || waves == 12 || waves == 16, "heat waves must be 8, 12 or 16");
+        CFD_REQUIRE(chunk >= 1 && chunk <= 65535, "heat chunk must be in [1, 65535]");
+        if (variant == 1 && !(m->uniform && heat_row_supported(m->Z, waves)))
+            throw Error(CFD_E_UNSUPPORTED,
+                        "the two-step heat form needs an exactly uniform mesh whose z extent is a multiple of 4 "
+                        "and at most 256");
+        m->heat_variant = variant;
+        m->heat_nw = waves;
+        m->heat_L = chunk;
+    });
+}
+
 int cfd_mesh_set_wave_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk, int xcd_remap) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");

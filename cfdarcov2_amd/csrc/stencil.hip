@@ -504,7 +504,17 @@ void launch_stab(const cfd_mesh_s* m, const void* phi, void* out, int mask, void
 template <class T>
 void launch_heat(const cfd_mesh_s* m, void* Ta, void* Tb, double alpha, const void* af, double dt, int64_t nsteps,
                  hipStream_t s) {
-    for (int64_t k = 0; k < nsteps; ++k) {
+    int64_t k0 = 0;
+    if (m->uniform && !af && m->heat_variant == 1) {
+        // two steps per launch, in pairs (Ta -> Tb -> Ta) so the state ends
+        // where single steps leave it; the remaining steps run one per launch
+        const auto& g = ugeo<T>(m);
+        for (; k0 + 4 <= nsteps; k0 += 4) {
+            if (!launch_heat_row2<T>(g, Ta, Tb, alpha, dt, m->heat_nw, m->heat_L, s)) break;
+            launch_heat_row2<T>(g, Tb, Ta, alpha, dt, m->heat_nw, m->heat_L, s);
+        }
+    }
+    for (int64_t k = k0; k < nsteps; ++k) {
         const T* in = (const T*)((k & 1) ? Tb : Ta);
         T* out = (T*)((k & 1) ? Ta : Tb);
         if (m->uniform) {

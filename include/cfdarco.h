@@ -328,6 +328,17 @@ int cfd_mesh_geometry(cfd_mesh_t mesh, cfd_geometry* out);
  * (cfdarcov2_amd/autotune.py). */
 int cfd_mesh_set_wave_shape(cfd_mesh_t mesh, int variant, int tile_rows, int chunk, int xcd_remap);
 
+/* Kernel form of the heat step (cfd_heat_run) on exactly uniform meshes (no
+ * counterpart in the reference): variant 0 = one thread per cell, one step per
+ * launch; variant 1 = two steps per launch in row form (z extent a multiple of
+ * 4 and at most 256, else CFD_E_UNSUPPORTED), `waves` (8, 12 or 16) waves per
+ * workgroup and `chunk` x-planes per tile; runs go in pairs of launches, and
+ * the remaining steps and runs with a diffusivity field take variant 0.  Both
+ * give bit-identical results.  Meshes start on variant 0; variant 1 pays on
+ * large meshes (it needs enough tiles to fill the GPU), and
+ * cfdarcov2_amd/autotune.py:tune_heat times both. */
+int cfd_mesh_set_heat_shape(cfd_mesh_t mesh, int variant, int waves, int chunk);
+
 #ifdef __cplusplus
 }
 #endif

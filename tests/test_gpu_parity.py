@@ -422,3 +422,47 @@ def test_wave_row_form_rejects_unsupported_rows(hip_ctx):
     with pytest.raises(capi.CfdError) as e:
         capi.HexMesh(hip_ctx, 8, 8, 64, 0.125, 0.125, 0.125, dtype=np.float64).set_wave_shape(11, 8, 8, 0)
     assert e.value.code == -4
+
+
+HEAT_ROW_CASES = [
+    # (extents, spacing, (waves, chunk)): rows per wave 3, 1, 64, 4, 4 (volume
+    # not a power of two) and 32 with two planes; partial tiles, chunks that do
+    # not divide x
+    ((13, 11, 68), (1.0 / 16, 1.0 / 16, 1.0 / 64), (8, 3)),
+    ((9, 30, 256), (1.0 / 16, 1.0 / 32, 1.0 / 256), (16, 4)),
+    ((6, 5, 4), (0.25, 0.25, 0.25), (8, 1)),
+    ((40, 33, 64), (1.0 / 64, 1.0 / 64, 1.0 / 64), (12, 16)),
+    ((13, 11, 60), (0.375, 0.25, 0.125), (8, 5)),
+    ((2, 7, 8), (0.125, 0.125, 0.125), (16, 2)),
+]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("case", HEAT_ROW_CASES, ids=[f"{c[0][0]}x{c[0][1]}x{c[0][2]}-w{c[2][0]}-L{c[2][1]}" for c in HEAT_ROW_CASES])
+def test_heat_two_step_row_form(hip_ctx, oracle_lib, dtype, case):
+    """Two heat steps per launch in row form: 9 steps (two pairs of two-step
+    launches and one single step) equal the oracle bit for bit."""
+    from cfdarcov2_amd import capi
+    n, d, (nw, L) = case
+    print(f"\n[case] heat two-step row form n={n} waves={nw} chunk={L} {np.dtype(dtype).name}", flush=True)
+    rng = np.random.default_rng(13)
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=dtype)
+    gm.set_heat_shape(1, nw, L)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=dtype)
+    T0 = rng.standard_normal(om.N).astype(dtype)
+    alpha = 1e-2
+    dt = 0.15 * min(d) ** 2 / alpha
+    a, b = _t(T0, dtype), torch.empty(om.N, dtype=_t(T0, dtype).dtype, device="cuda")
+    got = _np(gm.heat_run(a, b, alpha, dt, 9))
+    ref = om.heat_run(T0, alpha, dt, 9)
+    assert np.array_equal(got, ref), float(np.max(np.abs(got - ref)))
+
+
+def test_heat_shape_errors(hip_ctx):
+    from cfdarcov2_amd import capi
+    gm = capi.HexMesh(hip_ctx, 8, 8, 70, 0.125, 0.125, 0.125, dtype=np.float32)
+    with pytest.raises(capi.CfdError) as e:
+        gm.set_heat_shape(1, 8, 8)
+    assert e.value.code == -4
+    with pytest.raises(capi.CfdError):
+        capi.HexMesh(hip_ctx, 8, 8, 64, 0.125, 0.125, 0.125, dtype=np.float32).set_heat_shape(1, 10, 8)
