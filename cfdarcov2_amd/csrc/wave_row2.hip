@@ -273,6 +273,202 @@ __device__ inline void tile2(const UniformGeo<float>& g, const float* __restrict
     }
 }
 
+// Pressure update of a lane's four cells when a wave holds several rows: the
+// z walls are a row's first and last lane (zlo, zhi), first / last are the
+// lane's own y walls.
+template <bool PW>
+__device__ inline V4 pres4m(const V4& P, const V4& Um, const V4& U, const V4& Up, bool xlo, bool xhi, const V4& vn,
+                            const V4& vyl, const V4& vyh, bool first, bool last, const V4& wn, bool zlo, bool zhi,
+                            float Ax, float Ay, float Az, const VolDiv<float, PW>& Vd, float K, float dt) {
+    V4 wzl, wzh;
+    row::z_nbrs_row(wn, zlo, zhi, wzl, wzh);
+    V4 pn;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float fx_lo = xlo ? mlin(U[i], Um[i]) : 0.f;
+        const float fx_hi = xhi ? mlin(U[i], Up[i]) : 0.f;
+        const float fy_lo = first ? 0.f : mlin(vn[i], vyl[i]);
+        const float fy_hi = last ? 0.f : mlin(vn[i], vyh[i]);
+        const float fz_lo = (i == 0 && zlo) ? 0.f : mlin(wn[i], wzl[i]);
+        const float fz_hi = (i == 3 && zhi) ? 0.f : mlin(wn[i], wzh[i]);
+        float S = mggp(fx_lo, fx_hi, Ax, Vd);
+        S = S + mggp(fy_lo, fy_hi, Ay, Vd);
+        S = S + mggp(fz_lo, fz_hi, Az, Vd);
+        pn[i] = dt * (K * S) + P[i];
+    }
+    return pn;
+}
+
+// Two steps per pass with several rows per wave (z extent below 256).  A
+// workgroup of NW waves holds a band of NW*RW consecutive rows (RW = 256/Z per
+// wave); the output rows are the band without its four outer rows on each
+// side.  Every lane evaluates both steps on its cells: the first step is
+// correct on all but the band's two outer rows on each side, the second on the
+// output rows, which alone are stored.  The x pipeline (first step at plane x,
+// second at x-2, register queues) and the four LDS planes are those of tile2,
+// with row j of the band at element j*Z.
+template <int NW, bool PW, bool XB, bool NTL>
+__device__ inline void tile2m(const UniformGeo<float>& g, const float* __restrict__ p, const float* __restrict__ u,
+                              const float* __restrict__ v, const float* __restrict__ w, float* __restrict__ po,
+                              float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo, float K,
+                              float dt, int64_t x0, int64_t x1, int y0, int wv, bool nt, V4 (*sP1)[64],
+                              V4 (*sV1)[64], V4 (*sP2)[64], V4 (*sV2)[64]) {
+    const int64_t X = g.X;
+    const int Y = int(g.Y), Z = int(g.Z);
+    const int64_t SX = int64_t(Y) * Z;
+    const float Ax = g.area[0], Ay = g.area[1], Az = g.area[2];
+    const VolDiv<float, PW> Vd{g.vol, g.inv_vol};
+    const int lane = threadIdx.x;
+    const int RW = row::rows_per_wave(Z);
+    const int zq = Z / 4;
+    const int band = NW * RW;
+    const int TR = band - 8;
+    V4* const P1 = &sP1[0][0];
+    V4* const V1 = &sV1[0][0];
+    V4* const P2 = &sP2[0][0];
+    V4* const V2 = &sV2[0][0];
+    const row::LanePlace lp = row::lane_place(lane, Z, RW);
+    const int bi = wv * RW + lp.r;
+    const int y = y0 - 4 + bi;
+    const bool real = lp.span && y >= 0 && y < Y;
+    const bool out = real && y >= y0 && y < y0 + TR;
+    const bool live = __any(real);
+    const int own = wv * RW * zq + lane;
+    const bool first = y == 0, last = y == Y - 1;
+    // LDS neighbours: self at the y walls and at the band's ends; lanes past
+    // the wave's span read their own slot (their results are never used)
+    const int ilo = (!lp.span || first || bi == 0) ? own : own - zq;
+    const int ihi = (!lp.span || last || bi == band - 1) ? own : own + zq;
+    const bool zlo = lp.zc == 0, zhi = lp.zc + 4 == Z;
+    const int64_t rb = int64_t(real ? y : 0) * Z + lp.zc;
+
+    struct Fetch {
+        V4 a, b, c, d;
+    };
+    const V4 zero4 = {0.f, 0.f, 0.f, 0.f};
+    auto ld = [&](const float* base, int64_t off) -> V4 {
+        const V4* q = reinterpret_cast<const V4*>(base + off);
+        if constexpr (NTL) return __builtin_nontemporal_load(q);
+        else return *q;
+    };
+    // loads of plane xx: a = p(xx+2), b = u(xx+1), c = v(xx), d = w(xx)
+    auto fetch = [&](int64_t xx, Fetch& f) {
+        if (!real) return;
+        const int64_t pb = xx * SX;
+        if (!XB || xx + 2 < X) f.a = ld(p, pb + 2 * SX + rb);
+        if (!XB || xx + 1 < X) f.b = ld(u, pb + SX + rb);
+        f.c = ld(v, pb + rb);
+        f.d = ld(w, pb + rb);
+    };
+
+    const int64_t xs = x0 >= 2 ? x0 - 2 : 0;
+    const int64_t xe = x1 + 2;
+    const int64_t xs1 = xe < X ? xe : X;
+
+    V4 P0 = zero4, Pp1 = zero4, Um1 = zero4, U0 = zero4;
+    if (real) {
+        V4 Pm1 = zero4;
+        P0 = ld(p, xs * SX + rb);
+        if (xs >= 1) Pm1 = ld(p, (xs - 1) * SX + rb);
+        if (xs + 1 < X) Pp1 = ld(p, (xs + 1) * SX + rb);
+        if (xs >= 1) {
+            const V4 Pm2 = (xs >= 2) ? ld(p, (xs - 2) * SX + rb) : Pm1;
+            const V4 um = ld(u, (xs - 1) * SX + rb);
+            Um1 = vel4(Pm2, Pm1, P0, um, Ax, Vd, dt);
+        }
+        const V4 uc = ld(u, xs * SX + rb);
+        V4 lo = P0, hi = P0;
+        if (xs >= 1) lo = Pm1;
+        if (xs + 1 < X) hi = Pp1;
+        U0 = vel4(lo, P0, hi, uc, Ax, Vd, dt);
+    }
+    V4 Q2 = zero4, Q1 = zero4, Va2 = zero4, Va1 = zero4, Wa2 = zero4, Wa1 = zero4, R3 = zero4, R2 = zero4;
+
+    Fetch nx{zero4, zero4, zero4, zero4};
+    fetch(xs, nx);
+
+    for (int64_t x = xs; x < xe; ++x) {
+        const bool s1 = !XB || x < X;
+        const int64_t xp = x - 2;
+        const bool s2 = xp >= x0 && xp < x1;
+        const int64_t m = x - 1;
+        const bool su = m >= x0 - 1 && m >= 0 && m < X;
+        const Fetch cur = nx;
+        if (x + 1 < xs1) fetch(x + 1, nx);
+
+        // A. u'(x+1); p(x) and p'(x-2) into LDS
+        V4 Up1 = zero4;
+        if (live) {
+            if (s1) {
+                if (!XB || x + 1 < X) {
+                    const V4 hi = (!XB || x + 2 < X) ? cur.a : Pp1;
+                    Up1 = vel4(P0, Pp1, hi, cur.b, Ax, Vd, dt);
+                }
+                if (lp.span) P1[own] = P0;
+            }
+            if (s2 && lp.span) P2[own] = Q2;
+        }
+        __syncthreads();
+
+        // B. v'(x), w'(x); v''(x-2), w''(x-2)
+        V4 vn = zero4, wn = zero4, v2 = zero4, w2 = zero4;
+        if (live) {
+            if (s1) {
+                V4 pzl, pzh;
+                row::z_nbrs_row(P0, zlo, zhi, pzl, pzh);
+                vn = vel4(P1[ilo], P0, P1[ihi], cur.c, Ay, Vd, dt);
+                wn = vel4(pzl, P0, pzh, cur.d, Az, Vd, dt);
+                if (lp.span) V1[own] = vn;
+            }
+            if (s2) {
+                V4 qzl, qzh;
+                row::z_nbrs_row(Q2, zlo, zhi, qzl, qzh);
+                v2 = vel4(P2[ilo], Q2, P2[ihi], Va2, Ay, Vd, dt);
+                w2 = vel4(qzl, Q2, qzh, Wa2, Az, Vd, dt);
+                if (lp.span) V2[own] = v2;
+            }
+        }
+        __syncthreads();
+
+        // C. p'(x); u''(x-1); p''(x-2) and the outputs
+        if (live) {
+            V4 Q0 = zero4;
+            if (s1)
+                Q0 = pres4m(P0, Um1, U0, Up1, !XB || x > 0, !XB || x < X - 1, vn, V1[ilo], V1[ihi], first, last, wn,
+                            zlo, zhi, Ax, Ay, Az, Vd, K, dt);
+            V4 R1 = zero4;
+            if (su) {
+                const V4 lo = (!XB || m >= 1) ? Q2 : Q1;
+                const V4 hi = (!XB || x < X) ? Q0 : Q1;
+                R1 = vel4(lo, Q1, hi, Um1, Ax, Vd, dt);
+            }
+            if (s2) {
+                const V4 pn = pres4m(Q2, R3, R2, R1, !XB || xp > 0, !XB || xp < X - 1, v2, V2[ilo], V2[ihi], first,
+                                     last, w2, zlo, zhi, Ax, Ay, Az, Vd, K, dt);
+                if (out) {
+                    const int64_t o = xp * SX + rb;
+                    row::put(reinterpret_cast<V4*>(po + o), pn, nt);
+                    row::put(reinterpret_cast<V4*>(uo + o), R2, nt);
+                    row::put(reinterpret_cast<V4*>(vo + o), v2, nt);
+                    row::put(reinterpret_cast<V4*>(wo + o), w2, nt);
+                }
+            }
+            P0 = Pp1;
+            Pp1 = cur.a;
+            Um1 = U0;
+            U0 = Up1;
+            Q2 = Q1;
+            Q1 = Q0;
+            Va2 = Va1;
+            Va1 = vn;
+            Wa2 = Wa1;
+            Wa1 = wn;
+            R3 = R2;
+            R2 = R1;
+        }
+    }
+}
+
 }  // namespace row2
 
 template <int TY, bool PW, bool NTL>
@@ -324,12 +520,71 @@ static void launch_row2_ty(const UniformGeo<float>& g, void* const* in, void* co
     else launch_row2_k<TY, false>(g, in, out, K, dt, L, order, nt, s);
 }
 
+template <int NW, bool PW, bool NTL>
+__global__ void __launch_bounds__(64 * NW)
+wave_row2m_k(UniformGeo<float> g, const float* __restrict__ p, const float* __restrict__ u,
+             const float* __restrict__ v, const float* __restrict__ w, float* __restrict__ po, float* __restrict__ uo,
+             float* __restrict__ vo, float* __restrict__ wo, float K, float dt, int L, int nyt, int order, int nt) {
+    using row2::V4;
+    __shared__ V4 P1[NW][64];
+    __shared__ V4 V1[NW][64];
+    __shared__ V4 P2[NW][64];
+    __shared__ V4 V2[NW][64];
+    const int64_t nwg = gridDim.x;
+    int64_t lin = blockIdx.x;
+    if (order & 2) lin = nwg - 1 - lin;
+    if (order & 1) lin = (lin % 8) * (nwg / 8) + lin / 8;
+    const int yt = int(lin % nyt);
+    const int64_t xc = lin / nyt;
+    const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.y));
+    const int64_t X = g.X;
+    const int64_t x0 = xc * L;
+    const int64_t x1 = (x0 + L < X) ? x0 + L : X;
+    const int y0 = yt * (NW * row::rows_per_wave(g.Z) - 8);
+    if (x0 >= 4 && x1 + 4 <= X)
+        row2::tile2m<NW, PW, false, NTL>(g, p, u, v, w, po, uo, vo, wo, K, dt, x0, x1, y0, wv, nt != 0, P1, V1, P2,
+                                         V2);
+    else
+        row2::tile2m<NW, PW, true, NTL>(g, p, u, v, w, po, uo, vo, wo, K, dt, x0, x1, y0, wv, nt != 0, P1, V1, P2,
+                                        V2);
+}
+
+template <int NW>
+static void launch_row2m(const UniformGeo<float>& g, void* const* in, void* const* out, double K, double dt, int L,
+                         int order, bool nt, hipStream_t s) {
+    const int64_t rows = int64_t(NW) * row::rows_per_wave(g.Z) - 8;  // output rows per tile
+    CFD_REQUIRE(rows >= 1, "two-step tile without output rows");
+    const int nyt = int((g.Y + rows - 1) / rows);
+    const int64_t nxc = (g.X + L - 1) / L;
+    const int64_t nwg = int64_t(nyt) * nxc;
+    CFD_REQUIRE(nwg < (int64_t(1) << 31), "row march grid too large");
+    const int ord = ((order & 1) && nwg % 8 == 0 ? 1 : 0) | (order & 2);
+    const float* const* fi = (const float* const*)in;
+    float* const* fo = (float* const*)out;
+    if (g.vol_pow2)
+        wave_row2m_k<NW, true, true><<<dim3(unsigned(nwg)), dim3(64, NW), 0, s>>>(
+            g, fi[0], fi[1], fi[2], fi[3], fo[0], fo[1], fo[2], fo[3], float(K), float(dt), L, nyt, ord, nt ? 1 : 0);
+    else
+        wave_row2m_k<NW, false, true><<<dim3(unsigned(nwg)), dim3(64, NW), 0, s>>>(
+            g, fi[0], fi[1], fi[2], fi[3], fo[0], fo[1], fo[2], fo[3], float(K), float(dt), L, nyt, ord, nt ? 1 : 0);
+    CFD_HIP(hipGetLastError());
+}
+
 bool launch_wave_row2(const UniformGeo<float>& g, void* const* in, void* const* out, double K, double dt, int ty,
                       int L, int order, bool nt, hipStream_t s) {
     if (!wave_row_supported(g.Z)) return false;
     const uintptr_t al = 4 * sizeof(float) - 1;
     for (int i = 0; i < 4; ++i)
         if ((reinterpret_cast<uintptr_t>(in[i]) & al) || (reinterpret_cast<uintptr_t>(out[i]) & al)) return false;
+    if (row::rows_per_wave(g.Z) > 1) {  // several rows per wave: 8, 10 or 12 waves per workgroup
+        switch (ty) {
+            case 4: launch_row2m<8>(g, in, out, K, dt, L, order, nt, s); break;
+            case 8: launch_row2m<10>(g, in, out, K, dt, L, order, nt, s); break;
+            case 10: launch_row2m<12>(g, in, out, K, dt, L, order, nt, s); break;
+            default: throw Error(CFD_E_INVALID, "two-step row-form tiles have 4, 8 or 10 rows");
+        }
+        return true;
+    }
     switch (ty) {
         case 4: launch_row2_ty<4>(g, in, out, K, dt, L, order, nt, s); break;
         case 8: launch_row2_ty<8>(g, in, out, K, dt, L, order, nt, s); break;

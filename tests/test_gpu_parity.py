@@ -351,6 +351,10 @@ ROW_CASES = [
     ((30, 27, 256), (1.0 / 32, 1.0 / 32, 1.0 / 256), (11, 8, 4, 1)),
     ((6, 5, 4), (0.25, 0.25, 0.25), (11, 4, 1, 0)),
     ((13, 11, 60), (0.375, 0.25, 0.125), (12, 8, 2, 3)),
+    # two steps per pass with several rows per wave
+    ((10, 70, 128), (1.0 / 16, 1.0 / 128, 1.0 / 128), (11, 8, 6, 1)),
+    ((12, 50, 96), (1.0 / 16, 1.0 / 64, 1.0 / 128), (12, 10, 3, 2)),
+    ((7, 45, 32), (0.125, 1.0 / 64, 1.0 / 32), (11, 4, 16, 3)),
 ]
 
 
