@@ -18,6 +18,7 @@
 #include "kernels.hpp"
 #include "march.hpp"
 #include "row.hpp"
+#include "row2.hpp"
 
 namespace cfd {
 
@@ -120,6 +121,65 @@ __global__ void __launch_bounds__(64 * (TY + 2)) batch_row_k(const BatchCaseDev*
     }
 }
 
+// Two steps per launch (form 2): acoustic cases run the several-rows-per-wave
+// two-step tile, heat cases the two-step heat tile, both NW waves wide.
+template <int NW>
+__global__ void __launch_bounds__(64 * NW) batch_row2_k(const BatchCaseDev* __restrict__ cases, int ncases,
+                                                        int parity, int L) {
+    using V4 = row::Row4<float>::type;
+    __shared__ V4 s0[NW][64];
+    __shared__ V4 s1[NW][64];
+    __shared__ V4 s2[NW][64];
+    __shared__ V4 s3[NW][64];
+    const int64_t tile = blockIdx.x;
+    int lo = 0, hi = ncases - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cases[mid].tile0 <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchCaseDev& cs = cases[lo];
+    const int64_t t = tile - cs.tile0;
+    const int yt = int(t % cs.nyt);
+    const int64_t xc = t / cs.nyt;
+    const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.y));
+    const int64_t X = cs.g.X;
+    const int64_t x0 = xc * L;
+    const int64_t x1 = (x0 + L < X) ? x0 + L : X;
+    const int band = NW * row::rows_per_wave(cs.g.Z);
+    float* const* in = parity ? cs.b : cs.a;
+    float* const* out = parity ? cs.a : cs.b;
+    const bool pw = cs.g.vol_pow2 != 0;
+    if (cs.kind == CFD_CASE_HEAT) {
+        const int y0 = yt * (band - 4);
+        const bool inner = x0 >= 2 && x1 + 2 <= X;
+#define CFD_HEAT2(PW, XB) \
+    row::heat_row2_tile<float, NW, PW, XB, true>(cs.g, in[0], out[0], cs.coef, cs.dt, x0, x1, y0, wv, s0, s1)
+        if (pw) {
+            if (inner) CFD_HEAT2(true, false);
+            else CFD_HEAT2(true, true);
+        } else {
+            if (inner) CFD_HEAT2(false, false);
+            else CFD_HEAT2(false, true);
+        }
+#undef CFD_HEAT2
+    } else {
+        const int y0 = yt * (band - 8);
+        const bool inner = x0 >= 4 && x1 + 4 <= X;
+#define CFD_WAVE2(PW, XB)                                                                                  \
+    row2::tile2m<NW, PW, XB, true>(cs.g, in[0], in[1], in[2], in[3], out[0], out[1], out[2], out[3], cs.coef, \
+                                   cs.dt, x0, x1, y0, wv, false, s0, s1, s2, s3)
+        if (pw) {
+            if (inner) CFD_WAVE2(true, false);
+            else CFD_WAVE2(true, true);
+        } else {
+            if (inner) CFD_WAVE2(false, false);
+            else CFD_WAVE2(false, true);
+        }
+#undef CFD_WAVE2
+    }
+}
+
 }  // namespace cfd
 
 using namespace cfd;
@@ -130,8 +190,13 @@ struct cfd_batch_s {
     BatchCaseDev* dev = nullptr;
     int tile_rows = 4;
     int chunk = 8;
-    int form = 0;  // 0: 2.5D march tiles, 1: row form
+    int form = 0;  // 0: 2.5D march tiles, 1: row form, 2: row form, two steps per launch
     int64_t total_tiles = 0;
+    // form 2: single-step row-form layout (4-row tiles) for the steps left after
+    // the pairs of two-step launches
+    std::vector<BatchCaseDev> host1;
+    BatchCaseDev* dev1 = nullptr;
+    int64_t total_tiles1 = 0;
     bool dirty = true;
     hipStream_t capture = nullptr;
     // instantiated graphs keyed by (steps, tile rows, chunk)
@@ -140,20 +205,46 @@ struct cfd_batch_s {
 
 namespace cfd {
 
-static void batch_layout(cfd_batch_s* b, hipStream_t s) {
-    const int TY = b->tile_rows;
+// rows per tile of case c in `form` with tile parameter T (tile rows, or waves
+// for form 2)
+static int64_t tile_rows_of(const BatchCaseDev& c, int form, int T) {
+    const int64_t rw = row::rows_per_wave(c.g.Z);
+    if (form == 1) return int64_t(T) * rw;
+    if (form == 2) return int64_t(T) * rw - (c.kind == CFD_CASE_HEAT ? 4 : 8);
+    return T;
+}
+
+// the two-step form needs output rows in every case's tiles
+static void check_form2(const std::vector<BatchCaseDev>& cases, int T) {
+    for (const auto& c : cases)
+        if (tile_rows_of(c, 2, T) < 1)
+            throw Error(CFD_E_UNSUPPORTED, "two-step batch tiles of this width leave a case without output rows");
+}
+
+static int64_t layout_into(std::vector<BatchCaseDev>& cases, int form, int T, int L) {
     int64_t off = 0;
-    for (auto& c : b->host) {
-        c.nzt = b->form == 1 ? 1 : int((c.g.Z + 63) / 64);  // a row-form tile spans the z row
-        const int64_t rows = b->form == 1 ? int64_t(TY) * row::rows_per_wave(c.g.Z) : TY;  // rows per tile
+    for (auto& c : cases) {
+        c.nzt = form >= 1 ? 1 : int((c.g.Z + 63) / 64);  // a row-form tile spans the z row
+        const int64_t rows = tile_rows_of(c, form, T);
+        CFD_REQUIRE(rows >= 1, "batch tile without output rows");
         c.nyt = int((c.g.Y + rows - 1) / rows);
-        const int64_t nxc = (c.g.X + b->chunk - 1) / b->chunk;
+        const int64_t nxc = (c.g.X + L - 1) / L;
         c.tile0 = off;
         off += int64_t(c.nzt) * c.nyt * nxc;
     }
     CFD_REQUIRE(off < (int64_t(1) << 31), "batched grid too large");
-    b->total_tiles = off;
-    CFD_HIP(hipMemcpyAsync(b->dev, b->host.data(), b->host.size() * sizeof(BatchCaseDev), hipMemcpyHostToDevice, s));
+    return off;
+}
+
+static void batch_layout(cfd_batch_s* b, hipStream_t s) {
+    b->total_tiles = layout_into(b->host, b->form, b->tile_rows, b->chunk);
+    const size_t bytes = b->host.size() * sizeof(BatchCaseDev);
+    CFD_HIP(hipMemcpyAsync(b->dev, b->host.data(), bytes, hipMemcpyHostToDevice, s));
+    if (b->form == 2) {
+        b->host1 = b->host;
+        b->total_tiles1 = layout_into(b->host1, 1, 4, b->chunk);
+        CFD_HIP(hipMemcpyAsync(b->dev1, b->host1.data(), bytes, hipMemcpyHostToDevice, s));
+    }
     CFD_HIP(hipStreamSynchronize(s));
     b->dirty = false;
 }
@@ -161,6 +252,15 @@ static void batch_layout(cfd_batch_s* b, hipStream_t s) {
 static void batch_launch(const cfd_batch_s* b, int parity, hipStream_t s) {
     const int n = int(b->host.size());
     const dim3 grid(unsigned(b->total_tiles));
+    if (b->form == 2) {  // two steps per launch
+        switch (b->tile_rows) {
+            case 8: batch_row2_k<8><<<grid, dim3(64, 8), 0, s>>>(b->dev, n, parity, b->chunk); break;
+            case 10: batch_row2_k<10><<<grid, dim3(64, 10), 0, s>>>(b->dev, n, parity, b->chunk); break;
+            case 12: batch_row2_k<12><<<grid, dim3(64, 12), 0, s>>>(b->dev, n, parity, b->chunk); break;
+            default: throw Error(CFD_E_INVALID, "two-step batch tiles have 8, 10 or 12 waves");
+        }
+        return;
+    }
     if (b->form == 1) {
         switch (b->tile_rows) {
             case 4: batch_row_k<4><<<grid, dim3(64, 6), 0, s>>>(b->dev, n, parity, b->chunk); break;
@@ -175,6 +275,24 @@ static void batch_launch(const cfd_batch_s* b, int parity, hipStream_t s) {
         case 16: batch_step_k<4, 4><<<grid, dim3(64, 4), 0, s>>>(b->dev, n, parity, b->chunk); break;
         default: throw Error(CFD_E_INVALID, "batch tile rows must be 4, 8 or 16");
     }
+}
+
+// nsteps steps from state_a: in form 2, pairs of two-step launches (a -> b ->
+// a) and the remaining steps one per launch on the single-step layout
+static void batch_steps(const cfd_batch_s* b, int64_t nsteps, hipStream_t s) {
+    int64_t k0 = 0;
+    if (b->form == 2) {
+        for (; k0 + 4 <= nsteps; k0 += 4) {
+            batch_launch(b, 0, s);
+            batch_launch(b, 1, s);
+        }
+        const int n = int(b->host1.size());
+        const dim3 grid(unsigned(b->total_tiles1));
+        for (int64_t k = k0; k < nsteps; ++k)
+            batch_row_k<4><<<grid, dim3(64, 6), 0, s>>>(b->dev1, n, int(k & 1), b->chunk);
+        return;
+    }
+    for (int64_t k = 0; k < nsteps; ++k) batch_launch(b, int(k & 1), s);
 }
 
 static void drop_graphs(cfd_batch_s* b) {
@@ -230,8 +348,9 @@ int cfd_batch_create(cfd_ctx_t ctx, const cfd_batch_case* cases, int ncases, cfd
 int cfd_batch_set_form(cfd_batch_t b, int form) {
     return guarded([&] {
         CFD_REQUIRE(b != nullptr, "null batch");
-        CFD_REQUIRE(form == 0 || form == 1, "batch form must be 0 (march tiles) or 1 (row form)");
-        if (form == 1) {
+        CFD_REQUIRE(form >= 0 && form <= 2,
+                    "batch form must be 0 (march tiles), 1 (row form) or 2 (row form, two steps per launch)");
+        if (form >= 1) {
             for (const auto& c : b->host) {
                 if (!wave_row_supported(c.g.Z))
                     throw Error(CFD_E_UNSUPPORTED,
@@ -240,7 +359,14 @@ int cfd_batch_set_form(cfd_batch_t b, int form) {
                     if ((reinterpret_cast<uintptr_t>(c.a[f]) & 15) || (reinterpret_cast<uintptr_t>(c.b[f]) & 15))
                         throw Error(CFD_E_UNSUPPORTED, "the row form needs 16-byte aligned state fields");
             }
-            if (b->tile_rows != 4 && b->tile_rows != 8) b->tile_rows = 4;
+            if (form == 1 && b->tile_rows != 4 && b->tile_rows != 8) b->tile_rows = 4;
+            if (form == 2) {
+                if (b->tile_rows != 8 && b->tile_rows != 10 && b->tile_rows != 12) b->tile_rows = 12;
+                check_form2(b->host, b->tile_rows);
+                if (!b->dev1)
+                    b->dev1 = (BatchCaseDev*)pool_alloc(b->ctx, int64_t(b->host.size()) * int64_t(sizeof(BatchCaseDev)),
+                                                        nullptr);
+            }
         }
         if (form != b->form) {
             b->form = form;
@@ -252,8 +378,13 @@ int cfd_batch_set_form(cfd_batch_t b, int form) {
 int cfd_batch_set_shape(cfd_batch_t b, int tile_rows, int chunk) {
     return guarded([&] {
         CFD_REQUIRE(b != nullptr, "null batch");
-        CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || (tile_rows == 16 && b->form == 0),
-                    "batch tile rows must be 4, 8 or 16 (4 or 8 in the row form)");
+        if (b->form == 2) {
+            CFD_REQUIRE(tile_rows == 8 || tile_rows == 10 || tile_rows == 12,
+                        "two-step batch tiles have 8, 10 or 12 waves");
+            check_form2(b->host, tile_rows);
+        } else
+            CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || (tile_rows == 16 && b->form == 0),
+                        "batch tile rows must be 4, 8 or 16 (4 or 8 in the row form)");
         CFD_REQUIRE(chunk >= 1 && chunk <= 4096, "batch chunk must be in [1, 4096]");
         if (tile_rows != b->tile_rows || chunk != b->chunk) {
             b->tile_rows = tile_rows;
@@ -274,20 +405,20 @@ int cfd_batch_run(cfd_batch_t b, int64_t nsteps, int use_graph, void* stream) {
             batch_layout(b, s);
         }
         if (!use_graph) {
-            for (int64_t k = 0; k < nsteps; ++k) batch_launch(b, int(k & 1), s);
+            batch_steps(b, nsteps, s);
             CFD_HIP(hipGetLastError());
             return;
         }
         // a replay starts from state_a again, so the state must end there
         CFD_REQUIRE(nsteps % 2 == 0, "cfd_batch_run: graph runs need an even step count");
         if (nsteps == 0) return;
-        const auto key = std::make_tuple(nsteps, b->tile_rows * 2 + b->form, b->chunk);
+        const auto key = std::make_tuple(nsteps, b->tile_rows * 4 + b->form, b->chunk);
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {
             hipGraph_t graph = nullptr;
             CFD_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
             try {
-                for (int64_t k = 0; k < nsteps; ++k) batch_launch(b, int(k & 1), b->capture);
+                batch_steps(b, nsteps, b->capture);
             } catch (...) {
                 (void)hipStreamEndCapture(b->capture, &graph);
                 if (graph) (void)hipGraphDestroy(graph);
@@ -310,6 +441,7 @@ int cfd_batch_destroy(cfd_batch_t b) {
         drop_graphs(b);
         if (b->capture) (void)hipStreamDestroy(b->capture);
         pool_free(b->ctx, b->dev, nullptr);
+        if (b->dev1) pool_free(b->ctx, b->dev1, nullptr);
         CFD_HIP(hipStreamSynchronize(nullptr));
         delete b;
     });

@@ -73,7 +73,7 @@ class Ensemble:
     def configure(self, tile_rows=4, chunk=8, group_bytes=0, form=0):
         """Pack cases into groups of at most `group_bytes` of fields (0: one
         group), largest first, and set the tile form (0: 2.5D march tiles, 1:
-        row form) and shape."""
+        row form, 2: row form with two steps per launch) and shape."""
         self._destroy_groups()
         order = sorted(range(len(self.cases)), key=lambda i: -self.cases[i].nbytes)
         packs, cur, cur_b = [], [], 0
@@ -156,6 +156,8 @@ def autotune(ens: Ensemble, steps=20, candidates=None, verbose=False):
                       for gb in (0, 96 << 20, 192 << 20)]
         candidates += [(ty, L, gb, 1) for ty in (4, 8) for L in (4, 8, 16, 32)
                        for gb in (0, 96 << 20, 192 << 20)]
+        candidates += [(nw, L, gb, 2) for nw in (8, 10, 12) for L in (8, 16, 32)
+                       for gb in (0, 192 << 20)]
     results = []
     for cand in candidates:
         ty, L, gb = cand[:3]

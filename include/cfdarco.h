@@ -255,10 +255,13 @@ typedef struct cfd_batch_case {
 } cfd_batch_case;
 int cfd_batch_create(cfd_ctx_t ctx, const cfd_batch_case* cases, int ncases, cfd_batch_t* out);
 /* Tile form: 0 = 2.5D march tiles (any extents; tile_rows 4, 8 or 16 y-rows x
- * 64 z-columns per workgroup), 1 = row form (one wave per z row, tile_rows 4
- * or 8 output rows + 2 halo rows per workgroup; every case's z extent a
+ * 64 z-columns per workgroup), 1 = row form (256/Z z rows per wave, tile_rows
+ * 4 or 8 output waves + 2 halo waves per workgroup; every case's z extent a
  * multiple of 4 and at most 256 and its fields 16-byte aligned, else
- * CFD_E_UNSUPPORTED).  chunk = x-planes per tile. */
+ * CFD_E_UNSUPPORTED), 2 = row form with two steps per launch (tile_rows = 8, 10
+ * or 12 waves per workgroup; steps run in pairs of launches, the remaining
+ * steps one per launch in form 1 with 4-row tiles).  chunk = x-planes per
+ * tile. */
 int cfd_batch_set_form(cfd_batch_t batch, int form);
 int cfd_batch_set_shape(cfd_batch_t batch, int tile_rows, int chunk);
 int cfd_batch_run(cfd_batch_t batch, int64_t nsteps, int use_graph, void* stream);

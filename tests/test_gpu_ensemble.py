@@ -136,3 +136,30 @@ def test_batched_row_form_refuses_unsuited_cases(hip_ctx):
         assert e.value.code == -4
     finally:
         ens.close()
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 0), (10, 5, 0), (12, 3, 1 << 20), (12, 16, 0)])
+def test_batched_two_step_equals_sequential(hip_ctx, shape):
+    """Row form with two steps per launch: 9 steps (two pairs of two-step
+    launches and one single step) and graph replays of 10 steps equal the
+    single-case runs bit for bit."""
+    from cfdarcov2_amd.ensemble import Ensemble
+    print(f"\n[case] ensemble batched two-step shape={shape}", flush=True)
+    ens = Ensemble(hip_ctx, _row_specs())
+    try:
+        ens.run_sequential(9)
+        ref = _snapshot(ens, 9)
+        ens.reset()
+        ens.configure(*shape, form=2)
+        ens.run(9, use_graph=False)
+        _compare(_snapshot(ens, 9), ref)
+        ens.reset()
+        ens.run_sequential(10)
+        ref10 = _snapshot(ens, 10)
+        for _ in range(2):
+            ens.reset()
+            ens.run(10, use_graph=True)
+            torch.randn(1 << 16, device="cuda").sum().item()
+            _compare(_snapshot(ens, 10), ref10)
+    finally:
+        ens.close()
