@@ -233,7 +233,7 @@ PLAIN_WAVE_CANDIDATES = ([(0, 4, 8, 0)] + [(1, 4, L, r) for L in (4, 8, 16) for 
                          + [(2, 8, L, 0) for L in (8, 16)] + [(3, 2, L, 0) for L in (8, 16, 32)]
                          + [(4, 4, L, 0) for L in (8, 16)]
                          + [(v, ty, L, 3) for v in (7, 9) for ty in (4, 8) for L in (8, 16)]
-                         + [(11, 8, L, 3) for L in (8, 16, 32)])
+                         + [(v, ty, L, 3) for v in (11, 12) for ty in (4, 8, 10) for L in (8, 16, 32)])
 
 
 def ensemble_api_seconds(specs, steps, timeout=900):
@@ -273,12 +273,17 @@ def suite_ensemble(ctx, steps=500, verbose=False):
     from cfdarcov2_amd.ensemble import Ensemble, autotune as tune_batch, time_gpu
     specs, steps = workloads.ensemble_config(steps=steps)
     ens = Ensemble(ctx, specs)
-    plain_shapes = {}
+    plain_shapes, plain_heat = {}, {}
     for c in ens.cases:
         if c.spec.kind != "wave":
+            # each heat case with its fastest form (per-thread, or two steps per launch)
+            if c.spec.n not in plain_heat:
+                plain_heat[c.spec.n] = autotune.tune_heat(c.mesh, c.spec.coef, c.spec.dt)
+            h = plain_heat[c.spec.n]
+            c.mesh.set_heat_shape(h["variant"], h["waves"], h["chunk"])
             continue
         if c.spec.n not in plain_shapes:
-            plain_shapes[c.spec.n] = autotune.tune_wave(c.mesh, steps=10, use_cache=False,
+            plain_shapes[c.spec.n] = autotune.tune_wave(c.mesh, steps=20, use_cache=False,
                                                         candidates=PLAIN_WAVE_CANDIDATES)
         b = plain_shapes[c.spec.n]
         c.mesh.set_wave_shape(b["variant"], b["tile_rows"], b["chunk"], b["xcd_remap"])
@@ -297,6 +302,7 @@ def suite_ensemble(ctx, steps=500, verbose=False):
            "batched_ms": t_b * 1e3, "sequential_ms": t_seq * 1e3, "batch_shape": best,
            "plain_wave_shapes": {n: {k: v[k] for k in ("variant", "tile_rows", "chunk", "xcd_remap")}
                                  for n, v in plain_shapes.items()},
+           "plain_heat_shapes": {n: {k: v[k] for k in ("variant", "waves", "chunk")} for n, v in plain_heat.items()},
            "batched_gbs_algorithmic": ens.algorithmic_bytes(steps) / t_b / 1e9}
     ens.close()
     t_api = ensemble_api_seconds(specs, steps)
