@@ -80,7 +80,7 @@ __device__ inline void z_nbrs_row(const V4& c, bool zlo_wall, bool zhi_wall, V4&
 // y0+TY*RW, one row each), marching x over [x0, x1).  The LDS planes hold the
 // tile's rows with the halo, row j at element j*Z.  Used by the single-mesh
 // kernel (wave_row.hip) and the batched ensemble kernel.
-template <class T, int TY, bool PW, bool KF, bool XB, bool NTL>
+template <class T, int TY, bool PW, bool KF, bool XB, bool NTL, bool MR = true>
 __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p, const T* __restrict__ u,
                                 const T* __restrict__ v, const T* __restrict__ w, T* __restrict__ po,
                                 T* __restrict__ uo, T* __restrict__ vo, T* __restrict__ wo, T K,
@@ -102,7 +102,10 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
     const bool is_out = wv < TY;
     const bool lower = wv == TY;
     const LanePlace lp = is_out ? lane_place(lane, Z, RW) : LanePlace{0, 4 * lane, lane < zq};
-    const int y = is_out ? y0 + wv * RW + lp.r : (lower ? y0 - 1 : y0 + TY * RW);
+    // MR false: one row per wave (the caller has RW == 1), so the row is
+    // uniform across the wave and its wall tests are scalar
+    const int y = MR ? (is_out ? y0 + wv * RW + lp.r : (lower ? y0 - 1 : y0 + TY * RW))
+                     : __builtin_amdgcn_readfirstlane(is_out ? y0 + wv : (lower ? y0 - 1 : y0 + TY));
     const bool real = lp.span && y >= 0 && y < Y;
     const bool out = is_out && real;             // a cell of an output row
     // an output wave with no cell inside the mesh (a tile past the last row)

@@ -61,10 +61,17 @@ wave_row_k(UniformGeo<T> g, const T* __restrict__ p, const T* __restrict__ u, co
     const int64_t x1 = (x0 + L < X) ? x0 + L : X;
     const int y0 = yt * TY * row::rows_per_wave(g.Z);
     // chunks away from the x walls need no x-boundary tests
-    if (x0 >= 1 && x1 + 2 <= X)
-        row::row_tile<T, TY, PW, KF, false, NTL>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, x0, x1, y0, wv, nt != 0, sp, sv);
-    else
-        row::row_tile<T, TY, PW, KF, true, NTL>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, x0, x1, y0, wv, nt != 0, sp, sv);
+    const bool inner = x0 >= 1 && x1 + 2 <= X;
+#define CFD_ROW(XB, MR) \
+    row::row_tile<T, TY, PW, KF, XB, NTL, MR>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, x0, x1, y0, wv, nt != 0, sp, sv)
+    if (row::rows_per_wave(g.Z) == 1) {  // one row per wave: uniform rows
+        if (inner) CFD_ROW(false, false);
+        else CFD_ROW(true, false);
+    } else {
+        if (inner) CFD_ROW(false, true);
+        else CFD_ROW(true, true);
+    }
+#undef CFD_ROW
 }
 
 template <class T, int TY, bool PW, bool KF, bool NTL>
