@@ -189,6 +189,13 @@ template <class T> static void create_mesh(cfd_mesh_s* m, const cfd_hex_desc& d)
         u.vol_pow2 = (u.vol > T(0) && std::frexp(u.vol, &ex) == T(0.5)) ? 1 : 0;
         u.inv_vol = T(1) / u.vol;
         m->uniform = true;
+        // two heat steps per launch on large meshes (256^3: 1.5-1.7x the
+        // per-thread kernel); small ones stay on the per-thread kernel
+        if (d.nx * d.ny * d.nz >= (int64_t(1) << 23) && heat_row_supported(d.nz, 12)) {
+            m->heat_variant = 1;
+            m->heat_nw = 12;
+            m->heat_L = 16;
+        }
         if (wave_row_supported(d.nz)) {  // row forms, the fastest measured at 256^3
             if constexpr (sizeof(T) == 4) {  // two steps per pass
                 m->wave_variant = 11;

@@ -337,8 +337,9 @@ int cfd_mesh_set_wave_shape(cfd_mesh_t mesh, int variant, int tile_rows, int chu
  * 4 and at most 256, else CFD_E_UNSUPPORTED), `waves` (8, 12 or 16) waves per
  * workgroup and `chunk` x-planes per tile; runs go in pairs of launches, and
  * the remaining steps and runs with a diffusivity field take variant 0.  Both
- * give bit-identical results.  Meshes start on variant 0; variant 1 pays on
- * large meshes (it needs enough tiles to fill the GPU), and
+ * give bit-identical results.  Variant 1 pays on large meshes (it needs
+ * enough tiles to fill the GPU): meshes of at least 2^23 cells that allow it
+ * start on variant 1 (12 waves, 16 planes), others on variant 0;
  * cfdarcov2_amd/autotune.py:tune_heat times both. */
 int cfd_mesh_set_heat_shape(cfd_mesh_t mesh, int variant, int waves, int chunk);
 
