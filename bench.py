@@ -344,6 +344,53 @@ def suite_adjoint(ctx):
     return rec
 
 
+# The reference's published A100 figures (Mpts/s, KernelBuilder on) for the
+# same drivers and sizes: scripts/v2_results/
+# report_history_a100_cuda_full_no_data_move_even_better.csv:7,9,17,19,28,32.
+A100_MPTS = {("heat", 200): 6440.3, ("heat", 300): 6441.7, ("euler", 200): 264.7, ("euler", 300): 267.1,
+             ("wave", 200): 6559.7, ("wave", 400): 7258.6}
+
+
+def suite_drivers(reps=3):
+    """The reference's example drivers, compiled unchanged against the operator
+    API, run the way the reference's harness runs them
+    (scripts/benchmark_cfdarchov2.py:44: `-Lx n -Ly n -Lz n -dx 1 -dy 1 -dz 1
+    --visualize -t 200 -dt 0.005 -c`, the "Time difference" line, best of 3;
+    Mpts/s = n^3 * 200 / microseconds).  The wave example ships with a host-only
+    boundary callback (a host round trip of the field every step);
+    point_source_wave solves the same problem with its sources set on the
+    device as well."""
+    import re
+    import subprocess
+
+    from cfdarcov2_amd import build as hipbuild
+    runs = [("heat", "drivers/cfdARCO_heat", 200), ("heat", "drivers/cfdARCO_heat", 300),
+            ("euler", "drivers/cfdARCO_euler", 200), ("euler", "drivers/cfdARCO_euler", 300),
+            ("wave", "drivers/cfdARCO_wave", 200), ("wave", "drivers/cfdARCO_wave", 400),
+            ("wave", "bench/point_source_wave", 200), ("wave", "bench/point_source_wave", 400)]
+    out = []
+    for exp, rel, n in runs:
+        exe = os.path.join(hipbuild.BUILD_DIR, rel)
+        if not os.path.exists(exe):
+            out.append({"program": os.path.basename(rel), "n": n, "skipped": "not built"})
+            continue
+        best = None
+        for _ in range(reps):
+            print(f"[drivers] {os.path.basename(rel)} {n}^3", file=sys.stderr, flush=True)
+            res = subprocess.run([exe, "-Lx", str(n), "-Ly", str(n), "-Lz", str(n), "-dx", "1", "-dy", "1", "-dz", "1",
+                                  "-t", "200", "-dt", "0.005", "-c"], capture_output=True, text=True, timeout=300)
+            if res.returncode != 0:
+                raise RuntimeError(f"{rel} {n}^3 failed:\n{res.stderr[-2000:]}")
+            us = int(re.search(r"Time difference = (\d+)\[microseconds\]", res.stdout).group(1))
+            best = us if best is None else min(best, us)
+        sums = re.findall(r"At (\S+) Current sum = (\S+) Current max = (\S+)", res.stdout)
+        mpts = n ** 3 * 200 / best
+        out.append({"program": os.path.basename(rel), "n": n, "best_us": best, "mpts": mpts,
+                    "a100_published_mpts": A100_MPTS[(exp, n)], "vs_a100": mpts / A100_MPTS[(exp, n)],
+                    "final": {name: [float(s), float(m)] for name, s, m in sums}})
+    return {"workload": "reference example drivers, harness parameters, 200 steps, best of 3", "runs": out}
+
+
 def calibrate_copy(args):
     """A plain device-to-device copy of 4 fields' worth of bytes (what one fused
     launch reads), repeated --steps times: the known-size reference for
@@ -377,6 +424,8 @@ def main(argv=None):
                     help="run just one secondary workload and print its record (for profiling)")
     ap.add_argument("--no-suite", action="store_true",
                     help="skip the secondary workloads (heat FP64, wave FP64, Sod, ensemble, adjoint)")
+    ap.add_argument("--drivers", action="store_true",
+                    help="run only the reference's example drivers with the reference harness's parameters")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of the benchmark's read volume (PMC calibration)")
     args = ap.parse_args(argv)
@@ -392,6 +441,10 @@ def main(argv=None):
 
     if args.calibrate_copy:
         calibrate_copy(args)
+        return
+
+    if args.drivers:
+        print(json.dumps(suite_drivers()), flush=True)
         return
 
     if args.only:

@@ -12,6 +12,9 @@
 #include <cstddef>
 #include <initializer_list>
 #include <limits>
+#include <map>
+#include <mutex>
+#include <new>
 #include <numeric>
 #include <ostream>
 #include <stdexcept>
@@ -19,6 +22,87 @@
 #include <vector>
 
 namespace Eigen {
+
+namespace detail {
+
+// Process-wide reuse of large freed blocks.  Drivers that round-trip a field
+// through the host each step (a host boundary callback copies the field,
+// returns the copy, and the old storage is released) otherwise pay a fresh
+// mapping, and a page fault per page, for every such vector.  Blocks of at
+// least kMinBytes are kept by exact size, at most kMaxBlocks of them and
+// kMaxBytes in all; everything else goes straight to operator new / delete.
+class BlockCache {
+public:
+    static constexpr size_t kMinBytes = size_t(1) << 20;
+    static constexpr size_t kMaxBlocks = 8;
+    static constexpr size_t kMaxBytes = size_t(4) << 30;
+
+    static BlockCache& get() {
+        static BlockCache* c = new BlockCache;  // never destroyed: frees may run during exit
+        return *c;
+    }
+    void* take(size_t bytes) {
+        if (bytes >= kMinBytes) {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = free_.find(bytes);
+            if (it != free_.end()) {
+                void* p = it->second;
+                free_.erase(it);
+                held_ -= bytes;
+                return p;
+            }
+        }
+        return ::operator new(bytes);
+    }
+    void give(void* p, size_t bytes) {
+        if (bytes >= kMinBytes) {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (free_.size() < kMaxBlocks && held_ + bytes <= kMaxBytes) {
+                free_.emplace(bytes, p);
+                held_ += bytes;
+                return;
+            }
+        }
+        ::operator delete(p);
+    }
+
+private:
+    std::mutex mu_;
+    std::multimap<size_t, void*> free_;
+    size_t held_ = 0;
+};
+
+template <class T> struct BlockAllocator {
+    using value_type = T;
+    BlockAllocator() noexcept = default;
+    template <class U> BlockAllocator(const BlockAllocator<U>&) noexcept {}
+    T* allocate(size_t n) {
+        if (n > std::numeric_limits<size_t>::max() / sizeof(T)) throw std::bad_array_new_length();
+        return static_cast<T*>(BlockCache::get().take(n * sizeof(T)));
+    }
+    void deallocate(T* p, size_t n) noexcept { BlockCache::get().give(p, n * sizeof(T)); }
+    template <class U> bool operator==(const BlockAllocator<U>&) const noexcept { return true; }
+    template <class U> bool operator!=(const BlockAllocator<U>&) const noexcept { return false; }
+};
+
+// Sum of p[0..n): halves summed recursively, blocks of up to 128 left to
+// right.  A plain left-to-right float sum stalls once it reaches 2^24 times
+// the terms' size (large meshes of O(1) values); this one's error grows with
+// log2(n) instead of n.
+template <class S> S pairwise_sum(const S* p, size_t n) {
+    if (n <= 128) {
+        S s = S(0);
+        for (size_t i = 0; i < n; ++i) s += p[i];
+        return s;
+    }
+    const size_t h = n / 2;
+    return pairwise_sum(p, h) + pairwise_sum(p + h, n - h);
+}
+
+}  // namespace detail
+
+// coefficient storage of the dense types
+template <class S> using Buffer = std::vector<S, detail::BlockAllocator<S>>;
 
 using Index = std::ptrdiff_t;
 constexpr int Dynamic = -1;
@@ -99,8 +183,9 @@ public:
         return m;
     }
 
+    // sums are pairwise (as accurate as a vectorised reduction); the other
     // reductions run left to right
-    S sum() const { return std::accumulate(v_.begin(), v_.end(), S(0)); }
+    S sum() const { return detail::pairwise_sum(v_.data(), v_.size()); }
     S maxCoeff() const { return *std::max_element(v_.begin(), v_.end()); }
     S minCoeff() const { return *std::min_element(v_.begin(), v_.end()); }
     S squaredNorm() const {
@@ -147,8 +232,8 @@ public:
 
     Array1<S> array() const;
 
-    std::vector<S>& storage() { return v_; }
-    const std::vector<S>& storage() const { return v_; }
+    Buffer<S>& storage() { return v_; }
+    const Buffer<S>& storage() const { return v_; }
 
 private:
     template <class F> Matrix zip(const Matrix& o, F f) const {
@@ -164,7 +249,7 @@ private:
     }
 
     Index rows_ = 0, cols_ = 0;
-    std::vector<S> v_;
+    Buffer<S> v_;
 };
 
 template <class T, class S, int R, int C, int O, int MR, int MC,
@@ -181,16 +266,16 @@ Matrix<S, R, C, O, MR, MC> operator*(T s, const Matrix<S, R, C, O, MR, MC>& m) {
 template <class S> class Array1 {
 public:
     Array1() = default;
-    explicit Array1(std::vector<S> v) : v_(std::move(v)) {}
+    explicit Array1(Buffer<S> v) : v_(std::move(v)) {}
     Index size() const { return Index(v_.size()); }
     S& operator()(Index i) { return v_[size_t(i)]; }
     const S& operator()(Index i) const { return v_[size_t(i)]; }
     S& operator[](Index i) { return v_[size_t(i)]; }
     const S& operator[](Index i) const { return v_[size_t(i)]; }
-    const std::vector<S>& storage() const { return v_; }
-    std::vector<S>& storage() { return v_; }
+    const Buffer<S>& storage() const { return v_; }
+    Buffer<S>& storage() { return v_; }
     Matrix<S, Dynamic, 1> matrix() const { return Matrix<S, Dynamic, 1>(*this); }
-    S sum() const { return std::accumulate(v_.begin(), v_.end(), S(0)); }
+    S sum() const { return detail::pairwise_sum(v_.data(), v_.size()); }
     S maxCoeff() const { return *std::max_element(v_.begin(), v_.end()); }
     S minCoeff() const { return *std::min_element(v_.begin(), v_.end()); }
 
@@ -216,7 +301,7 @@ public:
     Array1 array() const { return *this; }
 
 private:
-    std::vector<S> v_;
+    Buffer<S> v_;
 };
 
 template <class S, int R, int C, int O, int MR, int MC>

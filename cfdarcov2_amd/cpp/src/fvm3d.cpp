@@ -573,7 +573,13 @@ void Equation::evaluate(std::vector<Variable*>& all_vars, EquationTemplate& equa
             if (var->has_boundary_conditions_cu) {
                 var->set_bound_cu(dt);
             } else {
-                var->current = var->current_cu.to_eigen();
+                // host boundary callback: the device state comes back first,
+                // into the host vector's own storage when the sizes agree
+                const size_t n = var->current_cu._size;
+                if (size_t(var->current.size()) == n && n)
+                    cfdarco::copy_d2h(var->current.data(), var->current_cu.data.get(), n * sizeof(float));
+                else
+                    var->current = var->current_cu.to_eigen();
                 var->set_bound(dt);
             }
             var->add_history();
