@@ -176,9 +176,10 @@ def suite_wave_f64(ctx, steps=200):
     from cfdarcov2_amd import autotune, capi, workloads
     case = workloads.wave_config(n=256, steps=steps)
     m = capi.HexMesh(ctx, *case.n, *case.spacing, dtype="f64")
-    shape = autotune.tune_wave(m, steps=10, use_cache=False,
+    shape = autotune.tune_wave(m, steps=12, use_cache=False,
                                candidates=[(1, ty, L, r) for ty in (4, 8) for L in (8, 16, 32) for r in (0, 1)]
                                + [(v, ty, L, r) for v in (7, 9) for ty in (4, 8) for L in (8, 16, 32) for r in (1, 3)]
+                               + [(v, 2, L, r) for v in (11, 12) for L in (8, 16, 32, 64) for r in (1, 3)]
                                + [(2, 8, L, 0) for L in (8, 16)])
     N = m.N
     A = [torch.from_numpy(case.p0).cuda()] + [torch.zeros(N, dtype=torch.float64, device="cuda") for _ in range(3)]
@@ -498,7 +499,7 @@ def main(argv=None):
     esize = 4 if args.dtype == "f32" else 8
     # the two-step kernels advance two steps per launch (in pairs of launches;
     # a K field or a step count that is not a multiple of 4 leaves single steps)
-    two_step = shape["variant"] >= 11 and args.dtype == "f32" and args.steps % 4 == 0
+    two_step = shape["variant"] >= 11 and args.steps % 4 == 0
     steps_per_launch = 2 if two_step else 1
     launch_ms = kernel_ms * steps_per_launch
     # algorithmic bytes: 32 B (FP32) per cell-update, so per launch 8 fields x N

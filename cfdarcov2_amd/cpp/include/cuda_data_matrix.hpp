@@ -1,6 +1,6 @@
 // Device vector of the operator API (include/cuda_data_matrix.hpp:35-190 of
 // the reference): a shared_ptr-owned buffer from the library's stream-ordered
-// pool.  Copy construction deep-copies, copy assignment shares (as the
+// pool, through a cache of freed blocks.  Copy construction deep-copies, copy assignment shares (as the
 // reference's implicit assignment does), from_eigen / to_eigen copy to and
 // from host vectors.  All transfers are ordered on the null stream.
 #pragma once
@@ -17,7 +17,7 @@
 
 namespace cfdarco {
 void* device_alloc(size_t bytes);
-void device_free(void* p);
+void device_free(void* p, size_t bytes);
 void copy_h2d(void* dst, const void* src, size_t bytes);
 void copy_d2h(void* dst, const void* src, size_t bytes);
 void copy_d2d(void* dst, const void* src, size_t bytes);
@@ -29,7 +29,7 @@ class CudaDeleter {
 public:
     explicit CudaDeleter(size_t size) : _size{size} {}
     void operator()(void* p) const {
-        if (_size > 0 && p != nullptr) cfdarco::device_free(p);
+        if (_size > 0 && p != nullptr) cfdarco::device_free(p, _size * sizeof(T));
     }
     size_t _size;
 };

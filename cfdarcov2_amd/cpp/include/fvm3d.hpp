@@ -266,7 +266,9 @@ public:
     ~KernelBuilder();
     // analysis and code generation only (no device): fills source()/kernel_name()
     void generate();
-    void build();
+    // compiles (or takes from the cache) the kernel; `announce` prints the
+    // reference's "Kernel build" line
+    void build(bool announce = true);
     void run();
     [[nodiscard]] const std::string& source() const;
     [[nodiscard]] const std::string& kernel_name() const;

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <sstream>
@@ -83,12 +84,22 @@ static bool scalar_value(const Node& n, float& v) {
     }
 }
 
+// a handle on the same device buffer (copy assignment shares; the copy
+// constructor would duplicate the field)
+static CudaDataMatrixD share(const CudaDataMatrixD& m) {
+    CudaDataMatrixD r;
+    r = m;
+    return r;
+}
+
+// Every consumer of a node's value reads it and writes a new buffer, so a
+// leaf or array is handed on as a shared handle.
 CudaDataMatrixD evaluate_node(const Node& n) {
     switch (n.kind) {
         case NodeKind::Leaf:
-            return n.leaf->current_cu;
+            return share(n.leaf->current_cu);
         case NodeKind::Array:
-            return n.array;
+            return share(n.array);
         case NodeKind::Const:
         case NodeKind::DtScalar:
         case NodeKind::Pointer: {
@@ -549,11 +560,32 @@ Equation::Equation(size_t timesteps_) : timesteps{timesteps_} {}
 
 void Equation::evaluate(std::vector<Variable*>& all_vars, EquationTemplate& equation_system, DT* dt, bool visualize,
                         std::vector<Variable*>) const {
+    // Equations flagged `true` run as fused kernels, as in the reference.  An
+    // equation flagged `false` whose target is a plain variable or d1t runs
+    // fused as well: the fused kernel and the operator-by-operator walk apply
+    // the same IEEE operations in the same order (the stabilisation and other
+    // unfusable sub-expressions are still evaluated operator by operator
+    // before the launch), so the results are identical, with fewer launches
+    // and no temporaries.  d2t keeps the per-operator path, whose warm-up rule
+    // differs from the fused one's.  CFDARCO_PER_OP=1 keeps every `false`
+    // equation per operator; an expression the generator cannot take stays
+    // per operator too.
+    const char* per_op_env = std::getenv("CFDARCO_PER_OP");
+    const bool per_op = per_op_env != nullptr && std::strcmp(per_op_env, "0") != 0;
     std::vector<std::unique_ptr<KernelBuilder>> krns(equation_system.size());
     for (size_t i = 0; i < equation_system.size(); ++i) {
-        if (std::get<3>(equation_system[i])) {
-            krns[i] = std::make_unique<KernelBuilder>(std::get<0>(equation_system[i]), &std::get<2>(equation_system[i]));
+        auto& eq = equation_system[i];
+        if (std::get<3>(eq)) {
+            krns[i] = std::make_unique<KernelBuilder>(std::get<0>(eq), &std::get<2>(eq));
             krns[i]->build();
+        } else if (!per_op && dynamic_cast<D2tVar*>(std::get<0>(eq)) == nullptr) {
+            try {
+                auto kb = std::make_unique<KernelBuilder>(std::get<0>(eq), &std::get<2>(eq));
+                kb->build(false);
+                krns[i] = std::move(kb);
+            } catch (const std::exception&) {
+                krns[i].reset();
+            }
         }
     }
     const auto t0 = std::chrono::steady_clock::now();

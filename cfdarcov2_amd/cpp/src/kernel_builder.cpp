@@ -361,7 +361,7 @@ void KernelBuilder::generate() {
     I.generate();
 }
 
-void KernelBuilder::build() {
+void KernelBuilder::build(bool announce) {
     generate();
     Impl& I = *impl_;
     auto* cm = dynamic_cast<CudaMesh3D*>(I.mesh);
@@ -389,7 +389,7 @@ void KernelBuilder::build() {
     if (std::getenv("CFDARCO_JIT_PRINT")) std::cout << "Building kernel:\n" << I.source << std::endl;
     cfdarco::check(cfd_program_compile(CFDArcoGlobalInit::context(), I.source.c_str(), I.kname.c_str(), &I.prog),
                    "KernelBuilder::build");
-    std::cout << "Kernel build" << std::endl;
+    if (announce) std::cout << "Kernel build" << std::endl;
 }
 
 void KernelBuilder::run() {
@@ -443,8 +443,11 @@ void KernelBuilder::run() {
     const uint32_t grid[3] = {uint32_t((I.geo.Z + 63) / 64), uint32_t((I.geo.Y + 3) / 4), uint32_t(I.geo.X)};
     const uint32_t block[3] = {64, 4, 1};
     cfdarco::check(cfd_program_launch(I.prog, grid, block, args.data(), nullptr), "KernelBuilder::run");
-    // new state in, old state becomes the next output (or the previous state of d2t)
-    CudaDataMatrixD old = st->current_cu;
+    // new state in, old state becomes the next output (or the previous state of
+    // d2t); the handle is shared, and a buffer someone else still holds is not
+    // reused as an output (use count above)
+    CudaDataMatrixD old;
+    old = st->current_cu;
     st->current_cu = I.next;
     if (I.target == Impl::Target::Leapfrog) {
         I.next = I.d2tvar->pre_current_cu;

@@ -167,8 +167,8 @@ __global__ void __launch_bounds__(64 * NW) batch_row2_k(const BatchCaseDev* __re
         const int y0 = yt * (band - 8);
         const bool inner = x0 >= 4 && x1 + 4 <= X;
 #define CFD_WAVE2(PW, XB)                                                                                  \
-    row2::tile2m<NW, PW, XB, true>(cs.g, in[0], in[1], in[2], in[3], out[0], out[1], out[2], out[3], cs.coef, \
-                                   cs.dt, x0, x1, y0, wv, false, s0, s1, s2, s3)
+    row2::tile2m<float, NW, PW, XB, true>(cs.g, in[0], in[1], in[2], in[3], out[0], out[1], out[2], out[3],    \
+                                          cs.coef, cs.dt, x0, x1, y0, wv, false, s0, s1, s2, s3)
         if (pw) {
             if (inner) CFD_WAVE2(true, false);
             else CFD_WAVE2(true, true);

@@ -66,10 +66,11 @@ template <class T>
 bool launch_heat_row2(const UniformGeo<T>& g, const void* in, void* out, double alpha, double dt, int nw, int L,
                       hipStream_t s);
 
-// Two steps per pass of the row form (wave_row2.hip), FP32 and constant K.
-// Returns false (nothing launched) for meshes or field pointers it cannot take.
-bool launch_wave_row2(const UniformGeo<float>& g, void* const* in, void* const* out, double K, double dt, int ty,
-                      int L, int order, bool nt, hipStream_t s);
+// Two steps per pass of the row form (wave_row2.hip), constant K.  Returns
+// false (nothing launched) for meshes or field pointers it cannot take.
+template <class T>
+bool launch_wave_row2(const UniformGeo<T>& g, void* const* in, void* const* out, double K, double dt, int ty, int L,
+                      int order, bool nt, hipStream_t s);
 
 template <class T>
 void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamma, double cfl, int64_t nsteps,

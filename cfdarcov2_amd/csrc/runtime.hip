@@ -710,9 +710,11 @@ int cfd_mesh_set_wave_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk,
         CFD_REQUIRE(variant >= 0 && variant <= 12, "wave variant must be 0 to 12");
         const bool row_form = variant >= 7;
         if (variant >= 11) {
-            CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || tile_rows == 10,
-                        "two-step row-form (11, 12) tile_rows must be 4, 8 or 10");
-            if (m->dtype != CFD_F32) throw Error(CFD_E_UNSUPPORTED, "the two-step row form is FP32 only");
+            if (m->dtype == CFD_F32)
+                CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || tile_rows == 10,
+                            "two-step row-form (11, 12) tile_rows must be 4, 8 or 10 in FP32");
+            else
+                CFD_REQUIRE(tile_rows == 2, "two-step row-form (11, 12) tile_rows must be 2 in FP64");
             if (!wave_row_supported(m->Z))
                 throw Error(CFD_E_UNSUPPORTED, "the row form needs a z extent that is a multiple of 4 and at most 256");
         } else if (row_form) {

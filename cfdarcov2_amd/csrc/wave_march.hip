@@ -307,20 +307,17 @@ void launch_wave_march(const cfd_mesh_s* m, void* const a[4], void* const b[4], 
     int64_t k0 = 0;
     if (variant >= 11) {
         // two steps per launch, in pairs (a -> b -> a) so the state ends where
-        // single steps leave it; constant K, FP32.  The rest of the steps, and
-        // every step of a case the two-step form cannot take, run the
-        // single-step row form.
-        if constexpr (sizeof(T) == 4) {
-            if (!Kf) {
-                const UniformGeo<float>& gf = *(const UniformGeo<float>*)&g;
-                for (; k0 + 4 <= nsteps; k0 += 4) {
-                    if (!launch_wave_row2(gf, a, b, K, dt, ty, L, remap & 1, variant == 12, s)) break;
-                    launch_wave_row2(gf, b, a, K, dt, ty, L, (remap & 1) | (remap & 2), variant == 12, s);
-                }
+        // single steps leave it; constant K.  The rest of the steps, and every
+        // step of a case the two-step form cannot take, run the single-step
+        // row form.
+        if (!Kf) {
+            for (; k0 + 4 <= nsteps; k0 += 4) {
+                if (!launch_wave_row2<T>(g, a, b, K, dt, ty, L, remap & 1, variant == 12, s)) break;
+                launch_wave_row2<T>(g, b, a, K, dt, ty, L, (remap & 1) | (remap & 2), variant == 12, s);
             }
         }
         variant = 9;
-        ty = ty == 10 ? 8 : ty;
+        ty = ty == 10 ? 8 : ty == 2 ? 4 : ty;
     }
     for (int64_t k = k0; k < nsteps; ++k) {
         void* const* in = (k & 1) ? b : a;

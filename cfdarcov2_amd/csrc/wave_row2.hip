@@ -1,5 +1,6 @@
-// Two time steps per pass of the row form (variants 11 and 12): FP32, constant
-// K, z extent a multiple of 4 and at most 256.
+// Two time steps per pass of the row form (variants 11 and 12): constant K, z
+// extent a multiple of 4 and at most 256; FP32 tiles of 4, 8 or 10 rows, FP64
+// tiles of 2 rows (registers).
 //
 // The single-step row form (wave_row.hip) reads and writes every field once per
 // step and runs at the memory's streaming rate.  This kernel advances the
@@ -37,12 +38,12 @@
 namespace cfd {
 
 
-template <int TY, bool PW, bool NTL>
+template <class T, int TY, bool PW, bool NTL>
 __global__ void __launch_bounds__(64 * (TY + 6))
-wave_row2_k(UniformGeo<float> g, const float* __restrict__ p, const float* __restrict__ u,
-            const float* __restrict__ v, const float* __restrict__ w, float* __restrict__ po, float* __restrict__ uo,
-            float* __restrict__ vo, float* __restrict__ wo, float K, float dt, int L, int nyt, int order, int nt) {
-    using row2::V4;
+wave_row2_k(UniformGeo<T> g, const T* __restrict__ p, const T* __restrict__ u, const T* __restrict__ v,
+            const T* __restrict__ w, T* __restrict__ po, T* __restrict__ uo, T* __restrict__ vo, T* __restrict__ wo,
+            T K, T dt, int L, int nyt, int order, int nt) {
+    using V4 = row2::V4T<T>;
     __shared__ V4 P1[TY + 6][64];
     __shared__ V4 V1[TY + 6][64];
     __shared__ V4 P2[TY + 6][64];
@@ -60,38 +61,40 @@ wave_row2_k(UniformGeo<float> g, const float* __restrict__ p, const float* __res
     const int y0 = yt * TY;
     // chunks four planes away from the x walls need no x-boundary tests
     if (x0 >= 4 && x1 + 4 <= X)
-        row2::tile2<TY, PW, false, NTL>(g, p, u, v, w, po, uo, vo, wo, K, dt, x0, x1, y0, j, nt != 0, P1, V1, P2, V2);
+        row2::tile2<T, TY, PW, false, NTL>(g, p, u, v, w, po, uo, vo, wo, K, dt, x0, x1, y0, j, nt != 0, P1, V1, P2,
+                                           V2);
     else
-        row2::tile2<TY, PW, true, NTL>(g, p, u, v, w, po, uo, vo, wo, K, dt, x0, x1, y0, j, nt != 0, P1, V1, P2, V2);
+        row2::tile2<T, TY, PW, true, NTL>(g, p, u, v, w, po, uo, vo, wo, K, dt, x0, x1, y0, j, nt != 0, P1, V1, P2,
+                                          V2);
 }
 
-template <int TY, bool PW>
-static void launch_row2_k(const UniformGeo<float>& g, void* const* in, void* const* out, double K, double dt, int L,
+template <class T, int TY, bool PW>
+static void launch_row2_k(const UniformGeo<T>& g, void* const* in, void* const* out, double K, double dt, int L,
                           int order, bool nt, hipStream_t s) {
     const int nyt = int((g.Y + TY - 1) / TY);
     const int64_t nxc = (g.X + L - 1) / L;
     const int64_t nwg = int64_t(nyt) * nxc;
     CFD_REQUIRE(nwg < (int64_t(1) << 31), "row march grid too large");
     const int ord = ((order & 1) && nwg % 8 == 0 ? 1 : 0) | (order & 2);
-    wave_row2_k<TY, PW, true><<<dim3(unsigned(nwg)), dim3(64, TY + 6), 0, s>>>(
-        g, (const float*)in[0], (const float*)in[1], (const float*)in[2], (const float*)in[3], (float*)out[0],
-        (float*)out[1], (float*)out[2], (float*)out[3], float(K), float(dt), L, nyt, ord, nt ? 1 : 0);
+    wave_row2_k<T, TY, PW, true><<<dim3(unsigned(nwg)), dim3(64, TY + 6), 0, s>>>(
+        g, (const T*)in[0], (const T*)in[1], (const T*)in[2], (const T*)in[3], (T*)out[0], (T*)out[1], (T*)out[2],
+        (T*)out[3], T(K), T(dt), L, nyt, ord, nt ? 1 : 0);
     CFD_HIP(hipGetLastError());
 }
 
-template <int TY>
-static void launch_row2_ty(const UniformGeo<float>& g, void* const* in, void* const* out, double K, double dt, int L,
+template <class T, int TY>
+static void launch_row2_ty(const UniformGeo<T>& g, void* const* in, void* const* out, double K, double dt, int L,
                            int order, bool nt, hipStream_t s) {
-    if (g.vol_pow2) launch_row2_k<TY, true>(g, in, out, K, dt, L, order, nt, s);
-    else launch_row2_k<TY, false>(g, in, out, K, dt, L, order, nt, s);
+    if (g.vol_pow2) launch_row2_k<T, TY, true>(g, in, out, K, dt, L, order, nt, s);
+    else launch_row2_k<T, TY, false>(g, in, out, K, dt, L, order, nt, s);
 }
 
-template <int NW, bool PW, bool NTL>
+template <class T, int NW, bool PW, bool NTL>
 __global__ void __launch_bounds__(64 * NW)
-wave_row2m_k(UniformGeo<float> g, const float* __restrict__ p, const float* __restrict__ u,
-             const float* __restrict__ v, const float* __restrict__ w, float* __restrict__ po, float* __restrict__ uo,
-             float* __restrict__ vo, float* __restrict__ wo, float K, float dt, int L, int nyt, int order, int nt) {
-    using row2::V4;
+wave_row2m_k(UniformGeo<T> g, const T* __restrict__ p, const T* __restrict__ u, const T* __restrict__ v,
+             const T* __restrict__ w, T* __restrict__ po, T* __restrict__ uo, T* __restrict__ vo, T* __restrict__ wo,
+             T K, T dt, int L, int nyt, int order, int nt) {
+    using V4 = row2::V4T<T>;
     __shared__ V4 P1[NW][64];
     __shared__ V4 V1[NW][64];
     __shared__ V4 P2[NW][64];
@@ -108,15 +111,15 @@ wave_row2m_k(UniformGeo<float> g, const float* __restrict__ p, const float* __re
     const int64_t x1 = (x0 + L < X) ? x0 + L : X;
     const int y0 = yt * (NW * row::rows_per_wave(g.Z) - 8);
     if (x0 >= 4 && x1 + 4 <= X)
-        row2::tile2m<NW, PW, false, NTL>(g, p, u, v, w, po, uo, vo, wo, K, dt, x0, x1, y0, wv, nt != 0, P1, V1, P2,
-                                         V2);
+        row2::tile2m<T, NW, PW, false, NTL>(g, p, u, v, w, po, uo, vo, wo, K, dt, x0, x1, y0, wv, nt != 0, P1, V1,
+                                            P2, V2);
     else
-        row2::tile2m<NW, PW, true, NTL>(g, p, u, v, w, po, uo, vo, wo, K, dt, x0, x1, y0, wv, nt != 0, P1, V1, P2,
-                                        V2);
+        row2::tile2m<T, NW, PW, true, NTL>(g, p, u, v, w, po, uo, vo, wo, K, dt, x0, x1, y0, wv, nt != 0, P1, V1,
+                                           P2, V2);
 }
 
-template <int NW>
-static void launch_row2m(const UniformGeo<float>& g, void* const* in, void* const* out, double K, double dt, int L,
+template <class T, int NW>
+static void launch_row2m(const UniformGeo<T>& g, void* const* in, void* const* out, double K, double dt, int L,
                          int order, bool nt, hipStream_t s) {
     const int64_t rows = int64_t(NW) * row::rows_per_wave(g.Z) - 8;  // output rows per tile
     CFD_REQUIRE(rows >= 1, "two-step tile without output rows");
@@ -125,39 +128,56 @@ static void launch_row2m(const UniformGeo<float>& g, void* const* in, void* cons
     const int64_t nwg = int64_t(nyt) * nxc;
     CFD_REQUIRE(nwg < (int64_t(1) << 31), "row march grid too large");
     const int ord = ((order & 1) && nwg % 8 == 0 ? 1 : 0) | (order & 2);
-    const float* const* fi = (const float* const*)in;
-    float* const* fo = (float* const*)out;
+    const T* const* fi = (const T* const*)in;
+    T* const* fo = (T* const*)out;
     if (g.vol_pow2)
-        wave_row2m_k<NW, true, true><<<dim3(unsigned(nwg)), dim3(64, NW), 0, s>>>(
-            g, fi[0], fi[1], fi[2], fi[3], fo[0], fo[1], fo[2], fo[3], float(K), float(dt), L, nyt, ord, nt ? 1 : 0);
+        wave_row2m_k<T, NW, true, true><<<dim3(unsigned(nwg)), dim3(64, NW), 0, s>>>(
+            g, fi[0], fi[1], fi[2], fi[3], fo[0], fo[1], fo[2], fo[3], T(K), T(dt), L, nyt, ord, nt ? 1 : 0);
     else
-        wave_row2m_k<NW, false, true><<<dim3(unsigned(nwg)), dim3(64, NW), 0, s>>>(
-            g, fi[0], fi[1], fi[2], fi[3], fo[0], fo[1], fo[2], fo[3], float(K), float(dt), L, nyt, ord, nt ? 1 : 0);
+        wave_row2m_k<T, NW, false, true><<<dim3(unsigned(nwg)), dim3(64, NW), 0, s>>>(
+            g, fi[0], fi[1], fi[2], fi[3], fo[0], fo[1], fo[2], fo[3], T(K), T(dt), L, nyt, ord, nt ? 1 : 0);
     CFD_HIP(hipGetLastError());
 }
 
-bool launch_wave_row2(const UniformGeo<float>& g, void* const* in, void* const* out, double K, double dt, int ty,
-                      int L, int order, bool nt, hipStream_t s) {
+template <class T>
+bool launch_wave_row2(const UniformGeo<T>& g, void* const* in, void* const* out, double K, double dt, int ty, int L,
+                      int order, bool nt, hipStream_t s) {
     if (!wave_row_supported(g.Z)) return false;
-    const uintptr_t al = 4 * sizeof(float) - 1;
+    const uintptr_t al = 4 * sizeof(T) - 1;
     for (int i = 0; i < 4; ++i)
         if ((reinterpret_cast<uintptr_t>(in[i]) & al) || (reinterpret_cast<uintptr_t>(out[i]) & al)) return false;
-    if (row::rows_per_wave(g.Z) > 1) {  // several rows per wave: 8, 10 or 12 waves per workgroup
+    const bool multi = row::rows_per_wave(g.Z) > 1;
+    if constexpr (sizeof(T) == 8) {
+        // FP64 keeps twice the registers per queue entry: only the tiles that
+        // fit without spilling, 8 waves per workgroup (2 output rows of full
+        // rows, or 8 waves of several rows each)
+        if (ty != 2) throw Error(CFD_E_INVALID, "FP64 two-step row-form tiles have 2 rows");
+        if (multi) launch_row2m<T, 8>(g, in, out, K, dt, L, order, nt, s);
+        else launch_row2_ty<T, 2>(g, in, out, K, dt, L, order, nt, s);
+        return true;
+    } else {
+        if (multi) {  // several rows per wave: 8, 10 or 12 waves per workgroup
+            switch (ty) {
+                case 4: launch_row2m<T, 8>(g, in, out, K, dt, L, order, nt, s); break;
+                case 8: launch_row2m<T, 10>(g, in, out, K, dt, L, order, nt, s); break;
+                case 10: launch_row2m<T, 12>(g, in, out, K, dt, L, order, nt, s); break;
+                default: throw Error(CFD_E_INVALID, "two-step row-form tiles have 4, 8 or 10 rows");
+            }
+            return true;
+        }
         switch (ty) {
-            case 4: launch_row2m<8>(g, in, out, K, dt, L, order, nt, s); break;
-            case 8: launch_row2m<10>(g, in, out, K, dt, L, order, nt, s); break;
-            case 10: launch_row2m<12>(g, in, out, K, dt, L, order, nt, s); break;
+            case 4: launch_row2_ty<T, 4>(g, in, out, K, dt, L, order, nt, s); break;
+            case 8: launch_row2_ty<T, 8>(g, in, out, K, dt, L, order, nt, s); break;
+            case 10: launch_row2_ty<T, 10>(g, in, out, K, dt, L, order, nt, s); break;
             default: throw Error(CFD_E_INVALID, "two-step row-form tiles have 4, 8 or 10 rows");
         }
         return true;
     }
-    switch (ty) {
-        case 4: launch_row2_ty<4>(g, in, out, K, dt, L, order, nt, s); break;
-        case 8: launch_row2_ty<8>(g, in, out, K, dt, L, order, nt, s); break;
-        case 10: launch_row2_ty<10>(g, in, out, K, dt, L, order, nt, s); break;
-        default: throw Error(CFD_E_INVALID, "two-step row-form tiles have 4, 8 or 10 rows");
-    }
-    return true;
 }
+
+template bool launch_wave_row2<float>(const UniformGeo<float>&, void* const*, void* const*, double, double, int, int,
+                                      int, bool, hipStream_t);
+template bool launch_wave_row2<double>(const UniformGeo<double>&, void* const*, void* const*, double, double, int,
+                                       int, int, bool, hipStream_t);
 
 }  // namespace cfd

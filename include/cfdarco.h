@@ -318,13 +318,13 @@ int cfd_mesh_geometry(cfd_mesh_t mesh, cfd_geometry* out);
  * workgroup, non-temporal output stores, variant 8 = variant 7 with ordinary
  * stores, variant 9 = variant 7 with non-temporal loads and ordinary stores,
  * variant 10 = non-temporal loads and stores, variant 11 = two steps per
- * launch (FP32 only; `tile_rows` 4, 8 or 10; steps run in pairs of launches,
- * the remainder and K-field runs as variant 9), variant 12 = variant 11 with
+ * launch (`tile_rows` 4, 8 or 10 in FP32, 2 in FP64; steps run in pairs of
+ * launches, the remainder and K-field runs as variant 9), variant 12 = variant 11 with
  * non-temporal stores; `chunk` x-planes per workgroup
  * (per wave for variant 3) and XCD-aware tile placement when bit 0 of
  * `xcd_remap` is set; for the row form (7 to 10), bit 1 of `xcd_remap` makes
  * every other step walk its tiles in reverse.  The row form returns
- * CFD_E_UNSUPPORTED for other z extents (and variants 11 and 12 for FP64).
+ * CFD_E_UNSUPPORTED for other z extents.
  * Uniform meshes whose z extent suits the row form start on variant 11 (FP32)
  * or 7 (FP64).
  * All variants give bit-identical results.  Set by the autotuner

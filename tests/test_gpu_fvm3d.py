@@ -28,8 +28,16 @@ f32 = np.float32
 
 
 def _run_case(case, n, d, dt, steps, fields, nout):
+    """Runs one system of tests/cpp/fvm3d_parity.cpp.  A case ending in
+    "_perop" is the same system with CFDARCO_PER_OP=1: equations flagged
+    `false` are then evaluated operator by operator instead of fused."""
     if not os.path.exists(PROG):
         pytest.fail(f"{PROG} not built (run __graft_entry__.build())")
+    env = dict(os.environ)
+    env.pop("CFDARCO_PER_OP", None)
+    if case.endswith("_perop"):
+        case = case[: -len("_perop")]
+        env["CFDARCO_PER_OP"] = "1"
     N = n[0] * n[1] * n[2]
     with tempfile.TemporaryDirectory() as tmp:
         inp, out = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
@@ -40,7 +48,7 @@ def _run_case(case, n, d, dt, steps, fields, nout):
             for a in fields:
                 np.ascontiguousarray(a, dtype=f32).tofile(f)
         print(f"[case] fvm3d {case} n={n} steps={steps}", flush=True)
-        res = subprocess.run([PROG, case, inp, out], capture_output=True, text=True, timeout=300)
+        res = subprocess.run([PROG, case, inp, out], capture_output=True, text=True, timeout=300, env=env)
         assert res.returncode == 0, res.stdout + res.stderr
         got = np.fromfile(out, dtype=f32)
     assert got.size == nout * N
@@ -54,7 +62,7 @@ def _mesh(oracle_lib, n, d, perturbed=False):
     return oracle_lib.OracleMesh(*n, *d, dtype=f32, perturb=off)
 
 
-@pytest.mark.parametrize("case", ["heat", "heat_nojit"])
+@pytest.mark.parametrize("case", ["heat", "heat_nojit", "heat_nojit_perop"])
 def test_heat_driver_structure(oracle_lib, case):
     n, d = (12, 10, 16), (1 / 16, 1 / 16, 1 / 16)
     om = _mesh(oracle_lib, n, d)
@@ -71,7 +79,8 @@ def test_heat_driver_structure(oracle_lib, case):
     assert np.array_equal(got[0], T)
 
 
-@pytest.mark.parametrize("case", ["wave", "wave_nojit", "wave_perturbed", "wave_perturbed_nojit"])
+@pytest.mark.parametrize("case", ["wave", "wave_nojit", "wave_perturbed", "wave_perturbed_nojit",
+                                  "wave_perturbed_nojit_perop"])
 def test_acoustic_system_through_api(oracle_lib, case):
     n, d = (10, 12, 14), (1 / 16, 1 / 16, 1 / 16)
     perturbed = "perturbed" in case
@@ -119,7 +128,7 @@ def test_second_order_time(oracle_lib, case):
     assert np.array_equal(got[0], u)
 
 
-@pytest.mark.parametrize("case", ["euler1d", "euler1d_nojit", "euler1d_perturbed"])
+@pytest.mark.parametrize("case", ["euler1d", "euler1d_nojit", "euler1d_perturbed", "euler1d_nojit_perop"])
 def test_euler_like_system(oracle_lib, case):
     """Assignment equations, DT inside expressions, products inside stencil
     operands, stabilisation, a stencil of a stencil and the device CFL policy."""
@@ -170,13 +179,17 @@ def _sum(a):
     return np.sum(a, dtype=np.float64)
 
 
-def _run_driver(name, args):
+def _run_driver(name, args, per_op=False):
     path = os.path.join(DRIVERS, name)
     if not os.path.exists(path):
         pytest.skip(f"{name}: reference driver binaries are built only where the reference sources exist")
-    print(f"[case] driver {name} {' '.join(args)}", flush=True)
+    print(f"[case] driver {name} {' '.join(args)}{' (per operator)' if per_op else ''}", flush=True)
+    env = dict(os.environ)
+    env.pop("CFDARCO_PER_OP", None)
+    if per_op:
+        env["CFDARCO_PER_OP"] = "1"
     with tempfile.TemporaryDirectory() as tmp:
-        res = subprocess.run([path, *args], capture_output=True, text=True, timeout=300, cwd=tmp)
+        res = subprocess.run([path, *args], capture_output=True, text=True, timeout=300, cwd=tmp, env=env)
     assert res.returncode == 0, res.stdout + res.stderr
     return res.stdout
 
@@ -218,6 +231,21 @@ def test_reference_drivers_run_unchanged(name, args):
     for var, s, mx in sums:
         assert np.isfinite(float(s)) and np.isfinite(float(mx)), (var, s, mx)
     assert "Time difference" in out
+
+
+@pytest.mark.parametrize("name,args", [
+    ("cfdARCO_euler", ["-Lx", "12", "-Ly", "10", "-Lz", "14", "-t", "12", "-dt", "0.1", "-c"]),
+    ("cfdARCO_sod_shock", ["-Lx", "64", "-Ly", "2", "-Lz", "2", "-t", "30", "-dt", "0.2", "-c"]),
+])
+def test_false_flagged_equations_fused_equal_per_operator(name, args):
+    """The drivers' equations flagged `false` (d1t with stabilisation) run as
+    fused kernels; every printed sum and maximum equals the run that
+    evaluates them operator by operator."""
+    def lines(out):
+        return re.findall(r"At \S+ Current sum = \S+ Current max = \S+", out)
+    fused = lines(_run_driver(name, args))
+    per_op = lines(_run_driver(name, args, per_op=True))
+    assert fused and fused == per_op
 
 
 def test_reference_sod_driver_unchanged(oracle_lib):

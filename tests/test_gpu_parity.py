@@ -243,6 +243,28 @@ def test_wave_full_size_properties(hip_ctx, oracle_lib):
     assert max(rel) < 1e-4
 
 
+def test_wave_two_step_f64_full_size(hip_ctx, oracle_lib):
+    """256^3 benchmark case in FP64: two passes of the two-step row form
+    (2-row tiles), forward and reversed tile order, equal four oracle steps
+    bit for bit."""
+    from cfdarcov2_amd import capi
+    print("\n[case] wave 256^3 FP64 two-step", flush=True)
+    case = workloads.wave_config(n=256, steps=4)
+    N = 256 ** 3
+    om = oracle_lib.OracleMesh(*case.n, *case.spacing, dtype=np.float64)
+    z = np.zeros(N)
+    ref = om.wave_run(case.p0, z, z, z, -1.0, case.dt, 4)
+    del om
+    g = capi.HexMesh(hip_ctx, *case.n, *case.spacing, dtype=np.float64)
+    for shape in ((11, 2, 16, 3), (12, 2, 32, 1)):
+        g.set_wave_shape(*shape)
+        A = [torch.from_numpy(case.p0).cuda()] + [torch.zeros(N, dtype=torch.float64, device="cuda") for _ in range(3)]
+        B = [torch.empty_like(A[0]) for _ in range(4)]
+        got = [_np(t) for t in g.wave_run(A, B, -case.c ** 2, case.dt, 4)]
+        for name, a, r in zip("puvw", got, ref):
+            assert np.array_equal(a, r), f"256^3 FP64 field {name}, shape {shape}"
+
+
 def test_ewise_and_reductions(hip_ctx, oracle_lib):
     from cfdarcov2_amd import capi
     rng = np.random.default_rng(11)
@@ -355,6 +377,13 @@ ROW_CASES = [
     ((10, 70, 128), (1.0 / 16, 1.0 / 128, 1.0 / 128), (11, 8, 6, 1)),
     ((12, 50, 96), (1.0 / 16, 1.0 / 64, 1.0 / 128), (12, 10, 3, 2)),
     ((7, 45, 32), (0.125, 1.0 / 64, 1.0 / 32), (11, 4, 16, 3)),
+    # two steps per pass, FP64 tiles (2 rows of full rows; 8 waves of several rows)
+    ((19, 21, 68), (1.0 / 32, 1.0 / 32, 1.0 / 64), (11, 2, 5, 0)),
+    ((30, 27, 256), (1.0 / 32, 1.0 / 32, 1.0 / 256), (12, 2, 4, 1)),
+    ((9, 30, 256), (1.0 / 16, 1.0 / 32, 1.0 / 256), (11, 2, 16, 2)),
+    ((6, 5, 4), (0.25, 0.25, 0.25), (12, 2, 1, 0)),
+    ((13, 11, 60), (0.375, 0.25, 0.125), (11, 2, 2, 3)),
+    ((10, 70, 128), (1.0 / 16, 1.0 / 128, 1.0 / 128), (12, 2, 6, 1)),
 ]
 
 
@@ -365,8 +394,10 @@ def test_wave_row_form(hip_ctx, oracle_lib, dtype, case):
     z-neighbours by lane shifts): the oracle's bits."""
     from cfdarcov2_amd import capi
     n, d, shape = case
-    if dtype == np.float64 and (shape[1] == 14 or shape[0] >= 11):
-        pytest.skip("14-row tiles and the two-step form are FP32 only")
+    if dtype == np.float64 and (shape[1] == 14 or (shape[0] >= 11 and shape[1] != 2)):
+        pytest.skip("14-row tiles and two-step tiles of 4 to 10 rows are FP32 only")
+    if dtype == np.float32 and shape[0] >= 11 and shape[1] == 2:
+        pytest.skip("two-step tiles of 2 rows are FP64 only")
     print(f"\n[case] wave row form n={n} shape={shape} {np.dtype(dtype).name}", flush=True)
     rng = np.random.default_rng(41)
     gm = capi.HexMesh(hip_ctx, *n, *d, dtype=dtype)
@@ -423,8 +454,13 @@ def test_wave_row_form_rejects_unsupported_rows(hip_ctx):
         capi.HexMesh(hip_ctx, 8, 8, 64, 0.125, 0.125, 0.125, dtype=np.float32).set_wave_shape(7, 6, 8, 0)
     with pytest.raises(capi.CfdError):
         capi.HexMesh(hip_ctx, 8, 8, 64, 0.125, 0.125, 0.125, dtype=np.float64).set_wave_shape(7, 14, 8, 0)
-    with pytest.raises(capi.CfdError) as e:
+    # two-step tiles: 4, 8 or 10 rows in FP32, 2 rows in FP64
+    with pytest.raises(capi.CfdError):
         capi.HexMesh(hip_ctx, 8, 8, 64, 0.125, 0.125, 0.125, dtype=np.float64).set_wave_shape(11, 8, 8, 0)
+    with pytest.raises(capi.CfdError):
+        capi.HexMesh(hip_ctx, 8, 8, 64, 0.125, 0.125, 0.125, dtype=np.float32).set_wave_shape(12, 2, 8, 0)
+    with pytest.raises(capi.CfdError) as e:
+        capi.HexMesh(hip_ctx, 8, 8, 70, 0.125, 0.125, 0.125, dtype=np.float64).set_wave_shape(11, 2, 8, 0)
     assert e.value.code == -4
 
 
