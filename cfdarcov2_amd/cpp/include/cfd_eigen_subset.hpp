@@ -85,18 +85,44 @@ template <class T> struct BlockAllocator {
     template <class U> bool operator!=(const BlockAllocator<U>&) const noexcept { return false; }
 };
 
-// Sum of p[0..n): halves summed recursively, blocks of up to 128 left to
-// right.  A plain left-to-right float sum stalls once it reaches 2^24 times
-// the terms' size (large meshes of O(1) values); this one's error grows with
-// log2(n) instead of n.
+// Sum of p[0..n): halves summed recursively down to blocks of at most 1024,
+// each block summed in eight interleaved partial sums (which the compiler
+// keeps in vector registers) combined pairwise.  A plain left-to-right float
+// sum stalls once it reaches 2^24 times the terms' size (large meshes of O(1)
+// values) and runs one dependent add per element; this one's error grows
+// with log2(n) and it streams at memory speed.
 template <class S> S pairwise_sum(const S* p, size_t n) {
-    if (n <= 128) {
-        S s = S(0);
-        for (size_t i = 0; i < n; ++i) s += p[i];
-        return s;
+    if (n <= 1024) {
+        S acc[8] = {S(0), S(0), S(0), S(0), S(0), S(0), S(0), S(0)};
+        size_t i = 0;
+        for (; i + 8 <= n; i += 8)
+            for (int k = 0; k < 8; ++k) acc[k] += p[i + k];
+        for (int k = 0; i < n; ++i, ++k) acc[k] += p[i];
+        return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     }
     const size_t h = n / 2;
     return pairwise_sum(p, h) + pairwise_sum(p + h, n - h);
+}
+
+// Largest (Greater = true) or smallest coefficient of p[0..n), n >= 1, in
+// eight interleaved lanes.  A NaN is kept only as the first element, as
+// std::max_element / std::min_element keep it.
+template <bool Greater, class S> S extreme(const S* p, size_t n) {
+    S m[8];
+    for (int k = 0; k < 8; ++k) m[k] = p[0];
+    size_t i = 1;
+    for (; i + 8 <= n; i += 8)
+        for (int k = 0; k < 8; ++k) {
+            const S x = p[i + k];
+            m[k] = (Greater ? m[k] < x : x < m[k]) ? x : m[k];
+        }
+    for (int k = 0; i < n; ++i, ++k) {
+        const S x = p[i];
+        m[k] = (Greater ? m[k] < x : x < m[k]) ? x : m[k];
+    }
+    S r = m[0];
+    for (int k = 1; k < 8; ++k) r = (Greater ? r < m[k] : m[k] < r) ? m[k] : r;
+    return r;
 }
 
 }  // namespace detail
@@ -183,11 +209,10 @@ public:
         return m;
     }
 
-    // sums are pairwise (as accurate as a vectorised reduction); the other
-    // reductions run left to right
+    // sums are pairwise; maxima and minima in eight lanes
     S sum() const { return detail::pairwise_sum(v_.data(), v_.size()); }
-    S maxCoeff() const { return *std::max_element(v_.begin(), v_.end()); }
-    S minCoeff() const { return *std::min_element(v_.begin(), v_.end()); }
+    S maxCoeff() const { return detail::extreme<true>(v_.data(), v_.size()); }
+    S minCoeff() const { return detail::extreme<false>(v_.data(), v_.size()); }
     S squaredNorm() const {
         S s = 0;
         for (const S& x : v_) s += x * x;
@@ -276,8 +301,8 @@ public:
     Buffer<S>& storage() { return v_; }
     Matrix<S, Dynamic, 1> matrix() const { return Matrix<S, Dynamic, 1>(*this); }
     S sum() const { return detail::pairwise_sum(v_.data(), v_.size()); }
-    S maxCoeff() const { return *std::max_element(v_.begin(), v_.end()); }
-    S minCoeff() const { return *std::min_element(v_.begin(), v_.end()); }
+    S maxCoeff() const { return detail::extreme<true>(v_.data(), v_.size()); }
+    S minCoeff() const { return detail::extreme<false>(v_.data(), v_.size()); }
 
     template <class F> Array1 map(F f) const {
         Array1 r = *this;
