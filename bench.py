@@ -38,10 +38,11 @@ def _traffic(n, dtype, shape):
         return None
     with open(path) as f:
         rec = json.load(f)
-    prof_shape = rec.get("kernel_shape")
-    if prof_shape is not None and any(prof_shape.get(k) != shape.get(k) for k in prof_shape):
-        return None
-    return rec.get("hbm_bytes_per_launch")
+    for entry in rec.get("shapes", [rec]):
+        prof_shape = entry.get("kernel_shape")
+        if prof_shape is None or all(prof_shape.get(k) == shape.get(k) for k in prof_shape):
+            return entry.get("hbm_bytes_per_launch")
+    return None
 
 
 def cpu_baseline(case, dtype, steps, seconds=12.0):
