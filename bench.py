@@ -188,8 +188,14 @@ def suite_wave_f64(ctx, steps=200):
     wall, ev = _timed(lambda: m.wave_run(A, B, -1.0, case.dt, steps))
     gbs = 64 * N / (ev * 1e-3 / steps) / 1e9
     m.close()
+    del A, B
+    # the same access pattern without arithmetic: 8 fields of 2N floats (the
+    # bytes of N doubles), ping-ponging; 1 GiB per pass, twice the last-level
+    # cache, so the FP64 step streams from HBM where the FP32 one partly hits
+    ceiling, how = stream_ceiling(nfloat=2 * N, reps=20)
     return {"workload": "acoustic wave 256^3 FP64", "mcups": N * steps / wall / 1e6,
             "kernel_us_per_step": ev * 1e3 / steps, "gbs_algorithmic": gbs, "roofline_frac": gbs / HBM_PEAK_GBS,
+            "stream_ceiling_gbs": ceiling, "stream_ceiling": how, "frac_of_stream_ceiling": gbs / ceiling,
             "kernel_shape": shape}
 
 
