@@ -485,14 +485,16 @@ int cfd_stab(cfd_mesh_t m, const void* phi, void* out, int mask, void* stream) {
         CFD_REQUIRE(m && phi && out, "cfd_stab: null argument");
         CFD_REQUIRE(mask >= 0 && mask <= 7, "cfd_stab: bad axis_mask");
         hipStream_t s = as_stream(stream);
-        void* scratch = pool_alloc(m->ctx, 3 * m->N * elem_size(m->dtype), s);
+        // the first stage's gradients go through 3N pool entries on general
+        // meshes; uniform meshes recompute them in one pass
+        void* scratch = m->uniform ? nullptr : pool_alloc(m->ctx, 3 * m->N * elem_size(m->dtype), s);
         try {
             DISPATCH(m, launch_stab, m, phi, out, mask, scratch, s);
         } catch (...) {
-            pool_free(m->ctx, scratch, s);
+            if (scratch) pool_free(m->ctx, scratch, s);
             throw;
         }
-        pool_free(m->ctx, scratch, s);
+        if (scratch) pool_free(m->ctx, scratch, s);
     });
 }
 

@@ -127,41 +127,60 @@ __global__ void __launch_bounds__(256) lapl_uniform_k(UniformGeo<T> g, const T* 
     out[c.id] = res;
 }
 
-// Outward Green-Gauss gradient, all three components (first stage of StabVar).
-template <class T>
-__global__ void __launch_bounds__(256) grad3_uniform_k(UniformGeo<T> g, const T* __restrict__ phi,
-                                                       T* __restrict__ gx, T* __restrict__ gy, T* __restrict__ gz) {
-    const Cell c = cell_of_thread(g);
-    if (!c.valid) return;
-    const T pc = phi[c.id];
-    T* outs[3] = {gx, gy, gz};
-    for (int a = 0; a < 3; ++a)
-        outs[a][c.id] = gg_pair(lin(pc, phi[nb_lo(g, c, a)]), lin(pc, phi[nb_hi(g, c, a)]), g.area[a], g.vol);
+// The cell across c's lower / upper face on axis a; c itself on a wall (its
+// id is then nb_lo / nb_hi's self index).
+template <class G> __device__ inline Cell cell_lo(const G& g, const Cell& c, int a) {
+    Cell n = c;
+    if (!at_lo(g, c, a)) {
+        if (a == 0) n.x -= 1;
+        else if (a == 1) n.y -= 1;
+        else n.z -= 1;
+        n.id = nb_lo(g, c, a);
+    }
+    return n;
+}
+template <class G> __device__ inline Cell cell_hi(const G& g, const Cell& c, int a) {
+    Cell n = c;
+    if (!at_hi(g, c, a)) {
+        if (a == 0) n.x += 1;
+        else if (a == 1) n.y += 1;
+        else n.z += 1;
+        n.id = nb_hi(g, c, a);
+    }
+    return n;
 }
 
-// Second stage of StabVar on a uniform mesh (src/fvm3d.cpp:915-936): upwind
-// face values phi + (len * g_d) * n_out_d, jump against the neighbour's
-// opposite face (self's opposite face on the boundary), 0.5 * jump, Gauss
-// with axis normals for component d.
+// Outward Green-Gauss gradient component a of cell n on a uniform mesh (the
+// first stage of StabVar).
+template <class T> __device__ inline T grad_comp(const UniformGeo<T>& g, const T* __restrict__ phi, const Cell& n,
+                                                 int a) {
+    const T pn = phi[n.id];
+    return gg_pair(lin(pn, phi[nb_lo(g, n, a)]), lin(pn, phi[nb_hi(g, n, a)]), g.area[a], g.vol);
+}
+
+// StabVar on a uniform mesh (src/fvm3d.cpp:909-936) in one pass.  Second
+// stage: upwind face values phi + (len * g_d) * n_out_d, jump against the
+// neighbour's opposite face (self's opposite face on the boundary), 0.5 *
+// jump, Gauss with axis normals for component d.  The gradient components it
+// reads (own and the two axis neighbours') are recomputed from phi at five
+// points per axis instead of going through a 3N buffer between two launches;
+// each is the same operation sequence as the first stage's.
 template <class T>
-__global__ void __launch_bounds__(256) stab_uniform_k(UniformGeo<T> g, const T* __restrict__ phi,
-                                                      const T* __restrict__ gx, const T* __restrict__ gy,
-                                                      const T* __restrict__ gz, T* __restrict__ out, int mask) {
+__global__ void __launch_bounds__(256) stab_fused_uniform_k(UniformGeo<T> g, const T* __restrict__ phi,
+                                                            T* __restrict__ out, int mask) {
     const Cell c = cell_of_thread(g);
     if (!c.valid) return;
-    const T* gs[3] = {gx, gy, gz};
     T res = T(0);
     for (int d = 0; d < 3; ++d) {
         if (!(mask & (1 << d))) continue;
         const T L = g.len[d];
-        const int64_t lo = nb_lo(g, c, d), hi = nb_hi(g, c, d);
-        const T pc = phi[c.id], gc = gs[d][c.id];
-        // lower face: own outward normal -1, neighbour's upper face +1
+        const Cell lo = cell_lo(g, c, d), hi = cell_hi(g, c, d);
+        const T pc = phi[c.id], gc = grad_comp(g, phi, c, d);
+        const T glo = grad_comp(g, phi, lo, d), ghi = grad_comp(g, phi, hi, d);
         const T own_lo = pc + (L * gc) * T(-1);
-        const T col_lo = phi[lo] + (L * gs[d][lo]) * T(1);
-        // upper face: own +1, neighbour's lower face -1
+        const T col_lo = phi[lo.id] + (L * glo) * T(1);
         const T own_hi = pc + (L * gc) * T(1);
-        const T col_hi = phi[hi] + (L * gs[d][hi]) * T(-1);
+        const T col_hi = phi[hi.id] + (L * ghi) * T(-1);
         const T f_lo = T(0.5) * (own_lo - col_lo);
         const T f_hi = T(0.5) * (own_hi - col_hi);
         res = res + (f_lo * g.area[d] + f_hi * g.area[d]) / g.vol;
@@ -482,17 +501,18 @@ template <class T> void launch_lapl(const cfd_mesh_s* m, const void* phi, void* 
     check_launch();
 }
 
+// scratch3 (3N entries, the first stage's gradients) is needed by the
+// general form only: uniform meshes run the one-pass kernel
 template <class T>
 void launch_stab(const cfd_mesh_s* m, const void* phi, void* out, int mask, void* scratch3, hipStream_t s) {
-    T* gx = (T*)scratch3;
-    T* gy = gx + m->N;
-    T* gz = gy + m->N;
     if (m->uniform) {
         const auto& g = ugeo<T>(m);
-        grad3_uniform_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)phi, gx, gy, gz);
-        check_launch();
-        stab_uniform_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)phi, gx, gy, gz, (T*)out, mask);
+        stab_fused_uniform_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)phi, (T*)out, mask);
     } else {
+        CFD_REQUIRE(scratch3 != nullptr, "stabilisation on a general mesh needs 3N scratch entries");
+        T* gx = (T*)scratch3;
+        T* gy = gx + m->N;
+        T* gz = gy + m->N;
         const auto& g = ggeo<T>(m);
         grad3_general_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)phi, gx, gy, gz);
         check_launch();

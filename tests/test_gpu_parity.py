@@ -28,6 +28,8 @@ def _np(t):
 MESHES = [
     # (name, shape, spacing, perturb seed, force_general)
     ("uniform", (9, 7, 10), (0.5, 0.25, 0.125), None, False),
+    # one- and two-cell extents: a cell's axis neighbours are itself or each other
+    ("uniform-thin", (1, 2, 5), (0.5, 0.25, 0.125), None, False),
     ("uniform-as-general", (9, 7, 10), (0.5, 0.25, 0.125), None, True),
     ("non-dyadic", (6, 8, 5), (0.1, 0.3, 0.7), None, False),
     ("perturbed", (8, 6, 7), (0.125, 0.25, 0.5), 42, False),
