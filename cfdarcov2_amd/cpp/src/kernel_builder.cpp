@@ -36,30 +36,52 @@ namespace {
 // same layout.
 struct JGeo {
     long long X, Y, Z;
-    float area[3], alpha[3], alpha_b[3], vol;
+    float area[3], alpha[3], alpha_b[3], len[3], vol;
     int uniform;
     const float* fn[3][3];
     const float* farea[3];
     const float* falpha[3];
     const float* fvol;
 };
-static_assert(sizeof(JGeo) == 200, "JGeo layout");
+static_assert(sizeof(JGeo) == 208, "JGeo layout");
 
 // hipRTC declares the HIP device built-ins itself, so the prelude includes no
 // runtime header.
 const char* kPrelude = R"PRELUDE(
 struct JGeo {
     long long X, Y, Z;
-    float area[3], alpha[3], alpha_b[3], vol;
+    float area[3], alpha[3], alpha_b[3], len[3], vol;
     int uniform;
     const float* fn[3][3];
     const float* farea[3];
     const float* falpha[3];
     const float* fvol;
 };
-static_assert(sizeof(JGeo) == 200, "JGeo layout");
+static_assert(sizeof(JGeo) == 208, "JGeo layout");
 // linear face interpolation (phi_c + phi_nb) * 0.5
 __device__ inline float jlin(float c, float n) { return (c + n) * 0.5f; }
+// uniform-mesh Green-Gauss pair of one axis: ((-f_lo) * A + f_hi * A) / V
+__device__ inline float jgg(float f_lo, float f_hi, float A, float V) {
+    const float t_lo = (-f_lo) * A;
+    const float t_hi = f_hi * A;
+    return (t_lo + t_hi) / V;
+}
+// one axis of StabVar on a uniform mesh, added to acc: p at the cell (pc),
+// its axis neighbours (pl, ph), and the neighbours' own neighbours (pll, plh
+// of the lower one, phl, phh of the upper one; self on walls)
+__device__ inline float jstab_axis(float acc, float pc, float pl, float ph, float pll, float plh, float phl,
+                                   float phh, float A, float V, float L) {
+    const float gc = jgg(jlin(pc, pl), jlin(pc, ph), A, V);
+    const float glo = jgg(jlin(pl, pll), jlin(pl, plh), A, V);
+    const float ghi = jgg(jlin(ph, phl), jlin(ph, phh), A, V);
+    const float own_lo = pc + (L * gc) * -1.0f;
+    const float col_lo = pl + (L * glo) * 1.0f;
+    const float own_hi = pc + (L * gc) * 1.0f;
+    const float col_hi = ph + (L * ghi) * -1.0f;
+    const float f_lo = 0.5f * (own_lo - col_lo);
+    const float f_hi = 0.5f * (own_hi - col_hi);
+    return acc + (f_lo * A + f_hi * A) / V;
+}
 // v[0] = cell value, v[1 + fc] = value across local face fc (z-, z+, y-, y+, x-, x+)
 __device__ inline bool jat(const JGeo& g, long long x, long long y, long long z, int fc) {
     switch (fc) {
@@ -173,6 +195,9 @@ struct KernelBuilder::Impl {
     std::vector<const Node*> materialized;      // evaluated operator by operator
     std::vector<const Node*> fused;             // stencils computed in registers
     std::map<const Node*, int> fused_id;
+    std::vector<const Node*> fused_stab;        // StabVar computed in registers (uniform meshes)
+    std::map<const Node*, int> stab_id;
+    bool uniform_mesh = false;                  // exactly uniform mesh on the device
 
     std::string source;
     std::string kname;
@@ -237,9 +262,25 @@ struct KernelBuilder::Impl {
                 }
                 return;
             case NodeKind::Stab:
-                materialize(n);
+                // on a uniform mesh a stabilisation of a pointwise operand
+                // runs in the kernel (the operand at 13 points); otherwise it
+                // is evaluated before the launch
+                if (uniform_mesh && !in_stencil && !has_stencil(*n.a)) {
+                    if (!stab_id.count(&n)) {
+                        stab_id[&n] = int(fused_stab.size());
+                        fused_stab.push_back(&n);
+                        walk(*n.a, true);
+                    }
+                } else {
+                    materialize(n);
+                }
                 return;
         }
+    }
+
+    static bool has_stencil(const Node& n) {
+        if (n.kind == NodeKind::Grad || n.kind == NodeKind::Grad2 || n.kind == NodeKind::Stab) return true;
+        return (n.a && has_stencil(*n.a)) || (n.b && has_stencil(*n.b));
     }
 
     // Kernels with one or two fused stencils march along x (one thread per
@@ -247,7 +288,7 @@ struct KernelBuilder::Impl {
     // Measured on the Euler driver's equations (three to eight stencils of
     // products), marching made them up to twice as slow, while the heat and
     // wave equations (one or two stencils of one field) gain 10-15% from it.
-    bool marches() const { return !fused.empty() && fused.size() <= 2; }
+    bool marches() const { return !fused.empty() && fused.size() <= 2 && fused_stab.empty(); }
 
     std::string render(const Node& n, const std::string& idx) const {
         switch (n.kind) {
@@ -269,6 +310,7 @@ struct KernelBuilder::Impl {
                 auto it = buffer_slot.find(buffer_key(n));
                 if (it != buffer_slot.end()) return "in" + std::to_string(it->second) + "[" + idx + "]";
                 if (idx != "c") throw std::logic_error("fused stencil read away from its cell");
+                if (n.kind == NodeKind::Stab) return "T" + std::to_string(stab_id.at(&n));
                 return "S" + std::to_string(fused_id.at(&n));
             }
             default:
@@ -316,6 +358,39 @@ struct KernelBuilder::Impl {
             body << ind << "const float S" << i << " = " << (n.kind == NodeKind::Grad ? "jgrad" : "jlapl")
                  << "(g, x, y, z, c, v" << i << ", " << n.mask << ");\n";
         };
+        // StabVar of a pointwise operand on a uniform mesh: per selected
+        // axis, the operand at the cell, its two neighbours and theirs (self
+        // on walls; each point evaluated only when it is not one of the
+        // others), then stab_fused_uniform_k's arithmetic
+        auto emit_stab = [&](size_t i) {
+            const Node& n = *fused_stab[i];
+            const std::string t = "T" + std::to_string(i), pc = "P" + std::to_string(i);
+            body << "    float " << t << " = 0.0f;\n"
+                 << "    const float " << pc << " = " << render(*n.a, "c") << ";\n";
+            static const char* stride[3] = {"g.Y * g.Z", "g.Z", "1LL"};
+            static const char* coord[3] = {"x", "y", "z"};
+            static const char* extent[3] = {"g.X", "g.Y", "g.Z"};
+            for (int a = 0; a < 3; ++a) {
+                if (!(n.mask & (1 << a))) continue;
+                const std::string sa = stride[a], ca = coord[a], na = extent[a];
+                body << "    {\n"
+                     << "        const long long sa = " << sa << ";\n"
+                     << "        const bool wl = " << ca << " == 0, wh = " << ca << " == " << na << " - 1;\n"
+                     << "        const long long il = wl ? c : c - sa;\n"
+                     << "        const long long ih = wh ? c : c + sa;\n"
+                     << "        const float pl = wl ? " << pc << " : " << render(*n.a, "il") << ";\n"
+                     << "        const float ph = wh ? " << pc << " : " << render(*n.a, "ih") << ";\n"
+                     << "        const float pll = wl ? " << pc << " : (" << ca << " == 1 ? pl : "
+                     << render(*n.a, "(il - sa)") << ");\n"
+                     << "        const float plh = wl ? ph : " << pc << ";\n"
+                     << "        const float phh = wh ? " << pc << " : (" << ca << " == " << na << " - 2 ? ph : "
+                     << render(*n.a, "(ih + sa)") << ");\n"
+                     << "        const float phl = wh ? pl : " << pc << ";\n"
+                     << "        " << t << " = jstab_axis(" << t << ", " << pc
+                     << ", pl, ph, pll, plh, phl, phh, g.area[" << a << "], g.vol, g.len[" << a << "]);\n"
+                     << "    }\n";
+            }
+        };
         if (!marches()) {
             // one thread per cell
             body << "    const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
@@ -331,6 +406,7 @@ struct KernelBuilder::Impl {
                  << "    const long long nb5 = x < g.X - 1 ? c + g.Y * g.Z : c;\n";
             for (size_t i = 0; i < fused.size(); ++i)
                 emit_stencil(i, "    ", render(*fused[i]->a, "nb4"), render(*fused[i]->a, "nb5"));
+            for (size_t i = 0; i < fused_stab.size(); ++i) emit_stab(i);
             emit_update("    ");
         } else {
             // One thread per (y, z) column marches xl_ consecutive x planes.
@@ -425,6 +501,13 @@ void KernelBuilder::generate() {
         throw std::runtime_error("equation target must be a state Variable");
     I.root = expression->node.get();
     I.mesh = I.state->mesh;
+    // stabilisation runs in the kernel only on an exactly uniform mesh, known
+    // once the mesh is on the device (generation without a device treats the
+    // mesh as general)
+    if (auto* cm = dynamic_cast<CudaMesh3D*>(I.mesh); cm != nullptr && cm->device_mesh() != nullptr) {
+        cfd_geometry cg{};
+        if (cfd_mesh_geometry(cm->device_mesh(), &cg) == CFD_OK) I.uniform_mesh = cg.uniform != 0;
+    }
     I.walk(*I.root, false);
     I.generate();
 }
@@ -447,6 +530,7 @@ void KernelBuilder::build(bool announce) {
         g.alpha_b[a] = float(cg.alpha_b[a]);
     }
     g.vol = float(cg.vol);
+    for (int a = 0; a < 3; ++a) g.len[a] = float(cg.len[a]);
     g.uniform = cg.uniform;
     for (int f = 0; f < 3; ++f) {
         for (int d = 0; d < 3; ++d) g.fn[f][d] = static_cast<const float*>(cg.face_normal[f][d]);

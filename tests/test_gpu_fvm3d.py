@@ -248,6 +248,24 @@ def test_false_flagged_equations_fused_equal_per_operator(name, args):
     assert fused and fused == per_op
 
 
+def test_stabilisation_runs_inside_fused_kernels():
+    """On a uniform mesh the Euler driver's stab_tot terms are computed inside
+    the generated kernels (no separate stabilisation launch); the printed
+    results equal the per-operator run (test above)."""
+    path = os.path.join(DRIVERS, "cfdARCO_euler")
+    if not os.path.exists(path):
+        pytest.skip("reference driver binaries are built only where the reference sources exist")
+    env = dict(os.environ, CFDARCO_JIT_PRINT="1")
+    env.pop("CFDARCO_PER_OP", None)
+    with tempfile.TemporaryDirectory() as tmp:
+        res = subprocess.run([path, "-Lx", "12", "-Ly", "10", "-Lz", "14", "-t", "3", "-dt", "0.1", "-c"],
+                             capture_output=True, text=True, timeout=300, cwd=tmp, env=env)
+    assert res.returncode == 0, res.stdout + res.stderr
+    sources = res.stdout.split("Building kernel:")[1:]
+    # kernel bodies that call the in-kernel stabilisation (the prelude defines it in every source)
+    assert sum(" = jstab_axis(T" in src for src in sources) == 5
+
+
 def test_reference_sod_driver_unchanged(oracle_lib):
     """examples/main_3d_sod_shock.cpp compiled unchanged (unit cells: the
     driver adds per-volume fluxes to cell totals, so it is written for dx = 1):
