@@ -142,7 +142,8 @@ int cfd_grad(cfd_mesh_t mesh, const void* phi, void* out, int axis_mask, int wal
  * src/cuda_operators.cu:490-569). */
 int cfd_lapl(cfd_mesh_t mesh, const void* phi, void* out, int axis_mask, void* stream);
 /* StabVar (stab_tot, stabx..) evaluate_cu + eval_stab (src/fvm3d.cpp:938-942,
- * src/cuda_operators.cu:307-487).  Uses a 3N scratch from the pool. */
+ * src/cuda_operators.cu:307-487).  Exactly uniform meshes run one pass; other
+ * meshes use a 3N scratch from the pool for the first stage's gradients. */
 int cfd_stab(cfd_mesh_t mesh, const void* phi, void* out, int axis_mask, void* stream);
 
 /* Elementwise kernels: add_mtrx/sub_mtrx/mul_mtrx/div_mtrx/neg_mtrx/
