@@ -180,7 +180,7 @@ def suite_wave_f64(ctx, steps=200):
     m = capi.HexMesh(ctx, *case.n, *case.spacing, dtype="f64")
     shape = autotune.tune_wave(m, steps=12, use_cache=False,
                                candidates=[(1, ty, L, r) for ty in (4, 8) for L in (8, 16, 32) for r in (0, 1)]
-                               + [(v, ty, L, r) for v in (7, 9) for ty in (4, 8) for L in (8, 16, 32) for r in (1, 3)]
+                               + [(v, ty, L, r) for v in (7, 9) for ty in (4, 8) for L in (8, 16, 32, 64) for r in (1, 3)]
                                + [(v, 2, L, r) for v in (11, 12) for L in (8, 16, 32, 64) for r in (1, 3)]
                                + [(2, 8, L, 0) for L in (8, 16)])
     N = m.N

@@ -25,7 +25,7 @@ DEFAULT_CANDIDATES = [(0, 16, 32, 1)] + [
 ] + [
     (v, 4, L, r) for v, L, r in itertools.product((5, 6), (4, 8, 16), (0, 1))
 ] + [
-    (v, ty, L, r) for v, ty, L, r in itertools.product((7, 8, 9, 10), (4, 8, 14), (4, 8, 16, 32), (0, 1, 2, 3))
+    (v, ty, L, r) for v, ty, L, r in itertools.product((7, 8, 9, 10), (4, 8, 14), (4, 8, 16, 32, 64), (0, 1, 2, 3))
 ] + [
     (v, ty, L, r) for v, ty, L, r in itertools.product((11, 12), (2, 4, 8, 10), (8, 16, 32, 64), (0, 1, 2, 3))
 ]
