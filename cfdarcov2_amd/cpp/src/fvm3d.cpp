@@ -583,7 +583,9 @@ void Equation::evaluate(std::vector<Variable*>& all_vars, EquationTemplate& equa
                 auto kb = std::make_unique<KernelBuilder>(std::get<0>(eq), &std::get<2>(eq));
                 kb->build(false);
                 krns[i] = std::move(kb);
-            } catch (const std::exception&) {
+            } catch (const std::exception& e) {
+                // stays operator by operator; say why, once per evaluate call
+                std::fprintf(stderr, "note: equation %zu runs operator by operator (%s)\n", i, e.what());
                 krns[i].reset();
             }
         }
