@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <sstream>
 #include <stdexcept>
@@ -262,10 +263,15 @@ struct KernelBuilder::Impl {
                 }
                 return;
             case NodeKind::Stab:
-                // on a uniform mesh a stabilisation of a pointwise operand
-                // runs in the kernel (the operand at 13 points); otherwise it
-                // is evaluated before the launch
-                if (uniform_mesh && !in_stencil && !has_stencil(*n.a)) {
+                // On a uniform mesh a stabilisation of a pointwise operand of
+                // three or more fields runs in the kernel (the operand at 13
+                // points); otherwise it is evaluated before the launch.  On the
+                // Euler driver at 200^3, stab_tot(E) of five fields took 210 us
+                // in-kernel against about 250 us materialised (eight
+                // elementwise launches, the one-pass stabilisation and the
+                // fused update), while stab_tot(rho * u) took 130 us in-kernel
+                // against about 100 us materialised.
+                if (uniform_mesh && !in_stencil && !has_stencil(*n.a) && distinct_leaves(*n.a) >= 3) {
                     if (!stab_id.count(&n)) {
                         stab_id[&n] = int(fused_stab.size());
                         fused_stab.push_back(&n);
@@ -278,17 +284,24 @@ struct KernelBuilder::Impl {
         }
     }
 
+    static size_t distinct_leaves(const Node& n) {
+        std::vector<const void*> seen;
+        std::function<void(const Node&)> visit = [&](const Node& m) {
+            if (m.kind == NodeKind::Leaf || m.kind == NodeKind::Array) {
+                const void* k = buffer_key(m);
+                if (std::find(seen.begin(), seen.end(), k) == seen.end()) seen.push_back(k);
+            }
+            if (m.a) visit(*m.a);
+            if (m.b) visit(*m.b);
+        };
+        visit(n);
+        return seen.size();
+    }
+
     static bool has_stencil(const Node& n) {
         if (n.kind == NodeKind::Grad || n.kind == NodeKind::Grad2 || n.kind == NodeKind::Stab) return true;
         return (n.a && has_stencil(*n.a)) || (n.b && has_stencil(*n.b));
     }
-
-    // Kernels with one or two fused stencils march along x (one thread per
-    // (y, z) column, xl_ planes); the others run one thread per cell.
-    // Measured on the Euler driver's equations (three to eight stencils of
-    // products), marching made them up to twice as slow, while the heat and
-    // wave equations (one or two stencils of one field) gain 10-15% from it.
-    bool marches() const { return !fused.empty() && fused.size() <= 2 && fused_stab.empty(); }
 
     std::string render(const Node& n, const std::string& idx) const {
         switch (n.kind) {
@@ -326,7 +339,7 @@ struct KernelBuilder::Impl {
         for (size_t i = 0; i < buffers.size(); ++i) sig << ", const float* __restrict__ in" << i;
         for (size_t i = 0; i < scalars.size(); ++i) sig << ", float s" << i;
         sig << ", float* __restrict__ out_, const float* __restrict__ cur_, const float* __restrict__ prev_, float dt_, "
-               "int stage_, long long xl_";
+               "int stage_";
         // kernel name from the source body (stable across runs)
         std::ostringstream body;
         // the right-hand side and the update of cell c
@@ -350,8 +363,7 @@ struct KernelBuilder::Impl {
         auto emit_stencil = [&](size_t i, const std::string& ind, const std::string& xm, const std::string& xp) {
             const Node& n = *fused[i];
             body << ind << "float v" << i << "[7];\n";
-            body << ind << "v" << i << "[0] = " << (marches() ? "w" + std::to_string(i) + "c" : render(*n.a, "c"))
-                 << ";\n";
+            body << ind << "v" << i << "[0] = " << render(*n.a, "c") << ";\n";
             for (int fc = 0; fc < 4; ++fc)
                 body << ind << "v" << i << "[" << fc + 1 << "] = " << render(*n.a, "nb" + std::to_string(fc)) << ";\n";
             body << ind << "v" << i << "[5] = " << xm << ";\n" << ind << "v" << i << "[6] = " << xp << ";\n";
@@ -391,73 +403,22 @@ struct KernelBuilder::Impl {
                      << "    }\n";
             }
         };
-        if (!marches()) {
-            // one thread per cell
-            body << "    const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
-                 << "    const long long y = (long long)blockIdx.y * blockDim.y + threadIdx.y;\n"
-                 << "    const long long x = blockIdx.z;\n"
-                 << "    if (z >= g.Z || y >= g.Y || x >= g.X) return;\n"
-                 << "    const long long c = (x * g.Y + y) * g.Z + z;\n"
-                 << "    const long long nb0 = z > 0 ? c - 1 : c;\n"
-                 << "    const long long nb1 = z < g.Z - 1 ? c + 1 : c;\n"
-                 << "    const long long nb2 = y > 0 ? c - g.Z : c;\n"
-                 << "    const long long nb3 = y < g.Y - 1 ? c + g.Z : c;\n"
-                 << "    const long long nb4 = x > 0 ? c - g.Y * g.Z : c;\n"
-                 << "    const long long nb5 = x < g.X - 1 ? c + g.Y * g.Z : c;\n";
-            for (size_t i = 0; i < fused.size(); ++i)
-                emit_stencil(i, "    ", render(*fused[i]->a, "nb4"), render(*fused[i]->a, "nb5"));
-            for (size_t i = 0; i < fused_stab.size(); ++i) emit_stab(i);
-            emit_update("    ");
-        } else {
-            // One thread per (y, z) column marches xl_ consecutive x planes.
-            // A fused stencil's operand is kept at planes x-1, x, x+1 in
-            // registers (each plane's value evaluated once), so its
-            // x-neighbours cost no loads; the y and z neighbours are evaluated
-            // on the plane.  Every value is the expression of the one-thread-
-            // per-cell form evaluated at the same index, so the results are
-            // identical.
-            body << "    const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
-                 << "    const long long y = (long long)blockIdx.y * blockDim.y + threadIdx.y;\n"
-                 << "    if (z >= g.Z || y >= g.Y) return;\n"
-                 << "    const long long SX = g.Y * g.Z;\n"
-                 << "    const long long x0 = (long long)blockIdx.z * xl_;\n"
-                 << "    const long long x1 = x0 + xl_ < g.X ? x0 + xl_ : g.X;\n"
-                 << "    const long long col = y * g.Z + z;\n";
-            for (size_t i = 0; i < fused.size(); ++i) {
-                const Node& n = *fused[i];
-                const std::string w = "w" + std::to_string(i);
-                body << "    float " << w << "m, " << w << "c, " << w << "p;\n"
-                     << "    {\n"
-                     << "        const long long c = x0 * SX + col;\n"
-                     << "        " << w << "c = " << render(*n.a, "c") << ";\n"
-                     << "        " << w << "m = x0 > 0 ? " << render(*n.a, "(c - SX)") << " : " << w << "c;\n"
-                     << "        " << w << "p = x0 + 1 < g.X ? " << render(*n.a, "(c + SX)") << " : " << w << "c;\n"
-                     << "    }\n";
-            }
-            body << "    for (long long x = x0; x < x1; ++x) {\n"
-                 << "        const long long c = x * SX + col;\n"
-                 << "        const long long nb0 = z > 0 ? c - 1 : c;\n"
-                 << "        const long long nb1 = z < g.Z - 1 ? c + 1 : c;\n"
-                 << "        const long long nb2 = y > 0 ? c - g.Z : c;\n"
-                 << "        const long long nb3 = y < g.Y - 1 ? c + g.Z : c;\n";
-            for (size_t i = 0; i < fused.size(); ++i)
-                emit_stencil(i, "        ", "w" + std::to_string(i) + "m", "w" + std::to_string(i) + "p");
-            emit_update("        ");
-            // slide the operand windows one plane (x + 2 is evaluated only
-            // when the next iteration exists; at the last plane of the mesh
-            // x + 1 is the cell itself)
-            body << "        if (x + 1 < x1) {\n";
-            for (size_t i = 0; i < fused.size(); ++i) {
-                const Node& n = *fused[i];
-                const std::string w = "w" + std::to_string(i);
-                body << "            " << w << "m = " << w << "c;\n"
-                     << "            " << w << "c = " << w << "p;\n"
-                     << "            " << w << "p = x + 2 < g.X ? " << render(*n.a, "(c + 2 * SX)") << " : " << w
-                     << "c;\n";
-            }
-            body << "        }\n"
-                 << "    }\n";
-        }
+        // one thread per cell
+        body << "    const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
+             << "    const long long y = (long long)blockIdx.y * blockDim.y + threadIdx.y;\n"
+             << "    const long long x = blockIdx.z;\n"
+             << "    if (z >= g.Z || y >= g.Y || x >= g.X) return;\n"
+             << "    const long long c = (x * g.Y + y) * g.Z + z;\n"
+             << "    const long long nb0 = z > 0 ? c - 1 : c;\n"
+             << "    const long long nb1 = z < g.Z - 1 ? c + 1 : c;\n"
+             << "    const long long nb2 = y > 0 ? c - g.Z : c;\n"
+             << "    const long long nb3 = y < g.Y - 1 ? c + g.Z : c;\n"
+             << "    const long long nb4 = x > 0 ? c - g.Y * g.Z : c;\n"
+             << "    const long long nb5 = x < g.X - 1 ? c + g.Y * g.Z : c;\n";
+        for (size_t i = 0; i < fused.size(); ++i)
+            emit_stencil(i, "    ", render(*fused[i]->a, "nb4"), render(*fused[i]->a, "nb5"));
+        for (size_t i = 0; i < fused_stab.size(); ++i) emit_stab(i);
+        emit_update("    ");
         const std::string b = body.str();
         uint64_t h = 1469598103934665603ull;
         for (unsigned char ch : sig.str() + b) {
@@ -592,13 +553,7 @@ void KernelBuilder::run() {
     args.push_back(&prev);
     args.push_back(&dtv);
     args.push_back(&stage);
-    // planes per thread of a marching kernel: about 8192 workgroups in all,
-    // at most 32 planes
-    const long long bpp = ((I.geo.Z + 63) / 64) * ((I.geo.Y + 3) / 4);
-    long long xl = I.marches() ? std::max(1LL, std::min(32LL, I.geo.X * bpp / 8192)) : 1LL;
-    args.push_back(&xl);
-    const uint32_t grid[3] = {uint32_t((I.geo.Z + 63) / 64), uint32_t((I.geo.Y + 3) / 4),
-                              uint32_t((I.geo.X + xl - 1) / xl)};
+    const uint32_t grid[3] = {uint32_t((I.geo.Z + 63) / 64), uint32_t((I.geo.Y + 3) / 4), uint32_t(I.geo.X)};
     const uint32_t block[3] = {64, 4, 1};
     cfdarco::check(cfd_program_launch(I.prog, grid, block, args.data(), nullptr), "KernelBuilder::run");
     // new state in, old state becomes the next output (or the previous state of
