@@ -248,6 +248,24 @@ def test_false_flagged_equations_fused_equal_per_operator(name, args):
     assert fused and fused == per_op
 
 
+def test_device_point_sources_equal_host_callback():
+    """cpp/bench/point_source_wave.cpp (the wave example's problem with its
+    point sources set by a device callback) prints the same final sum and
+    maximum as the example as shipped (host callback, a host round trip of the
+    field every step)."""
+    psw = os.path.join(BUILD, "bench", "point_source_wave")
+    if not os.path.exists(psw):
+        pytest.fail(f"{psw} not built (run __graft_entry__.build())")
+    args = ["-Lx", "20", "-Ly", "18", "-Lz", "16", "-t", "30", "-c"]
+    with tempfile.TemporaryDirectory() as tmp:
+        res = subprocess.run([psw, *args], capture_output=True, text=True, timeout=300, cwd=tmp)
+    assert res.returncode == 0, res.stdout + res.stderr
+    dev = re.findall(r"At u Current sum = \S+ Current max = \S+", res.stdout)
+    assert dev, res.stdout
+    host = re.findall(r"At u Current sum = \S+ Current max = \S+", _run_driver("cfdARCO_wave", args))
+    assert dev == host
+
+
 def test_stabilisation_runs_inside_fused_kernels():
     """On a uniform mesh the Euler driver's stab_tot(E) (an operand of five
     fields) is computed inside its generated kernel, while the stabilisation
