@@ -40,6 +40,100 @@ __global__ void __launch_bounds__(256) ewise_k(int op, const T* __restrict__ a, 
     }
 }
 
+// One element of an elementwise op (the operation orders of the reference's
+// kernels, src/cuda_operators.cu:27-136, and the DtVar / D2tVar updates).
+template <int OP, class T> __device__ inline T ew_one(T a, T b, T aux, T s) {
+    switch (OP) {
+        case CFD_EW_ADD: return a + b;
+        case CFD_EW_SUB: return a - b;
+        case CFD_EW_MUL: return a * b;
+        case CFD_EW_DIV: return a / b;
+        case CFD_EW_NEG: return -a;
+        case CFD_EW_MULS: return a * s;
+        case CFD_EW_DIVS: return a / s;
+        case CFD_EW_SMUL: return s * a;
+        case CFD_EW_AXPY: return s * a + b;
+        case CFD_EW_LEAP: return ((s * s) * a + T(2) * b) - aux;
+        case CFD_EW_FILL: return s;
+        case CFD_EW_ADDS: return a + s;
+        case CFD_EW_SADD: return s + a;
+        case CFD_EW_SUBS: return a - s;
+        case CFD_EW_SSUB: return s - a;
+        case CFD_EW_SDIV: return s / a;
+        default: return a;  // CFD_EW_COPY
+    }
+}
+
+constexpr bool ew_reads_a(int op) { return op != CFD_EW_FILL; }
+constexpr bool ew_reads_b(int op) {
+    return op == CFD_EW_ADD || op == CFD_EW_SUB || op == CFD_EW_MUL || op == CFD_EW_DIV || op == CFD_EW_AXPY ||
+           op == CFD_EW_LEAP;
+}
+
+// Vector form: 16 bytes per access (4 floats or 2 doubles), one vector per
+// thread, the op fixed at compile time; the tail of n that does not fill a
+// vector is handled by the last vector's threads one element at a time.
+// Fields from the pool are 256-byte aligned; callers with other alignments
+// take ewise_k.
+template <int OP, class T>
+__global__ void __launch_bounds__(256) ewise_vec_k(const T* __restrict__ a, const T* __restrict__ b,
+                                                   const T* __restrict__ aux, T s, T* __restrict__ out, int64_t n) {
+    constexpr int W = 16 / sizeof(T);
+    using V = T __attribute__((ext_vector_type(W)));
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t nv = n / W;
+    if (i < nv) {
+        V va = {}, vb = {}, vx = {}, r;
+        if constexpr (ew_reads_a(OP)) va = reinterpret_cast<const V*>(a)[i];
+        if constexpr (ew_reads_b(OP)) vb = reinterpret_cast<const V*>(b)[i];
+        if constexpr (OP == CFD_EW_LEAP) vx = reinterpret_cast<const V*>(aux)[i];
+#pragma unroll
+        for (int k = 0; k < W; ++k) r[k] = ew_one<OP, T>(va[k], vb[k], vx[k], s);
+        reinterpret_cast<V*>(out)[i] = r;
+    } else if (i == nv) {
+        for (int64_t j = nv * W; j < n; ++j) {
+            const T ea = ew_reads_a(OP) ? a[j] : T(0);
+            const T eb = ew_reads_b(OP) ? b[j] : T(0);
+            const T ex = OP == CFD_EW_LEAP ? aux[j] : T(0);
+            out[j] = ew_one<OP, T>(ea, eb, ex, s);
+        }
+    }
+}
+
+template <class T, int OP>
+static void launch_vec(const void* a, const void* b, const void* aux, double s, void* out, int64_t n, hipStream_t st) {
+    constexpr int W = 16 / sizeof(T);
+    const int64_t threads = n / W + 1;  // one more for the tail
+    const int64_t blocks = (threads + 255) / 256;
+    CFD_REQUIRE(blocks < (int64_t(1) << 31), "elementwise launch too large");
+    ewise_vec_k<OP, T><<<dim3(unsigned(blocks)), 256, 0, st>>>((const T*)a, (const T*)b, (const T*)aux, T(s),
+                                                                 (T*)out, n);
+}
+
+template <class T>
+static void dispatch_vec(int op, const void* a, const void* b, const void* aux, double s, void* out, int64_t n,
+                         hipStream_t st) {
+    switch (op) {
+        case CFD_EW_ADD: launch_vec<T, CFD_EW_ADD>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_SUB: launch_vec<T, CFD_EW_SUB>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_MUL: launch_vec<T, CFD_EW_MUL>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_DIV: launch_vec<T, CFD_EW_DIV>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_NEG: launch_vec<T, CFD_EW_NEG>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_MULS: launch_vec<T, CFD_EW_MULS>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_DIVS: launch_vec<T, CFD_EW_DIVS>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_SMUL: launch_vec<T, CFD_EW_SMUL>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_AXPY: launch_vec<T, CFD_EW_AXPY>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_LEAP: launch_vec<T, CFD_EW_LEAP>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_FILL: launch_vec<T, CFD_EW_FILL>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_ADDS: launch_vec<T, CFD_EW_ADDS>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_SADD: launch_vec<T, CFD_EW_SADD>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_SUBS: launch_vec<T, CFD_EW_SUBS>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_SSUB: launch_vec<T, CFD_EW_SSUB>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_SDIV: launch_vec<T, CFD_EW_SDIV>(a, b, aux, s, out, n, st); break;
+        default: launch_vec<T, CFD_EW_COPY>(a, b, aux, s, out, n, st); break;
+    }
+}
+
 static int blocks_for(int64_t n) {
     int64_t b = (n + 255) / 256;
     if (b > 65536) b = 65536;
@@ -50,7 +144,11 @@ template <class T>
 void launch_ewise(int op, const void* a, const void* b, const void* aux, double s, void* out, int64_t n,
                   hipStream_t st) {
     if (n <= 0) return;
-    ewise_k<T><<<blocks_for(n), 256, 0, st>>>(op, (const T*)a, (const T*)b, (const T*)aux, T(s), (T*)out, n);
+    auto aligned = [](const void* p) { return p == nullptr || (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    if (aligned(a) && aligned(b) && aligned(aux) && aligned(out))
+        dispatch_vec<T>(op, a, b, aux, s, out, n, st);
+    else
+        ewise_k<T><<<blocks_for(n), 256, 0, st>>>(op, (const T*)a, (const T*)b, (const T*)aux, T(s), (T*)out, n);
     CFD_HIP(hipGetLastError());
 }
 

@@ -270,11 +270,15 @@ def test_wave_two_step_f64_full_size(hip_ctx, oracle_lib):
 def test_ewise_and_reductions(hip_ctx, oracle_lib):
     from cfdarcov2_amd import capi
     rng = np.random.default_rng(11)
-    for dtype in (np.float32, np.float64):
-        a, b, c = (rng.standard_normal(1000).astype(dtype) for _ in range(3))
+    # lengths that fill whole 16-byte vectors and ones that leave a tail;
+    # offset views (not 16-byte aligned) take the one-element-per-thread kernel
+    for dtype, n, off in [(np.float32, 1000, 0), (np.float64, 1000, 0), (np.float32, 1003, 0),
+                          (np.float64, 1001, 0), (np.float32, 7, 0), (np.float32, 999, 1), (np.float64, 999, 1)]:
+        a, b, c = (rng.standard_normal(n + off).astype(dtype) for _ in range(3))
         b = np.where(np.abs(b) < 0.1, 0.5, b).astype(dtype)
-        ta, tb, tc = _t(a, dtype), _t(b, dtype), _t(c, dtype)
-        out = torch.empty_like(ta)
+        ta, tb, tc = _t(a, dtype)[off:], _t(b, dtype)[off:], _t(c, dtype)[off:]
+        a, b, c = a[off:], b[off:], c[off:]
+        out = torch.empty(n + off, dtype=ta.dtype, device="cuda")[off:]
         s = dtype(0.37)
         expect = {"add": a + b, "sub": a - b, "mul": a * b, "div": a / b, "neg": -a, "muls": a * s,
                   "divs": a / s, "smul": s * a, "axpy": s * a + b, "leap": ((s * s) * a + dtype(2) * b) - c,
@@ -285,7 +289,7 @@ def test_ewise_and_reductions(hip_ctx, oracle_lib):
                 hip_ctx.ewise(op, tb, None, None, float(s), out)
             else:
                 hip_ctx.ewise(op, ta, tb, tc, float(s), out)
-            assert np.array_equal(_np(out), ref.astype(dtype)), op
+            assert np.array_equal(_np(out), ref.astype(dtype)), (op, np.dtype(dtype).name, n, off)
     # CFL minimum against the oracle (3D and 1D forms)
     n, d = (8, 8, 8), (0.5, 0.25, 0.125)
     gm = capi.HexMesh(hip_ctx, *n, *d, dtype=np.float64)
