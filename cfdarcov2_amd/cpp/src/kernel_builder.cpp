@@ -14,6 +14,7 @@
 // The new state goes to a separate buffer that is swapped in after the launch
 // (Jacobi within the equation), fixing the reference's in-place JIT update.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -37,51 +38,54 @@ namespace {
 // same layout.
 struct JGeo {
     long long X, Y, Z;
-    float area[3], alpha[3], alpha_b[3], len[3], vol;
-    int uniform;
+    float area[3], alpha[3], alpha_b[3], len[3], vol, inv_vol;
+    int uniform, vol_pow2;
     const float* fn[3][3];
     const float* farea[3];
     const float* falpha[3];
     const float* fvol;
 };
-static_assert(sizeof(JGeo) == 208, "JGeo layout");
+static_assert(sizeof(JGeo) == 216, "JGeo layout");
 
 // hipRTC declares the HIP device built-ins itself, so the prelude includes no
 // runtime header.
 const char* kPrelude = R"PRELUDE(
 struct JGeo {
     long long X, Y, Z;
-    float area[3], alpha[3], alpha_b[3], len[3], vol;
-    int uniform;
+    float area[3], alpha[3], alpha_b[3], len[3], vol, inv_vol;
+    int uniform, vol_pow2;
     const float* fn[3][3];
     const float* farea[3];
     const float* falpha[3];
     const float* fvol;
 };
-static_assert(sizeof(JGeo) == 208, "JGeo layout");
+static_assert(sizeof(JGeo) == 216, "JGeo layout");
 // linear face interpolation (phi_c + phi_nb) * 0.5
 __device__ inline float jlin(float c, float n) { return (c + n) * 0.5f; }
+// x / V of a uniform mesh: for a power-of-two V the product with its exact
+// reciprocal, which rounds to the same value as the quotient
+__device__ inline float jdivv(const JGeo& g, float x) { return g.vol_pow2 ? x * g.inv_vol : x / g.vol; }
 // uniform-mesh Green-Gauss pair of one axis: ((-f_lo) * A + f_hi * A) / V
-__device__ inline float jgg(float f_lo, float f_hi, float A, float V) {
+__device__ inline float jgg(const JGeo& g, float f_lo, float f_hi, float A) {
     const float t_lo = (-f_lo) * A;
     const float t_hi = f_hi * A;
-    return (t_lo + t_hi) / V;
+    return jdivv(g, t_lo + t_hi);
 }
 // one axis of StabVar on a uniform mesh, added to acc: p at the cell (pc),
 // its axis neighbours (pl, ph), and the neighbours' own neighbours (pll, plh
 // of the lower one, phl, phh of the upper one; self on walls)
-__device__ inline float jstab_axis(float acc, float pc, float pl, float ph, float pll, float plh, float phl,
-                                   float phh, float A, float V, float L) {
-    const float gc = jgg(jlin(pc, pl), jlin(pc, ph), A, V);
-    const float glo = jgg(jlin(pl, pll), jlin(pl, plh), A, V);
-    const float ghi = jgg(jlin(ph, phl), jlin(ph, phh), A, V);
+__device__ inline float jstab_axis(const JGeo& g, float acc, float pc, float pl, float ph, float pll, float plh,
+                                   float phl, float phh, float A, float L) {
+    const float gc = jgg(g, jlin(pc, pl), jlin(pc, ph), A);
+    const float glo = jgg(g, jlin(pl, pll), jlin(pl, plh), A);
+    const float ghi = jgg(g, jlin(ph, phl), jlin(ph, phh), A);
     const float own_lo = pc + (L * gc) * -1.0f;
     const float col_lo = pl + (L * glo) * 1.0f;
     const float own_hi = pc + (L * gc) * 1.0f;
     const float col_hi = ph + (L * ghi) * -1.0f;
     const float f_lo = 0.5f * (own_lo - col_lo);
     const float f_hi = 0.5f * (own_hi - col_hi);
-    return acc + (f_lo * A + f_hi * A) / V;
+    return acc + jdivv(g, f_lo * A + f_hi * A);
 }
 // v[0] = cell value, v[1 + fc] = value across local face fc (z-, z+, y-, y+, x-, x+)
 __device__ inline bool jat(const JGeo& g, long long x, long long y, long long z, int fc) {
@@ -131,7 +135,7 @@ __device__ inline float jgrad(const JGeo& g, long long x, long long y, long long
             const float f_lo = jlin(v[0], v[lo]), f_hi = jlin(v[0], v[hi]);
             const float t_lo = (-f_lo) * g.area[a];
             const float t_hi = f_hi * g.area[a];
-            res = res + (t_lo + t_hi) / g.vol;
+            res = res + jdivv(g, t_lo + t_hi);
         }
         return res;
     }
@@ -153,7 +157,7 @@ __device__ inline float jlapl(const JGeo& g, long long x, long long y, long long
             const float al_hi = jat(g, x, y, z, fhi) ? g.alpha_b[a] : g.alpha[a];
             const float s_lo = (-v[0] + v[1 + flo]) * al_lo;
             const float s_hi = (-v[0] + v[1 + fhi]) * al_hi;
-            res = res + (s_lo * g.area[a] + s_hi * g.area[a]) / g.vol;
+            res = res + jdivv(g, s_lo * g.area[a] + s_hi * g.area[a]);
         }
         return res;
     }
@@ -199,6 +203,7 @@ struct KernelBuilder::Impl {
     std::vector<const Node*> fused_stab;        // StabVar computed in registers (uniform meshes)
     std::map<const Node*, int> stab_id;
     bool uniform_mesh = false;                  // exactly uniform mesh on the device
+    std::string idx_t = "long long";            // cell index type of the generated code
 
     std::string source;
     std::string kname;
@@ -379,17 +384,17 @@ struct KernelBuilder::Impl {
             const std::string t = "T" + std::to_string(i), pc = "P" + std::to_string(i);
             body << "    float " << t << " = 0.0f;\n"
                  << "    const float " << pc << " = " << render(*n.a, "c") << ";\n";
-            static const char* stride[3] = {"g.Y * g.Z", "g.Z", "1LL"};
+            static const char* stride[3] = {"g.Y * g.Z", "g.Z", "1"};
             static const char* coord[3] = {"x", "y", "z"};
             static const char* extent[3] = {"g.X", "g.Y", "g.Z"};
             for (int a = 0; a < 3; ++a) {
                 if (!(n.mask & (1 << a))) continue;
                 const std::string sa = stride[a], ca = coord[a], na = extent[a];
                 body << "    {\n"
-                     << "        const long long sa = " << sa << ";\n"
+                     << "        const " << idx_t << " sa = " << sa << ";\n"
                      << "        const bool wl = " << ca << " == 0, wh = " << ca << " == " << na << " - 1;\n"
-                     << "        const long long il = wl ? c : c - sa;\n"
-                     << "        const long long ih = wh ? c : c + sa;\n"
+                     << "        const " << idx_t << " il = wl ? c : c - sa;\n"
+                     << "        const " << idx_t << " ih = wh ? c : c + sa;\n"
                      << "        const float pl = wl ? " << pc << " : " << render(*n.a, "il") << ";\n"
                      << "        const float ph = wh ? " << pc << " : " << render(*n.a, "ih") << ";\n"
                      << "        const float pll = wl ? " << pc << " : (" << ca << " == 1 ? pl : "
@@ -398,23 +403,29 @@ struct KernelBuilder::Impl {
                      << "        const float phh = wh ? " << pc << " : (" << ca << " == " << na << " - 2 ? ph : "
                      << render(*n.a, "(ih + sa)") << ");\n"
                      << "        const float phl = wh ? pl : " << pc << ";\n"
-                     << "        " << t << " = jstab_axis(" << t << ", " << pc
-                     << ", pl, ph, pll, plh, phl, phh, g.area[" << a << "], g.vol, g.len[" << a << "]);\n"
+                     << "        " << t << " = jstab_axis(g, " << t << ", " << pc
+                     << ", pl, ph, pll, plh, phl, phh, g.area[" << a << "], g.len[" << a << "]);\n"
                      << "    }\n";
             }
         };
         // one thread per cell
-        body << "    const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
-             << "    const long long y = (long long)blockIdx.y * blockDim.y + threadIdx.y;\n"
-             << "    const long long x = blockIdx.z;\n"
-             << "    if (z >= g.Z || y >= g.Y || x >= g.X) return;\n"
-             << "    const long long c = (x * g.Y + y) * g.Z + z;\n"
-             << "    const long long nb0 = z > 0 ? c - 1 : c;\n"
-             << "    const long long nb1 = z < g.Z - 1 ? c + 1 : c;\n"
-             << "    const long long nb2 = y > 0 ? c - g.Z : c;\n"
-             << "    const long long nb3 = y < g.Y - 1 ? c + g.Z : c;\n"
-             << "    const long long nb4 = x > 0 ? c - g.Y * g.Z : c;\n"
-             << "    const long long nb5 = x < g.X - 1 ? c + g.Y * g.Z : c;\n";
+        // cell indices in 32 bits when the mesh allows (fewer VALU
+        // instructions for the address arithmetic), else 64
+        const std::string I = idx_t;
+        body << "    const " << I << " Xg = " << I << "(g.X), Yg = " << I << "(g.Y), Zg = " << I << "(g.Z);\n"
+             << "    const " << I << " z = " << I << "(blockIdx.x) * " << I << "(blockDim.x) + " << I
+             << "(threadIdx.x);\n"
+             << "    const " << I << " y = " << I << "(blockIdx.y) * " << I << "(blockDim.y) + " << I
+             << "(threadIdx.y);\n"
+             << "    const " << I << " x = " << I << "(blockIdx.z);\n"
+             << "    if (z >= Zg || y >= Yg || x >= Xg) return;\n"
+             << "    const " << I << " c = (x * Yg + y) * Zg + z;\n"
+             << "    const " << I << " nb0 = z > 0 ? c - 1 : c;\n"
+             << "    const " << I << " nb1 = z < Zg - 1 ? c + 1 : c;\n"
+             << "    const " << I << " nb2 = y > 0 ? c - Zg : c;\n"
+             << "    const " << I << " nb3 = y < Yg - 1 ? c + Zg : c;\n"
+             << "    const " << I << " nb4 = x > 0 ? c - Yg * Zg : c;\n"
+             << "    const " << I << " nb5 = x < Xg - 1 ? c + Yg * Zg : c;\n";
         for (size_t i = 0; i < fused.size(); ++i)
             emit_stencil(i, "    ", render(*fused[i]->a, "nb4"), render(*fused[i]->a, "nb5"));
         for (size_t i = 0; i < fused_stab.size(); ++i) emit_stab(i);
@@ -469,6 +480,9 @@ void KernelBuilder::generate() {
         cfd_geometry cg{};
         if (cfd_mesh_geometry(cm->device_mesh(), &cg) == CFD_OK) I.uniform_mesh = cg.uniform != 0;
     }
+    // 32-bit cell indices when every index the kernel forms (up to two planes
+    // past a cell, and the face numbers of the general form) stays below 2^31
+    if (I.mesh != nullptr && uint64_t(I.mesh->_num_nodes) < (uint64_t(1) << 28)) I.idx_t = "int";
     I.walk(*I.root, false);
     I.generate();
 }
@@ -491,6 +505,14 @@ void KernelBuilder::build(bool announce) {
         g.alpha_b[a] = float(cg.alpha_b[a]);
     }
     g.vol = float(cg.vol);
+    {
+        int e = 0;
+        const float m = std::frexp(g.vol, &e);
+        const float inv = 1.0f / g.vol;
+        // a power of two whose reciprocal is a normal float
+        g.vol_pow2 = (m == 0.5f && std::isnormal(inv) && std::isnormal(g.vol)) ? 1 : 0;
+        g.inv_vol = g.vol_pow2 ? inv : 0.0f;
+    }
     for (int a = 0; a < 3; ++a) g.len[a] = float(cg.len[a]);
     g.uniform = cg.uniform;
     for (int f = 0; f < 3; ++f) {

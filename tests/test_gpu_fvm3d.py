@@ -282,7 +282,7 @@ def test_stabilisation_runs_inside_fused_kernels():
     assert res.returncode == 0, res.stdout + res.stderr
     sources = res.stdout.split("Building kernel:")[1:]
     # kernel bodies that call the in-kernel stabilisation (the prelude defines it in every source)
-    assert sum(" = jstab_axis(T" in src for src in sources) == 1
+    assert sum(" = jstab_axis(g, T" in src for src in sources) == 1
 
 
 def test_reference_sod_driver_unchanged(oracle_lib):
