@@ -25,24 +25,38 @@ namespace Eigen {
 
 namespace detail {
 
+// Optional page-locked host memory for large blocks, provided by the
+// operator-API library once the device is enabled (weak: a program built
+// without it, or a call before the device is up, gets ordinary memory).  The
+// copies between the device and page-locked memory run at DMA speed instead of
+// through the runtime's staging buffers.
+extern "C" {
+__attribute__((weak)) void* cfdarco_pinned_alloc(size_t bytes);
+__attribute__((weak)) void cfdarco_pinned_free(void* p, size_t bytes);
+}
+
 // Process-wide reuse of large freed blocks.  Drivers that round-trip a field
 // through the host each step (a host boundary callback copies the field,
 // returns the copy, and the old storage is released) otherwise pay a fresh
 // mapping, and a page fault per page, for every such vector.  Blocks of at
-// least kMinBytes are kept by exact size, at most kMaxBlocks of them and
-// kMaxBytes in all; everything else goes straight to operator new / delete.
+// least kMinBytes carry a 64-byte header (size and whether the memory is
+// page-locked), so any copy of this code can release any block; they are kept
+// by exact size, at most kMaxBlocks of them and kMaxBytes in all.  Smaller
+// blocks go straight to operator new / delete.
 class BlockCache {
 public:
     static constexpr size_t kMinBytes = size_t(1) << 20;
     static constexpr size_t kMaxBlocks = 8;
     static constexpr size_t kMaxBytes = size_t(4) << 30;
+    static constexpr size_t kHeader = 64;
 
     static BlockCache& get() {
         static BlockCache* c = new BlockCache;  // never destroyed: frees may run during exit
         return *c;
     }
     void* take(size_t bytes) {
-        if (bytes >= kMinBytes) {
+        if (bytes < kMinBytes) return ::operator new(bytes);
+        {
             std::lock_guard<std::mutex> lk(mu_);
             auto it = free_.find(bytes);
             if (it != free_.end()) {
@@ -52,10 +66,24 @@ public:
                 return p;
             }
         }
-        return ::operator new(bytes);
+        void* raw = nullptr;
+        int pinned = 0;
+        if (cfdarco_pinned_alloc != nullptr) {
+            raw = cfdarco_pinned_alloc(bytes + kHeader);
+            pinned = raw != nullptr;
+        }
+        if (raw == nullptr) raw = ::operator new(bytes + kHeader);
+        Header* h = static_cast<Header*>(raw);
+        h->bytes = bytes;
+        h->pinned = pinned;
+        return static_cast<char*>(raw) + kHeader;
     }
     void give(void* p, size_t bytes) {
-        if (bytes >= kMinBytes) {
+        if (bytes < kMinBytes) {
+            ::operator delete(p);
+            return;
+        }
+        {
             std::lock_guard<std::mutex> lk(mu_);
             if (free_.size() < kMaxBlocks && held_ + bytes <= kMaxBytes) {
                 free_.emplace(bytes, p);
@@ -63,10 +91,20 @@ public:
                 return;
             }
         }
-        ::operator delete(p);
+        void* raw = static_cast<char*>(p) - kHeader;
+        if (!static_cast<Header*>(raw)->pinned)
+            ::operator delete(raw);
+        else if (cfdarco_pinned_free != nullptr)
+            cfdarco_pinned_free(raw, bytes + kHeader);
+        // (page-locked memory is never handed to operator delete)
     }
 
 private:
+    struct Header {
+        size_t bytes;
+        int pinned;
+    };
+    static_assert(sizeof(Header) <= kHeader, "block header");
     std::mutex mu_;
     std::multimap<size_t, void*> free_;
     size_t held_ = 0;

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -141,6 +142,38 @@ void fill_device(void* dst, const void* value, size_t elem_bytes, size_t n) {
 }
 
 }  // namespace cfdarco
+
+// Page-locked host blocks for the Eigen subset's large vectors (declared weak
+// in cfd_eigen_subset.hpp): field copies between the device and these run at
+// DMA speed (256 MB: 4.7 ms against 15.6 ms device-to-host and 9.3 ms
+// host-to-device through pageable memory).  Only once the device is enabled,
+// and at most kPinnedCap bytes outstanding; beyond that, or on any failure,
+// the caller falls back to ordinary memory.
+namespace {
+constexpr size_t kPinnedCap = size_t(8) << 30;
+std::atomic<size_t> pinned_held{0};
+}  // namespace
+
+extern "C" void* cfdarco_pinned_alloc(size_t bytes) {
+    if (CFDArcoGlobalInit::context_or_null() == nullptr) return nullptr;
+    if (pinned_held.fetch_add(bytes) + bytes > kPinnedCap) {
+        pinned_held.fetch_sub(bytes);
+        return nullptr;
+    }
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || p == nullptr) {
+        (void)hipGetLastError();
+        pinned_held.fetch_sub(bytes);
+        return nullptr;
+    }
+    return p;
+}
+
+extern "C" void cfdarco_pinned_free(void* p, size_t bytes) {
+    if (p == nullptr) return;
+    (void)hipHostFree(p);
+    pinned_held.fetch_sub(bytes);
+}
 
 void CFDArcoGlobalInit::initialize(int, char**, bool skip_history_, const fs::path& store_path) {
     world_size = 1;

@@ -256,7 +256,9 @@ def test_device_point_sources_equal_host_callback():
     psw = os.path.join(BUILD, "bench", "point_source_wave")
     if not os.path.exists(psw):
         pytest.fail(f"{psw} not built (run __graft_entry__.build())")
-    args = ["-Lx", "20", "-Ly", "18", "-Lz", "16", "-t", "30", "-c"]
+    # fields of 1.3 MiB: the host vectors take the large-block path
+    # (page-locked once the device is enabled)
+    args = ["-Lx", "72", "-Ly", "72", "-Lz", "64", "-t", "30", "-c"]
     with tempfile.TemporaryDirectory() as tmp:
         res = subprocess.run([psw, *args], capture_output=True, text=True, timeout=300, cwd=tmp)
     assert res.returncode == 0, res.stdout + res.stderr
