@@ -72,13 +72,21 @@ __device__ inline int axis_family(int a) { return 2 - a; }  // family 0 = z-face
 // uniform-mesh building blocks
 // ---------------------------------------------------------------------------
 
+// x / V on a uniform mesh.  For a power-of-two volume the quotient equals the
+// product with the exact reciprocal bit for bit (both are the correctly
+// rounded value of the same real number), which is one instruction instead of
+// a division sequence; the branch is uniform across the launch.
+template <class T> __device__ inline T udiv(const UniformGeo<T>& g, T x) {
+    return g.vol_pow2 ? x * g.inv_vol : x / g.vol;
+}
+
 // Green-Gauss component along axis a from the two face values on that axis:
 // acc = (f_lo * -1) * A + (f_hi * +1) * A, then / V (gauss_grad_cu_k order,
 // include/custom_cuda_functions.hpp:277-295; the other four faces add zeros).
-template <class T> __device__ inline T gg_pair(T f_lo, T f_hi, T A, T V) {
+template <class T> __device__ inline T gg_pair_u(const UniformGeo<T>& g, T f_lo, T f_hi, T A) {
     const T t_lo = (-f_lo) * A;
     const T t_hi = f_hi * A;
-    return (t_lo + t_hi) / V;
+    return udiv(g, t_lo + t_hi);
 }
 
 // Linear face interpolation, (phi_c + phi_nb) * 0.5 (grad_utils.hpp:23-36).
@@ -103,7 +111,7 @@ __global__ void __launch_bounds__(256) grad_uniform_k(UniformGeo<T> g, const T* 
             if (at_lo(g, c, a)) f_lo = T(0);
             if (at_hi(g, c, a)) f_hi = T(0);
         }
-        res = res + gg_pair(f_lo, f_hi, g.area[a], g.vol);
+        res = res + gg_pair_u(g, f_lo, f_hi, g.area[a]);
     }
     out[c.id] = res;
 }
@@ -122,7 +130,7 @@ __global__ void __launch_bounds__(256) lapl_uniform_k(UniformGeo<T> g, const T* 
         const T s_lo = (-pc + phi[nb_lo(g, c, a)]) * al_lo;
         const T s_hi = (-pc + phi[nb_hi(g, c, a)]) * al_hi;
         // axis normals are +1 on both faces of the pair
-        res = res + (s_lo * g.area[a] + s_hi * g.area[a]) / g.vol;
+        res = res + udiv(g, s_lo * g.area[a] + s_hi * g.area[a]);
     }
     out[c.id] = res;
 }
@@ -155,7 +163,7 @@ template <class G> __device__ inline Cell cell_hi(const G& g, const Cell& c, int
 template <class T> __device__ inline T grad_comp(const UniformGeo<T>& g, const T* __restrict__ phi, const Cell& n,
                                                  int a) {
     const T pn = phi[n.id];
-    return gg_pair(lin(pn, phi[nb_lo(g, n, a)]), lin(pn, phi[nb_hi(g, n, a)]), g.area[a], g.vol);
+    return gg_pair_u(g, lin(pn, phi[nb_lo(g, n, a)]), lin(pn, phi[nb_hi(g, n, a)]), g.area[a]);
 }
 
 // StabVar on a uniform mesh (src/fvm3d.cpp:909-936) in one pass.  Second
@@ -183,7 +191,7 @@ __global__ void __launch_bounds__(256) stab_fused_uniform_k(UniformGeo<T> g, con
         const T col_hi = phi[hi.id] + (L * ghi) * T(-1);
         const T f_lo = T(0.5) * (own_lo - col_lo);
         const T f_hi = T(0.5) * (own_hi - col_hi);
-        res = res + (f_lo * g.area[d] + f_hi * g.area[d]) / g.vol;
+        res = res + udiv(g, f_lo * g.area[d] + f_hi * g.area[d]);
     }
     out[c.id] = res;
 }
@@ -379,7 +387,7 @@ __device__ inline T vel_update(const UniformGeo<T>& g, const T* __restrict__ p, 
     const T pc = p[j];
     const T pl = lo_wall ? pc : p[j - stride];
     const T ph = hi_wall ? pc : p[j + stride];
-    const T G = gg_pair(lin(pc, pl), lin(pc, ph), g.area[a], g.vol);
+    const T G = gg_pair_u(g, lin(pc, pl), lin(pc, ph), g.area[a]);
     return dt * (-G) + q[j];
 }
 
@@ -413,7 +421,7 @@ __global__ void __launch_bounds__(256) wave_uniform_k(UniformGeo<T> g, const T* 
             const T qh = vel_update(g, p, vel[a], j, str[a], false, crd[a] + 1 == ext[a] - 1, a, dt);
             f_hi = lin(qc, qh);
         }
-        const T G = gg_pair(f_lo, f_hi, g.area[a], g.vol);
+        const T G = gg_pair_u(g, f_lo, f_hi, g.area[a]);
         S = (a == 0) ? G : S + G;
     }
     const T k = Kf ? Kf[c.id] : K;
