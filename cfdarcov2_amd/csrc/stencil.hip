@@ -135,65 +135,103 @@ __global__ void __launch_bounds__(256) lapl_uniform_k(UniformGeo<T> g, const T* 
     out[c.id] = res;
 }
 
-// The cell across c's lower / upper face on axis a; c itself on a wall (its
-// id is then nb_lo / nb_hi's self index).
-template <class G> __device__ inline Cell cell_lo(const G& g, const Cell& c, int a) {
-    Cell n = c;
-    if (!at_lo(g, c, a)) {
-        if (a == 0) n.x -= 1;
-        else if (a == 1) n.y -= 1;
-        else n.z -= 1;
-        n.id = nb_lo(g, c, a);
-    }
-    return n;
-}
-template <class G> __device__ inline Cell cell_hi(const G& g, const Cell& c, int a) {
-    Cell n = c;
-    if (!at_hi(g, c, a)) {
-        if (a == 0) n.x += 1;
-        else if (a == 1) n.y += 1;
-        else n.z += 1;
-        n.id = nb_hi(g, c, a);
-    }
-    return n;
-}
-
-// Outward Green-Gauss gradient component a of cell n on a uniform mesh (the
-// first stage of StabVar).
-template <class T> __device__ inline T grad_comp(const UniformGeo<T>& g, const T* __restrict__ phi, const Cell& n,
-                                                 int a) {
-    const T pn = phi[n.id];
-    return gg_pair_u(g, lin(pn, phi[nb_lo(g, n, a)]), lin(pn, phi[nb_hi(g, n, a)]), g.area[a]);
-}
-
-// StabVar on a uniform mesh (src/fvm3d.cpp:909-936) in one pass.  Second
-// stage: upwind face values phi + (len * g_d) * n_out_d, jump against the
-// neighbour's opposite face (self's opposite face on the boundary), 0.5 *
-// jump, Gauss with axis normals for component d.  The gradient components it
-// reads (own and the two axis neighbours') are recomputed from phi at five
-// points per axis instead of going through a 3N buffer between two launches;
-// each is the same operation sequence as the first stage's.
+// One axis d of StabVar on a uniform mesh (src/fvm3d.cpp:909-936), added to
+// res.  Second stage: upwind face values phi + (len * g_d) * n_out_d, jump
+// against the neighbour's opposite face (self's opposite face on the
+// boundary), 0.5 * jump, Gauss with axis normals for component d.  The first
+// stage's gradient components it reads (own and the two axis neighbours') are
+// recomputed here from phi at five points on the axis: pc at the cell, pl / ph
+// at its lower / upper neighbour, pll / plh at the lower neighbour's own lower
+// / upper neighbour, phl / phh likewise for the upper one (self on walls).
 template <class T>
-__global__ void __launch_bounds__(256) stab_fused_uniform_k(UniformGeo<T> g, const T* __restrict__ phi,
-                                                            T* __restrict__ out, int mask) {
-    const Cell c = cell_of_thread(g);
-    if (!c.valid) return;
-    T res = T(0);
-    for (int d = 0; d < 3; ++d) {
-        if (!(mask & (1 << d))) continue;
-        const T L = g.len[d];
-        const Cell lo = cell_lo(g, c, d), hi = cell_hi(g, c, d);
-        const T pc = phi[c.id], gc = grad_comp(g, phi, c, d);
-        const T glo = grad_comp(g, phi, lo, d), ghi = grad_comp(g, phi, hi, d);
-        const T own_lo = pc + (L * gc) * T(-1);
-        const T col_lo = phi[lo.id] + (L * glo) * T(1);
-        const T own_hi = pc + (L * gc) * T(1);
-        const T col_hi = phi[hi.id] + (L * ghi) * T(-1);
-        const T f_lo = T(0.5) * (own_lo - col_lo);
-        const T f_hi = T(0.5) * (own_hi - col_hi);
-        res = res + udiv(g, f_lo * g.area[d] + f_hi * g.area[d]);
+__device__ inline T stab_axis_u(const UniformGeo<T>& g, T res, int d, T pc, T pl, T ph, T pll, T plh, T phl,
+                                T phh) {
+    const T L = g.len[d];
+    const T gc = gg_pair_u(g, lin(pc, pl), lin(pc, ph), g.area[d]);
+    const T glo = gg_pair_u(g, lin(pl, pll), lin(pl, plh), g.area[d]);
+    const T ghi = gg_pair_u(g, lin(ph, phl), lin(ph, phh), g.area[d]);
+    const T own_lo = pc + (L * gc) * T(-1);
+    const T col_lo = pl + (L * glo) * T(1);
+    const T own_hi = pc + (L * gc) * T(1);
+    const T col_hi = ph + (L * ghi) * T(-1);
+    const T f_lo = T(0.5) * (own_lo - col_lo);
+    const T f_hi = T(0.5) * (own_hi - col_hi);
+    return res + udiv(g, f_lo * g.area[d] + f_hi * g.area[d]);
+}
+
+// The five points of one axis from a fetch function P(offset) (offset in
+// cells along the axis, -2..2), coordinate k on an axis of n cells.
+template <class T, class F>
+__device__ inline T stab_axis_at(const UniformGeo<T>& g, T res, int d, int64_t k, int64_t n, T pc, F P) {
+    const bool wl = k == 0, wh = k == n - 1;
+    const T pl = wl ? pc : P(-1);
+    const T ph = wh ? pc : P(1);
+    const T pll = wl ? pc : (k == 1 ? pl : P(-2));
+    const T plh = wl ? ph : pc;
+    const T phh = wh ? pc : (k == n - 2 ? ph : P(2));
+    const T phl = wh ? pl : pc;
+    return stab_axis_u(g, res, d, pc, pl, ph, pll, plh, phl, phh);
+}
+
+// StabVar on a uniform mesh in one pass, as a march along x: a workgroup
+// holds TY rows of 64 z-cells and walks `L` planes.  Each thread keeps its
+// column's values at x-2 .. x+2 in registers; the plane x goes to LDS with
+// two halo rows and columns on each side, so the y and z points come from
+// there.  Every value passed to the arithmetic is the one the cell-indexed
+// rule picks (self on walls), so the result is the per-cell evaluation's.
+template <class T, int TY>
+__global__ void __launch_bounds__(64 * TY) stab_march_uniform_k(UniformGeo<T> g, const T* __restrict__ phi,
+                                                                T* __restrict__ out, int mask, int L, int nzt,
+                                                                int nyt) {
+    __shared__ T sh[TY + 4][64 + 4];
+    const int64_t X = g.X, Y = g.Y, Z = g.Z, SX = Y * Z;
+    const int64_t lin_ = blockIdx.x;
+    const int64_t zt = lin_ % nzt, yt = (lin_ / nzt) % nyt, xc = lin_ / (int64_t(nzt) * nyt);
+    const int lz = threadIdx.x, ly = threadIdx.y;
+    const int64_t z = zt * 64 + lz, y = yt * TY + ly;
+    const bool valid = z < Z && y < Y;
+    const int64_t col = valid ? y * Z + z : 0;
+    const int64_t x0 = xc * L, x1 = (x0 + L < X) ? x0 + L : X;
+    // halo rows (threads of the first / last two rows) and columns (lanes
+    // 0, 1, 62, 63) of this thread, -1 where outside the mesh
+    int64_t hy = -1, hz = -1;
+    int hy_row = 0, hz_col = 0;
+    if (z < Z) {
+        int64_t yy = -1;
+        if (ly < 2) { yy = yt * TY - 2 + ly; hy_row = ly; }
+        else if (ly >= TY - 2) { yy = yt * TY + TY + (ly - (TY - 2)); hy_row = ly + 4; }
+        if (yy >= 0 && yy < Y) hy = yy * Z + z;
     }
-    out[c.id] = res;
+    if (y < Y) {
+        int64_t zz = -1;
+        if (lz < 2) { zz = zt * 64 - 2 + lz; hz_col = lz; }
+        else if (lz >= 62) { zz = zt * 64 + 64 + (lz - 62); hz_col = lz + 4; }
+        if (zz >= 0 && zz < Z) hz = y * Z + zz;
+    }
+    T w[5] = {T(0), T(0), T(0), T(0), T(0)};  // column values at x-2 .. x+2
+    if (valid)
+        for (int k = 0; k < 5; ++k) {
+            const int64_t xx = x0 - 2 + k;
+            if (xx >= 0 && xx < X) w[k] = phi[xx * SX + col];
+        }
+    for (int64_t x = x0; x < x1; ++x) {
+        const int64_t pb = x * SX;
+        if (valid) sh[ly + 2][lz + 2] = w[2];
+        if (hy >= 0) sh[hy_row][lz + 2] = phi[pb + hy];
+        if (hz >= 0) sh[ly + 2][hz_col] = phi[pb + hz];
+        __syncthreads();
+        if (valid) {
+            const T pc = w[2];
+            T res = T(0);
+            if (mask & 1) res = stab_axis_at(g, res, 0, x, X, pc, [&](int o) { return w[2 + o]; });
+            if (mask & 2) res = stab_axis_at(g, res, 1, y, Y, pc, [&](int o) { return sh[ly + 2 + o][lz + 2]; });
+            if (mask & 4) res = stab_axis_at(g, res, 2, z, Z, pc, [&](int o) { return sh[ly + 2][lz + 2 + o]; });
+            out[pb + col] = res;
+            w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4];
+            w[4] = (x + 3 < X) ? phi[(x + 3) * SX + col] : T(0);
+        }
+        __syncthreads();
+    }
 }
 
 // --- general (per-face geometry) forms -------------------------------------
@@ -515,7 +553,13 @@ template <class T>
 void launch_stab(const cfd_mesh_s* m, const void* phi, void* out, int mask, void* scratch3, hipStream_t s) {
     if (m->uniform) {
         const auto& g = ugeo<T>(m);
-        stab_fused_uniform_k<T><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)phi, (T*)out, mask);
+        constexpr int TY = 4;
+        const int L = 8;
+        const int nzt = int((g.Z + 63) / 64), nyt = int((g.Y + TY - 1) / TY);
+        const int64_t nwg = int64_t(nzt) * nyt * ((g.X + L - 1) / L);
+        CFD_REQUIRE(nwg < (int64_t(1) << 31), "stabilisation grid too large");
+        stab_march_uniform_k<T, TY><<<dim3(unsigned(nwg)), dim3(64, TY), 0, s>>>(g, (const T*)phi, (T*)out, mask,
+                                                                                 L, nzt, nyt);
     } else {
         CFD_REQUIRE(scratch3 != nullptr, "stabilisation on a general mesh needs 3N scratch entries");
         T* gx = (T*)scratch3;

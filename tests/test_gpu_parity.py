@@ -30,6 +30,9 @@ MESHES = [
     ("uniform", (9, 7, 10), (0.5, 0.25, 0.125), None, False),
     # one- and two-cell extents: a cell's axis neighbours are itself or each other
     ("uniform-thin", (1, 2, 5), (0.5, 0.25, 0.125), None, False),
+    # several 64-cell z tiles, 4-row y tiles and 8-plane x chunks of the
+    # marching stabilisation, with partial ones at the ends
+    ("uniform-wide", (19, 11, 133), (0.5, 0.25, 0.125), None, False),
     ("uniform-as-general", (9, 7, 10), (0.5, 0.25, 0.125), None, True),
     ("non-dyadic", (6, 8, 5), (0.1, 0.3, 0.7), None, False),
     ("perturbed", (8, 6, 7), (0.125, 0.25, 0.5), 42, False),
