@@ -378,7 +378,7 @@ struct KernelBuilder::Impl {
         // StabVar of a pointwise operand on a uniform mesh: per selected
         // axis, the operand at the cell, its two neighbours and theirs (self
         // on walls; each point evaluated only when it is not one of the
-        // others), then stab_fused_uniform_k's arithmetic
+        // others), then the arithmetic of stab_axis_u in csrc/stencil.hip
         auto emit_stab = [&](size_t i) {
             const Node& n = *fused_stab[i];
             const std::string t = "T" + std::to_string(i), pc = "P" + std::to_string(i);
