@@ -272,6 +272,9 @@ public:
     void run();
     [[nodiscard]] const std::string& source() const;
     [[nodiscard]] const std::string& kernel_name() const;
+    // kernels computing stabilised pointwise operands before the launch
+    [[nodiscard]] const std::vector<std::string>& operand_sources() const;
+    [[nodiscard]] const std::vector<std::string>& operand_kernel_names() const;
 
     Variable* expression;
     Variable* left_val;

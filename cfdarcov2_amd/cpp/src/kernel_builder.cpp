@@ -202,7 +202,13 @@ struct KernelBuilder::Impl {
     std::map<const Node*, int> fused_id;
     std::vector<const Node*> fused_stab;        // StabVar computed in registers (uniform meshes)
     std::map<const Node*, int> stab_id;
+    // StabVar whose pointwise operand is computed by one generated kernel
+    // (operand_src / operand_name / operand_prog), then stabilised in one pass
+    std::vector<const Node*> operand_stab;
+    std::vector<std::string> operand_src, operand_name;
+    std::vector<cfd_program_t> operand_prog;
     bool uniform_mesh = false;                  // exactly uniform mesh on the device
+    bool stab_in_kernel = false;                // CFDARCO_STAB_IN_KERNEL=1: the 13-point in-kernel form
     std::string idx_t = "long long";            // cell index type of the generated code
 
     std::string source;
@@ -268,15 +274,28 @@ struct KernelBuilder::Impl {
                 }
                 return;
             case NodeKind::Stab:
-                // On a uniform mesh a stabilisation of a pointwise operand of
-                // three or more fields runs in the kernel (the operand at 13
-                // points); otherwise it is evaluated before the launch.  On the
+                // A stabilisation is evaluated before the launch and read as a
+                // buffer.  A pointwise operand with two or more operators is
+                // computed by one generated kernel of its own (one read of
+                // each field, one write) and stabilised by the one-pass
+                // kernel; other operands run operator by operator.
+                //
+                // CFDARCO_STAB_IN_KERNEL=1 selects the in-kernel form instead:
+                // on a uniform mesh an operand of three or more fields is
+                // evaluated at 13 points inside the generated kernel.  On the
                 // Euler driver at 200^3, stab_tot(E) of five fields took 210 us
-                // in-kernel against about 250 us materialised (eight
+                // that way against about 250 us operator by operator (eight
                 // elementwise launches, the one-pass stabilisation and the
                 // fused update), while stab_tot(rho * u) took 130 us in-kernel
-                // against about 100 us materialised.
-                if (uniform_mesh && !in_stencil && !has_stencil(*n.a) && distinct_leaves(*n.a) >= 3) {
+                // against about 100 us operator by operator.
+                if (!stab_in_kernel && !has_stencil(*n.a) && count_ops(*n.a) >= 2) {
+                    if (!buffer_slot.count(buffer_key(n))) {
+                        operand_stab.push_back(&n);
+                        walk(*n.a, true);
+                        materialize(n);
+                    }
+                } else if (stab_in_kernel && uniform_mesh && !in_stencil && !has_stencil(*n.a) &&
+                           distinct_leaves(*n.a) >= 3) {
                     if (!stab_id.count(&n)) {
                         stab_id[&n] = int(fused_stab.size());
                         fused_stab.push_back(&n);
@@ -301,6 +320,11 @@ struct KernelBuilder::Impl {
         };
         visit(n);
         return seen.size();
+    }
+
+    static size_t count_ops(const Node& n) {
+        const size_t own = (n.kind == NodeKind::Binary || n.kind == NodeKind::Neg) ? 1 : 0;
+        return own + (n.a ? count_ops(*n.a) : 0) + (n.b ? count_ops(*n.b) : 0);
     }
 
     static bool has_stencil(const Node& n) {
@@ -343,6 +367,7 @@ struct KernelBuilder::Impl {
         sig << "JGeo g";
         for (size_t i = 0; i < buffers.size(); ++i) sig << ", const float* __restrict__ in" << i;
         for (size_t i = 0; i < scalars.size(); ++i) sig << ", float s" << i;
+        const std::string sig_inputs = sig.str();
         sig << ", float* __restrict__ out_, const float* __restrict__ cur_, const float* __restrict__ prev_, float dt_, "
                "int stage_";
         // kernel name from the source body (stable across runs)
@@ -442,6 +467,26 @@ struct KernelBuilder::Impl {
         k << "extern \"C\" __global__ void __launch_bounds__(256) " << kname << "(" << sig.str() << ") {\n"
           << b << "}\n";
         source = k.str();
+        // operand kernels: the main kernel's inputs (a stabilised buffer is
+        // not ready yet and is not read), one output, one thread per cell
+        for (const Node* st : operand_stab) {
+            const std::string ob = "    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;\n"
+                                   "    if (c >= n_) return;\n"
+                                   "    res_[c] = " + render(*st->a, "c") + ";\n";
+            const std::string os = sig_inputs + ", float* __restrict__ res_, long long n_";
+            uint64_t ho = 1469598103934665603ull;
+            for (unsigned char ch : os + ob) {
+                ho ^= ch;
+                ho *= 1099511628211ull;
+            }
+            char on[64];
+            std::snprintf(on, sizeof(on), "cfd_operand_%016llx", (unsigned long long)ho);
+            std::ostringstream ok;
+            ok << kPrelude << "extern \"C\" __global__ void __launch_bounds__(256) " << on << "(" << os << ") {\n"
+               << ob << "}\n";
+            operand_name.push_back(on);
+            operand_src.push_back(ok.str());
+        }
     }
 };
 
@@ -453,6 +498,8 @@ KernelBuilder::~KernelBuilder() = default;
 
 const std::string& KernelBuilder::source() const { return impl_->source; }
 const std::string& KernelBuilder::kernel_name() const { return impl_->kname; }
+const std::vector<std::string>& KernelBuilder::operand_sources() const { return impl_->operand_src; }
+const std::vector<std::string>& KernelBuilder::operand_kernel_names() const { return impl_->operand_name; }
 
 void KernelBuilder::generate() {
     Impl& I = *impl_;
@@ -483,6 +530,8 @@ void KernelBuilder::generate() {
     // 32-bit cell indices when every index the kernel forms (up to two planes
     // past a cell, and the face numbers of the general form) stays below 2^31
     if (I.mesh != nullptr && uint64_t(I.mesh->_num_nodes) < (uint64_t(1) << 28)) I.idx_t = "int";
+    const char* sik = std::getenv("CFDARCO_STAB_IN_KERNEL");
+    I.stab_in_kernel = sik != nullptr && std::strcmp(sik, "1") == 0;
     I.walk(*I.root, false);
     I.generate();
 }
@@ -521,9 +570,17 @@ void KernelBuilder::build(bool announce) {
         g.falpha[f] = static_cast<const float*>(cg.face_alpha[f]);
     }
     g.fvol = static_cast<const float*>(cg.cell_vol);
-    if (std::getenv("CFDARCO_JIT_PRINT")) std::cout << "Building kernel:\n" << I.source << std::endl;
+    if (std::getenv("CFDARCO_JIT_PRINT")) {
+        std::cout << "Building kernel:\n" << I.source << std::endl;
+        for (const std::string& src : I.operand_src) std::cout << "Building kernel:\n" << src << std::endl;
+    }
     cfdarco::check(cfd_program_compile(CFDArcoGlobalInit::context(), I.source.c_str(), I.kname.c_str(), &I.prog),
                    "KernelBuilder::build");
+    I.operand_prog.assign(I.operand_src.size(), nullptr);
+    for (size_t j = 0; j < I.operand_src.size(); ++j)
+        cfdarco::check(cfd_program_compile(CFDArcoGlobalInit::context(), I.operand_src[j].c_str(),
+                                           I.operand_name[j].c_str(), &I.operand_prog[j]),
+                       "KernelBuilder::build (stabilised operand)");
     if (announce) std::cout << "Kernel build" << std::endl;
 }
 
@@ -532,25 +589,47 @@ void KernelBuilder::run() {
     if (I.prog == nullptr) throw std::runtime_error("KernelBuilder::run before build");
     Variable* st = I.state;
     const size_t n = st->current_cu._size;
-    // sub-expressions evaluated operator by operator
-    std::vector<CudaDataMatrixD> mats;
-    mats.reserve(I.materialized.size());
-    std::map<const Node*, const float*> mat_ptr;
-    for (const Node* m : I.materialized) {
-        mats.push_back(cfdarco::evaluate_node(*m));
-        mat_ptr[m] = mats.back().data.get();
-    }
-    std::vector<const float*> bufs(I.buffers.size());
+    std::vector<const float*> bufs(I.buffers.size(), nullptr);
     for (size_t i = 0; i < I.buffers.size(); ++i) {
         const Node& b = *I.buffers[i];
         if (b.kind == NodeKind::Leaf) bufs[i] = b.leaf->current_cu.data.get();
         else if (b.kind == NodeKind::Array) bufs[i] = b.array.data.get();
-        else bufs[i] = mat_ptr.at(&b);
     }
     std::vector<float> scal(I.scalars.size());
     for (size_t i = 0; i < I.scalars.size(); ++i) {
         const Node& s = *I.scalars[i];
         scal[i] = s.kind == NodeKind::DtScalar ? s.dt->_dt : *s.ptr;
+    }
+    // sub-expressions evaluated before the launch: operator by operator, or a
+    // stabilised operand from its generated kernel
+    std::vector<CudaDataMatrixD> mats;
+    mats.reserve(I.materialized.size());
+    for (const Node* m : I.materialized) {
+        const auto it = std::find(I.operand_stab.begin(), I.operand_stab.end(), m);
+        if (it == I.operand_stab.end()) {
+            mats.push_back(cfdarco::evaluate_node(*m));
+        } else {
+            const size_t j = size_t(it - I.operand_stab.begin());
+            auto* cm = dynamic_cast<CudaMesh3D*>(I.mesh);
+            if (cm == nullptr || j >= I.operand_prog.size() || I.operand_prog[j] == nullptr)
+                throw std::runtime_error("KernelBuilder::run: stabilised operand kernel not built");
+            CudaDataMatrixD operand{n};
+            float* res = operand.data.get();
+            long long cells = static_cast<long long>(n);
+            std::vector<void*> oargs;
+            oargs.reserve(3 + bufs.size() + scal.size());
+            oargs.push_back(&I.geo);
+            for (auto& p : bufs) oargs.push_back(&p);
+            for (auto& s : scal) oargs.push_back(&s);
+            oargs.push_back(&res);
+            oargs.push_back(&cells);
+            const uint32_t ogrid[3] = {uint32_t((n + 255) / 256), 1, 1};
+            const uint32_t oblock[3] = {256, 1, 1};
+            cfdarco::check(cfd_program_launch(I.operand_prog[j], ogrid, oblock, oargs.data(), nullptr),
+                           "KernelBuilder::run (stabilised operand)");
+            mats.push_back(eval_stab(cm, operand, m->mask & 1, m->mask & 2, m->mask & 4));
+        }
+        bufs[size_t(I.buffer_slot.at(Impl::buffer_key(*m)))] = mats.back().data.get();
     }
     if (I.next._size != n || I.next.data.use_count() > 1) I.next = CudaDataMatrixD{n};
     float* out = I.next.data.get();

@@ -1,7 +1,7 @@
 // Emits the fused-kernel sources KernelBuilder generates for the equation
 // systems of fvm3d_parity.cpp, on the host only (no device), one file per
-// equation: <outdir>/<kernel_name>.hip.  tests/test_jit_codegen.py compiles
-// them with hipRTC.
+// equation: <outdir>/<kernel_name>.hip, and one file per kernel that computes
+// a stabilised operand.  tests/test_jit_codegen.py compiles them with hipRTC.
 #include <fstream>
 #include <iostream>
 #include <memory>
@@ -22,6 +22,11 @@ int emit(const std::string& dir, EquationTemplate& eqs) {
         std::ofstream f(dir + "/" + kb.kernel_name() + ".hip");
         f << kb.source();
         std::cout << kb.kernel_name() << "\n";
+        for (size_t j = 0; j < kb.operand_sources().size(); ++j) {
+            std::ofstream o(dir + "/" + kb.operand_kernel_names()[j] + ".hip");
+            o << kb.operand_sources()[j];
+            std::cout << kb.operand_kernel_names()[j] << "\n";
+        }
         ++n;
     }
     return n;
@@ -65,6 +70,7 @@ int main(int argc, char** argv) {
         {&rho, '=', rho - 0.5 * dt * (u * d1dx(rho) + rho * d1dx(u)), true},
         {&mass_tmp, '=', 0 * mass_tmp - (d1dx((E + p) * u)), true},
         {d1t(mass), '=', mass_tmp - stabx(rho * u) * 2.0f, true},
+        {d1t(mass), '=', mass_tmp - stab_tot(E) * 2.0f, true},
         {&mass_tmp, '=', lapl(d1dx(rho)) + d2dy(mass) + pv * rho, true},
         {&T, '=', -(-T) / (1.0f / u) - 0.0f * T, true},
     };

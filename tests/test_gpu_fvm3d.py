@@ -179,15 +179,19 @@ def _sum(a):
     return np.sum(a, dtype=np.float64)
 
 
-def _run_driver(name, args, per_op=False):
+def _run_driver(name, args, per_op=False, stab_in_kernel=False):
     path = os.path.join(DRIVERS, name)
     if not os.path.exists(path):
         pytest.skip(f"{name}: reference driver binaries are built only where the reference sources exist")
-    print(f"[case] driver {name} {' '.join(args)}{' (per operator)' if per_op else ''}", flush=True)
+    print(f"[case] driver {name} {' '.join(args)}{' (per operator)' if per_op else ''}"
+          f"{' (stabilisation in kernel)' if stab_in_kernel else ''}", flush=True)
     env = dict(os.environ)
     env.pop("CFDARCO_PER_OP", None)
+    env.pop("CFDARCO_STAB_IN_KERNEL", None)
     if per_op:
         env["CFDARCO_PER_OP"] = "1"
+    if stab_in_kernel:
+        env["CFDARCO_STAB_IN_KERNEL"] = "1"
     with tempfile.TemporaryDirectory() as tmp:
         res = subprocess.run([path, *args], capture_output=True, text=True, timeout=300, cwd=tmp, env=env)
     assert res.returncode == 0, res.stdout + res.stderr
@@ -239,13 +243,16 @@ def test_reference_drivers_run_unchanged(name, args):
 ])
 def test_false_flagged_equations_fused_equal_per_operator(name, args):
     """The drivers' equations flagged `false` (d1t with stabilisation) run as
-    fused kernels; every printed sum and maximum equals the run that
-    evaluates them operator by operator."""
+    fused kernels, with multi-operator stabilised operands from kernels of
+    their own or (CFDARCO_STAB_IN_KERNEL=1) evaluated inside the fused kernel;
+    every printed sum and maximum of both equals the run that evaluates them
+    operator by operator."""
     def lines(out):
         return re.findall(r"At \S+ Current sum = \S+ Current max = \S+", out)
     fused = lines(_run_driver(name, args))
+    in_kernel = lines(_run_driver(name, args, stab_in_kernel=True))
     per_op = lines(_run_driver(name, args, per_op=True))
-    assert fused and fused == per_op
+    assert fused and fused == per_op and in_kernel == per_op
 
 
 def test_device_point_sources_equal_host_callback():
@@ -268,23 +275,29 @@ def test_device_point_sources_equal_host_callback():
     assert dev == host
 
 
-def test_stabilisation_runs_inside_fused_kernels():
-    """On a uniform mesh the Euler driver's stab_tot(E) (an operand of five
-    fields) is computed inside its generated kernel, while the stabilisation
-    of one- and two-field operands runs as its own one-pass launch; the
-    printed results equal the per-operator run (test above)."""
+@pytest.mark.parametrize("in_kernel", [False, True])
+def test_stabilised_operand_placement(in_kernel):
+    """The Euler driver's stab_tot(E) (E of five fields and eight operators)
+    has its operand computed by one generated kernel, while rho * u (one
+    operator) runs operator by operator; with CFDARCO_STAB_IN_KERNEL=1 on a
+    uniform mesh, E is evaluated inside the fused kernel instead.  The printed
+    results equal the per-operator run (test above)."""
     path = os.path.join(DRIVERS, "cfdARCO_euler")
     if not os.path.exists(path):
         pytest.skip("reference driver binaries are built only where the reference sources exist")
     env = dict(os.environ, CFDARCO_JIT_PRINT="1")
     env.pop("CFDARCO_PER_OP", None)
+    env.pop("CFDARCO_STAB_IN_KERNEL", None)
+    if in_kernel:
+        env["CFDARCO_STAB_IN_KERNEL"] = "1"
     with tempfile.TemporaryDirectory() as tmp:
         res = subprocess.run([path, "-Lx", "12", "-Ly", "10", "-Lz", "14", "-t", "3", "-dt", "0.1", "-c"],
                              capture_output=True, text=True, timeout=300, cwd=tmp, env=env)
     assert res.returncode == 0, res.stdout + res.stderr
     sources = res.stdout.split("Building kernel:")[1:]
     # kernel bodies that call the in-kernel stabilisation (the prelude defines it in every source)
-    assert sum(" = jstab_axis(g, T" in src for src in sources) == 1
+    assert sum(" = jstab_axis(g, T" in src for src in sources) == (1 if in_kernel else 0)
+    assert sum("void __launch_bounds__(256) cfd_operand_" in src for src in sources) == (0 if in_kernel else 1)
 
 
 def test_reference_sod_driver_unchanged(oracle_lib):
