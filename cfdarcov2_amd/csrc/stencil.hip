@@ -177,12 +177,15 @@ __device__ inline T stab_axis_at(const UniformGeo<T>& g, T res, int d, int64_t k
 // holds TY rows of 64 z-cells and walks `L` planes.  Each thread keeps its
 // column's values at x-2 .. x+2 in registers; the plane x goes to LDS with
 // two halo rows and columns on each side, so the y and z points come from
-// there.  Every value passed to the arithmetic is the one the cell-indexed
-// rule picks (self on walls), so the result is the per-cell evaluation's.
+// there; the halo values of the next plane are loaded into registers while
+// the current one is computed.  Every value passed to the arithmetic is the
+// one the cell-indexed rule picks (self on walls), so the result is the
+// per-cell evaluation's.
 template <class T, int TY>
 __global__ void __launch_bounds__(64 * TY) stab_march_uniform_k(UniformGeo<T> g, const T* __restrict__ phi,
                                                                 T* __restrict__ out, int mask, int L, int nzt,
                                                                 int nyt) {
+    static_assert(TY >= 4, "halo rows come from two distinct leading and trailing rows");
     __shared__ T sh[TY + 4][64 + 4];
     const int64_t X = g.X, Y = g.Y, Z = g.Z, SX = Y * Z;
     const int64_t lin_ = blockIdx.x;
@@ -214,11 +217,18 @@ __global__ void __launch_bounds__(64 * TY) stab_march_uniform_k(UniformGeo<T> g,
             const int64_t xx = x0 - 2 + k;
             if (xx >= 0 && xx < X) w[k] = phi[xx * SX + col];
         }
+    // halo values of plane x (x0 < x1 <= X whenever the block has planes)
+    T hyv = (hy >= 0 && x0 < x1) ? phi[x0 * SX + hy] : T(0);
+    T hzv = (hz >= 0 && x0 < x1) ? phi[x0 * SX + hz] : T(0);
     for (int64_t x = x0; x < x1; ++x) {
         const int64_t pb = x * SX;
         if (valid) sh[ly + 2][lz + 2] = w[2];
-        if (hy >= 0) sh[hy_row][lz + 2] = phi[pb + hy];
-        if (hz >= 0) sh[ly + 2][hz_col] = phi[pb + hz];
+        if (hy >= 0) sh[hy_row][lz + 2] = hyv;
+        if (hz >= 0) sh[ly + 2][hz_col] = hzv;
+        if (x + 1 < x1) {
+            if (hy >= 0) hyv = phi[pb + SX + hy];
+            if (hz >= 0) hzv = phi[pb + SX + hz];
+        }
         __syncthreads();
         if (valid) {
             const T pc = w[2];
@@ -553,8 +563,15 @@ template <class T>
 void launch_stab(const cfd_mesh_s* m, const void* phi, void* out, int mask, void* scratch3, hipStream_t s) {
     if (m->uniform) {
         const auto& g = ugeo<T>(m);
-        constexpr int TY = 4;
-        const int L = 8;
+#ifndef CFD_STAB_TY
+#define CFD_STAB_TY 4
+#endif
+#ifndef CFD_STAB_PLANES
+#define CFD_STAB_PLANES 8
+#endif
+        // workgroup rows and x-planes per workgroup (build-time tunables)
+        constexpr int TY = CFD_STAB_TY;
+        const int L = CFD_STAB_PLANES;
         const int nzt = int((g.Z + 63) / 64), nyt = int((g.Y + TY - 1) / TY);
         const int64_t nwg = int64_t(nzt) * nyt * ((g.X + L - 1) / L);
         CFD_REQUIRE(nwg < (int64_t(1) << 31), "stabilisation grid too large");
