@@ -654,7 +654,13 @@ void KernelBuilder::run() {
     args.push_back(&prev);
     args.push_back(&dtv);
     args.push_back(&stage);
-    const uint32_t grid[3] = {uint32_t((I.geo.Z + 63) / 64), uint32_t((I.geo.Y + 3) / 4), uint32_t(I.geo.X)};
+    // workgroups are dealt round-robin to the 8 XCDs; with a multiple of 8
+    // per x-plane the same (y, z) tile of every plane runs on one XCD, whose
+    // L2 then already holds the x-neighbours (rows past Y exit at once)
+    const uint32_t gz = uint32_t((I.geo.Z + 63) / 64);
+    uint32_t gy = uint32_t((I.geo.Y + 3) / 4);
+    while ((uint64_t(gz) * gy) % 8 != 0) ++gy;
+    const uint32_t grid[3] = {gz, gy, uint32_t(I.geo.X)};
     const uint32_t block[3] = {64, 4, 1};
     cfdarco::check(cfd_program_launch(I.prog, grid, block, args.data(), nullptr), "KernelBuilder::run");
     // new state in, old state becomes the next output (or the previous state of
