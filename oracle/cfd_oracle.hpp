@@ -371,10 +371,65 @@ void stab(const HexMesh<T>& m, const T* phi, int axis_mask, T* out) {
 // DtVar::extract (src/fvm3d.cpp:690-692): dt * rhs + current.
 template <class T> inline T euler_update(T dt, T rhs, T cur) { return dt * rhs + cur; }
 
-// D2tVar::extract (src/fvm3d.cpp:735-740) once two history entries exist:
-// dt * dt * rhs + 2 * current - previous.
+// D2tVar::extract (src/fvm3d.cpp:735-740) once two history entries exist, and
+// the fused form's second stage (:719-731): dt * dt * rhs + 2 * current - previous.
 template <class T> inline T leapfrog_update(T dt, T rhs, T cur, T prev) {
     return ((dt * dt) * rhs + T(2) * cur) - prev;
+}
+
+// When does D2tVar leapfrog at step t = 0, 1, ... of Equation::evaluate?
+// Before that it takes the Euler form dt * rhs + current (the same
+// DtVar-shaped update for a second-order equation).
+//  * fused (KernelBuilder) equations: set_dt shifts
+//    second_stage <- second_stage_next <- second_stage_next_next <- 1 before
+//    every launch (include/fvm3d.hpp:290-295), all three starting at 0
+//    (:306-308), so the
+//    kernel leapfrogs from its third launch on;
+//  * per-operator equations with stored history (--store): extract leapfrogs
+//    once the history holds more than one entry (src/fvm3d.cpp:736), and
+//    Equation::evaluate appends one entry per step after the boundary
+//    callbacks (:1048, add_history :301-306), so at step t it holds t entries;
+//  * per-operator equations without history (skip_history, the default when
+//    --store is absent, include/utils3d.hpp:126): never.
+// In both leapfrogging forms "previous" is the state at the start of step
+// t - 1: the fused kernel saved it as pre_current during that launch, and the
+// history's second-to-last entry is the state after step t - 2's callbacks.
+enum D2tRule { D2T_FUSED = 0, D2T_HISTORY = 1, D2T_NO_HISTORY = 2 };
+
+inline bool d2t_leapfrogs(int64_t t, int rule) {
+    if (rule == D2T_FUSED) {
+        int64_t stage = 0, next = 0, next_next = 0;  // D2tVar members, zero-initialised
+        for (int64_t k = 0; k <= t; ++k) {           // set_dt before launch k
+            stage = next;
+            next = next_next;
+            next_next = 1;
+        }
+        return stage != 0;
+    }
+    if (rule == D2T_HISTORY) return t > 1;  // history.size() == t
+    return false;
+}
+
+// One step of {d2t(u), '=', k * (d2dx(u) + d2dy(u) [+ d2dz(u)])}: Grad2Var of
+// each selected axis (src/fvm3d.cpp:880-900), summed x, y, z from the left,
+// times k (the drivers' c * c), then the D2tVar update of the rule above.
+template <class T>
+void d2t_step(const HexMesh<T>& m, const T* u, const T* prev, T k, int axis_mask, T dt, bool leapfrog, T* out) {
+    const int64_t N = m.N;
+    std::vector<T> L[3], sum(N);
+    bool first = true;
+    for (int d = 0; d < 3; ++d) {
+        if (!(axis_mask & (1 << d))) continue;
+        L[d].resize(N);
+        lapl(m, u, 1 << d, L[d].data());
+        for (int64_t i = 0; i < N; ++i) sum[i] = first ? L[d][i] : sum[i] + L[d][i];
+        first = false;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        const T rhs = k * sum[i];
+        out[i] = leapfrog ? leapfrog_update(dt, rhs, u[i], prev[i]) : euler_update(dt, rhs, u[i]);
+    }
 }
 
 // UpdatePolicies::CourantFriedrichsLewy3D (src/fvm3d.cpp:512-527): gamma = 5/3

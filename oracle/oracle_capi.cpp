@@ -187,4 +187,20 @@ double oracle_euler_dt(void* h, const void* U, double gamma, double CFL) {
     return cfdo::euler_dt(om->d, S, gamma, CFL);
 }
 
+// D2tVar rule: 1 when step t (0-based) leapfrogs under `rule` (0 fused, 1
+// per-operator with history, 2 per-operator without history).
+int oracle_d2t_leapfrogs(int64_t t, int rule) { return cfdo::d2t_leapfrogs(t, rule) ? 1 : 0; }
+
+// One step of {d2t(u), '=', k * (sum of d2d<axis>(u) over axis_mask)}; `prev`
+// is read only when `leapfrog` is set.
+void oracle_d2t_step(void* h, const void* u, const void* prev, double k, int axis_mask, double dt, int leapfrog,
+                     void* out) {
+    auto* om = static_cast<OMesh*>(h);
+    ORACLE_DISPATCH(om,
+                    cfdo::d2t_step(om->f, (const float*)u, (const float*)prev, float(k), axis_mask, float(dt),
+                                   leapfrog != 0, (float*)out),
+                    cfdo::d2t_step(om->d, (const double*)u, (const double*)prev, k, axis_mask, dt, leapfrog != 0,
+                                   (double*)out));
+}
+
 }  // extern "C"

@@ -52,12 +52,25 @@ def lib():
         L.oracle_euler_run.argtypes = [vp, vp, d, d, i64, vp]
         L.oracle_euler_dt.argtypes = [vp, vp, d, d]
         L.oracle_euler_dt.restype = d
+        L.oracle_d2t_leapfrogs.argtypes = [i64, i]
+        L.oracle_d2t_leapfrogs.restype = i
+        L.oracle_d2t_step.argtypes = [vp, vp, vp, d, i, d, i, vp]
         _lib = L
     return _lib
 
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+# D2tVar rules (cfd_oracle.hpp d2t_leapfrogs): fused kernels, per-operator
+# evaluation with stored history (--store), per-operator without history
+D2T_FUSED, D2T_HISTORY, D2T_NO_HISTORY = 0, 1, 2
+
+
+def d2t_leapfrogs(t: int, rule: int) -> bool:
+    """Whether step t (0-based) of a d2t equation leapfrogs under `rule`."""
+    return bool(lib().oracle_d2t_leapfrogs(int(t), int(rule)))
 
 
 _INT_TABLES = {"ids", "ids_bf", "face_verts", "face_cells", "cell_faces", "cell_verts"}
@@ -159,3 +172,28 @@ class OracleMesh:
         U = np.ascontiguousarray(U, dtype=self.dtype)
         assert U.shape == (5, self.N), U.shape
         return lib().oracle_euler_dt(self._h, _ptr(U), float(gamma), float(CFL))
+
+    def d2t_step(self, u, prev, k, mask, dt, leapfrog):
+        """One step of {d2t(u), '=', k * (sum of d2d<axis>(u) over mask)}:
+        leapfrog ((dt*dt)*rhs + 2u) - prev, or the Euler form dt*rhs + u."""
+        u = self._arr(u)
+        prev = self._arr(prev) if prev is not None else np.zeros_like(u)
+        out = np.empty_like(u)
+        lib().oracle_d2t_step(self._h, _ptr(u), _ptr(prev), float(k), int(mask), float(dt), int(bool(leapfrog)),
+                              _ptr(out))
+        return out
+
+    def d2t_run(self, u0, k, mask, dt, steps, rule, bc=None, prev0=None):
+        """`steps` steps of a d2t equation under a D2tVar rule.  bc(u, t)
+        (optional) is the boundary callback applied after step t; the state at
+        the start of a step is the next step's "previous".  prev0 replaces the
+        previous state of step 0 (for a leapfrog started from two levels)."""
+        u = self._arr(u0).copy()
+        prev = np.zeros_like(u) if prev0 is None else self._arr(prev0).copy()
+        for t in range(int(steps)):
+            leap = rule is True or (rule is not False and d2t_leapfrogs(t, rule))
+            new = self.d2t_step(u, prev, k, mask, dt, leap)
+            prev, u = u, new
+            if bc is not None:
+                u = self._arr(bc(u, t))
+        return u

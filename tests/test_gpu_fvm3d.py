@@ -104,11 +104,12 @@ def test_acoustic_system_through_api(oracle_lib, case):
         assert np.array_equal(g, r), name
 
 
-@pytest.mark.parametrize("case", ["d2t", "d2t_nojit"])
-def test_second_order_time(oracle_lib, case):
-    """d2t: fused equations run two Euler steps then leapfrog; operator-by-
-    operator equations leapfrog only with stored history, which is off here,
-    so they stay Euler (the reference's two behaviours)."""
+@pytest.mark.parametrize("case,rule", [("d2t", 0), ("d2t_nojit", 2), ("d2t_nojit_store", 1)])
+def test_second_order_time(oracle_lib, case, rule):
+    """d2t: fused equations run two Euler-form steps then leapfrog; operator-by-
+    operator equations leapfrog only with stored history (--store, here the
+    "store" case), else stay in the Euler form (the reference's behaviours,
+    restated by the oracle's D2tVar rules)."""
     n, d = (16, 14, 6), (1 / 16, 1 / 16, 1 / 16)
     om = _mesh(oracle_lib, n, d)
     X, Y, Z = workloads.cell_centres(*n, *d)
@@ -116,15 +117,7 @@ def test_second_order_time(oracle_lib, case):
     dt, steps = f32(0.05), 20
     got = _run_case(case, n, d, float(dt), steps, [u0], 1)
     c = f32(0.3)
-    k = c * c
-    u, prev = u0.copy(), np.zeros_like(u0)
-    for t in range(steps):
-        rhs = k * (om.lapl(u, 1) + om.lapl(u, 2))
-        if case == "d2t" and t >= 2:
-            new = ((dt * dt) * rhs + f32(2.0) * u) - prev
-        else:
-            new = dt * rhs + u
-        prev, u = u, new
+    u = om.d2t_run(u0, float(c * c), 3, float(dt), steps, rule)
     assert np.array_equal(got[0], u)
 
 
@@ -222,18 +215,152 @@ def test_reference_heat_driver_unchanged(oracle_lib):
     assert "Time difference" in out
 
 
-@pytest.mark.parametrize("name,args", [
-    ("cfdARCO_wave", ["-Lx", "20", "-Ly", "20", "-Lz", "20", "-t", "30", "-dt", "0.1", "-c"]),
-    ("cfdARCO_gauss_pulse", ["-Lx", "20", "-Ly", "20", "-Lz", "4", "-dx", "0.05", "-dy", "0.05", "-dz", "0.05",
-                             "-t", "30", "-dt", "0.01", "-c"]),
-    ("cfdARCO_euler", ["-Lx", "12", "-Ly", "12", "-Lz", "12", "-t", "10", "-dt", "0.1", "-c"]),
-])
-def test_reference_drivers_run_unchanged(name, args):
-    out = _run_driver(name, args)
-    sums = re.findall(r"At (\S+) Current sum = (\S+) Current max = (\S+)", out)
-    assert sums, out
-    for var, s, mx in sums:
-        assert np.isfinite(float(s)) and np.isfinite(float(mx)), (var, s, mx)
+def _printed(out):
+    return {v: (float(s), float(m)) for v, s, m in re.findall(r"At (\S+) Current sum = (\S+) Current max = (\S+)",
+                                                             out)}
+
+
+def _check_printed(printed, name, ref):
+    """The driver prints its FP32 sum and maximum with six significant digits:
+    allow half a unit in the sixth digit plus the FP32 sum's own rounding
+    (bounded by a few ulp of the sum of magnitudes)."""
+    s, mx = printed[name]
+    ref = np.asarray(ref, dtype=f32)
+    assert np.all(np.isfinite(ref)), name
+    want_s, want_m = float(_sum(ref)), float(ref.max())
+    mag = float(np.sum(np.abs(ref), dtype=np.float64))
+    assert abs(s - want_s) <= 5e-6 * abs(want_s) + 1e-6 * mag + 1e-30, (name, s, want_s)
+    assert abs(mx - want_m) <= 5e-6 * abs(want_m) + 1e-30, (name, mx, want_m)
+
+
+def test_reference_wave_driver_unchanged(oracle_lib):
+    """examples/main_3d_wave.cu compiled unchanged: d2t(u) = c*c*(d2dx(u) +
+    d2dy(u)) fused (two Euler-form steps, then leapfrog), dt = 0.1 fixed by
+    the driver, point sources at cells (0.1 n) and (0.9 n) set by its host
+    callback to sin(float(step) * 0.2) after every step.  Printed sum and
+    maximum equal the oracle's run of the same equation and callback."""
+    n, steps = (20, 18, 16), 30
+    out = _run_driver("cfdARCO_wave", ["-Lx", str(n[0]), "-Ly", str(n[1]), "-Lz", str(n[2]), "-t", str(steps),
+                                       "-dt", "0.7", "-c"])
+    om = _mesh(oracle_lib, n, (1.0, 1.0, 1.0))
+    X, Y, Z = n
+    cid = lambda x, y, z: Z * Y * x + Z * y + z
+    src = [cid(int(X * 0.1), int(Y * 0.1), int(Z * 0.1)), cid(int(X * 0.9), int(Y * 0.9), int(Z * 0.9))]
+
+    def sources(u, t):  # _current_time_step_int is t + 1 when the callback runs
+        u = u.copy()
+        u[src] = f32(np.sin(np.float64(f32(t + 1)) * 0.2))
+        return u
+
+    c = f32(0.3)
+    u = om.d2t_run(np.zeros(om.N, f32), float(c * c), 3, float(f32(0.1)), steps, oracle_lib.D2T_FUSED, bc=sources)
+    assert np.count_nonzero(u) > om.N // 4       # the sources have filled a good part of the box
+    _check_printed(_printed(out), "u", u)
+    assert "Time difference" in out
+
+
+@pytest.mark.parametrize("store", [False, True])
+def test_reference_gauss_pulse_driver_unchanged(oracle_lib, store):
+    """examples/main_3d_wave_gauss_pulse.cu compiled unchanged, 2D via one z
+    layer: the standing-wave initial state sampled at i*dx, the x/y boundary
+    layer zeroed (boundary_zero, by cell centre against dx + 0.001 and
+    lx - dx - 0.001), d2t flagged `false`.  Without --store the per-operator
+    D2tVar keeps the Euler form; with it, it leapfrogs from the third step.
+    Printed sum and maximum equal the oracle's run of the same rule."""
+    n, d, steps, dt = (40, 36, 1), 1.0 / 32, 60, 0.01
+    args = ["-Lx", str(n[0]), "-Ly", str(n[1]), "-Lz", "1", "-dx", str(d), "-dy", str(d), "-dz", str(d),
+            "-t", str(steps), "-dt", str(dt), "-c"] + (["-s"] if store else [])
+    out = _run_driver("cfdARCO_gauss_pulse", args)
+    om = _mesh(oracle_lib, n, (d, d, d))
+    X, Y = n[0], n[1]
+    dxf = f32(d)
+    lx, ly = f32(dxf * f32(X)), f32(dxf * f32(Y))
+    ix, iy = (a.ravel() for a in np.meshgrid(np.arange(X), np.arange(Y), indexing="ij"))
+    xv = f32(lx / f32(X)) * ix.astype(f32)
+    yv = f32(ly / f32(Y)) * iy.astype(f32)
+    u0 = (np.sin(np.pi * xv.astype(np.float64)) * np.sin(np.pi * yv.astype(np.float64))).astype(f32)
+    xl, xu = f32(float(dxf) + 0.001), f32(float(f32(lx - dxf)) - 0.001)
+    yl, yu = f32(float(dxf) + 0.001), f32(float(f32(ly - dxf)) - 0.001)
+    cx, cy = om.table("ccx"), om.table("ccy")
+    keep = (xl < cx) & (cx < xu) & (yl < cy) & (cy < yu)
+    zero_bc = lambda u, t: np.where(keep, u, f32(0)).astype(f32)
+    u0 = zero_bc(u0, -1)
+    c = f32(0.3)
+    rule = oracle_lib.D2T_HISTORY if store else oracle_lib.D2T_NO_HISTORY
+    u = om.d2t_run(u0, float(c * c), 3, float(f32(dt)), steps, rule, bc=zero_bc)
+    _check_printed(_printed(out), "u", u)
+
+
+def test_reference_euler_driver_unchanged(oracle_lib):
+    """examples/main_3d.cpp compiled unchanged (10 variables, 15 fused
+    equations and 5 stabilised d1t equations flagged `false`, 3D CFL policy
+    with gamma 5/3, boundary layers held at their initial values): every
+    printed sum and maximum equals the oracle's run of the same twenty
+    equations in the same order."""
+    n, steps, cfl = (12, 10, 14), 8, 0.1
+    out = _run_driver("cfdARCO_euler", ["-Lx", str(n[0]), "-Ly", str(n[1]), "-Lz", str(n[2]), "-t", str(steps),
+                                        "-dt", str(cfl), "-c"])
+    printed = _printed(out)
+    om = _mesh(oracle_lib, n, (1.0, 1.0, 1.0))
+    cx, cy, cz, vol = om.table("ccx"), om.table("ccy"), om.table("ccz"), om.table("vol")
+    lx, ly, lz = (f32(f32(1.0) * f32(k)) for k in n)
+    lim = lambda a, L: f32(a * float(L))
+    rho0 = np.where((lim(0.4, ly) < cy) & (cy < lim(0.6, ly)) & (lim(0.4, lx) < cx) & (cx < lim(0.6, lx))
+                    & (cz < lim(1.0, lz)), f32(5), f32(1)).astype(f32)
+    # u and v alternate -0.1, +0.1 in ten z bands: band b (1..10) is the first
+    # with z < 0.b * lz (compared in double), value (-1)^b * 0.1
+    zc = cz.astype(np.float64)
+    band = np.full(om.N, 10)
+    for b in range(9, 0, -1):
+        band = np.where(zc < b / 10 * float(lz), b, band)
+    u0 = np.where(band % 2 == 1, f32(-0.1), f32(0.1)).astype(f32)
+    v0 = u0.copy()
+    w0 = np.full(om.N, f32(0.6), f32)
+    p0 = np.full(om.N, f32(2.5), f32)
+    gamma = f32(5.0 / 3.0)
+    g1 = gamma - f32(1)
+    half, zero, two = f32(0.5), f32(0), f32(2)
+    mass0 = rho0 * vol
+    ru0, rv0, rw0 = (rho0 * u0) * vol, (rho0 * v0) * vol, (rho0 * w0) * vol
+    E0 = ((p0 / g1) + ((half * rho0) * (((u0 * u0) + (v0 * v0)) + (w0 * w0)))) * vol
+    init = dict(rho=rho0, u=u0, v=v0, w=w0, p=p0, mass=mass0, rho_u=ru0, rho_v=rv0, rho_w=rw0, rho_e=E0)
+    S = {k: a.copy() for k, a in init.items()}
+    tmp = dict(mt=mass0.copy(), rut=ru0.copy(), rvt=rv0.copy(), rwt=rw0.copy(), ret=E0.copy())
+    bnd = om.table("is_bnd").astype(f32)
+    rev = f32(1) - bnd
+    Gx, Gy, Gz = (lambda a: om.grad(a, 1)), (lambda a: om.grad(a, 2)), (lambda a: om.grad(a, 4))
+    for _ in range(steps):
+        rho, u, v, w, p = S["rho"], S["u"], S["v"], S["w"], S["p"]
+        dt = f32(om.cfl3d(cfl, u, v, w, p, rho))
+        hd = half * dt
+        rho = S["mass"] / vol
+        u = (S["rho_u"] / rho) / vol
+        v = (S["rho_v"] / rho) / vol
+        w = (S["rho_w"] / rho) / vol
+        p = ((S["rho_e"] / vol) - ((half * rho) * (((u * u) + (v * v)) + (w * w)))) * g1
+        rho = rho - hd * ((((((u * Gx(rho)) + (rho * Gx(u))) + (v * Gy(rho))) + (rho * Gy(v))) + (w * Gz(rho)))
+                          + (rho * Gz(w)))
+        u = u - hd * ((((u * Gx(u)) + (v * Gy(u))) + (w * Gz(u))) + ((f32(1) / rho) * Gx(p)))
+        v = v - hd * ((((u * Gx(v)) + (v * Gy(v))) + (w * Gz(v))) + ((f32(1) / rho) * Gy(p)))
+        w = w - hd * ((((u * Gx(w)) + (v * Gy(w))) + (w * Gz(w))) + ((f32(1) / rho) * Gz(p)))
+        p = p - hd * (((((gamma * p) * ((Gx(u) + Gy(v)) + Gz(w))) + (u * Gx(p))) + (v * Gy(p))) + (w * Gz(p)))
+        E = (p / g1) + ((half * rho) * (((u * u) + (v * v)) + (w * w)))
+        tmp["mt"] = (zero * tmp["mt"]) - ((Gx(rho * u) + Gy(rho * v)) + Gz(rho * w))
+        tmp["rut"] = (zero * tmp["rut"]) - ((Gx(((rho * u) * u) + p) + Gy((rho * v) * u)) + Gz((rho * w) * u))
+        tmp["rvt"] = (zero * tmp["rvt"]) - ((Gx((rho * v) * u) + Gy(((rho * v) * v) + p)) + Gz((rho * v) * w))
+        tmp["rwt"] = (zero * tmp["rwt"]) - ((Gx((rho * w) * u) + Gy((rho * w) * v)) + Gz(((rho * w) * w) + p))
+        tmp["ret"] = (zero * tmp["ret"]) - ((Gx((E + p) * u) + Gy((E + p) * v)) + Gz((E + p) * w))
+        S["mass"] = dt * (tmp["mt"] - (om.stab(rho, 7) * two)) + S["mass"]
+        S["rho_u"] = dt * (tmp["rut"] - (om.stab(rho * u, 7) * two)) + S["rho_u"]
+        S["rho_v"] = dt * (tmp["rvt"] - (om.stab(rho * v, 7) * two)) + S["rho_v"]
+        S["rho_w"] = dt * (tmp["rwt"] - (om.stab(rho * w, 7) * two)) + S["rho_w"]
+        S["rho_e"] = dt * (tmp["ret"] - (om.stab(E, 7) * two)) + S["rho_e"]
+        S.update(rho=rho, u=u, v=v, w=w, p=p)
+        for k in S:   # boundary_copy_cu: arr * rev + initial * bnd
+            S[k] = (S[k] * rev) + (init[k] * bnd)
+    assert len(printed) == 10
+    for k, ref in S.items():
+        _check_printed(printed, k, ref)
     assert "Time difference" in out
 
 

@@ -316,3 +316,201 @@ def test_splitmix64_known_answers():
     scal = np.array([g1.uniform() for _ in range(17)])
     vec = g2.uniform_array(17)
     assert np.array_equal(scal, vec) and g1.state == g2.state
+
+
+# ---------------------------------------------------------------------------
+# stabilisation at the walls (include/grad_utils.hpp:38-80 with
+# use_bound_free = false, src/fvm3d.cpp:915-936)
+# ---------------------------------------------------------------------------
+def _stab_linear_profile(n, a):
+    """stab_d(a * x_d) along one axis of n >= 4 cells, computed by hand.
+
+    Cell gradients are a inside and a/2 in the two wall cells (one-sided
+    half difference).  The upwind face values are phi + (h/2) g n_out.  On a
+    wall face the collected value is the cell's own opposite face (the
+    neighbour is the cell itself), so the lower wall cell gets
+    0.5 * ((phi - a h/4) - (phi + a h/4)) = -a h/4 on its wall face and
+    0.5 * ((phi + a h/4) - (phi + h a - a h/2)) = -a h/8 on its inner face;
+    with +axis normals on both faces and A/V = 1/h the sum is -3a/8.  The
+    next cell sees +a h/8 on its lower face only, +a/8.  The upper wall is
+    the mirror image with the signs reversed; interior cells give 0."""
+    out = np.zeros(n)
+    out[0], out[1], out[-2], out[-1] = -3 * a / 8, a / 8, -a / 8, 3 * a / 8
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_stab_wall_rule_by_hand(oracle_lib, dtype):
+    """A slip in the wall rule (reading the wall face itself, 0 instead of the
+    opposite face, the bound-free -1 rule, half instead of full len) changes
+    the wall-cell values: -3a/8 becomes -a/8, -a/4 or other."""
+    m = _mesh(oracle_lib, N3, D3, dtype)
+    X, Y, Z = _coords()
+    ix, iy, iz = (a.ravel() for a in _idx())
+    a, b, c = 1.5, -2.0, 0.75
+    phi = (a * X + b * Y + c * Z + 3.0).astype(dtype)
+    tol = dict(rtol=1e-10, atol=1e-10) if dtype == np.float64 else dict(rtol=2e-4, atol=2e-4)
+    px = _stab_linear_profile(N3[0], a)[ix]
+    py = _stab_linear_profile(N3[1], b)[iy]
+    pz = _stab_linear_profile(N3[2], c)[iz]
+    np.testing.assert_allclose(m.stab(phi, 1), px, **tol)
+    np.testing.assert_allclose(m.stab(phi, 2), py, **tol)
+    np.testing.assert_allclose(m.stab(phi, 4), pz, **tol)
+    np.testing.assert_allclose(m.stab(phi, 7), px + py + pz, **tol)
+
+
+# ---------------------------------------------------------------------------
+# geometry of one perturbed, non-planar cell pair, computed by hand
+# (src/mesh3d.cpp:43-62 face, :69-105 cell, :230-312 tables)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_perturbed_cell_pair_by_hand(oracle_lib, dtype):
+    """Two unit cells along x.  Vertex (1,0,0) moves by (0.4, 0, 0): it is the
+    fourth vertex of the shared face's quad, so the face is no longer planar
+    but its normal (first three vertices only) stays -x.  Vertex (2,1,1)
+    moves by (0.8, 0, 0.8), tilting the centre-to-centre vector.
+
+    By hand: cell centres c0 = (0.55, 0.5, 0.5), c1 = (1.65, 0.5, 0.6); shared
+    face centre (1.1, 0.5, 0.5); d = (1.1, 0, 0.1).
+      alpha_d = 1 / (cos(phi) |d|) = 1 / (n.d) = 1/1.1 from both sides (without
+        the cos(phi) factor it would be 1/|d| = 1/sqrt(1.22));
+      len = 0.55 (cell 0) and sqrt(0.55^2 + 0.1^2) (cell 1);
+      area (sum of components) = 0.5 (|-1 - 0.4| + |1 + 0.4|) = 1.4.
+    Far wall face of cell 1 (quad (2,0,1), (2.8,1,1.8), (2,1,0), (2,0,0)):
+      AB x BC = (-1.8, 0.8, 0.8), axis normal (1.8, -0.8, -0.8)/sqrt(4.52);
+      centre (2.2, 0.5, 0.7); to-face vector d = (0.55, 0, 0.1);
+      n.d = (0.99 - 0.08)/sqrt(4.52), so the boundary
+      alpha_d = |1/cos(phi)| / (2|d|) = 1 / (2 n.d) = 1 / (2 * 0.91 / sqrt(4.52));
+      area = 0.5 (|-1 + 0.8| + |1.8 - 0.8|) = 0.6.
+    Near wall face of cell 0 (x = 0): d = (-0.55, 0, 0), alpha_d = 1/1.1,
+    len = 0.55."""
+    off = np.zeros((12, 3))              # vertices (x, y, z), z fastest, 3 x 2 x 2
+    vid = lambda x, y, z: (x * 2 + y) * 2 + z
+    off[vid(1, 0, 0)] = (0.4, 0.0, 0.0)
+    off[vid(2, 1, 1)] = (0.8, 0.0, 0.8)
+    m = _mesh(oracle_lib, (2, 1, 1), (1.0, 1.0, 1.0), dtype, perturb=off)
+    rt = 1e-12 if dtype == np.float64 else 2e-6
+    ok = lambda got, want: np.testing.assert_allclose(got, want, rtol=rt, atol=rt)
+    ok(m.table("ccx"), [0.55, 1.65])
+    ok(m.table("ccz"), [0.5, 0.6])
+    alpha, ln, area = m.table("alpha"), m.table("len"), m.table("area")
+    nx, ny, nz = m.table("nx"), m.table("ny"), m.table("nz")
+    nax = m.table("nax")
+    # shared face: cell 0 fc 5 (x+), cell 1 fc 4 (x-)
+    ok(alpha[5, 0], 1 / 1.1)
+    ok(alpha[4, 1], 1 / 1.1)
+    ok(ln[5, 0], 0.55)
+    ok(ln[4, 1], np.sqrt(0.55 ** 2 + 0.1 ** 2))
+    ok(area[5, 0], 1.4)
+    ok(area[4, 1], 1.4)
+    ok([nx[5, 0], ny[5, 0], nz[5, 0]], [1, 0, 0])       # axis normal: -(first-three normal)
+    ok([nax[5, 0], nax[4, 1]], [1, -1])                 # outward from each side
+    # far wall face of cell 1 (fc 5): tilted first-three normal
+    s = np.sqrt(4.52)
+    ok([nx[5, 1], ny[5, 1], nz[5, 1]], [1.8 / s, -0.8 / s, -0.8 / s])
+    ok(alpha[5, 1], 1 / (2 * 0.91 / s))
+    ok(ln[5, 1], np.sqrt(0.3125))
+    ok(area[5, 1], 0.6)
+    # near wall face of cell 0 (fc 4)
+    ok(alpha[4, 0], 1 / 1.1)
+    ok(ln[4, 0], 0.55)
+
+
+# ---------------------------------------------------------------------------
+# second-order time (D2tVar, src/fvm3d.cpp:710-765, include/fvm3d.hpp:278-313)
+# ---------------------------------------------------------------------------
+def _dirichlet_box(oracle_lib, X, Y, h, dtype=np.float64):
+    """A 2D mesh (X, Y, 1) whose outermost cell layer in x and y is held at
+    zero after every step (the gauss-pulse driver's boundary_zero)."""
+    m = _mesh(oracle_lib, (X, Y, 1), (h, h, h), dtype)
+    i, j = (a.ravel() for a in np.meshgrid(np.arange(X), np.arange(Y), indexing="ij"))
+    keep = (i > 0) & (i < X - 1) & (j > 0) & (j < Y - 1)
+    bc = lambda u, t: np.where(keep, u, 0).astype(dtype)
+    return m, i, j, bc
+
+
+def _mode(i, j, X, Y):
+    """The discrete eigenmode sin(pi i/(X-1)) sin(pi j/(Y-1)): zero on both
+    held layers, and (phi_{i-1} + phi_{i+1} - 2 phi_i) = -4 sin^2(pi/(2(X-1))) phi_i
+    per axis, so lapl = lam * phi with lam below."""
+    return np.sin(np.pi * i / (X - 1)) * np.sin(np.pi * j / (Y - 1))
+
+
+def _lam(X, Y, h):
+    return -(4 / h ** 2) * (np.sin(np.pi / (2 * (X - 1))) ** 2 + np.sin(np.pi / (2 * (Y - 1))) ** 2)
+
+
+def _amplitude(n, r, theta):
+    """Closed form of the mode amplitude after n steps: two Euler-form steps
+    (a_1 = r, a_2 = r^2 with r = 1 + dt k lam), then the leapfrog
+    a_{n+1} = (2 + dt^2 k lam) a_n - a_{n-1}, whose solution from (a_1, a_2) is
+    a_n = A cos((n-2) theta) + B sin((n-2) theta), cos(theta) = 1 + dt^2 k lam / 2,
+    A = r^2, B = (r^2 cos(theta) - r) / sin(theta)."""
+    if n <= 2:
+        return r ** n
+    A, B = r * r, (r * r * np.cos(theta) - r) / np.sin(theta)
+    return A * np.cos((n - 2) * theta) + B * np.sin((n - 2) * theta)
+
+
+@pytest.mark.parametrize("rule", [0, 1])
+def test_d2t_warmup_then_leapfrog_discrete_standing_wave(oracle_lib, rule):
+    """Fused kernels and per-operator runs with stored history both take two
+    Euler-form steps and then leapfrog.  On the discrete eigenmode the
+    amplitude has the closed form above; one warm-up step too many or too few,
+    dt instead of dt^2, a wrong sign of the previous level or a wrong Laplacian
+    eigenvalue each fail it."""
+    X, Y, h = 18, 14, 1.0 / 16
+    m, i, j, bc = _dirichlet_box(oracle_lib, X, Y, h)
+    k, dt = 0.09, 0.01
+    lam = _lam(X, Y, h)
+    r = 1 + dt * k * lam
+    theta = np.arccos(1 + dt * dt * k * lam / 2)
+    u0 = _mode(i, j, X, Y)
+    u0 = bc(u0, -1)
+    for n in (1, 2, 3, 4, 57):
+        u = m.d2t_run(u0, k, 3, dt, n, rule, bc=bc)
+        np.testing.assert_allclose(u, _amplitude(n, r, theta) * u0, rtol=0, atol=1e-12)
+
+
+def test_d2t_without_history_stays_euler(oracle_lib):
+    """Per-operator d2t without stored history (no --store) keeps the Euler
+    form dt * rhs + u at every step: the mode decays as r^n."""
+    X, Y, h = 18, 14, 1.0 / 16
+    m, i, j, bc = _dirichlet_box(oracle_lib, X, Y, h)
+    k, dt = 0.09, 0.01
+    r = 1 + dt * k * _lam(X, Y, h)
+    u0 = bc(_mode(i, j, X, Y), -1)
+    u = m.d2t_run(u0, k, 3, dt, 40, oracle_lib.D2T_NO_HISTORY, bc=bc)
+    np.testing.assert_allclose(u, r ** 40 * u0, rtol=0, atol=1e-12)
+
+
+def test_d2t_continuum_standing_wave(oracle_lib):
+    """The standing wave sin(pi x) sin(pi y) cos(sqrt(2) pi c t) that the
+    reference's own comparison uses (reports/compare_gauss_pulse.py:57-77,
+    c = 0.3 as in examples/main_3d_wave_gauss_pulse.cu:187).
+
+    * Leapfrog started from the two exact levels (zero initial velocity)
+      converges to it: second order in h and dt.
+    * The reference's warm-up (two steps of dt * rhs + u) gives the mode the
+      initial velocity c^2 lapl(u0) = -omega^2 u0 instead of 0, so the same
+      scheme converges to sin(pi x) sin(pi y) (cos(omega t) - omega sin(omega t)),
+      with an O(dt) error from the warm-up."""
+    c = 0.3
+    k = c * c
+    omega = np.sqrt(2) * np.pi * c
+    errs = []
+    for X in (33, 65):
+        h = 1.0 / (X - 1)                 # held layers at x = 0 and x = 1
+        m, i, j, bc = _dirichlet_box(oracle_lib, X, X, h)
+        x, y = i * h, j * h               # the driver samples the sine at i * dx
+        shape = np.sin(np.pi * x) * np.sin(np.pi * y)
+        u0 = bc(shape, -1)
+        dt, steps = 0.25 * h, int(round(1.0 / (0.25 * h)))
+        t = steps * dt
+        prev0 = np.cos(omega * dt) * u0   # u(-dt) of the zero-velocity solution
+        u = m.d2t_run(u0, k, 3, dt, steps, True, bc=bc, prev0=prev0)
+        errs.append(np.max(np.abs(u - shape * np.cos(omega * t))))
+        warm = m.d2t_run(u0, k, 3, dt, steps, oracle_lib.D2T_FUSED, bc=bc)
+        want = shape * (np.cos(omega * t) - omega * np.sin(omega * t))
+        assert np.max(np.abs(warm - want)) < 4 * omega ** 2 * dt
+    assert errs[1] < 1e-3 and errs[0] / errs[1] > 3.5   # second-order convergence
