@@ -254,7 +254,11 @@ def test_reference_wave_driver_unchanged(oracle_lib):
 
     c = f32(0.3)
     u = om.d2t_run(np.zeros(om.N, f32), float(c * c), 3, float(f32(0.1)), steps, oracle_lib.D2T_FUSED, bc=sources)
-    assert np.count_nonzero(u) > om.N // 4       # the sources have filled a good part of the box
+    # the operator has no z term, so the sources spread only within their two
+    # z layers (2 * 20 * 18 cells); after 30 steps they cover most of them
+    zlayer = np.arange(om.N) % Z
+    assert np.all(u[(zlayer != src[0] % Z) & (zlayer != src[1] % Z)] == 0)
+    assert np.count_nonzero(u) > 300
     _check_printed(_printed(out), "u", u)
     assert "Time difference" in out
 
@@ -267,7 +271,7 @@ def test_reference_gauss_pulse_driver_unchanged(oracle_lib, store):
     lx - dx - 0.001), d2t flagged `false`.  Without --store the per-operator
     D2tVar keeps the Euler form; with it, it leapfrogs from the third step.
     Printed sum and maximum equal the oracle's run of the same rule."""
-    n, d, steps, dt = (40, 36, 1), 1.0 / 32, 60, 0.01
+    n, d, steps, dt = (40, 36, 1), 1.0 / 32, 60, 0.002
     args = ["-Lx", str(n[0]), "-Ly", str(n[1]), "-Lz", "1", "-dx", str(d), "-dy", str(d), "-dz", str(d),
             "-t", str(steps), "-dt", str(dt), "-c"] + (["-s"] if store else [])
     out = _run_driver("cfdARCO_gauss_pulse", args)
