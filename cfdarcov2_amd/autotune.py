@@ -173,3 +173,40 @@ def tune_heat(mesh, alpha, dt, steps=20, candidates=None, verbose=False) -> dict
     mesh.set_heat_shape(best["variant"], best["waves"], best["chunk"])
     del a, b
     return dict(best, table=table)
+
+
+ADJOINT_CANDIDATES = [(0, 4, 8)] + [(1, ty, L) for ty in (4, 8) for L in (2, 4, 8, 16, 32)]
+
+
+def tune_adjoint(problem, steps=20, candidates=None, verbose=False) -> dict:
+    """Pick the fastest kernel shape of the transposed acoustic step for the
+    problem's mesh (constant K, no K gradient: the dL/dp0 path), from runs on
+    scratch fields; forms the mesh does not allow are skipped."""
+    import torch
+
+    from . import adjoint
+    from .capi import CfdError
+    cands = candidates or ADJOINT_CANDIDATES
+    mesh = problem.mesh
+    short = adjoint.WaveProblem(mesh, problem.sensors, problem.dt, steps, K=problem.K, every=problem.every)
+    g = torch.zeros((steps, problem.sensors.size), dtype=mesh.torch_dtype, device="cuda")
+    stream = torch.cuda.current_stream()
+    table = []
+    for form, ty, L in cands:
+        try:
+            mesh.set_adjoint_shape(form, ty, L)
+        except CfdError:
+            continue
+        adjoint.adjoint(short, g)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        adjoint.adjoint(short, g)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        table.append({"form": form, "tile_rows": ty, "chunk": L, "ms_per_step": ms})
+        if verbose:
+            print(f"  adjoint shape form={form} ty={ty} L={L}: {ms * 1e3:.1f} us/step", flush=True)
+    best = min(table, key=lambda r: r["ms_per_step"])
+    mesh.set_adjoint_shape(best["form"], best["tile_rows"], best["chunk"])
+    return dict(best, table=table)

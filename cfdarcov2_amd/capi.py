@@ -35,7 +35,7 @@ EXPORTS = [
     "cfd_mesh_create_hex", "cfd_mesh_destroy", "cfd_mesh_get_info", "cfd_mesh_table", "cfd_hex_table",
     "cfd_grad", "cfd_lapl", "cfd_stab", "cfd_ewise", "cfd_cfl_min", "cfd_count_nonfinite",
     "cfd_heat_run", "cfd_wave_run", "cfd_euler_run", "cfd_mesh_set_wave_shape", "cfd_mesh_set_euler_shape",
-    "cfd_mesh_set_heat_shape",
+    "cfd_mesh_set_heat_shape", "cfd_mesh_set_adjoint_shape",
     "cfd_wave_traces", "cfd_wave_adjoint",
     "cfd_batch_create", "cfd_batch_set_form", "cfd_batch_set_shape", "cfd_batch_run", "cfd_batch_destroy",
     "cfd_program_compile", "cfd_program_compile_only", "cfd_program_launch", "cfd_program_destroy",
@@ -103,6 +103,7 @@ def load_library(path: str = LIB_PATH):
         "cfd_mesh_set_wave_shape": [vp, i, i, i, i],
         "cfd_mesh_set_euler_shape": [vp, i, i, i],
         "cfd_mesh_set_heat_shape": [vp, i, i, i],
+        "cfd_mesh_set_adjoint_shape": [vp, i, i, i],
         "cfd_program_compile": [vp, ctypes.c_char_p, ctypes.c_char_p, pp],
         "cfd_program_compile_only": [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)],
         "cfd_program_launch": [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), pp, vp],
@@ -338,6 +339,11 @@ class HexMesh:
 
     def set_heat_shape(self, variant=1, waves=8, chunk=16):
         check(self.lib.cfd_mesh_set_heat_shape(self.h, int(variant), int(waves), int(chunk)))
+
+    def set_adjoint_shape(self, form=1, tile_rows=4, chunk=8):
+        """Kernel form of wave_traces / wave_adjoint: 0 = one thread per cell,
+        1 = row form."""
+        check(self.lib.cfd_mesh_set_adjoint_shape(self.h, int(form), int(tile_rows), int(chunk)))
 
     def wave_run(self, state_a, state_b, K, dt, nsteps, K_field=None):
         """Fused acoustic steps on (p, u, v, w); returns the tuple holding the final state."""

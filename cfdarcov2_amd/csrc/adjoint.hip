@@ -30,8 +30,16 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "row.hpp"
 
 namespace cfd {
+
+// row form (adj_row.hip)
+template <class T> bool adj_row_usable(const cfd_mesh_s* m, const void* const* fields, int n);
+template <class T>
+void launch_wave_row_aux(const cfd_mesh_s* m, int mode, const void* const in[4], void* const out[4], double K,
+                         const void* Kf, double dt, const SensorList& sens, const row::RowAux<T>& aux,
+                         hipStream_t s);
 
 namespace {
 
@@ -218,6 +226,8 @@ SensorList make_sensors(const cfd_mesh_s* m, const int64_t* cells, int n) {
         for (int i = 0; i < j; ++i) CFD_REQUIRE(cells[i] != cells[j], "duplicate sensor cell");
         s.cell[j] = cells[j];
         s.row[j] = int(cells[j] / m->Z);
+        s.x[j] = int(cells[j] / (m->Z * m->Y));
+        s.y[j] = int((cells[j] / m->Z) % m->Y);
         s.z[j] = int(cells[j] % m->Z);
     }
     return s;
@@ -264,6 +274,8 @@ void run_wave_traces(const cfd_mesh_s* m, void* const a[4], void* const b[4], do
                      int64_t nsteps, const SensorList& sens, void* traces, void* ckpt, int64_t every,
                      hipStream_t s) {
     const size_t field = size_t(m->N) * sizeof(T);
+    const void* fields[9] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3], Kf};
+    const bool rowf = adj_row_usable<T>(m, fields, 9);
     for (int64_t t = 0; t < nsteps; ++t) {
         void* const* in = (t & 1) ? b : a;
         void* const* out = (t & 1) ? a : b;
@@ -273,9 +285,73 @@ void run_wave_traces(const cfd_mesh_s* m, void* const a[4], void* const b[4], do
                 CFD_HIP(hipMemcpyAsync(dst + f * field, in[f], field, hipMemcpyDeviceToDevice, s));
         }
         T* tr = traces ? (T*)traces + t * sens.n : nullptr;
-        launch_wave_trace_step<T>(m, (const void* const*)in, out, K, Kf, dt, sens, tr, s);
+        if (rowf) {
+            const row::RowAux<T> aux{T(0), tr, nullptr, nullptr, nullptr, nullptr};
+            launch_wave_row_aux<T>(m, 1, (const void* const*)in, out, K, Kf, dt, sens, aux, s);
+        } else {
+            launch_wave_trace_step<T>(m, (const void* const*)in, out, K, Kf, dt, sens, tr, s);
+        }
     }
     CFD_HIP(hipGetLastError());
+}
+
+// Reverse loop in row form (constant K).  With grad_K, each checkpoint
+// segment is recomputed forward in row form, keeping every step's divergence S
+// (one field per step) for the K-gradient term (lam_p dt) S_t, and the state
+// in two ping-pong buffers.
+template <class T>
+static void run_wave_adjoint_row(const cfd_mesh_s* m, void* const la[4], void* const lb[4], double K, double dt,
+                                 int64_t nsteps, const SensorList& sens, const void* tgrad, const void* ckpt,
+                                 int64_t every, void* gK, hipStream_t s) {
+    const size_t field = size_t(m->N) * sizeof(T);
+    const T dK = T(T(dt) * T(K));
+    void* ws = nullptr;     // S of each step of one segment: every fields
+    void* st = nullptr;     // recomputed state: 2 x {p, u, v, w}
+    if (gK) {
+        ws = pool_alloc(m->ctx, int64_t(every) * int64_t(field), s);
+        st = pool_alloc(m->ctx, 8 * int64_t(field), s);
+    }
+    try {
+        int64_t done = 0;
+        auto adj_step = [&](int64_t t, const void* s_in) {
+            void* const* in = (done & 1) ? lb : la;
+            void* const* out = (done & 1) ? la : lb;
+            const T* src = (tgrad && t >= 2) ? (const T*)tgrad + (t - 2) * sens.n : nullptr;
+            const row::RowAux<T> aux{dK, nullptr, nullptr, src, (const T*)s_in, s_in ? (T*)gK : nullptr};
+            launch_wave_row_aux<T>(m, 2, (const void* const*)in, out, K, nullptr, dt, sens, aux, s);
+            ++done;
+        };
+        int64_t t = nsteps;
+        while (t >= 1) {
+            if (!gK) {
+                adj_step(t, nullptr);
+                --t;
+                continue;
+            }
+            const int64_t seg = (t - 1) / every, t0 = seg * every, len = t - t0;
+            const char* ck = (const char*)ckpt + size_t(seg) * 4 * field;
+            for (int64_t k = 0; k < len; ++k) {  // state t0+k -> t0+k+1
+                const void* in[4];
+                void* out[4];
+                for (int f = 0; f < 4; ++f) {
+                    in[f] = k == 0 ? (const void*)(ck + f * field) : (const void*)((char*)st + (size_t(k & 1) * 4 + f) * field);
+                    out[f] = (char*)st + (size_t((k + 1) & 1) * 4 + f) * field;
+                }
+                const row::RowAux<T> aux{T(0), nullptr, (T*)((char*)ws + size_t(k) * field), nullptr, nullptr,
+                                         nullptr};
+                launch_wave_row_aux<T>(m, 1, in, out, K, nullptr, dt, sens, aux, s);
+            }
+            for (int64_t tt = t; tt > t0; --tt) adj_step(tt, (char*)ws + size_t(tt - t0 - 1) * field);
+            t = t0;
+        }
+        CFD_HIP(hipGetLastError());
+    } catch (...) {
+        pool_free(m->ctx, ws, s);
+        pool_free(m->ctx, st, s);
+        throw;
+    }
+    pool_free(m->ctx, ws, s);
+    pool_free(m->ctx, st, s);
 }
 
 // Reverse loop.  adj_a holds the adjoint of the final state (without the last
@@ -290,6 +366,13 @@ void run_wave_adjoint(const cfd_mesh_s* m, void* const la[4], void* const lb[4],
     const int64_t N = m->N;
     const size_t field = size_t(N) * sizeof(T);
     if (tgrad && nsteps >= 1) launch_scatter_add<T>(la[0], sens, (const T*)tgrad + (nsteps - 1) * sens.n, s);
+    {
+        const void* fields[10] = {la[0], la[1], la[2], la[3], lb[0], lb[1], lb[2], lb[3], gK, ckpt};
+        if (!Kf && adj_row_usable<T>(m, fields, 10)) {
+            run_wave_adjoint_row<T>(m, la, lb, K, dt, nsteps, sens, tgrad, ckpt, every, gK, s);
+            return;
+        }
+    }
     void* ws = nullptr;       // recomputed velocities of one segment: every x 3 fields
     void* pbuf = nullptr;     // recomputed pressures (ping-pong)
     if (gK) {

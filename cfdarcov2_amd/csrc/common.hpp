@@ -144,6 +144,13 @@ struct cfd_mesh_s {
     int euler_variant = 1;
     int euler_ty = 4;
     int euler_L = 16;
+    // adjoint of the acoustic step (cfd_wave_traces / cfd_wave_adjoint):
+    // form 0 = one thread per cell, 1 = row form with `adj_ty` output waves
+    // and `adj_L` planes per workgroup (where the mesh allows it; constant K
+    // for the transposed step)
+    int adj_form = 1;
+    int adj_ty = 4;
+    int adj_L = 8;
     // Euler stepper geometry (device block allocated on first use)
     cfd::FvGeo<float> fv32{};
     cfd::FvGeo<double> fv64{};

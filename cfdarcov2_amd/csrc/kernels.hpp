@@ -8,11 +8,13 @@
 namespace cfd {
 
 // Sensor cells of the trace-recording forward step and the adjoint step,
-// passed by value as a kernel argument (row = cell / Z, z = cell % Z).
+// passed by value as a kernel argument (row = cell / Z = x * Y + y, z = cell % Z).
 constexpr int kMaxSensors = 64;
 struct SensorList {
     int n;
     int row[kMaxSensors];
+    int x[kMaxSensors];
+    int y[kMaxSensors];
     int z[kMaxSensors];
     int64_t cell[kMaxSensors];
 };

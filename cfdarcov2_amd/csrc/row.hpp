@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
+#include "kernels.hpp"
 #include "march.hpp"
 
 namespace cfd {
@@ -75,17 +76,43 @@ __device__ inline void z_nbrs_row(const V4& c, bool zlo_wall, bool zhi_wall, V4&
     if (zhi_wall) hi[3] = c[3];
 }
 
+// Extra inputs and outputs of the row tile's trace-recording forward step
+// (MODE 1) and transposed step (MODE 2).
+template <class T> struct RowAux {
+    T dK;              // MODE 2: dt * K; the transposed step's q = dK * lam_p
+    T* trace;          // MODE 1: p' at the sensor cells (nullable)
+    T* s_out;          // MODE 1: the velocity divergence S of every cell (nullable)
+    const T* src;      // MODE 2: added to nu_p at the sensor cells (nullable)
+    const T* s_in;     // MODE 2: the forward step's S, read with gK
+    T* gK;             // MODE 2: gK += (lam_p dt) S (nullable)
+};
+
 // One (y tile, x chunk) of the single-step acoustic row form: TY output waves
 // of RW rows each (rows y0 .. y0+TY*RW-1) and two halo waves (rows y0-1 and
 // y0+TY*RW, one row each), marching x over [x0, x1).  The LDS planes hold the
 // tile's rows with the halo, row j at element j*Z.  Used by the single-mesh
-// kernel (wave_row.hip) and the batched ensemble kernel.
-template <class T, int TY, bool PW, bool KF, bool XB, bool NTL, bool MR = true>
+// kernel (wave_row.hip), the batched ensemble kernel and the adjoint kernels
+// (adj_row.hip).
+//
+// MODE 0 is the fused step.  MODE 1 is the same step that also stores p' at
+// the sensor cells and, when aux->s_out is set, the divergence S (the value
+// that multiplies dt K).  MODE 2 is the transposed step of the adjoint: with
+// (p, u, v, w) read as (lam_p, lam_u, lam_v, lam_w) and q = dK lam_p,
+//   mu_a = lam_a - G_a(q),  nu_p = dt (Dx mu_x + Dy mu_y + Dz mu_z) + lam_p,
+// which is the fused step with q in place of p, the velocity update's
+// dt * (-G) + u replaced by lam - G(q), and the pressure update's
+// dt * (K * S) + p by dt * S + lam_p (KF must be false); aux->src is added to
+// nu_p at the sensors and aux->gK, when set, gains (lam_p dt) S_t from the
+// forward step's aux->s_in.
+template <class T, int TY, bool PW, bool KF, bool XB, bool NTL, bool MR = true, int MODE = 0>
 __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p, const T* __restrict__ u,
                                 const T* __restrict__ v, const T* __restrict__ w, T* __restrict__ po,
                                 T* __restrict__ uo, T* __restrict__ vo, T* __restrict__ wo, T K,
                                 const T* __restrict__ Kf, T dt, int64_t x0, int64_t x1, int y0, int wv, bool nt,
-                                typename Row4<T>::type (*sp)[64], typename Row4<T>::type (*sv)[64]) {
+                                typename Row4<T>::type (*sp)[64], typename Row4<T>::type (*sv)[64],
+                                const SensorList* sens = nullptr, const RowAux<T>* aux = nullptr) {
+    constexpr bool ADJ = MODE == 2;
+    static_assert(!(ADJ && KF), "the transposed row step takes a constant K");
     using V4 = typename Row4<T>::type;
     const int64_t X = g.X;
     const int Y = int(g.Y), Z = int(g.Z);
@@ -124,6 +151,37 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
     const int64_t rb = int64_t(real ? y : 0) * Z + lp.zc;   // first cell of the lane's run
     const int64_t ob = int64_t(real ? orow : 0) * Z + lp.zc;
 
+    // MODE 2: pressure-slot loads become q = dK * lam_p, and the velocity
+    // update is lam - G(q)
+    T dK = T(0);
+    if constexpr (ADJ) dK = aux->dK;
+    auto qv = [&](const V4& a) -> V4 {
+        if constexpr (ADJ) {
+            V4 r;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) r[i] = dK * a[i];
+            return r;
+        } else {
+            return a;
+        }
+    };
+    auto vel = [&](T pl, T pc, T ph, T q, T A) -> T {
+        if constexpr (ADJ) return q - mggp(mlin(pc, pl), mlin(pc, ph), A, Vd);
+        else return mvel(pl, pc, ph, q, A, Vd, dt);
+    };
+    // MODE 1/2: the sensors in this output wave's rows and x chunk (bit j =
+    // sensor j; wave-uniform, built once on the scalar unit)
+    const int wy0 = y0 + wv * RW;
+    uint64_t mine = 0;
+    if constexpr (MODE != 0) {
+        const bool want = ADJ ? aux->src != nullptr : aux->trace != nullptr;
+        if (is_out && want) {
+            for (int j = 0; j < sens->n; ++j)
+                if (sens->x[j] >= x0 && sens->x[j] < x1 && sens->y[j] >= wy0 && sens->y[j] < wy0 + RW)
+                    mine |= uint64_t(1) << j;
+        }
+    }
+
     // Per-plane loads as (field, plane offset, row) slots:
     //   output cells: a = p(xx+2), b = u(xx+1), c = v(xx), d = w(xx), k = K(xx);
     //   halo cells:   a = p(xx), b = p(xx) of the outer row, c = v(xx).
@@ -131,7 +189,7 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
     const int da = out ? 2 : 0, db = out ? 1 : 0;
     const int64_t rbb = out ? rb : ob;
     struct Fetch {
-        V4 a, b, c, d, k;
+        V4 a, b, c, d, k, gk;
     };
     const V4 zero4 = {T(0), T(0), T(0), T(0)};
     // NTL: non-temporal loads (streamed past the caches' retention, which then
@@ -151,33 +209,48 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
         if constexpr (KF) {
             if (out) f.k = ld(Kf, pb + rb);
         }
+        if constexpr (ADJ) {  // the K gradient's inputs: S_t and gK
+            if (out && aux->gK) {
+                f.k = ld(aux->s_in, pb + rb);
+                f.gk = ld(aux->gK, pb + rb);
+            }
+        }
     };
 
     // ---- warm-up of the output cells' x queue ---------------------------------
     V4 P0 = zero4, Pp1 = zero4, Um1 = zero4, U0 = zero4;
+    V4 R0 = zero4, Rp1 = zero4;  // MODE 2: lam_p(x), lam_p(x+1) as loaded (P0, Pp1 hold q)
     if (out) {
         V4 Pm1 = zero4;
         P0 = ld(p, x0 * SX + rb);
         if (x0 >= 1) Pm1 = ld(p, (x0 - 1) * SX + rb);
         if (x0 + 1 < X) Pp1 = ld(p, (x0 + 1) * SX + rb);
+        if constexpr (ADJ) {
+            R0 = P0;
+            Rp1 = Pp1;
+            P0 = qv(P0);
+            Pp1 = qv(Pp1);
+            Pm1 = qv(Pm1);
+        }
         if (x0 >= 1) {
-            const V4 Pm2 = (x0 >= 2) ? ld(p, (x0 - 2) * SX + rb) : Pm1;
+            const V4 Pm2 = (x0 >= 2) ? qv(ld(p, (x0 - 2) * SX + rb)) : Pm1;
             const V4 um = ld(u, (x0 - 1) * SX + rb);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) Um1[i] = mvel(Pm2[i], Pm1[i], P0[i], um[i], Ax, Vd, dt);
+            for (int i = 0; i < 4; ++i) Um1[i] = vel(Pm2[i], Pm1[i], P0[i], um[i], Ax);
         }
         const V4 uc = ld(u, x0 * SX + rb);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            U0[i] = mvel(x0 >= 1 ? Pm1[i] : P0[i], P0[i], x0 + 1 < X ? Pp1[i] : P0[i], uc[i], Ax, Vd, dt);
+            U0[i] = vel(x0 >= 1 ? Pm1[i] : P0[i], P0[i], x0 + 1 < X ? Pp1[i] : P0[i], uc[i], Ax);
     }
 
-    Fetch nx{zero4, zero4, zero4, zero4, zero4};
+    Fetch nx{zero4, zero4, zero4, zero4, zero4, zero4};
     fetch(x0, nx);
 
     for (int64_t x = x0; x < x1; ++x) {
         const Fetch cur = nx;
         if (x + 1 < x1) fetch(x + 1, nx);
+        const V4 Qa = qv(cur.a);  // p(x+2) (output cells) or p(x) (halo cells), as q in MODE 2
 
         // 1. u'(x+1); p into the LDS plane
         V4 Up1 = zero4;
@@ -185,11 +258,11 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
             if (!XB || x + 1 < X) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    Up1[i] = mvel(P0[i], Pp1[i], (!XB || x + 2 < X) ? cur.a[i] : Pp1[i], cur.b[i], Ax, Vd, dt);
+                    Up1[i] = vel(P0[i], Pp1[i], (!XB || x + 2 < X) ? Qa[i] : Pp1[i], cur.b[i], Ax);
             }
             if (lp.span) SP[own] = P0;
         } else if (halo_v) {
-            SP[own] = cur.a;
+            SP[own] = Qa;
         }
         __syncthreads();
 
@@ -202,17 +275,18 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
             z_nbrs_row(P0, zlo, zhi, pzl, pzh);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                vn[i] = mvel(pyl[i], P0[i], pyh[i], cur.c[i], Ay, Vd, dt);
-                wn[i] = mvel(pzl[i], P0[i], pzh[i], cur.d[i], Az, Vd, dt);
+                vn[i] = vel(pyl[i], P0[i], pyh[i], cur.c[i], Ay);
+                wn[i] = vel(pzl[i], P0[i], pzh[i], cur.d[i], Az);
             }
             if (lp.span) SV[own] = vn;
         } else if (halo_v) {
             const V4 in = SP[iin];
-            const V4 pl = lower ? cur.b : in;
-            const V4 ph = lower ? in : cur.b;
+            const V4 Qb = qv(cur.b);
+            const V4 pl = lower ? Qb : in;
+            const V4 ph = lower ? in : Qb;
             V4 vh;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) vh[i] = mvel(pl[i], cur.a[i], ph[i], cur.c[i], Ay, Vd, dt);
+            for (int i = 0; i < 4; ++i) vh[i] = vel(pl[i], Qa[i], ph[i], cur.c[i], Ay);
             SV[own] = vh;
         }
         __syncthreads();
@@ -223,7 +297,7 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
             const V4 vyh = SV[ihi];
             V4 wzl, wzh;
             z_nbrs_row(wn, zlo, zhi, wzl, wzh);
-            V4 pn;
+            V4 pn, Sv;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const T fx_lo = (XB && x == 0) ? T(0) : mlin(U0[i], Um1[i]);
@@ -235,18 +309,53 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
                 T S = mggp(fx_lo, fx_hi, Ax, Vd);
                 S = S + mggp(fy_lo, fy_hi, Ay, Vd);
                 S = S + mggp(fz_lo, fz_hi, Az, Vd);
-                const T k = KF ? cur.k[i] : K;
-                pn[i] = dt * (k * S) + P0[i];
+                Sv[i] = S;
+                if constexpr (ADJ) {
+                    pn[i] = dt * S + R0[i];
+                } else {
+                    const T k = KF ? cur.k[i] : K;
+                    pn[i] = dt * (k * S) + P0[i];
+                }
+            }
+            const int64_t o = x * SX + rb;
+            if constexpr (MODE != 0) {
+                // sensors of this plane: record p' (MODE 1) or add the trace
+                // gradient (MODE 2) in the lane that holds the cell
+                uint64_t m = mine;
+                while (m) {
+                    const int j = __builtin_ctzll(m);
+                    m &= m - 1;
+                    if (sens->x[j] != x) continue;
+                    const int zz = sens->z[j];
+                    if (out && lp.r == sens->y[j] - wy0 && zz >= lp.zc && zz < lp.zc + 4) {
+                        const int i = zz - lp.zc;
+                        if constexpr (ADJ) pn[i] = pn[i] + aux->src[j];
+                        else aux->trace[j] = pn[i];
+                    }
+                }
+                if constexpr (ADJ) {
+                    if (out && aux->gK) {
+                        V4 gk;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) gk[i] = cur.gk[i] + (R0[i] * dt) * cur.k[i];
+                        *reinterpret_cast<V4*>(aux->gK + o) = gk;
+                    }
+                } else {
+                    if (out && aux->s_out) *reinterpret_cast<V4*>(aux->s_out + o) = Sv;
+                }
             }
             if (out) {
-                const int64_t o = x * SX + rb;
                 put(reinterpret_cast<V4*>(po + o), pn, nt);
                 put(reinterpret_cast<V4*>(uo + o), U0, nt);
                 put(reinterpret_cast<V4*>(vo + o), vn, nt);
                 put(reinterpret_cast<V4*>(wo + o), wn, nt);
             }
             P0 = Pp1;
-            Pp1 = cur.a;
+            Pp1 = Qa;
+            if constexpr (ADJ) {
+                R0 = Rp1;
+                Rp1 = cur.a;
+            }
             Um1 = U0;
             U0 = Up1;
         }

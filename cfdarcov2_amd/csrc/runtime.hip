@@ -295,6 +295,7 @@ static void hex_table(int64_t X_, int64_t Y_, int64_t Z_, double dx, double dy, 
 
 static bool valid_table(int t) { return t >= CFD_T_IDS && t <= CFD_T_FV_VOL; }
 
+
 }  // namespace cfd
 
 using namespace cfd;
@@ -487,14 +488,14 @@ int cfd_stab(cfd_mesh_t m, const void* phi, void* out, int mask, void* stream) {
         hipStream_t s = as_stream(stream);
         // the first stage's gradients go through 3N pool entries on general
         // meshes; uniform meshes recompute them in one pass
-        void* scratch = m->uniform ? nullptr : pool_alloc(m->ctx, 3 * m->N * elem_size(m->dtype), s);
+        void* work = m->uniform ? nullptr : pool_alloc(m->ctx, 3 * m->N * elem_size(m->dtype), s);
         try {
-            DISPATCH(m, launch_stab, m, phi, out, mask, scratch, s);
+            DISPATCH(m, launch_stab, m, phi, out, mask, work, s);
         } catch (...) {
-            if (scratch) pool_free(m->ctx, scratch, s);
+            if (work) pool_free(m->ctx, work, s);
             throw;
         }
-        if (scratch) pool_free(m->ctx, scratch, s);
+        if (work) pool_free(m->ctx, work, s);
     });
 }
 
@@ -525,8 +526,8 @@ int cfd_cfl_min(cfd_ctx_t ctx, int dtype, const void* u, const void* v, const vo
         hipStream_t s = as_stream(stream);
         const int nb = int(std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 2048));
         const int es = elem_size(dtype);
-        void* scratch = pool_alloc(ctx, int64_t(nb + 1) * es, s);
-        char* base = (char*)scratch;
+        void* work = pool_alloc(ctx, int64_t(nb + 1) * es, s);
+        char* base = (char*)work;
         double host = 0;
         try {
             if (dtype == CFD_F32) {
@@ -543,10 +544,10 @@ int cfd_cfl_min(cfd_ctx_t ctx, int dtype, const void* u, const void* v, const vo
                 host = cfl * r;
             }
         } catch (...) {
-            pool_free(ctx, scratch, s);
+            pool_free(ctx, work, s);
             throw;
         }
-        pool_free(ctx, scratch, s);
+        pool_free(ctx, work, s);
         *dt_out = host;
     });
 }
@@ -585,18 +586,18 @@ int cfd_count_nonfinite(cfd_ctx_t ctx, int dtype, const void* x, int64_t n, int6
         CFD_REQUIRE(ctx && x && bad, "cfd_count_nonfinite: null argument");
         check_dtype(dtype);
         hipStream_t s = as_stream(stream);
-        void* scratch = pool_alloc(ctx, 1025 * 8, s);
+        void* work = pool_alloc(ctx, 1025 * 8, s);
         unsigned long long total = 0;
         try {
-            if (dtype == CFD_F32) launch_nonfinite<float>(x, n, (unsigned long long*)scratch, s);
-            else launch_nonfinite<double>(x, n, (unsigned long long*)scratch, s);
-            CFD_HIP(hipMemcpyAsync(&total, (char*)scratch + 1024 * 8, 8, hipMemcpyDeviceToHost, s));
+            if (dtype == CFD_F32) launch_nonfinite<float>(x, n, (unsigned long long*)work, s);
+            else launch_nonfinite<double>(x, n, (unsigned long long*)work, s);
+            CFD_HIP(hipMemcpyAsync(&total, (char*)work + 1024 * 8, 8, hipMemcpyDeviceToHost, s));
             CFD_HIP(hipStreamSynchronize(s));
         } catch (...) {
-            pool_free(ctx, scratch, s);
+            pool_free(ctx, work, s);
             throw;
         }
-        pool_free(ctx, scratch, s);
+        pool_free(ctx, work, s);
         *bad = int64_t(total);
     });
 }
@@ -703,6 +704,22 @@ int cfd_mesh_set_heat_shape(cfd_mesh_t m, int variant, int waves, int chunk) {
         m->heat_variant = variant;
         m->heat_nw = waves;
         m->heat_L = chunk;
+    });
+}
+
+int cfd_mesh_set_adjoint_shape(cfd_mesh_t m, int form, int tile_rows, int chunk) {
+    return guarded([&] {
+        CFD_REQUIRE(m != nullptr, "null mesh");
+        CFD_REQUIRE(form == 0 || form == 1, "adjoint form must be 0 or 1");
+        CFD_REQUIRE(tile_rows == 4 || tile_rows == 8, "adjoint row tiles have 4 or 8 output waves");
+        CFD_REQUIRE(chunk >= 1 && chunk <= 65535, "adjoint chunk must be in [1, 65535]");
+        if (form == 1 && !(m->uniform && wave_row_supported(m->Z)))
+            throw Error(CFD_E_UNSUPPORTED,
+                        "the adjoint row form needs an exactly uniform mesh whose z extent is a multiple of 4 and "
+                        "at most 256");
+        m->adj_form = form;
+        m->adj_ty = tile_rows;
+        m->adj_L = chunk;
     });
 }
 

@@ -236,6 +236,18 @@ int cfd_wave_adjoint(cfd_mesh_t mesh, void* const adj_a[4], void* const adj_b[4]
                      const void* K_field, double dt, int64_t nsteps, const int64_t* sensor_cells, int nsensors,
                      const void* trace_grad, const void* checkpoints, int64_t every, void* grad_K, void* stream);
 
+/* Kernel form of cfd_wave_traces / cfd_wave_adjoint (no reference
+ * counterpart): form 0 = one thread per cell; form 1 = row form (uniform
+ * meshes whose z extent is a multiple of 4 and at most 256, else
+ * CFD_E_UNSUPPORTED) with `tile_rows` (4 or 8) output waves and `chunk`
+ * x-planes per workgroup.  The row form's forward records the traces in the
+ * fused step itself; its transposed step needs a constant K (runs with a K
+ * field take form 0 for the transposed steps) and its K gradient keeps one
+ * divergence field per recomputed step instead of three velocity fields.  Both
+ * forms give bit-identical traces and gradients.  Uniform meshes that allow it
+ * start on form 1 (4 waves, 8 planes). */
+int cfd_mesh_set_adjoint_shape(cfd_mesh_t mesh, int form, int tile_rows, int chunk);
+
 /* ---- batched ensemble (FP32 uniform meshes; no reference counterpart) -----
  * Many independent heat / acoustic cases advance together: one launch per
  * step covers every case's tiles, and cfd_batch_run(use_graph = 1) replays n

@@ -168,3 +168,73 @@ def test_K_gradient_with_recompute_against_finite_differences(hip_ctx):
         fd = (_loss(pr, p0, obs, K0 + h * e) - _loss(pr, p0, obs, K0 - h * e)) / (2 * h)
         dd = float(torch.dot(gK, e))
         assert fd == pytest.approx(dd, rel=1e-6), k
+
+
+ROW_CASES = [((20, 18, 64), (4, 8)), ((13, 11, 256), (8, 3)), ((9, 30, 32), (4, 1)), ((17, 5, 4), (8, 16)),
+             ((6, 7, 128), (4, 2))]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n,shape", ROW_CASES)
+def test_row_form_equals_per_thread_form(hip_ctx, dtype, n, shape):
+    """The row-form forward (traces recorded in the fused step) and transposed
+    step (trace gradients added in the step, K gradient from one divergence
+    field per recomputed step) against the per-thread kernels, bit for bit:
+    traces, checkpoints, dL/dp0 and dL/dK, constant K, with sensors on the
+    walls, the corners and chunk borders, an uneven last checkpoint segment
+    and an odd step count."""
+    from cfdarcov2_amd import adjoint
+    N = n[0] * n[1] * n[2]
+    d = (1 / 32,) * 3
+    rng = np.random.default_rng(N)
+    X, Y, Z = n
+    cid = lambda x, y, z: (x * Y + y) * Z + z
+    fixed = {0, N - 1, cid(X - 1, 0, Z - 1), cid(0, Y - 1, 0), cid(min(shape[1], X - 1), Y // 2, Z // 2),
+             cid(X // 2, Y - 1, Z - 1), cid(X - 1, Y - 1, 3 % Z)}
+    sens = sorted(fixed | set(rng.choice(N, 30, replace=False).tolist()))[:64]
+    steps, dt, K = 11, 0.4 / 32, -1.3
+    p0 = torch.from_numpy(rng.standard_normal(N).astype(dtype)).cuda()
+    res = {}
+    for form in (0, 1):
+        print(f"\n[case] adjoint row vs per-thread n={n} shape={shape} dtype={dtype.__name__} form={form}", flush=True)
+        pr = _problem(hip_ctx, n, d, dt, steps, sens, every=4, dtype=dtype, K=K)
+        pr.mesh.set_adjoint_shape(form, *shape)
+        tr, final, ck = adjoint.forward_traces(pr, p0, checkpoints=True)
+        g = torch.from_numpy(rng.standard_normal(tuple(tr.shape)).astype(dtype)).cuda() if form == 0 else res[0][4]
+        gp, gK = adjoint.adjoint(pr, g, None, ck, want_K=True)
+        gp2, _ = adjoint.adjoint(pr, g)      # no K gradient: no recomputation
+        torch.cuda.synchronize()
+        res[form] = (tr, [t.clone() for t in final], ck, gp.clone(), g, gK, gp2)
+        pr.mesh.close()
+    a, b = res[0], res[1]
+    assert torch.equal(a[0], b[0])
+    for x, y in zip(a[1], b[1]):
+        assert torch.equal(x, y)
+    assert torch.equal(a[2], b[2])
+    assert torch.equal(a[3], b[3])
+    assert torch.equal(a[5], b[5])
+    assert torch.equal(a[6], b[6]) and torch.equal(a[3], a[6])
+
+
+def test_constant_K_gradient_equals_K_field_gradient(hip_ctx):
+    """dL/dK at a constant K through the row form (constant-K transposed
+    steps) equals the gradient at the equal K field (per-thread transposed
+    steps with a K field), bit for bit."""
+    from cfdarcov2_amd import adjoint
+    n, d = (24, 20, 32), (1 / 32,) * 3
+    N = n[0] * n[1] * n[2]
+    rng = np.random.default_rng(3)
+    sens = rng.choice(N, 20, replace=False)
+    K = -0.9
+    pr = _problem(hip_ctx, n, d, 0.4 / 32, 16, sens, every=5, dtype=np.float64, K=K)
+    p0 = torch.from_numpy(rng.standard_normal(N)).cuda()
+    Kf = torch.full((N,), K, dtype=torch.float64, device="cuda")
+    print("\n[case] constant-K gradient vs K-field gradient", flush=True)
+    tr, _, ck = adjoint.forward_traces(pr, p0, checkpoints=True)
+    trf, _, ckf = adjoint.forward_traces(pr, p0, Kf, checkpoints=True)
+    assert torch.equal(tr, trf) and torch.equal(ck, ckf)
+    g = torch.from_numpy(rng.standard_normal(tuple(tr.shape))).cuda()
+    gp, gK = adjoint.adjoint(pr, g, None, ck, want_K=True)
+    gpf, gKf = adjoint.adjoint(pr, g, Kf, ckf, want_K=True)
+    torch.cuda.synchronize()
+    assert torch.equal(gp, gpf) and torch.equal(gK, gKf)
