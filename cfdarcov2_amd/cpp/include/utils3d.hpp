@@ -18,14 +18,14 @@ inline argparse::ArgumentParser parse_args_base(int argc, char** argv) {
     argparse::ArgumentParser program("cfdARCO");
     program.add_argument("-v", "--visualize").default_value(false).implicit_value(true);
     program.add_argument("--create_plot").default_value(false).implicit_value(true);
-    program.add_argument("-Lx").help("square size").default_value(200).scan<'i', int>();
-    program.add_argument("-Ly").help("square size").default_value(200).scan<'i', int>();
-    program.add_argument("-Lz").help("square size").default_value(200).scan<'i', int>();
-    program.add_argument("-dx").help("square size").default_value(1.0).scan<'g', float>();
-    program.add_argument("-dy").help("square size").default_value(1.0).scan<'g', float>();
-    program.add_argument("-dz").help("square size").default_value(1.0).scan<'g', float>();
-    program.add_argument("-t", "--timesteps").help("timesteps").default_value(1000).scan<'i', int>();
-    program.add_argument("-dt").help("dt").default_value(0.005).scan<'g', float>();
+    program.add_argument("-Lx").help("cells along x").default_value(200).scan<'i', int>();
+    program.add_argument("-Ly").help("cells along y").default_value(200).scan<'i', int>();
+    program.add_argument("-Lz").help("cells along z").default_value(200).scan<'i', int>();
+    program.add_argument("-dx").help("cell spacing along x").default_value(1.0).scan<'g', float>();
+    program.add_argument("-dy").help("cell spacing along y").default_value(1.0).scan<'g', float>();
+    program.add_argument("-dz").help("cell spacing along z").default_value(1.0).scan<'g', float>();
+    program.add_argument("-t", "--timesteps").help("number of time steps").default_value(1000).scan<'i', int>();
+    program.add_argument("-dt").help("time step, or the CFL number where a driver sets a CFL policy").default_value(0.005).scan<'g', float>();
     program.add_argument("-c", "--cuda_enable").default_value(false).implicit_value(true);
     program.add_argument("-hip", "--hip_enable").default_value(false).implicit_value(true);
     program.add_argument("--cuda_ranks").default_value(1).scan<'i', int>();
@@ -88,7 +88,7 @@ public:
         store_stepping = program.get<bool>("store_stepping");
         store_last = program.get<bool>("store_last");
         store = program.get<bool>("store");
-        if (cuda_enable && hip_enable) throw std::runtime_error{"Can`t run HIP and CUDA simultaneously"};
+        if (cuda_enable && hip_enable) throw std::runtime_error{"-c/--cuda_enable and -hip/--hip_enable exclude each other"};
         if (!cuda_enable && !hip_enable)
             throw std::runtime_error{
                 "this build evaluates on the MI355X only: pass -c/--cuda_enable or -hip/--hip_enable"};
@@ -104,17 +104,30 @@ public:
         dynamic_cast<CudaMesh3D*>(mesh.get())->host_to_device();
     }
 
-    void init_store(const std::vector<Variable*>& vars_to_store) { store_vars = vars_to_store; }
+    // With -s, the stored variables' history goes to .vti files at finalize();
+    // with -s -st, mesh.json is written here and every step's state to .bin
+    // files as the run goes (the reference's stepping writers,
+    // include/io_operators_tmpl.hpp:105-159, whose calls it leaves commented
+    // out at include/utils3d.hpp:182,197-198 and src/fvm3d.cpp:1057)
+    void init_store(const std::vector<Variable*>& vars_to_store) {
+        store_vars = vars_to_store;
+        if (store && store_stepping) init_store_history_stepping(store_vars, mesh.get());
+    }
 
     void finalize() {
-        if (store && !store_stepping) store_history(store_vars, mesh.get());
+        if (store) {
+            if (store_stepping) finalize_history_stepping();
+            else store_history(store_vars, mesh.get());
+        }
         CFDArcoGlobalInit::finalize();
     }
 };
 
+// Stores val at cell (x, y, z) of a field in the reference cell order: `rows`
+// is the z extent and `cols` the y extent, so the cell id is
+// (x * cols + y) * rows + z.
 inline void sett(Eigen::Matrix<float, -1, 1>& v, int rows, int cols, int x, int y, int z, float val) {
-    const size_t idx = x * rows * cols + y * rows + z;
-    v[idx] = val;
+    v[(size_t(x) * size_t(cols) + size_t(y)) * size_t(rows) + size_t(z)] = val;
 }
 
 template <typename MeshClass>

@@ -1,6 +1,7 @@
 // Variable / operator expression API and the time loop of the explicit
 // finite-volume path (src/fvm3d.cpp of the reference), evaluated on the MI355X.
 #include "fvm3d.hpp"
+#include "io_operators_tmpl.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -618,6 +619,9 @@ void Equation::evaluate(std::vector<Variable*>& all_vars, EquationTemplate& equa
             }
             var->add_history();
         }
+        // -s -st: the stored variables' state after this step, to
+        // <store_dir>/<var>/<t>.bin (asynchronous device-to-host snapshot)
+        if (cfdarco::io::stepping_active()) cfdarco::io::stepping_snapshot(int(t));
         if (visualize && CFDArcoGlobalInit::get_rank() == 0) {
             const int pct = int(100.0 * double(t + 1) / double(timesteps));
             if (pct != last_pct) {

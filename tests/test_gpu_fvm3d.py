@@ -172,7 +172,7 @@ def _sum(a):
     return np.sum(a, dtype=np.float64)
 
 
-def _run_driver(name, args, per_op=False, stab_in_kernel=False):
+def _run_driver(name, args, per_op=False, stab_in_kernel=False, cwd=None):
     path = os.path.join(DRIVERS, name)
     if not os.path.exists(path):
         pytest.skip(f"{name}: reference driver binaries are built only where the reference sources exist")
@@ -185,8 +185,11 @@ def _run_driver(name, args, per_op=False, stab_in_kernel=False):
         env["CFDARCO_PER_OP"] = "1"
     if stab_in_kernel:
         env["CFDARCO_STAB_IN_KERNEL"] = "1"
-    with tempfile.TemporaryDirectory() as tmp:
-        res = subprocess.run([path, *args], capture_output=True, text=True, timeout=300, cwd=tmp, env=env)
+    if cwd is not None:
+        res = subprocess.run([path, *args], capture_output=True, text=True, timeout=300, cwd=cwd, env=env)
+    else:
+        with tempfile.TemporaryDirectory() as tmp:
+            res = subprocess.run([path, *args], capture_output=True, text=True, timeout=300, cwd=tmp, env=env)
     assert res.returncode == 0, res.stdout + res.stderr
     return res.stdout
 
@@ -213,6 +216,52 @@ def test_reference_heat_driver_unchanged(oracle_lib):
     assert float(m.group(1)) == pytest.approx(float(_sum(T)), rel=1e-5)
     assert float(m.group(2)) == pytest.approx(float(T.max()), rel=1e-5)
     assert "Time difference" in out
+
+
+@pytest.mark.parametrize("stepping", [False, True])
+def test_reference_heat_driver_store(oracle_lib, stepping):
+    """examples/main_3d_heat.cpp with -s (history as <var>/res_<i>.vti at the
+    end, every entry but the last) and with -s -st (mesh.json first, then
+    <var>/<i>.bin after every step, written from asynchronous snapshots):
+    every file read back equals the oracle's state after that many steps, bit
+    for bit, and dumps/run_latest points at the run."""
+    import base64
+    import json
+    n, d, steps, dt = (12, 10, 16), (1.0, 1.0, 1.0), 9, 0.01
+    X, Y, Z = n
+    om = _mesh(oracle_lib, n, d)
+    Xc, Yc, Zc = workloads.cell_centres(*n, *d)
+    T0 = np.where(Zc.astype(f32) < f32(2) * f32(1.0), f32(100), f32(0)).astype(f32)
+    bnd = om.table("is_bnd").astype(f32)
+    rev = f32(1) - bnd
+    states, T, dtf = [], T0.copy(), f32(dt)
+    for _ in range(steps):
+        T = dtf * (f32(3.0) * om.lapl(T)) + T
+        T = T * rev + T0 * bnd
+        states.append(T.copy())
+    args = ["-Lx", str(X), "-Ly", str(Y), "-Lz", str(Z), "-t", str(steps), "-dt", str(dt), "-c", "-s"]
+    with tempfile.TemporaryDirectory() as tmp:
+        _run_driver("cfdARCO_heat", args + (["-st"] if stepping else []), cwd=tmp)
+        latest = os.path.join(tmp, "dumps", "run_latest")
+        assert os.path.islink(latest)
+        run = os.path.realpath(latest)
+        assert os.path.basename(run).startswith("run_")
+        tdir = os.path.join(run, "T")
+        if stepping:
+            with open(os.path.join(run, "mesh.json")) as f:
+                mj = json.load(f)
+            assert (mj["x"], mj["y"], mj["z"]) == n and len(mj["nodes"]) == X * Y * Z
+            assert sorted(os.listdir(tdir)) == sorted(f"{i}.bin" for i in range(steps))
+            for i in range(steps):
+                got = np.fromfile(os.path.join(tdir, f"{i}.bin"), dtype=f32)
+                assert np.array_equal(got, states[i]), i
+        else:
+            assert sorted(os.listdir(tdir)) == sorted(f"res_{i}.vti" for i in range(steps - 1))
+            for i in range(steps - 1):
+                with open(os.path.join(tdir, f"res_{i}.vti")) as f:
+                    blob = re.search(r'format="binary">\s*(\S+)\s*</DataArray>', f.read()).group(1)
+                img = np.frombuffer(base64.b64decode(blob[12:]), dtype=f32).reshape(Z, Y, X)
+                assert np.array_equal(img.transpose(2, 1, 0).ravel(), states[i]), i
 
 
 def _printed(out):
