@@ -110,18 +110,39 @@ def build_test_programs(verbose: bool = False) -> list:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile the library; returns the path of the shared object."""
+    """Compile the library; returns the path of the shared object.  Each
+    translation unit compiles to its own object in parallel, then one link."""
     if not os.path.exists(HIPCC) and shutil.which("hipcc") is None:
         raise RuntimeError("hipcc not found; the HIP library cannot be built")
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(BUILD_DIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+
+    def compile_one(src):
+        obj = os.path.join(objdir, os.path.splitext(os.path.basename(src))[0] + ".o")
+        cmd = [HIPCC, *HIP_FLAGS, "-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        return obj, res
+
+    workers = max(1, min(len(sources()), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(workers) as pool:
+        results = list(pool.map(compile_one, sources()))
+    failed = [r for _, r in results if r.returncode != 0]
+    if failed:
+        for r in failed:
+            sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libcfdarco_hip.so")
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *sources(),
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *[o for o, _ in results],
            f"-Wl,-rpath,{ROCM_LLVM_LIB}", "-lhiprtc"]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libcfdarco_hip.so")
+        raise RuntimeError("hipcc failed linking libcfdarco_hip.so")
     os.replace(tmp, LIB)
     return LIB
 

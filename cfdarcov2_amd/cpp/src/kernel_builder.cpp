@@ -657,9 +657,12 @@ void KernelBuilder::run() {
     // workgroups are dealt round-robin to the 8 XCDs; with a multiple of 8
     // per x-plane the same (y, z) tile of every plane runs on one XCD, whose
     // L2 then already holds the x-neighbours (rows past Y exit at once)
+    // (padding is skipped where it would push gridDim.y past 65535)
     const uint32_t gz = uint32_t((I.geo.Z + 63) / 64);
     uint32_t gy = uint32_t((I.geo.Y + 3) / 4);
-    while ((uint64_t(gz) * gy) % 8 != 0) ++gy;
+    uint32_t gyp = gy;
+    while ((uint64_t(gz) * gyp) % 8 != 0) ++gyp;
+    if (gyp <= 65535u) gy = gyp;
     const uint32_t grid[3] = {gz, gy, uint32_t(I.geo.X)};
     const uint32_t block[3] = {64, 4, 1};
     cfdarco::check(cfd_program_launch(I.prog, grid, block, args.data(), nullptr), "KernelBuilder::run");

@@ -199,6 +199,10 @@ struct cfd_batch_s {
     int64_t total_tiles1 = 0;
     bool dirty = true;
     hipStream_t capture = nullptr;
+    // stream of the last cfd_batch_run: destroy waits on it before freeing the
+    // case descriptors its kernels read
+    hipStream_t last_run = nullptr;
+    bool ran = false;
     // instantiated graphs keyed by (steps, tile rows, chunk)
     std::map<std::tuple<int64_t, int, int>, hipGraphExec_t> graphs;
 };
@@ -336,6 +340,9 @@ int cfd_batch_create(cfd_ctx_t ctx, const cfd_batch_case* cases, int ncases, cfd
         b->dev = (BatchCaseDev*)pool_alloc(ctx, int64_t(ncases) * int64_t(sizeof(BatchCaseDev)), nullptr);
         try {
             batch_layout(b.get(), nullptr);
+            // the descriptors are written on the null stream and read on the
+            // caller's stream, which may be a non-blocking one
+            CFD_HIP(hipStreamSynchronize(nullptr));
             CFD_HIP(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
         } catch (...) {
             pool_free(ctx, b->dev, nullptr);
@@ -363,9 +370,12 @@ int cfd_batch_set_form(cfd_batch_t b, int form) {
             if (form == 2) {
                 if (b->tile_rows != 8 && b->tile_rows != 10 && b->tile_rows != 12) b->tile_rows = 12;
                 check_form2(b->host, b->tile_rows);
-                if (!b->dev1)
+                if (!b->dev1) {
                     b->dev1 = (BatchCaseDev*)pool_alloc(b->ctx, int64_t(b->host.size()) * int64_t(sizeof(BatchCaseDev)),
                                                         nullptr);
+                    // allocated on the null stream, first written on the run's
+                    CFD_HIP(hipStreamSynchronize(nullptr));
+                }
             }
         }
         if (form != b->form) {
@@ -399,6 +409,9 @@ int cfd_batch_run(cfd_batch_t b, int64_t nsteps, int use_graph, void* stream) {
         CFD_REQUIRE(b != nullptr, "null batch");
         CFD_REQUIRE(nsteps >= 0, "cfd_batch_run: negative step count");
         hipStream_t s = as_stream(stream);
+        if (b->ran && b->last_run != s) CFD_HIP(hipStreamSynchronize(b->last_run));
+        b->last_run = s;
+        b->ran = true;
         if (b->dirty) {  // new tile shape: new layout, and graphs of the old one are stale
             CFD_HIP(hipStreamSynchronize(s));
             drop_graphs(b);
@@ -438,6 +451,8 @@ int cfd_batch_run(cfd_batch_t b, int64_t nsteps, int use_graph, void* stream) {
 int cfd_batch_destroy(cfd_batch_t b) {
     return guarded([&] {
         if (!b) return;
+        // in-flight kernels of the last run still read the descriptors
+        if (b->ran) CFD_HIP(hipStreamSynchronize(b->last_run));
         drop_graphs(b);
         if (b->capture) (void)hipStreamDestroy(b->capture);
         pool_free(b->ctx, b->dev, nullptr);
