@@ -39,7 +39,7 @@ EXPORTS = [
     "cfd_wave_traces", "cfd_wave_adjoint",
     "cfd_batch_create", "cfd_batch_set_form", "cfd_batch_set_shape", "cfd_batch_run", "cfd_batch_destroy",
     "cfd_program_compile", "cfd_program_compile_only", "cfd_program_launch", "cfd_program_destroy",
-    "cfd_mesh_geometry",
+    "cfd_mesh_geometry", "cfd_stab_noalloc",
 ]
 
 
@@ -89,6 +89,7 @@ def load_library(path: str = LIB_PATH):
         "cfd_mesh_get_info": [vp, ctypes.POINTER(MeshInfo)], "cfd_mesh_table": [vp, i, vp],
         "cfd_hex_table": [ctypes.POINTER(HexDesc), i, vp],
         "cfd_grad": [vp, vp, vp, i, i, vp], "cfd_lapl": [vp, vp, vp, i, vp], "cfd_stab": [vp, vp, vp, i, vp],
+        "cfd_stab_noalloc": [vp, vp, vp, i, vp],
         "cfd_ewise": [vp, i, i, vp, vp, vp, d, vp, i64, vp],
         "cfd_cfl_min": [vp, i, vp, vp, vp, vp, vp, i64, d, d, d, ctypes.POINTER(ctypes.c_double), vp],
         "cfd_mesh_geometry": [vp, vp],
@@ -309,6 +310,14 @@ class HexMesh:
         self._check_field(phi)
         out = torch.empty_like(phi) if out is None else out
         check(self.lib.cfd_stab(self.h, _dptr(phi), _dptr(out), int(axis_mask), _stream_ptr()))
+        return out
+
+    def stab_noalloc(self, phi, out, axis_mask=7):
+        """cfd_stab_noalloc: the stabilisation into a caller-given `out`, with
+        the library's fixed per-device workspace instead of a pool block."""
+        self._check_field(phi)
+        self._check_field(out)
+        check(self.lib.cfd_stab_noalloc(self.h, _dptr(phi), _dptr(out), int(axis_mask), _stream_ptr()))
         return out
 
     def cfl_min(self, u, p, rho, v=None, w=None, cfl=1.0, gamma=5.0 / 3.0) -> float:

@@ -295,6 +295,38 @@ static void hex_table(int64_t X_, int64_t Y_, int64_t Z_, double dx, double dy, 
 
 static bool valid_table(int t) { return t >= CFD_T_IDS && t <= CFD_T_FV_VOL; }
 
+// ---------------------------------------------------------------------------
+// Fixed workspace of cfd_stab_noalloc: the first stage's three gradient fields
+// of the two-launch stabilisation on general meshes (3 N entries of the mesh
+// dtype).  Each GPU that loads the library holds its own copy until the
+// process ends.  The array's name is a host-side stand-in, never passed to a
+// kernel or a copy: its device address is looked up once per device, in
+// cfd_init with that device current, and only that address is used.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocBytes = int64_t(48) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[kMaxDevices] = {};
+
+// Resolves and records `scratch`'s address on `device` (the current device).
+static void resolve_scratch(int device) {
+    CFD_REQUIRE(device >= 0 && device < kMaxDevices, "device index beyond the workspace table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_addr[device]) return;
+    void* ptr = nullptr;
+    CFD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    CFD_REQUIRE(ptr != nullptr, "hipGetSymbolAddress returned a null workspace address");
+    g_scratch_addr[device] = ptr;
+}
+
+static void* scratch_addr(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    void* p = (device >= 0 && device < kMaxDevices) ? g_scratch_addr[device] : nullptr;
+    CFD_REQUIRE(p != nullptr, "workspace address not resolved for this device (cfd_init not called on it)");
+    return p;
+}
 
 }  // namespace cfd
 
@@ -325,6 +357,7 @@ int cfd_init(int device, cfd_ctx_t* out) {
         hipDeviceProp_t prop;
         CFD_HIP(hipGetDeviceProperties(&prop, device));
         ctx->num_cu = prop.multiProcessorCount;
+        resolve_scratch(device);
         *out = ctx.release();
     });
 }
@@ -496,6 +529,27 @@ int cfd_stab(cfd_mesh_t m, const void* phi, void* out, int mask, void* stream) {
             throw;
         }
         if (work) pool_free(m->ctx, work, s);
+    });
+}
+
+int cfd_stab_noalloc(cfd_mesh_t m, const void* phi, void* out, int mask, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(m && phi && out, "cfd_stab_noalloc: null argument");
+        CFD_REQUIRE(mask >= 0 && mask <= 7, "cfd_stab_noalloc: bad axis_mask");
+        void* work = nullptr;
+        if (!m->uniform) {
+            // the kernels write all 3 N entries before reading any, so the
+            // previous call's contents need no clearing
+            const int64_t need = 3 * m->N * elem_size(m->dtype);
+            if (need > kNoAllocBytes)
+                throw Error(CFD_E_INVALID, "cfd_stab_noalloc: a general mesh of " + std::to_string(m->N) +
+                                               " cells needs " + std::to_string(need) +
+                                               " workspace bytes, beyond the fixed " +
+                                               std::to_string(kNoAllocBytes) +
+                                               " (at most 2097152 FP64 or 4194304 FP32 cells)");
+            work = scratch_addr(m->ctx->device);
+        }
+        DISPATCH(m, launch_stab, m, phi, out, mask, work, as_stream(stream));
     });
 }
 

@@ -145,6 +145,16 @@ int cfd_lapl(cfd_mesh_t mesh, const void* phi, void* out, int axis_mask, void* s
  * src/cuda_operators.cu:307-487).  Exactly uniform meshes run one pass; other
  * meshes use a 3N scratch from the pool for the first stage's gradients. */
 int cfd_stab(cfd_mesh_t mesh, const void* phi, void* out, int axis_mask, void* stream);
+/* cfd_stab without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  General meshes keep the
+ * first stage's gradients in a fixed 48 MiB workspace that the library holds
+ * on each GPU until the process ends (3 N entries of the mesh dtype: at most
+ * 2,097,152 cells in FP64 and 4,194,304 in FP32; larger general meshes get
+ * CFD_E_INVALID before anything is launched).  Uniform meshes need no
+ * workspace.  Every call on one GPU shares its workspace, so calls that
+ * overlap in time on one GPU (other streams, other threads) race. */
+int cfd_stab_noalloc(cfd_mesh_t mesh, const void* phi, void* out, int axis_mask, void* stream);
 
 /* Elementwise kernels: add_mtrx/sub_mtrx/mul_mtrx/div_mtrx/neg_mtrx/
  * mul_mtrx(float)/div_const (src/cuda_operators.cu:27-136) plus the DtVar /
