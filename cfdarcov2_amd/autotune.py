@@ -137,7 +137,8 @@ def tune_euler(mesh, U, gamma=1.4, cfl=0.3, steps=10, candidates=None, verbose=F
     return dict(best, table=table)
 
 
-HEAT_CANDIDATES = [(0, 8, 16)] + [(1, w, L) for w in (8, 12, 16) for L in (1, 2, 4, 8, 16)]
+HEAT_CANDIDATES = [(0, 8, 16)] + [(1, w, L) for w in (8, 12, 16) for L in (1, 2, 4, 8, 16)] + [
+    (2, B, K) for B in (4, 8) for K in (2, 4)]
 
 
 def tune_heat(mesh, alpha, dt, steps=20, candidates=None, verbose=False) -> dict:

@@ -40,6 +40,7 @@ EXPORTS = [
     "cfd_batch_create", "cfd_batch_set_form", "cfd_batch_set_shape", "cfd_batch_run", "cfd_batch_destroy",
     "cfd_program_compile", "cfd_program_compile_only", "cfd_program_launch", "cfd_program_destroy",
     "cfd_mesh_geometry", "cfd_stab_noalloc",
+    "cfd_mesh_create_unstructured", "cfd_mesh_permutation", "cfd_unstructured_table",
 ]
 
 
@@ -59,6 +60,39 @@ class HexDesc(ctypes.Structure):
                 ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
                 ("dtype", ctypes.c_int), ("force_general", ctypes.c_int),
                 ("vertex_offsets", ctypes.c_void_p)]
+
+
+class UMeshDesc(ctypes.Structure):
+    _fields_ = [("n_cells", ctypes.c_int64), ("n_faces", ctypes.c_int64), ("n_vertices", ctypes.c_int64),
+                ("vertices", ctypes.c_void_p), ("face_verts", ctypes.c_void_p), ("face_cells", ctypes.c_void_p),
+                ("cell_faces", ctypes.c_void_p), ("cell_verts", ctypes.c_void_p), ("dtype", ctypes.c_int),
+                ("reorder", ctypes.c_int)]
+
+
+def umesh_desc(conn, dtype, reorder):
+    """cfd_umesh_desc of a connectivity dict (cfdarcov2_amd.meshio layout); the
+    returned arrays must outlive the descriptor's use."""
+    arrs = {k: np.ascontiguousarray(conn[k], dtype=np.float64 if k == "vertices" else np.int64)
+            for k in ("vertices", "face_verts", "face_cells", "cell_faces", "cell_verts")}
+    N, F, V = arrs["cell_faces"].size // 6, arrs["face_verts"].size // 4, arrs["vertices"].size // 3
+    d = UMeshDesc(N, F, V, arrs["vertices"].ctypes.data, arrs["face_verts"].ctypes.data,
+                  arrs["face_cells"].ctypes.data, arrs["cell_faces"].ctypes.data, arrs["cell_verts"].ctypes.data,
+                  _dtype_code(dtype), int(bool(reorder)))
+    return d, arrs
+
+
+def unstructured_table(conn, name, dtype="f64"):
+    """Reference-layout table of an unstructured mesh in the caller's numbering,
+    computed on the host (no device)."""
+    L = load_library()
+    d, keep = umesh_desc(conn, dtype, False)
+    N = d.n_cells
+    n = N if name in _N_TABLES else 6 * N
+    code = _dtype_code(dtype)
+    dt = np.int64 if name in ("ids", "ids_bf") else (np.float32 if code == F32 else np.float64)
+    out = np.empty(n, dtype=dt)
+    check(L.cfd_unstructured_table(ctypes.byref(d), TABLES[name], out.ctypes.data))
+    return out if n == N else out.reshape(6, N)
 
 
 class MeshInfo(ctypes.Structure):
@@ -90,6 +124,9 @@ def load_library(path: str = LIB_PATH):
         "cfd_hex_table": [ctypes.POINTER(HexDesc), i, vp],
         "cfd_grad": [vp, vp, vp, i, i, vp], "cfd_lapl": [vp, vp, vp, i, vp], "cfd_stab": [vp, vp, vp, i, vp],
         "cfd_stab_noalloc": [vp, vp, vp, i, vp],
+        "cfd_mesh_create_unstructured": [vp, ctypes.POINTER(UMeshDesc), pp],
+        "cfd_mesh_permutation": [vp, vp, vp],
+        "cfd_unstructured_table": [ctypes.POINTER(UMeshDesc), i, vp],
         "cfd_ewise": [vp, i, i, vp, vp, vp, d, vp, i64, vp],
         "cfd_cfl_min": [vp, i, vp, vp, vp, vp, vp, i64, d, d, d, ctypes.POINTER(ctypes.c_double), vp],
         "cfd_mesh_geometry": [vp, vp],
@@ -377,3 +414,34 @@ class HexMesh:
         check(self.lib.cfd_euler_run(self.h, A, B, float(gamma), float(cfl), int(nsteps), _dptr(dt_hist),
                                      _stream_ptr()))
         return tuple(state_a) if nsteps % 2 == 0 else tuple(state_b)
+
+
+class UnstructuredMesh(HexMesh):
+    """cfd_mesh_create_unstructured: a hex mesh given by its connectivity (a
+    cfdarcov2_amd.meshio dict).  With reorder, the cells are renumbered by
+    reverse Cuthill-McKee: fields are in that numbering, `perm[new] = old`
+    maps them to the caller's.  grad, lapl, heat_run and table apply."""
+
+    def __init__(self, ctx: Context, conn, dtype="f64", reorder=True):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.code = _dtype_code(dtype)
+        d, self._arrs = umesh_desc(conn, dtype, reorder)
+        self.N, self.F = int(d.n_cells), int(d.n_faces)
+        self.shape = None
+        h = ctypes.c_void_p()
+        check(self.lib.cfd_mesh_create_unstructured(ctx.h, ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        self.perm = np.empty(self.N, dtype=np.int64)
+        self.face_perm = np.empty(self.F, dtype=np.int64)
+        check(self.lib.cfd_mesh_permutation(self.h, self.perm.ctypes.data, self.face_perm.ctypes.data))
+        self.iperm = np.empty_like(self.perm)
+        self.iperm[self.perm] = np.arange(self.N)
+
+    def to_mesh_order(self, a):
+        """Caller's numbering -> the mesh's."""
+        return a[self.perm]
+
+    def to_caller_order(self, a):
+        """The mesh's numbering -> the caller's."""
+        return a[self.iperm]

@@ -14,6 +14,7 @@
 #pragma once
 
 #include <filesystem>
+#include <string>
 #include <vector>
 
 class Variable;
@@ -29,6 +30,16 @@ void store_history_stepping(const std::vector<Variable*>& vars_to_store, const M
 void finalize_history_stepping(const fs::path& store_path = "./dumps");
 
 namespace cfdarco::io {
+// A mesh.json of lattice connectivity (the -m/--mesh input): the extents, the
+// spacing and the vertex offsets (empty when the vertices sit on the lattice)
+// of the Mesh3D it describes.  Throws for other connectivity.
+struct LatticeMeshFile {
+    size_t x = 0, y = 0, z = 0;
+    float dx = 0, dy = 0, dz = 0;
+    std::vector<double> vertex_offsets;
+};
+LatticeMeshFile read_lattice_mesh_json(const std::string& path);
+
 // One field on an X x Y x Z lattice (values in the mesh's cell order) as VTK
 // XML ImageData point data named "ImageScalars", inline base64 binary.
 void write_vti(const fs::path& file, size_t X, size_t Y, size_t Z, const float* cells);

@@ -92,10 +92,17 @@ public:
         if (!cuda_enable && !hip_enable)
             throw std::runtime_error{
                 "this build evaluates on the MI355X only: pass -c/--cuda_enable or -hip/--hip_enable"};
+        const std::string mesh_file = program.get<std::string>("mesh");
+        std::vector<double> file_offsets;
+        if (!mesh_file.empty()) {  // -m: extents, spacing and vertices from a mesh.json
+            auto mf = cfdarco::io::read_lattice_mesh_json(mesh_file);
+            Lx = mf.x; Ly = mf.y; Lz = mf.z;
+            dx = mf.dx; dy = mf.dy; dz = mf.dz;
+            file_offsets = std::move(mf.vertex_offsets);
+        }
         mesh = std::make_shared<CudaMesh3D>(Lx, Ly, Lz, dx, dy, dz);
-        if (!program.get<std::string>("mesh").empty())
-            throw std::runtime_error{"-m/--mesh: mesh files are not read by this build"};
         mesh->init_basic_internals();
+        if (!file_offsets.empty()) mesh->set_vertex_offsets(std::move(file_offsets));
         const double perturb = program.get<double>("perturb");
         if (perturb > 0) mesh->perturb_interior_vertices(uint64_t(program.get<int>("perturb_seed")), perturb);
         mesh->compute();

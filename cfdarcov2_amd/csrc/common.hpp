@@ -108,6 +108,30 @@ template <class T> struct FvGeo {
     const T* vol;
 };
 
+// Connectivity of an unstructured hex mesh (cfd_mesh_create_unstructured), in
+// the caller's numbering.
+struct UConn {
+    int64_t N = 0, F = 0, V = 0;
+    std::vector<double> verts;        // V x 3
+    std::vector<int64_t> face_verts;  // F x 4
+    std::vector<int64_t> face_cells;  // F x 2, -1 when absent
+    std::vector<int64_t> cell_faces;  // N x 6 (local order z-, z+, y-, y+, x-, x+)
+    std::vector<int64_t> cell_verts;  // N x 8
+};
+
+// Device geometry of an unstructured hex mesh, cells and faces in the mesh's
+// (reordered) numbering.  Per (cell, local face) arrays are fc-major (6 x N).
+template <class T> struct UnstructGeo {
+    int64_t N, F;
+    const int32_t* nbr;    // neighbour across the face, -1 on the boundary
+    const int32_t* face;   // face id
+    const int8_t* osign;   // outward normal of the face = osign * its normal
+    const T* fn[3];        // per face: normal (first three vertices)
+    const T* area;
+    const T* alpha;        // alpha_d (the same from both cells)
+    const T* vol;          // per cell
+};
+
 }  // namespace cfd
 
 struct cfd_mesh_s {
@@ -151,6 +175,14 @@ struct cfd_mesh_s {
     int adj_form = 1;
     int adj_ty = 4;
     int adj_L = 8;
+    // unstructured meshes: connectivity in the caller's numbering, the cell
+    // and face orders the device arrays use (perm[new] = old, iperm[old] =
+    // new) and the device geometry (in dev_block)
+    bool unstructured = false;
+    cfd::UConn uconn;
+    std::vector<int64_t> cell_perm, cell_iperm, face_perm;
+    cfd::UnstructGeo<float> uu32{};
+    cfd::UnstructGeo<double> uu64{};
     // Euler stepper geometry (device block allocated on first use)
     cfd::FvGeo<float> fv32{};
     cfd::FvGeo<double> fv64{};

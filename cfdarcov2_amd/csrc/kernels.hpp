@@ -74,6 +74,13 @@ template <class T>
 bool launch_wave_row2(const UniformGeo<T>& g, void* const* in, void* const* out, double K, double dt, int ty, int L,
                       int order, bool nt, hipStream_t s);
 
+// Up to K heat steps per launch in LDS (heat_multi.hip): B x B x min(Z, 64-2K)
+// output boxes, B 4 or 8, K 2 or 4 (B 8 with K 4 in FP32 only), constant alpha.
+bool heat_multi_supported(int B, int K);
+template <class T>
+void launch_heat_multi(const UniformGeo<T>& g, const void* in, void* out, double alpha, double dt, int B, int K,
+                       int ns, hipStream_t s);
+
 template <class T>
 void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamma, double cfl, int64_t nsteps,
                   void* dt_hist, hipStream_t s);

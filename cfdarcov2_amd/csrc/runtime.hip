@@ -295,6 +295,37 @@ static void hex_table(int64_t X_, int64_t Y_, int64_t Z_, double dx, double dy, 
 
 static bool valid_table(int t) { return t >= CFD_T_IDS && t <= CFD_T_FV_VOL; }
 
+// unstructured meshes (umesh.hip)
+template <class T> void create_unstructured(cfd_mesh_s* m, int reorder);
+template <class T>
+void unstructured_table(const UConn& c, const int64_t* perm, const int64_t* iperm, int which, void* out);
+template <class T> void launch_ugrad(const cfd_mesh_s* m, const void* phi, void* out, int mask, int wall, hipStream_t s);
+template <class T> void launch_ulapl(const cfd_mesh_s* m, const void* phi, void* out, int mask, hipStream_t s);
+template <class T>
+void launch_uheat(const cfd_mesh_s* m, void* Ta, void* Tb, double alpha, const void* af, double dt, int64_t nsteps,
+                  hipStream_t s);
+
+static void require_lattice(const cfd_mesh_s* m, const char* what) {
+    if (m->unstructured)
+        throw Error(CFD_E_UNSUPPORTED, std::string(what) +
+                                           ": unstructured meshes take cfd_grad, cfd_lapl and cfd_heat_run only");
+}
+
+static UConn uconn_of(const cfd_umesh_desc* d) {
+    CFD_REQUIRE(d != nullptr, "null unstructured mesh description");
+    CFD_REQUIRE(d->n_cells > 0 && d->n_faces > 0 && d->n_vertices > 0, "unstructured mesh: empty mesh");
+    CFD_REQUIRE(d->vertices && d->face_verts && d->face_cells && d->cell_faces && d->cell_verts,
+                "unstructured mesh: null connectivity array");
+    UConn c;
+    c.N = d->n_cells; c.F = d->n_faces; c.V = d->n_vertices;
+    c.verts.assign(d->vertices, d->vertices + 3 * c.V);
+    c.face_verts.assign(d->face_verts, d->face_verts + 4 * c.F);
+    c.face_cells.assign(d->face_cells, d->face_cells + 2 * c.F);
+    c.cell_faces.assign(d->cell_faces, d->cell_faces + 6 * c.N);
+    c.cell_verts.assign(d->cell_verts, d->cell_verts + 8 * c.N);
+    return c;
+}
+
 // ---------------------------------------------------------------------------
 // Fixed workspace of cfd_stab_noalloc: the first stage's three gradient fields
 // of the two-launch stabilisation on general meshes (3 N entries of the mesh
@@ -475,6 +506,16 @@ int cfd_mesh_get_info(cfd_mesh_t m, cfd_mesh_info* info) {
 
 int cfd_mesh_table(cfd_mesh_t m, int table, void* host_out) {
     return guarded([&] {
+        if (m && m->unstructured) {
+            CFD_REQUIRE(host_out != nullptr, "cfd_mesh_table: null output");
+            CFD_REQUIRE(table >= CFD_T_IDS && table <= CFD_T_CCZ,
+                        "unstructured meshes have the reference tables 0 to 15");
+            if (m->dtype == CFD_F32)
+                unstructured_table<float>(m->uconn, m->cell_perm.data(), m->cell_iperm.data(), table, host_out);
+            else
+                unstructured_table<double>(m->uconn, m->cell_perm.data(), m->cell_iperm.data(), table, host_out);
+            return;
+        }
         CFD_REQUIRE(m != nullptr && host_out != nullptr, "null argument");
         CFD_REQUIRE(valid_table(table), "unknown mesh table id");
         const double* off = m->has_offsets ? m->vertex_offsets.data() : nullptr;
@@ -498,11 +539,55 @@ int cfd_hex_table(const cfd_hex_desc* d, int table, void* host_out) {
 
 #define DISPATCH(m, F, ...) ((m)->dtype == CFD_F32 ? F<float>(__VA_ARGS__) : F<double>(__VA_ARGS__))
 
+int cfd_mesh_create_unstructured(cfd_ctx_t ctx, const cfd_umesh_desc* desc, cfd_mesh_t* out) {
+    return guarded([&] {
+        CFD_REQUIRE(ctx != nullptr && desc != nullptr && out != nullptr,
+                    "cfd_mesh_create_unstructured: null argument");
+        check_dtype(desc->dtype);
+        CFD_HIP(hipSetDevice(ctx->device));
+        auto m = std::make_unique<cfd_mesh_s>();
+        m->ctx = ctx;
+        m->dtype = desc->dtype;
+        m->unstructured = true;
+        m->uniform = false;
+        m->uconn = uconn_of(desc);
+        m->N = desc->n_cells; m->F = desc->n_faces; m->V = desc->n_vertices;
+        if (m->dtype == CFD_F32) create_unstructured<float>(m.get(), desc->reorder);
+        else create_unstructured<double>(m.get(), desc->reorder);
+        *out = m.release();
+    });
+}
+
+int cfd_mesh_permutation(cfd_mesh_t m, int64_t* cell_perm, int64_t* face_perm) {
+    return guarded([&] {
+        CFD_REQUIRE(m != nullptr, "null mesh");
+        if (!m->unstructured) {  // lattice meshes keep the reference numbering
+            for (int64_t i = 0; cell_perm && i < m->N; ++i) cell_perm[i] = i;
+            for (int64_t f = 0; face_perm && f < m->F; ++f) face_perm[f] = f;
+            return;
+        }
+        if (cell_perm) std::copy(m->cell_perm.begin(), m->cell_perm.end(), cell_perm);
+        if (face_perm) std::copy(m->face_perm.begin(), m->face_perm.end(), face_perm);
+    });
+}
+
+int cfd_unstructured_table(const cfd_umesh_desc* d, int table, void* host_out) {
+    return guarded([&] {
+        CFD_REQUIRE(host_out != nullptr, "cfd_unstructured_table: null output");
+        CFD_REQUIRE(table >= CFD_T_IDS && table <= CFD_T_CCZ, "unstructured meshes have the reference tables 0 to 15");
+        check_dtype(d ? d->dtype : -1);
+        const UConn c = uconn_of(d);
+        if (d->dtype == CFD_F32) unstructured_table<float>(c, nullptr, nullptr, table, host_out);
+        else unstructured_table<double>(c, nullptr, nullptr, table, host_out);
+    });
+}
+
 int cfd_grad(cfd_mesh_t m, const void* phi, void* out, int mask, int wall, void* stream) {
     return guarded([&] {
         CFD_REQUIRE(m && phi && out, "cfd_grad: null argument");
         CFD_REQUIRE(mask >= 0 && mask <= 7 && wall >= -1 && wall <= 2, "cfd_grad: bad axis_mask or wall_axis");
-        DISPATCH(m, launch_grad, m, phi, out, mask, wall, as_stream(stream));
+        if (m->unstructured) DISPATCH(m, launch_ugrad, m, phi, out, mask, wall, as_stream(stream));
+        else DISPATCH(m, launch_grad, m, phi, out, mask, wall, as_stream(stream));
     });
 }
 
@@ -510,13 +595,15 @@ int cfd_lapl(cfd_mesh_t m, const void* phi, void* out, int mask, void* stream) {
     return guarded([&] {
         CFD_REQUIRE(m && phi && out, "cfd_lapl: null argument");
         CFD_REQUIRE(mask >= 0 && mask <= 7, "cfd_lapl: bad axis_mask");
-        DISPATCH(m, launch_lapl, m, phi, out, mask, as_stream(stream));
+        if (m->unstructured) DISPATCH(m, launch_ulapl, m, phi, out, mask, as_stream(stream));
+        else DISPATCH(m, launch_lapl, m, phi, out, mask, as_stream(stream));
     });
 }
 
 int cfd_stab(cfd_mesh_t m, const void* phi, void* out, int mask, void* stream) {
     return guarded([&] {
         CFD_REQUIRE(m && phi && out, "cfd_stab: null argument");
+        require_lattice(m, "cfd_stab");
         CFD_REQUIRE(mask >= 0 && mask <= 7, "cfd_stab: bad axis_mask");
         hipStream_t s = as_stream(stream);
         // the first stage's gradients go through 3N pool entries on general
@@ -535,6 +622,7 @@ int cfd_stab(cfd_mesh_t m, const void* phi, void* out, int mask, void* stream) {
 int cfd_stab_noalloc(cfd_mesh_t m, const void* phi, void* out, int mask, void* stream) {
     return guarded([&] {
         CFD_REQUIRE(m && phi && out, "cfd_stab_noalloc: null argument");
+        require_lattice(m, "cfd_stab_noalloc");
         CFD_REQUIRE(mask >= 0 && mask <= 7, "cfd_stab_noalloc: bad axis_mask");
         void* work = nullptr;
         if (!m->uniform) {
@@ -609,6 +697,7 @@ int cfd_cfl_min(cfd_ctx_t ctx, int dtype, const void* u, const void* v, const vo
 int cfd_mesh_geometry(cfd_mesh_t m, cfd_geometry* out) {
     return guarded([&] {
         CFD_REQUIRE(m && out, "cfd_mesh_geometry: null argument");
+        require_lattice(m, "cfd_mesh_geometry");
         cfd_geometry g{};
         g.uniform = m->uniform ? 1 : 0;
         g.dtype = m->dtype;
@@ -662,7 +751,8 @@ int cfd_heat_run(cfd_mesh_t m, void* Ta, void* Tb, double alpha, const void* af,
         CFD_REQUIRE(m && Ta && Tb, "cfd_heat_run: null argument");
         CFD_REQUIRE(Ta != Tb, "cfd_heat_run: ping-pong buffers must differ");
         CFD_REQUIRE(nsteps >= 0, "cfd_heat_run: negative step count");
-        DISPATCH(m, launch_heat, m, Ta, Tb, alpha, af, dt, nsteps, as_stream(stream));
+        if (m->unstructured) DISPATCH(m, launch_uheat, m, Ta, Tb, alpha, af, dt, nsteps, as_stream(stream));
+        else DISPATCH(m, launch_heat, m, Ta, Tb, alpha, af, dt, nsteps, as_stream(stream));
     });
 }
 
@@ -670,6 +760,7 @@ int cfd_wave_run(cfd_mesh_t m, void* const a[4], void* const b[4], double K, con
                  int64_t nsteps, void* stream) {
     return guarded([&] {
         CFD_REQUIRE(m && a && b, "cfd_wave_run: null argument");
+        require_lattice(m, "cfd_wave_run");
         for (int i = 0; i < 4; ++i) {
             CFD_REQUIRE(a[i] && b[i], "cfd_wave_run: null state pointer");
             for (int j = 0; j < 4; ++j) CFD_REQUIRE(a[i] != b[j], "cfd_wave_run: state_a and state_b overlap");
@@ -683,6 +774,7 @@ int cfd_euler_run(cfd_mesh_t m, void* const a[5], void* const b[5], double gamma
                   void* dt_hist, void* stream) {
     return guarded([&] {
         CFD_REQUIRE(m && a && b, "cfd_euler_run: null argument");
+        require_lattice(m, "cfd_euler_run");
         for (int i = 0; i < 5; ++i) {
             CFD_REQUIRE(a[i] && b[i], "cfd_euler_run: null state pointer");
             for (int j = 0; j < 5; ++j) CFD_REQUIRE(a[i] != b[j], "cfd_euler_run: state_a and state_b overlap");
@@ -707,6 +799,7 @@ int cfd_wave_traces(cfd_mesh_t m, void* const a[4], void* const b[4], double K, 
                     int64_t every, void* stream) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "cfd_wave_traces: null mesh");
+        require_lattice(m, "cfd_wave_traces");
         check_state4(a, b, "cfd_wave_traces");
         if (!m->uniform) throw Error(CFD_E_UNSUPPORTED, "cfd_wave_traces: uniform meshes only");
         CFD_REQUIRE(nsteps >= 0, "cfd_wave_traces: negative step count");
@@ -722,6 +815,7 @@ int cfd_wave_adjoint(cfd_mesh_t m, void* const adj_a[4], void* const adj_b[4], d
                      const void* checkpoints, int64_t every, void* grad_K, void* stream) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "cfd_wave_adjoint: null mesh");
+        require_lattice(m, "cfd_wave_adjoint");
         check_state4(adj_a, adj_b, "cfd_wave_adjoint");
         if (!m->uniform) throw Error(CFD_E_UNSUPPORTED, "cfd_wave_adjoint: uniform meshes only");
         CFD_REQUIRE(nsteps >= 0, "cfd_wave_adjoint: negative step count");
@@ -736,6 +830,7 @@ int cfd_wave_adjoint(cfd_mesh_t m, void* const adj_a[4], void* const adj_b[4], d
 int cfd_mesh_set_euler_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");
+        require_lattice(m, "cfd_mesh_set_euler_shape");
         CFD_REQUIRE(variant == 0 || variant == 1, "euler variant must be 0 or 1");
         CFD_REQUIRE(tile_rows == 2 || tile_rows == 4 || tile_rows == 8, "euler tile_rows must be 2, 4 or 8");
         CFD_REQUIRE(chunk >= 1 && chunk <= 65535, "euler chunk must be in [1, 65535]");
@@ -748,7 +843,18 @@ int cfd_mesh_set_euler_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk
 int cfd_mesh_set_heat_shape(cfd_mesh_t m, int variant, int waves, int chunk) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");
-        CFD_REQUIRE(variant == 0 || variant == 1, "heat variant must be 0 or 1");
+        require_lattice(m, "cfd_mesh_set_heat_shape");
+        CFD_REQUIRE(variant >= 0 && variant <= 2, "heat variant must be 0, 1 or 2");
+        if (variant == 2) {
+            CFD_REQUIRE(heat_multi_supported(waves, chunk) && !(m->dtype == CFD_F64 && waves == 8 && chunk == 4),
+                        "multi-step heat boxes: 4 or 8 cells wide (waves), 2 or 4 steps per launch (chunk); "
+                        "8 with 4 steps in FP32 only");
+            CFD_REQUIRE(m->uniform, "the multi-step heat form needs an exactly uniform mesh");
+            m->heat_variant = variant;
+            m->heat_nw = waves;
+            m->heat_L = chunk;
+            return;
+        }
         CFD_REQUIRE(waves == 8 || waves == 12 || waves == 16, "heat waves must be 8, 12 or 16");
         CFD_REQUIRE(chunk >= 1 && chunk <= 65535, "heat chunk must be in [1, 65535]");
         if (variant == 1 && !(m->uniform && heat_row_supported(m->Z, waves)))
@@ -764,6 +870,7 @@ int cfd_mesh_set_heat_shape(cfd_mesh_t m, int variant, int waves, int chunk) {
 int cfd_mesh_set_adjoint_shape(cfd_mesh_t m, int form, int tile_rows, int chunk) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");
+        require_lattice(m, "cfd_mesh_set_adjoint_shape");
         CFD_REQUIRE(form == 0 || form == 1, "adjoint form must be 0 or 1");
         CFD_REQUIRE(tile_rows == 4 || tile_rows == 8, "adjoint row tiles have 4 or 8 output waves");
         CFD_REQUIRE(chunk >= 1 && chunk <= 65535, "adjoint chunk must be in [1, 65535]");
@@ -780,6 +887,7 @@ int cfd_mesh_set_adjoint_shape(cfd_mesh_t m, int form, int tile_rows, int chunk)
 int cfd_mesh_set_wave_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk, int xcd_remap) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");
+        require_lattice(m, "cfd_mesh_set_wave_shape");
         CFD_REQUIRE(variant >= 0 && variant <= 12, "wave variant must be 0 to 12");
         const bool row_form = variant >= 7;
         if (variant >= 11) {

@@ -594,6 +594,23 @@ template <class T>
 void launch_heat(const cfd_mesh_s* m, void* Ta, void* Tb, double alpha, const void* af, double dt, int64_t nsteps,
                  hipStream_t s) {
     int64_t k0 = 0;
+    if (m->uniform && !af && m->heat_variant == 2 && nsteps > 0) {
+        // up to K steps per launch in LDS, reading one buffer and writing the
+        // other: the number of launches has the parity of nsteps, so the
+        // state ends where single steps leave it, and the steps are spread
+        // evenly over the launches
+        const auto& g = ugeo<T>(m);
+        const int K = m->heat_L;
+        int64_t launches = (nsteps + K - 1) / K;
+        if ((launches - nsteps) % 2 != 0) ++launches;
+        for (int64_t l = 0; l < launches; ++l) {
+            const int64_t done = nsteps * l / launches, next = nsteps * (l + 1) / launches;
+            const void* in = (l & 1) ? Tb : Ta;
+            void* out = (l & 1) ? Ta : Tb;
+            launch_heat_multi<T>(g, in, out, alpha, dt, m->heat_nw, K, int(next - done), s);
+        }
+        return;
+    }
     if (m->uniform && !af && m->heat_variant == 1) {
         // two steps per launch, in pairs (Ta -> Tb -> Ta) so the state ends
         // where single steps leave it; the remaining steps run one per launch

@@ -98,6 +98,39 @@ int cfd_mesh_create_hex(cfd_ctx_t ctx, const cfd_hex_desc* desc, cfd_mesh_t* out
 int cfd_mesh_destroy(cfd_mesh_t mesh);
 int cfd_mesh_get_info(cfd_mesh_t mesh, cfd_mesh_info* info);
 
+/* ---- unstructured hex meshes ----------------------------------------------
+ * A hex mesh given by its connectivity, in the layout of the reference's
+ * mesh.json (include/io_operators_tmpl.hpp:161-212; the -m/--mesh input the
+ * reference's readers take, include/io_operators_tmpl.hpp:215-247,
+ * legacy/io_operators.cpp:170-205): vertex coordinates; per face its 4
+ * vertices (the first three give its normal) and its cells (the second -1 on
+ * the boundary); per cell its 6 faces in the local order z-, z+, y-, y+, x-,
+ * x+ and its 8 vertices.  The library computes the reference geometry
+ * (src/mesh3d.cpp:43-105,151-321) from it in the mesh dtype.  reorder = 1
+ * renumbers the cells by reverse Cuthill-McKee and the faces by their lowest
+ * new cell; fields passed to the operators are in that numbering
+ * (cfd_mesh_permutation gives perm[new] = old; reorder = 0 keeps the
+ * caller's).  cfd_grad, cfd_lapl, cfd_heat_run and cfd_mesh_table take such a
+ * mesh; the other entry points return CFD_E_UNSUPPORTED for it.  Results equal
+ * the reference semantics bit for bit in any numbering. */
+typedef struct cfd_umesh_desc {
+    int64_t n_cells, n_faces, n_vertices;
+    const double* vertices;      /* V x 3 */
+    const int64_t* face_verts;   /* F x 4 */
+    const int64_t* face_cells;   /* F x 2 */
+    const int64_t* cell_faces;   /* N x 6 */
+    const int64_t* cell_verts;   /* N x 8 */
+    int dtype;
+    int reorder;
+} cfd_umesh_desc;
+int cfd_mesh_create_unstructured(cfd_ctx_t ctx, const cfd_umesh_desc* desc, cfd_mesh_t* out);
+/* perm[new] = old for the cells (N long) and the faces (F long); either may
+ * be NULL.  Lattice meshes give the identity. */
+int cfd_mesh_permutation(cfd_mesh_t mesh, int64_t* cell_perm, int64_t* face_perm);
+/* The reference-layout tables 0..15 of an unstructured mesh in the caller's
+ * numbering, computed on the host without a device. */
+int cfd_unstructured_table(const cfd_umesh_desc* desc, int table, void* host_out);
+
 /* Reference-layout tables (N x 6 column-major, element (i,fc) at fc*N+i) copied
  * to host memory, for parity checks against Mesh3D's tables
  * (include/mesh3d.hpp:248-269).  ids/ids_bf are int64; the others are in the mesh
@@ -359,8 +392,12 @@ int cfd_mesh_set_wave_shape(cfd_mesh_t mesh, int variant, int tile_rows, int chu
  * launch; variant 1 = two steps per launch in row form (z extent a multiple of
  * 4 and at most 256, else CFD_E_UNSUPPORTED), `waves` (8, 12 or 16) waves per
  * workgroup and `chunk` x-planes per tile; runs go in pairs of launches, and
- * the remaining steps and runs with a diffusivity field take variant 0.  Both
- * give bit-identical results.  Variant 1 pays on large meshes (it needs
+ * the remaining steps and runs with a diffusivity field take variant 0;
+ * variant 2 = up to `chunk` (2 or 4) steps per launch in LDS boxes of `waves`
+ * (4 or 8) x `waves` x min(Z, 64 - 2 chunk) output cells, each workgroup
+ * computing its box widened by `chunk` cells for the steps in between (8-wide
+ * boxes with 4 steps in FP32 only); the form for small meshes whose single
+ * step lasts about a launch.  All give bit-identical results.  Variant 1 pays on large meshes (it needs
  * enough tiles to fill the GPU): meshes of at least 2^23 cells that allow it
  * start on variant 1 (12 waves, 16 planes), others on variant 0;
  * cfdarcov2_amd/autotune.py:tune_heat times both. */

@@ -146,6 +146,29 @@ void build_topology(HexMesh<T>& m, int64_t X, int64_t Y, int64_t Z, T dx, T dy, 
             }
 }
 
+// A hex mesh from its connectivity (the mesh.json layout written by
+// store_mesh, include/io_operators_tmpl.hpp:161-212, and read back by the -m
+// readers, :215-247 and legacy/io_operators.cpp:170-205): the same objects
+// init_basic_internals builds, taken as given.  face_cells holds the face's
+// cells in the order they list it, -1 when absent.
+template <class T>
+void build_from_connectivity(HexMesh<T>& m, int64_t N, int64_t F, int64_t V, const double* verts,
+                             const int64_t* face_verts, const int64_t* face_cells, const int64_t* cell_faces,
+                             const int64_t* cell_verts) {
+    m.X = m.Y = m.Z = 0;
+    m.N = N; m.F = F; m.V = V;
+    m.vx.resize(V); m.vy.resize(V); m.vz.resize(V);
+    for (int64_t v = 0; v < V; ++v) {
+        m.vx[v] = T(verts[3 * v]); m.vy[v] = T(verts[3 * v + 1]); m.vz[v] = T(verts[3 * v + 2]);
+    }
+    m.face_verts.assign(face_verts, face_verts + 4 * F);
+    m.face_cells.assign(face_cells, face_cells + 2 * F);
+    m.face_ncells.assign(F, 0);
+    for (int64_t f = 0; f < F; ++f) m.face_ncells[f] = face_cells[2 * f + 1] >= 0 ? 2 : 1;
+    m.cell_faces.assign(cell_faces, cell_faces + 6 * N);
+    m.cell_verts.assign(cell_verts, cell_verts + 8 * N);
+}
+
 // Face3D::compute (src/mesh3d.cpp:43-62), Quadrangle3D::compute (:69-105) and the
 // table assembly of Mesh3D::compute (:151-321).
 template <class T> void compute_geometry(HexMesh<T>& m) {
@@ -519,6 +542,39 @@ void wave_run(const HexMesh<T>& m, T* p, T* u, T* v, T* w, T K, const T* K_field
     }
 }
 
+// The same steps as wave_run in one fused pass per stage: each cell forms its
+// own face values and Gauss sums in the operators' order (grad's res = 0 + g),
+// so the results equal wave_run bit for bit without the whole-mesh
+// temporaries.  The CPU baseline's honest best case.
+template <class T>
+void wave_run_fused(const HexMesh<T>& m, T* p, T* u, T* v, T* w, T K, const T* K_field, T dt, int64_t steps) {
+    const int64_t N = m.N;
+    T* vel[3] = {u, v, w};
+    for (int64_t s = 0; s < steps; ++s) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < N; ++i) {
+            T fv[6];
+            for (int fc = 0; fc < 6; ++fc) fv[fc] = face_linear(m, p, i, fc, -1);
+            for (int d = 0; d < 3; ++d) {
+                const T g = T(0) + gauss_component(m, fv, i, d, false);
+                vel[d][i] = euler_update(dt, -g, vel[d][i]);
+            }
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < N; ++i) {
+            T gd[3];
+            for (int d = 0; d < 3; ++d) {
+                T fv[6];
+                for (int fc = 0; fc < 6; ++fc) fv[fc] = face_linear(m, vel[d], i, fc, d);
+                gd[d] = T(0) + gauss_component(m, fv, i, d, false);
+            }
+            const T div = (gd[0] + gd[1]) + gd[2];
+            const T k = K_field ? K_field[i] : K;
+            p[i] = euler_update(dt, k * div, p[i]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Compressible Euler with a Rusanov face flux.  No reference counterpart: the
 // reference's Sod driver (examples/main_3d_sod_shock.cpp:150-167) uses a
@@ -538,7 +594,8 @@ void wave_run(const HexMesh<T>& m, T* p, T* u, T* v, T* w, T K, const T* K_field
 //   lam = max(|un_L| + a_L, |un_R| + a_R),
 //   G_k = (0.5 ((F_L,k + F_R,k) - lam (U_R,k - U_L,k))) A.
 // Update: net_k = ((Gx+ - Gx-) + (Gy+ - Gy-)) + (Gz+ - Gz-),
-//         U_k <- U_k - (dt net_k) / V.
+//         U_k <- U_k - (dt net_k) (1 / V)  (one division per cell; the
+//         primitive velocities and sound speed use 1 / rho the same way).
 // Time step, taken from the state at the start of each step:
 //   dt = CFL min_i V_i / s_i,  s_i = sum over fc = 0..5 of A_f (|un_f| + a_i).
 // ---------------------------------------------------------------------------
@@ -587,14 +644,17 @@ template <class T> void compute_fv_geometry(HexMesh<T>& m) {
 
 template <class T> struct EulerPrim { T ux, uy, uz, p, a; };
 
+// One division per state: the reciprocal density multiplies the momenta and
+// the sound-speed argument.
 template <class T> inline EulerPrim<T> euler_prim(T gamma, T r, T mx, T my, T mz, T E) {
     EulerPrim<T> q;
-    q.ux = mx / r;
-    q.uy = my / r;
-    q.uz = mz / r;
+    const T ir = T(1) / r;
+    q.ux = mx * ir;
+    q.uy = my * ir;
+    q.uz = mz * ir;
     const T q2 = (q.ux * q.ux + q.uy * q.uy) + q.uz * q.uz;
     q.p = (gamma - T(1)) * (E - (T(0.5) * r) * q2);
-    q.a = std::sqrt((gamma * q.p) / r);
+    q.a = std::sqrt((gamma * q.p) * ir);
     return q;
 }
 
@@ -664,10 +724,11 @@ template <class T> void euler_step(const HexMesh<T>& m, T* const U[5], T gamma, 
             if (fc & 1) rusanov_flux(Ui, qi, Un, qn, nx, ny, nz, A, Gf[fc]);   // i is the lower cell
             else rusanov_flux(Un, qn, Ui, qi, nx, ny, nz, A, Gf[fc]);          // i is the upper cell
         }
+        const T iV = T(1) / m.svol[i];
         for (int k = 0; k < 5; ++k) {
             // fc 4/5 = x-/x+, 2/3 = y-/y+, 0/1 = z-/z+
             const T net = ((Gf[5][k] - Gf[4][k]) + (Gf[3][k] - Gf[2][k])) + (Gf[1][k] - Gf[0][k]);
-            U[k][i] = Ui[k] - (dt * net) / m.svol[i];
+            U[k][i] = Ui[k] - (dt * net) * iV;
         }
     }
 }

@@ -7,6 +7,8 @@
 #include <cstring>
 #include <string>
 
+#include <omp.h>
+
 #include "cfd_oracle.hpp"
 
 namespace {
@@ -88,6 +90,21 @@ void* oracle_mesh_create(int64_t X, int64_t Y, int64_t Z, double dx, double dy, 
     return om;
 }
 
+void* oracle_mesh_create_conn(int64_t N, int64_t F, int64_t V, const double* verts, const int64_t* face_verts,
+                              const int64_t* face_cells, const int64_t* cell_faces, const int64_t* cell_verts,
+                              int dtype) {
+    auto* om = new OMesh();
+    om->dtype = dtype;
+    if (dtype == 0) {
+        cfdo::build_from_connectivity<float>(om->f, N, F, V, verts, face_verts, face_cells, cell_faces, cell_verts);
+        cfdo::compute_geometry<float>(om->f);
+    } else {
+        cfdo::build_from_connectivity<double>(om->d, N, F, V, verts, face_verts, face_cells, cell_faces, cell_verts);
+        cfdo::compute_geometry<double>(om->d);
+    }
+    return om;
+}
+
 void oracle_mesh_destroy(void* h) { delete static_cast<OMesh*>(h); }
 
 void oracle_mesh_sizes(void* h, int64_t* out) {
@@ -137,6 +154,24 @@ void oracle_wave_run(void* h, void* p, void* u, void* v, void* w, double K, cons
                                    (const float*)K_field, float(dt), steps),
                     cfdo::wave_run(om->d, (double*)p, (double*)u, (double*)v, (double*)w, K,
                                    (const double*)K_field, dt, steps));
+}
+
+void oracle_wave_run_fused(void* h, void* p, void* u, void* v, void* w, double K, const void* K_field, double dt,
+                           int64_t steps) {
+    auto* om = static_cast<OMesh*>(h);
+    ORACLE_DISPATCH(om,
+                    cfdo::wave_run_fused(om->f, (float*)p, (float*)u, (float*)v, (float*)w, float(K),
+                                         (const float*)K_field, float(dt), steps),
+                    cfdo::wave_run_fused(om->d, (double*)p, (double*)u, (double*)v, (double*)w, K,
+                                         (const double*)K_field, dt, steps));
+}
+
+// OpenMP threads of the oracle's loops (the CPU baseline's 1-thread and
+// all-thread runs); returns the previous setting.
+int oracle_set_threads(int n) {
+    const int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
 }
 
 double oracle_cfl3d(void* h, double CFL, const void* u, const void* v, const void* w, const void* p,

@@ -36,6 +36,8 @@ def lib():
         vp, i64, d, i = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int
         L.oracle_mesh_create.restype = vp
         L.oracle_mesh_create.argtypes = [i64, i64, i64, d, d, d, i, vp]
+        L.oracle_mesh_create_conn.restype = vp
+        L.oracle_mesh_create_conn.argtypes = [i64, i64, i64, vp, vp, vp, vp, vp, i]
         L.oracle_mesh_destroy.argtypes = [vp]
         L.oracle_mesh_sizes.argtypes = [vp, vp]
         L.oracle_mesh_table.argtypes = [vp, ctypes.c_char_p, vp]
@@ -45,6 +47,9 @@ def lib():
         L.oracle_stab.argtypes = [vp, vp, i, vp]
         L.oracle_heat_run.argtypes = [vp, vp, d, vp, d, i64]
         L.oracle_wave_run.argtypes = [vp, vp, vp, vp, vp, d, vp, d, i64]
+        L.oracle_wave_run_fused.argtypes = [vp, vp, vp, vp, vp, d, vp, d, i64]
+        L.oracle_set_threads.argtypes = [i]
+        L.oracle_set_threads.restype = i
         L.oracle_cfl3d.restype = d
         L.oracle_cfl3d.argtypes = [vp, d, vp, vp, vp, vp, vp]
         L.oracle_cfl1d.restype = d
@@ -57,6 +62,11 @@ def lib():
         L.oracle_d2t_step.argtypes = [vp, vp, vp, d, i, d, i, vp]
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's loops; returns the previous setting."""
+    return int(lib().oracle_set_threads(int(n)))
 
 
 def _ptr(a):
@@ -91,6 +101,26 @@ class OracleMesh:
         sizes = np.zeros(3, dtype=np.int64)
         lib().oracle_mesh_sizes(self._h, _ptr(sizes))
         self.N, self.F, self.V = (int(s) for s in sizes)
+
+    @classmethod
+    def from_connectivity(cls, conn, dtype=np.float64):
+        """A mesh from its connectivity: a dict (or object with the same
+        attributes) holding vertices (V x 3), face_verts (F x 4), face_cells
+        (F x 2, -1 when absent), cell_faces (N x 6), cell_verts (N x 8)."""
+        get = (lambda k: conn[k]) if isinstance(conn, dict) else (lambda k: getattr(conn, k))
+        arrs = {k: np.ascontiguousarray(get(k), dtype=np.float64 if k == "vertices" else np.int64)
+                for k in ("vertices", "face_verts", "face_cells", "cell_faces", "cell_verts")}
+        self = cls.__new__(cls)
+        self.dtype = np.dtype(dtype)
+        self.shape = None
+        self.spacing = None
+        code = _F32 if self.dtype == np.float32 else _F64
+        N, F, V = arrs["cell_faces"].size // 6, arrs["face_verts"].size // 4, arrs["vertices"].size // 3
+        self._h = lib().oracle_mesh_create_conn(N, F, V, _ptr(arrs["vertices"]), _ptr(arrs["face_verts"]),
+                                                _ptr(arrs["face_cells"]), _ptr(arrs["cell_faces"]),
+                                                _ptr(arrs["cell_verts"]), code)
+        self.N, self.F, self.V = N, F, V
+        return self
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -149,6 +179,14 @@ class OracleMesh:
         kf = None if K_field is None else self._arr(K_field)
         lib().oracle_wave_run(self._h, _ptr(p), _ptr(u), _ptr(v), _ptr(w), float(K), _ptr(kf),
                               float(dt), int(steps))
+        return p, u, v, w
+
+    def wave_run_fused(self, p, u, v, w, K, dt, steps, K_field=None):
+        """wave_run's steps, one fused pass per stage (bit-identical)."""
+        p, u, v, w = (self._arr(a).copy() for a in (p, u, v, w))
+        kf = None if K_field is None else self._arr(K_field)
+        lib().oracle_wave_run_fused(self._h, _ptr(p), _ptr(u), _ptr(v), _ptr(w), float(K), _ptr(kf),
+                                    float(dt), int(steps))
         return p, u, v, w
 
     def cfl3d(self, CFL, u, v, w, p, rho):

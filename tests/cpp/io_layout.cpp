@@ -34,6 +34,27 @@ int main(int argc, char** argv) {
     FILE* o = std::fopen((out / "offsets.f64").c_str(), "wb");
     std::fwrite(off.data(), sizeof(double), off.size(), o);
     std::fclose(o);
+    // -m input: the same file read back gives the extents, spacing and vertices
+    const auto mf = cfdarco::io::read_lattice_mesh_json((run / "mesh.json").string());
+    if (mf.x != mesh._x || mf.y != mesh._y || mf.z != mesh._z || mf.dx != mesh._dx || mf.dy != mesh._dy ||
+        mf.dz != mesh._dz || mf.vertex_offsets.size() != off.size()) {
+        std::cerr << "mesh.json read back with other extents or spacing\n";
+        return 1;
+    }
+    const float d[3] = {mesh._dx, mesh._dy, mesh._dz};
+    size_t v = 0;
+    for (size_t x = 0; x <= mesh._x; ++x)
+        for (size_t y = 0; y <= mesh._y; ++y)
+            for (size_t z = 0; z <= mesh._z; ++z, ++v) {
+                const size_t lat[3] = {x, y, z};
+                for (int k = 0; k < 3; ++k) {
+                    const float base = float(lat[k]) * d[k];
+                    if (base + float(mf.vertex_offsets[3 * v + k]) != base + float(off[3 * v + k])) {
+                        std::cerr << "vertex " << v << " read back elsewhere\n";
+                        return 1;
+                    }
+                }
+            }
     std::cout << "ok\n";
     return 0;
 }

@@ -264,6 +264,29 @@ def test_reference_heat_driver_store(oracle_lib, stepping):
                 assert np.array_equal(img.transpose(2, 1, 0).ravel(), states[i]), i
 
 
+def test_reference_heat_driver_mesh_input():
+    """-m/--mesh: the heat driver on a perturbed mesh writes mesh.json with
+    -s -st; run again from that file (-m, extents, spacing and vertices from
+    it), it writes the same fields after every step, bit for bit."""
+    steps = 6
+    common = ["-t", str(steps), "-dt", "0.001", "-c", "-s", "-st"]
+    with tempfile.TemporaryDirectory() as tmp:
+        a, b = os.path.join(tmp, "a"), os.path.join(tmp, "b")
+        os.makedirs(a)
+        os.makedirs(b)
+        out_a = _run_driver("cfdARCO_heat", ["-Lx", "10", "-Ly", "8", "-Lz", "12", "-dx", "0.125", "-dy", "0.125",
+                                             "-dz", "0.0625", "--perturb", "0.2", "--perturb_seed", "3"] + common,
+                            cwd=a)
+        run_a = os.path.realpath(os.path.join(a, "dumps", "run_latest"))
+        out_b = _run_driver("cfdARCO_heat", ["-m", os.path.join(run_a, "mesh.json")] + common, cwd=b)
+        run_b = os.path.realpath(os.path.join(b, "dumps", "run_latest"))
+        assert _printed(out_a) == _printed(out_b)
+        for i in range(steps):
+            fa = np.fromfile(os.path.join(run_a, "T", f"{i}.bin"), dtype=f32)
+            fb = np.fromfile(os.path.join(run_b, "T", f"{i}.bin"), dtype=f32)
+            assert fa.size == 10 * 8 * 12 and np.array_equal(fa, fb), i
+
+
 def _printed(out):
     return {v: (float(s), float(m)) for v, s, m in re.findall(r"At (\S+) Current sum = (\S+) Current max = (\S+)",
                                                              out)}

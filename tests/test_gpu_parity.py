@@ -515,3 +515,48 @@ def test_heat_shape_errors(hip_ctx):
     assert e.value.code == -4
     with pytest.raises(capi.CfdError):
         capi.HexMesh(hip_ctx, 8, 8, 64, 0.125, 0.125, 0.125, dtype=np.float32).set_heat_shape(1, 10, 8)
+
+
+HEAT_MULTI_CASES = [
+    ((64, 64, 64), (1.0 / 64,) * 3, (4, 4), 200),      # the 64^3 workload's extents and length
+    ((13, 11, 100), (1.0 / 16, 1.0 / 16, 1.0 / 64), (4, 2), 7),
+    ((9, 10, 130), (0.375, 0.25, 0.125), (8, 2), 9),
+    ((5, 3, 4), (0.25, 0.25, 0.25), (4, 4), 6),
+    ((20, 17, 61), (1.0 / 32,) * 3, (8, 4), 11),
+    ((8, 8, 64), (0.125,) * 3, (4, 2), 1),
+    ((8, 8, 64), (0.125,) * 3, (4, 4), 2),
+]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("case", HEAT_MULTI_CASES,
+                         ids=[f"{c[0][0]}x{c[0][1]}x{c[0][2]}-B{c[2][0]}-K{c[2][1]}-s{c[3]}" for c in HEAT_MULTI_CASES])
+def test_heat_multi_step_form(hip_ctx, oracle_lib, dtype, case):
+    """Up to K heat steps per launch in LDS boxes: boxes that reach one, two or
+    no walls per axis, z extents below, at and above 64, odd and even step
+    counts (the launches split so the state ends in the right buffer): equal
+    to the oracle bit for bit."""
+    from cfdarcov2_amd import capi
+    n, d, (B, K), steps = case
+    if dtype == np.float64 and (B, K) == (8, 4):
+        pytest.skip("8-wide boxes with 4 steps do not fit the LDS in FP64 (rejected by set_heat_shape)")
+    print(f"\n[case] heat multi-step n={n} B={B} K={K} steps={steps} {np.dtype(dtype).name}", flush=True)
+    rng = np.random.default_rng(17)
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=dtype)
+    gm.set_heat_shape(2, B, K)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=dtype)
+    T0 = rng.standard_normal(om.N).astype(dtype)
+    alpha = 1e-2
+    dt = 0.15 * min(d) ** 2 / alpha
+    a, b = _t(T0, dtype), torch.empty(om.N, dtype=_t(T0, dtype).dtype, device="cuda")
+    got = _np(gm.heat_run(a, b, alpha, dt, steps))
+    ref = om.heat_run(T0, alpha, dt, steps)
+    assert np.array_equal(got, ref), float(np.max(np.abs(got - ref)))
+
+
+def test_heat_multi_step_shape_errors(hip_ctx):
+    from cfdarcov2_amd import capi
+    gm = capi.HexMesh(hip_ctx, 8, 8, 8, 0.125, 0.125, 0.125, dtype=np.float64)
+    for shape in ((2, 8, 4), (2, 6, 2), (2, 4, 3)):
+        with pytest.raises(capi.CfdError):
+            gm.set_heat_shape(*shape)

@@ -1,0 +1,105 @@
+"""Unstructured hex meshes on the CPU (no device): the library's reference
+tables computed from connectivity alone (cfd_unstructured_table) against the
+oracle built from the same connectivity, for the lattice mesh (where they
+also equal the structured tables), a perturbed one, the same mesh in a
+shuffled numbering (tables must follow the renumbering exactly), and a mesh
+read back from a mesh.json file."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from cfdarcov2_amd import capi, meshio
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TABLES = ["ids", "ids_bf", "nx", "ny", "nz", "nax", "nay", "naz", "area", "alpha", "len", "vol", "is_bnd",
+          "ccx", "ccy", "ccz"]
+
+
+def shuffle(conn, seed):
+    """The same mesh with cells, faces and vertices renumbered at random (each
+    face keeps its vertex order and each cell its local face order).
+    Returns the new connectivity and the maps old -> new."""
+    rng = np.random.default_rng(seed)
+    N, F, V = len(conn["cell_faces"]), len(conn["face_verts"]), len(conn["vertices"])
+    pc, pf, pv = rng.permutation(N), rng.permutation(F), rng.permutation(V)   # old -> new
+    out = {"vertices": np.empty_like(conn["vertices"]), "face_verts": np.empty_like(conn["face_verts"]),
+           "face_cells": np.empty_like(conn["face_cells"]), "cell_faces": np.empty_like(conn["cell_faces"]),
+           "cell_verts": np.empty_like(conn["cell_verts"])}
+    out["vertices"][pv] = conn["vertices"]
+    out["face_verts"][pf] = pv[conn["face_verts"]]
+    fcells = conn["face_cells"]
+    out["face_cells"][pf] = np.where(fcells >= 0, pc[np.maximum(fcells, 0)], -1)
+    out["cell_faces"][pc] = pf[conn["cell_faces"]]
+    out["cell_verts"][pc] = pv[conn["cell_verts"]]
+    return out, pc
+
+
+def _check_all(conn, oracle_lib, dtype):
+    om = oracle_lib.OracleMesh.from_connectivity(conn, dtype=dtype)
+    for t in TABLES:
+        got = capi.unstructured_table(conn, t, "f32" if dtype == np.float32 else "f64")
+        assert np.array_equal(got, om.table(t)), t
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_lattice_tables_equal_structured(oracle_lib, dtype):
+    n, d = (5, 4, 6), (0.25, 0.5, 0.125)
+    conn = meshio.lattice_connectivity(*n, *d, dtype=dtype)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=dtype)
+    for t in TABLES:
+        got = capi.unstructured_table(conn, t, "f32" if dtype == np.float32 else "f64")
+        assert np.array_equal(got, om.table(t)), t
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_perturbed_and_shuffled_tables(oracle_lib, dtype):
+    from cfdarcov2_amd import workloads
+    n, d = (6, 5, 7), (1 / 8, 1 / 8, 1 / 8)
+    off = workloads.perturbed_vertex_offsets(*n, *d, seed=9, frac=0.2)
+    conn = meshio.lattice_connectivity(*n, *d, vertex_offsets=off, dtype=dtype)
+    _check_all(conn, oracle_lib, dtype)
+    sh, pc = shuffle(conn, 4)
+    _check_all(sh, oracle_lib, dtype)
+    # the shuffled mesh's tables are the original's, renumbered
+    tag = "f32" if dtype == np.float32 else "f64"
+    for t in ("vol", "ccx", "area", "alpha", "nax", "len"):
+        a, b = capi.unstructured_table(conn, t, tag), capi.unstructured_table(sh, t, tag)
+        if a.ndim == 1:
+            assert np.array_equal(b[pc], a), t
+        else:
+            assert np.array_equal(b[:, pc], a), t
+    ids_a, ids_b = capi.unstructured_table(conn, "ids", tag), capi.unstructured_table(sh, "ids", tag)
+    assert np.array_equal(ids_b[:, pc], pc[ids_a])
+
+
+def test_mesh_json_round_trip(oracle_lib):
+    """mesh.json written by the drivers' writer (tests/cpp/io_layout) reads back
+    to the connectivity of the same perturbed mesh."""
+    prog = os.path.join(ROOT, "cfdarcov2_amd", "_build", "tests", "io_layout")
+    if not os.path.exists(prog):
+        pytest.fail(f"{prog} not built (run __graft_entry__.build())")
+    with tempfile.TemporaryDirectory() as tmp:
+        res = subprocess.run([prog, tmp], capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stdout + res.stderr
+        conn = meshio.read_mesh_json(os.path.join(tmp, "run_test", "mesh.json"))
+        off = np.fromfile(os.path.join(tmp, "offsets.f64"), dtype=np.float64)
+    assert conn["extents"] == (3, 2, 4)
+    want = meshio.lattice_connectivity(3, 2, 4, 0.5, 0.25, 0.125, vertex_offsets=off, dtype=np.float32)
+    for k in meshio.KEYS:
+        assert np.array_equal(conn[k], want[k]), k
+    _check_all(conn, oracle_lib, np.float32)
+
+
+def test_bad_connectivity_is_refused():
+    conn = meshio.lattice_connectivity(2, 2, 2, 0.5, 0.5, 0.5)
+    bad = {k: v.copy() for k, v in conn.items()}
+    bad["cell_faces"][0, 0] = bad["cell_faces"][1, 5]     # a face that does not list the cell
+    with pytest.raises(capi.CfdError):
+        capi.unstructured_table(bad, "vol")
+    bad = {k: v.copy() for k, v in conn.items()}
+    bad["face_verts"][3, 1] = 10 ** 6
+    with pytest.raises(capi.CfdError):
+        capi.unstructured_table(bad, "vol")
