@@ -24,12 +24,18 @@ template <class T, int TY, bool PW, bool KF, int MODE>
 __global__ void __launch_bounds__(64 * (TY + 2))
 wave_row_aux_k(UniformGeo<T> g, const T* __restrict__ p, const T* __restrict__ u, const T* __restrict__ v,
                const T* __restrict__ w, T* __restrict__ po, T* __restrict__ uo, T* __restrict__ vo,
-               T* __restrict__ wo, T K, const T* __restrict__ Kf, T dt, int L, int nyt, SensorList sens,
+               T* __restrict__ wo, T K, const T* __restrict__ Kf, T dt, int L, int nyt, int order, SensorList sens,
                row::RowAux<T> aux) {
     using V4 = typename row::Row4<T>::type;
     __shared__ V4 sp[TY + 2][64];
     __shared__ V4 sv[TY + 2][64];
-    const int64_t lin = blockIdx.x;
+    // tile order as the fused row kernel's: bit 1 walks the tiles last to
+    // first (alternate steps, so a step starts where the previous one ended),
+    // bit 0 gives each XCD a contiguous run of tiles
+    const int64_t nwg = gridDim.x;
+    int64_t lin = blockIdx.x;
+    if (order & 2) lin = nwg - 1 - lin;
+    if (order & 1) lin = (lin % 8) * (nwg / 8) + lin / 8;
     const int yt = int(lin % nyt);
     const int64_t xc = lin / nyt;
     const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.y));
@@ -55,30 +61,31 @@ namespace {
 
 template <class T, int TY, bool PW, bool KF, int MODE>
 void launch_aux(const UniformGeo<T>& g, const void* const in[4], void* const out[4], T K, const void* Kf, T dt,
-                int L, const SensorList& sens, const row::RowAux<T>& aux, hipStream_t s) {
+                int L, int reverse, const SensorList& sens, const row::RowAux<T>& aux, hipStream_t s) {
     const int64_t rows = int64_t(TY) * row::rows_per_wave(g.Z);
     const int nyt = int((g.Y + rows - 1) / rows);
     const int64_t nwg = int64_t(nyt) * ((g.X + L - 1) / L);
     CFD_REQUIRE(nwg < (int64_t(1) << 31), "adjoint row grid too large");
+    const int order = (nwg % 8 == 0 ? 1 : 0) | (reverse ? 2 : 0);
     wave_row_aux_k<T, TY, PW, KF, MODE><<<dim3(unsigned(nwg)), dim3(64, TY + 2), 0, s>>>(
         g, (const T*)in[0], (const T*)in[1], (const T*)in[2], (const T*)in[3], (T*)out[0], (T*)out[1], (T*)out[2],
-        (T*)out[3], K, (const T*)Kf, dt, L, nyt, sens, aux);
+        (T*)out[3], K, (const T*)Kf, dt, L, nyt, order, sens, aux);
     CFD_HIP(hipGetLastError());
 }
 
 template <class T, int TY, int MODE>
 void launch_aux_ty(const UniformGeo<T>& g, const void* const in[4], void* const out[4], T K, const void* Kf, T dt,
-                   int L, const SensorList& sens, const row::RowAux<T>& aux, hipStream_t s) {
+                   int L, int rev, const SensorList& sens, const row::RowAux<T>& aux, hipStream_t s) {
     if constexpr (MODE == 2) {
-        if (g.vol_pow2) launch_aux<T, TY, true, false, 2>(g, in, out, K, nullptr, dt, L, sens, aux, s);
-        else launch_aux<T, TY, false, false, 2>(g, in, out, K, nullptr, dt, L, sens, aux, s);
+        if (g.vol_pow2) launch_aux<T, TY, true, false, 2>(g, in, out, K, nullptr, dt, L, rev, sens, aux, s);
+        else launch_aux<T, TY, false, false, 2>(g, in, out, K, nullptr, dt, L, rev, sens, aux, s);
     } else {
         if (g.vol_pow2) {
-            if (Kf) launch_aux<T, TY, true, true, MODE>(g, in, out, K, Kf, dt, L, sens, aux, s);
-            else launch_aux<T, TY, true, false, MODE>(g, in, out, K, Kf, dt, L, sens, aux, s);
+            if (Kf) launch_aux<T, TY, true, true, MODE>(g, in, out, K, Kf, dt, L, rev, sens, aux, s);
+            else launch_aux<T, TY, true, false, MODE>(g, in, out, K, Kf, dt, L, rev, sens, aux, s);
         } else {
-            if (Kf) launch_aux<T, TY, false, true, MODE>(g, in, out, K, Kf, dt, L, sens, aux, s);
-            else launch_aux<T, TY, false, false, MODE>(g, in, out, K, Kf, dt, L, sens, aux, s);
+            if (Kf) launch_aux<T, TY, false, true, MODE>(g, in, out, K, Kf, dt, L, rev, sens, aux, s);
+            else launch_aux<T, TY, false, false, MODE>(g, in, out, K, Kf, dt, L, rev, sens, aux, s);
         }
     }
 }
@@ -103,16 +110,16 @@ bool adj_row_usable(const cfd_mesh_s* m, const void* const* fields, int n) {
 
 template <class T>
 void launch_wave_row_aux(const cfd_mesh_s* m, int mode, const void* const in[4], void* const out[4], double K,
-                         const void* Kf, double dt, const SensorList& sens, const row::RowAux<T>& aux,
+                         const void* Kf, double dt, const SensorList& sens, const row::RowAux<T>& aux, int rev,
                          hipStream_t s) {
     const UniformGeo<T>& g = ugeo_of<T>(m);
     const int L = m->adj_L;
     CFD_REQUIRE(mode == 1 || (mode == 2 && Kf == nullptr), "transposed row step: constant K only");
     switch (m->adj_ty * 4 + mode) {
-        case 4 * 4 + 1: launch_aux_ty<T, 4, 1>(g, in, out, T(K), Kf, T(dt), L, sens, aux, s); break;
-        case 4 * 4 + 2: launch_aux_ty<T, 4, 2>(g, in, out, T(K), Kf, T(dt), L, sens, aux, s); break;
-        case 8 * 4 + 1: launch_aux_ty<T, 8, 1>(g, in, out, T(K), Kf, T(dt), L, sens, aux, s); break;
-        case 8 * 4 + 2: launch_aux_ty<T, 8, 2>(g, in, out, T(K), Kf, T(dt), L, sens, aux, s); break;
+        case 4 * 4 + 1: launch_aux_ty<T, 4, 1>(g, in, out, T(K), Kf, T(dt), L, rev, sens, aux, s); break;
+        case 4 * 4 + 2: launch_aux_ty<T, 4, 2>(g, in, out, T(K), Kf, T(dt), L, rev, sens, aux, s); break;
+        case 8 * 4 + 1: launch_aux_ty<T, 8, 1>(g, in, out, T(K), Kf, T(dt), L, rev, sens, aux, s); break;
+        case 8 * 4 + 2: launch_aux_ty<T, 8, 2>(g, in, out, T(K), Kf, T(dt), L, rev, sens, aux, s); break;
         default: throw Error(CFD_E_INVALID, "adjoint row tiles have 4 or 8 output waves");
     }
 }
@@ -120,7 +127,7 @@ void launch_wave_row_aux(const cfd_mesh_s* m, int mode, const void* const in[4],
 #define CFD_AUX_INSTANTIATE(T)                                                                                   \
     template bool adj_row_usable<T>(const cfd_mesh_s*, const void* const*, int);                                \
     template void launch_wave_row_aux<T>(const cfd_mesh_s*, int, const void* const[4], void* const[4], double,  \
-                                         const void*, double, const SensorList&, const row::RowAux<T>&,          \
+                                         const void*, double, const SensorList&, const row::RowAux<T>&, int,     \
                                          hipStream_t);
 CFD_AUX_INSTANTIATE(float)
 CFD_AUX_INSTANTIATE(double)

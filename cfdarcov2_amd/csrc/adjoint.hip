@@ -38,7 +38,7 @@ namespace cfd {
 template <class T> bool adj_row_usable(const cfd_mesh_s* m, const void* const* fields, int n);
 template <class T>
 void launch_wave_row_aux(const cfd_mesh_s* m, int mode, const void* const in[4], void* const out[4], double K,
-                         const void* Kf, double dt, const SensorList& sens, const row::RowAux<T>& aux,
+                         const void* Kf, double dt, const SensorList& sens, const row::RowAux<T>& aux, int rev,
                          hipStream_t s);
 
 namespace {
@@ -287,7 +287,7 @@ void run_wave_traces(const cfd_mesh_s* m, void* const a[4], void* const b[4], do
         T* tr = traces ? (T*)traces + t * sens.n : nullptr;
         if (rowf) {
             const row::RowAux<T> aux{T(0), tr, nullptr, nullptr, nullptr, nullptr};
-            launch_wave_row_aux<T>(m, 1, (const void* const*)in, out, K, Kf, dt, sens, aux, s);
+            launch_wave_row_aux<T>(m, 1, (const void* const*)in, out, K, Kf, dt, sens, aux, int(t & 1), s);
         } else {
             launch_wave_trace_step<T>(m, (const void* const*)in, out, K, Kf, dt, sens, tr, s);
         }
@@ -318,7 +318,7 @@ static void run_wave_adjoint_row(const cfd_mesh_s* m, void* const la[4], void* c
             void* const* out = (done & 1) ? la : lb;
             const T* src = (tgrad && t >= 2) ? (const T*)tgrad + (t - 2) * sens.n : nullptr;
             const row::RowAux<T> aux{dK, nullptr, nullptr, src, (const T*)s_in, s_in ? (T*)gK : nullptr};
-            launch_wave_row_aux<T>(m, 2, (const void* const*)in, out, K, nullptr, dt, sens, aux, s);
+            launch_wave_row_aux<T>(m, 2, (const void* const*)in, out, K, nullptr, dt, sens, aux, int(done & 1), s);
             ++done;
         };
         int64_t t = nsteps;
@@ -339,7 +339,7 @@ static void run_wave_adjoint_row(const cfd_mesh_s* m, void* const la[4], void* c
                 }
                 const row::RowAux<T> aux{T(0), nullptr, (T*)((char*)ws + size_t(k) * field), nullptr, nullptr,
                                          nullptr};
-                launch_wave_row_aux<T>(m, 1, in, out, K, nullptr, dt, sens, aux, s);
+                launch_wave_row_aux<T>(m, 1, in, out, K, nullptr, dt, sens, aux, int(k & 1), s);
             }
             for (int64_t tt = t; tt > t0; --tt) adj_step(tt, (char*)ws + size_t(tt - t0 - 1) * field);
             t = t0;
