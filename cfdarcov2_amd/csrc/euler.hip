@@ -7,7 +7,7 @@
 // evaluates the six face fluxes in the lower-cell -> upper-cell orientation (so
 // both cells of a face compute bit-identical fluxes and the scheme stays
 // conservative without scatter or atomics on the fields), and updates
-//   U <- U - (dt * net) / V,  net = ((Gx+ - Gx-) + (Gy+ - Gy-)) + (Gz+ - Gz-).
+//   U <- U - (dt * net) * (1 / V),  net = ((Gx+ - Gx-) + (Gy+ - Gy-)) + (Gz+ - Gz-).
 //
 // Time step, all on the device: the step kernel reads dt from a rotating slot,
 // and from the cell's new state computes V / sum_f A_f (|u.n_f| + a), reduces
@@ -40,14 +40,18 @@ template <class T> struct EOut { T* u[5]; };
 
 template <class T> struct Prim { T ux, uy, uz, p, a; };
 
+// Primitive state with one division: the reciprocal density multiplies the
+// momenta and the sound-speed argument (the oracle's euler_prim, operation
+// for operation).
 template <class T> __device__ inline Prim<T> prim(T gamma, const T U[5]) {
     Prim<T> q;
-    q.ux = U[1] / U[0];
-    q.uy = U[2] / U[0];
-    q.uz = U[3] / U[0];
+    const T ir = T(1) / U[0];
+    q.ux = U[1] * ir;
+    q.uy = U[2] * ir;
+    q.uz = U[3] * ir;
     const T q2 = (q.ux * q.ux + q.uy * q.uy) + q.uz * q.uz;
     q.p = (gamma - T(1)) * (U[4] - (T(0.5) * U[0]) * q2);
-    q.a = sqrt((gamma * q.p) / U[0]);
+    q.a = sqrt((gamma * q.p) * ir);
     return q;
 }
 
@@ -193,10 +197,11 @@ __global__ void __launch_bounds__(256) euler_step_k(FvGeo<T> g, EIn<T> in, EOut<
             }
         }
         const T V = g.vol[c];
+        const T iV = T(1) / V;
         T Un[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            Un[k] = Ui[k] - (dt * net[k]) / V;
+            Un[k] = Ui[k] - (dt * net[k]) * iV;
             out.u[k][c] = Un[k];
         }
         loc = local_dt(g, x, y, z, c, prim(gamma, Un));
@@ -364,11 +369,12 @@ __global__ void __launch_bounds__(64 * TY) euler_march_k(FvGeo<T> g, EIn<T> in, 
             rusanov(Uc, qc, Um, qm, fg[1][0], fg[1][1], fg[1][2], fg[1][3], G2);
             const int64_t c = x * SX + col;
             const T V = g.vol[c];
+            const T iV = T(1) / V;
             T Unew[5];
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
                 const T net = acc[k] + (G2[k] - G1[k]);
-                Unew[k] = Uc[k] - (dt * net) / V;
+                Unew[k] = Uc[k] - (dt * net) * iV;
                 out.u[k][c] = Unew[k];
             }
             loc = fmin(loc, local_dt_faces(V, prim(gamma, Unew), fg));
@@ -454,21 +460,16 @@ void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamm
                 continue;
             }
             const dim3 mgrid{unsigned(nwg), 1u, 1u};
+#define CFD_EULER_MARCH(TYV)                                                                                    \
+    euler_march_k<T, TYV><<<mgrid, dim3(64, TYV), 0, s>>>(g, in, out, T(gamma), T(cfl), cur, nxt, rst, (T*)dt_hist, \
+                                                         st, L, nzt, nyt)
             switch (ty) {
-                case 2:
-                    euler_march_k<T, 2><<<mgrid, dim3(64, 2), 0, s>>>(g, in, out, T(gamma), T(cfl), cur, nxt, rst,
-                                                                      (T*)dt_hist, st, L, nzt, nyt);
-                    break;
-                case 4:
-                    euler_march_k<T, 4><<<mgrid, dim3(64, 4), 0, s>>>(g, in, out, T(gamma), T(cfl), cur, nxt, rst,
-                                                                      (T*)dt_hist, st, L, nzt, nyt);
-                    break;
-                case 8:
-                    euler_march_k<T, 8><<<mgrid, dim3(64, 8), 0, s>>>(g, in, out, T(gamma), T(cfl), cur, nxt, rst,
-                                                                      (T*)dt_hist, st, L, nzt, nyt);
-                    break;
+                case 2: CFD_EULER_MARCH(2); break;
+                case 4: CFD_EULER_MARCH(4); break;
+                case 8: CFD_EULER_MARCH(8); break;
                 default: throw Error(CFD_E_INVALID, "euler march tile rows must be 2, 4 or 8");
             }
+#undef CFD_EULER_MARCH
         }
         CFD_HIP(hipGetLastError());
     } catch (...) {

@@ -68,8 +68,8 @@ SHAPES = [(0, 4, 16), (1, 2, 5), (1, 4, 16), (1, 8, 3), (1, 4, 64)]
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("shape", SHAPES, ids=[f"v{a}-ty{b}-L{c}" for a, b, c in SHAPES])
 def test_euler_kernel_shapes(hip_ctx, oracle_lib, shape, dtype):
-    """Per-thread and 2.5D-march forms, partial tiles in y and z and chunks
-    that do not divide x: the oracle's bits, fields and time steps."""
+    """Per-thread and 2.5D-march forms, partial tiles in y and z, two z tiles
+    and chunks that do not divide x: the oracle's bits, fields and time steps."""
     n, steps = (23, 9, 70), 9
     print(f"\n[case] euler shape {shape} {np.dtype(dtype).name}", flush=True)
     d = (1.0 / 23,) * 3
