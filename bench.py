@@ -45,9 +45,13 @@ def _traffic(n, dtype, shape):
     return None
 
 
-def cpu_baseline(case, dtype, steps, seconds=12.0):
-    """The oracle (reference-faithful CPU evaluation: every operator materialised
-    over the whole mesh, equations in order) on `steps` steps of the same mesh."""
+def cpu_baseline(case, dtype, steps, seconds=7.0):
+    """The CPU restatement of the path on the same mesh, timed on this host:
+    the fused loop (one pass per stage) with every core -- the reported
+    baseline, the CPU's honest best -- next to the operator-by-operator
+    evaluation (every operator materialised over the whole mesh, as the
+    reference's CPU path evaluates) and the fused loop on one core.  Each
+    variant runs a sample of the benchmark's steps sized to about `seconds`."""
     import numpy as np
 
     from oracle import pyoracle
@@ -56,18 +60,35 @@ def cpu_baseline(case, dtype, steps, seconds=12.0):
     om = pyoracle.OracleMesh(*case.n, *case.spacing, dtype=npdt)
     z = np.zeros(om.N, dtype=npdt)
     p0 = case.p0.astype(npdt)
-    # a short probe sizes the sample to about `seconds` of CPU work
-    t0 = time.perf_counter()
-    om.wave_run(p0, z, z, z, -case.c ** 2, case.dt, steps)
-    probe = time.perf_counter() - t0
-    steps = max(steps, min(500, int(seconds * steps / max(probe, 1e-9))))
-    t0 = time.perf_counter()
-    om.wave_run(p0, z, z, z, -case.c ** 2, case.dt, steps)
-    t = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": om.N * steps / t / 1e6, "unit": "MCUPS", "cores": cores, "kind": "port",
-            "sample": f"oracle wave_run, {case.n[0]}^3 {dtype}, first {steps} steps of the benchmark workload, "
-                      f"{t:.1f} s"}
+
+    def timed(fn, base_steps):
+        t0 = time.perf_counter()
+        fn(base_steps)
+        probe = time.perf_counter() - t0
+        n = max(base_steps, min(500, int(seconds * base_steps / max(probe, 1e-9))))
+        t0 = time.perf_counter()
+        fn(n)
+        t = time.perf_counter() - t0
+        return om.N * n / t / 1e6, n, t
+
+    run_op = lambda n: om.wave_run(p0, z, z, z, -case.c ** 2, case.dt, n)
+    run_fused = lambda n: om.wave_run_fused(p0, z, z, z, -case.c ** 2, case.dt, n)
+    fused, nf, tf = timed(run_fused, steps)
+    op, no, to = timed(run_op, steps)
+    prev = pyoracle.set_threads(1)
+    try:
+        one, n1, t1 = timed(run_fused, 1)
+    finally:
+        pyoracle.set_threads(prev)
+    return {"value": fused, "unit": "MCUPS", "cores": cores, "kind": "port",
+            "sample": f"fused CPU loop (oracle wave_run_fused), {case.n[0]}^3 {dtype}, first {nf} steps of the "
+                      f"benchmark workload, {tf:.1f} s",
+            "variants": {"fused_all_cores": fused, "operator_by_operator_all_cores": op,
+                         "fused_one_core": one,
+                         "samples": {"fused_all_cores": f"{nf} steps, {tf:.1f} s",
+                                     "operator_by_operator_all_cores": f"{no} steps, {to:.1f} s",
+                                     "fused_one_core": f"{n1} steps, {t1:.1f} s"}}}
 
 
 def _events():
