@@ -514,3 +514,27 @@ def test_d2t_continuum_standing_wave(oracle_lib):
         want = shape * (np.cos(omega * t) - omega * np.sin(omega * t))
         assert np.max(np.abs(warm - want)) < 4 * omega ** 2 * dt
     assert errs[1] < 1e-3 and errs[0] / errs[1] > 3.5   # second-order convergence
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_fused_cpu_stepper_equals_operator_stepper(oracle_lib, dtype):
+    """The CPU baseline's fused loop (wave_run_fused) takes the same steps as
+    the operator-by-operator stepper bit for bit, on a perturbed mesh with a K
+    field, with one thread and with several."""
+    from cfdarcov2_amd import workloads
+    n, d = (9, 7, 11), (1 / 8,) * 3
+    off = workloads.perturbed_vertex_offsets(*n, *d, seed=2, frac=0.2)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=dtype, perturb=off)
+    rng = np.random.default_rng(3)
+    p, u, v = (rng.standard_normal(om.N).astype(dtype) for _ in range(3))
+    w = np.zeros_like(p)
+    kf = -(0.5 + rng.random(om.N)).astype(dtype)
+    ref = om.wave_run(p, u, v, w, 0.0, 0.01, 5, K_field=kf)
+    prev = oracle_lib.set_threads(1)
+    try:
+        one = om.wave_run_fused(p, u, v, w, 0.0, 0.01, 5, K_field=kf)
+    finally:
+        oracle_lib.set_threads(prev)
+    many = om.wave_run_fused(p, u, v, w, 0.0, 0.01, 5, K_field=kf)
+    for a, b, c in zip(ref, one, many):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
