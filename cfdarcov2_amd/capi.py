@@ -20,7 +20,8 @@ F32, F64 = 0, 1
 H2D, D2H, D2D = 1, 2, 3
 
 EW = {"add": 0, "sub": 1, "mul": 2, "div": 3, "neg": 4, "muls": 5, "divs": 6, "smul": 7,
-      "axpy": 8, "leap": 9, "fill": 10, "copy": 11, "adds": 12, "sadd": 13, "subs": 14, "ssub": 15, "sdiv": 16}
+      "axpy": 8, "leap": 9, "fill": 10, "copy": 11, "adds": 12, "sadd": 13, "subs": 14, "ssub": 15, "sdiv": 16,
+      "exp": 17, "abs": 18}
 
 TABLES = {"ids": 0, "ids_bf": 1, "nx": 2, "ny": 3, "nz": 4, "nax": 5, "nay": 6, "naz": 7,
           "area": 8, "alpha": 9, "len": 10, "vol": 11, "is_bnd": 12, "ccx": 13, "ccy": 14, "ccz": 15,

@@ -16,6 +16,10 @@ CudaDataMatrixD mul_mtrx(const CudaDataMatrixD& a, float b);
 CudaDataMatrixD div_mtrx(const CudaDataMatrixD& a, const CudaDataMatrixD& b);
 CudaDataMatrixD neg_mtrx(const CudaDataMatrixD& a);
 CudaDataMatrixD div_const(const CudaDataMatrixD& a, float b);
+// exp and abs of a field: no device form in the reference (its exp/abs run
+// on the host only, src/fvm3d.cpp:472-494)
+CudaDataMatrixD exp_mtrx(const CudaDataMatrixD& a);
+CudaDataMatrixD abs_mtrx(const CudaDataMatrixD& a);
 
 CudaDataMatrixD eval_grad2(CudaMesh3D* mesh, const CudaDataMatrixD& a, bool clc_x, bool clc_y, bool clc_z);
 CudaDataMatrixD eval_grad(CudaMesh3D* mesh, const CudaDataMatrixD& a, bool clc_x, bool clc_y, bool clc_z);

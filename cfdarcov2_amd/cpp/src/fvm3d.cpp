@@ -136,9 +136,12 @@ CudaDataMatrixD evaluate_node(const Node& n) {
         case NodeKind::Neg:
             return neg_mtrx(evaluate_node(*n.a));
         case NodeKind::Exp:
+            // the reference evaluates exp and abs on the host only and throws
+            // on its device path (src/fvm3d.cpp:472-494); here they are
+            // elementwise kernels
+            return exp_mtrx(evaluate_node(*n.a));
         case NodeKind::Abs:
-            // as the reference's device path (src/fvm3d.cpp:472-494)
-            throw std::runtime_error("Not implemented");
+            return abs_mtrx(evaluate_node(*n.a));
         case NodeKind::Grad:
             return eval_grad(cuda_mesh(node_mesh(n)), evaluate_node(*n.a), n.mask & 1, n.mask & 2, n.mask & 4);
         case NodeKind::Grad2:

@@ -258,11 +258,10 @@ struct KernelBuilder::Impl {
                 walk(*n.b, in_stencil);
                 return;
             case NodeKind::Neg:
-                walk(*n.a, in_stencil);
-                return;
             case NodeKind::Exp:
             case NodeKind::Abs:
-                throw std::runtime_error("Not implemented");
+                walk(*n.a, in_stencil);
+                return;
             case NodeKind::Grad:
             case NodeKind::Grad2:
                 if (in_stencil) {
@@ -323,7 +322,10 @@ struct KernelBuilder::Impl {
     }
 
     static size_t count_ops(const Node& n) {
-        const size_t own = (n.kind == NodeKind::Binary || n.kind == NodeKind::Neg) ? 1 : 0;
+        const size_t own = (n.kind == NodeKind::Binary || n.kind == NodeKind::Neg || n.kind == NodeKind::Exp ||
+                            n.kind == NodeKind::Abs)
+                               ? 1
+                               : 0;
         return own + (n.a ? count_ops(*n.a) : 0) + (n.b ? count_ops(*n.b) : 0);
     }
 
@@ -346,6 +348,11 @@ struct KernelBuilder::Impl {
                 return "(" + render(*n.a, idx) + " " + n.op + " " + render(*n.b, idx) + ")";
             case NodeKind::Neg:
                 return "(-" + render(*n.a, idx) + ")";
+            case NodeKind::Exp:
+                // the device libm's exp, as the operator-by-operator path
+                return "expf(" + render(*n.a, idx) + ")";
+            case NodeKind::Abs:
+                return "fabsf(" + render(*n.a, idx) + ")";
             case NodeKind::Grad:
             case NodeKind::Grad2:
             case NodeKind::Stab: {

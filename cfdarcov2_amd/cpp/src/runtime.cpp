@@ -256,6 +256,8 @@ CudaDataMatrixD mul_mtrx(const CudaDataMatrixD& a, float b) { return ewise_s(CFD
 CudaDataMatrixD div_mtrx(const CudaDataMatrixD& a, const CudaDataMatrixD& b) { return ewise2(CFD_EW_DIV, a, b); }
 CudaDataMatrixD neg_mtrx(const CudaDataMatrixD& a) { return ewise_s(CFD_EW_NEG, a, 0.f); }
 CudaDataMatrixD div_const(const CudaDataMatrixD& a, float b) { return ewise_s(CFD_EW_DIVS, a, b); }
+CudaDataMatrixD exp_mtrx(const CudaDataMatrixD& a) { return ewise_s(CFD_EW_EXP, a, 0.f); }
+CudaDataMatrixD abs_mtrx(const CudaDataMatrixD& a) { return ewise_s(CFD_EW_ABS, a, 0.f); }
 
 CudaDataMatrixD eval_grad(CudaMesh3D* mesh, const CudaDataMatrixD& a, bool x, bool y, bool z) {
     CudaDataMatrixD res{a._size};

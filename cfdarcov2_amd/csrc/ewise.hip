@@ -11,6 +11,21 @@
 
 namespace cfd {
 
+// exp from the device libm in the operand's precision; abs as the sign bit
+// cleared (exact, -0 -> +0, NaN stays NaN)
+template <class T> __device__ inline T ew_exp(T a) {
+    if constexpr (sizeof(T) == 4)
+        return expf(a);
+    else
+        return ::exp(a);
+}
+template <class T> __device__ inline T ew_abs(T a) {
+    if constexpr (sizeof(T) == 4)
+        return fabsf(a);
+    else
+        return ::fabs(a);
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) ewise_k(int op, const T* __restrict__ a, const T* __restrict__ b,
                                                const T* __restrict__ aux, T s, T* __restrict__ out, int64_t n) {
@@ -34,6 +49,8 @@ __global__ void __launch_bounds__(256) ewise_k(int op, const T* __restrict__ a, 
             case CFD_EW_SUBS: r = a[i] - s; break;
             case CFD_EW_SSUB: r = s - a[i]; break;
             case CFD_EW_SDIV: r = s / a[i]; break;
+            case CFD_EW_EXP: r = ew_exp(a[i]); break;
+            case CFD_EW_ABS: r = ew_abs(a[i]); break;
             default: r = a[i]; break;  // CFD_EW_COPY
         }
         out[i] = r;
@@ -60,6 +77,8 @@ template <int OP, class T> __device__ inline T ew_one(T a, T b, T aux, T s) {
         case CFD_EW_SUBS: return a - s;
         case CFD_EW_SSUB: return s - a;
         case CFD_EW_SDIV: return s / a;
+        case CFD_EW_EXP: return ew_exp(a);
+        case CFD_EW_ABS: return ew_abs(a);
         default: return a;  // CFD_EW_COPY
     }
 }
@@ -130,6 +149,8 @@ static void dispatch_vec(int op, const void* a, const void* b, const void* aux, 
         case CFD_EW_SUBS: launch_vec<T, CFD_EW_SUBS>(a, b, aux, s, out, n, st); break;
         case CFD_EW_SSUB: launch_vec<T, CFD_EW_SSUB>(a, b, aux, s, out, n, st); break;
         case CFD_EW_SDIV: launch_vec<T, CFD_EW_SDIV>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_EXP: launch_vec<T, CFD_EW_EXP>(a, b, aux, s, out, n, st); break;
+        case CFD_EW_ABS: launch_vec<T, CFD_EW_ABS>(a, b, aux, s, out, n, st); break;
         default: launch_vec<T, CFD_EW_COPY>(a, b, aux, s, out, n, st); break;
     }
 }

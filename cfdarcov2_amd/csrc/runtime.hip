@@ -646,7 +646,7 @@ int cfd_ewise(cfd_ctx_t ctx, int op, int dtype, const void* a, const void* b, co
     return guarded([&] {
         CFD_REQUIRE(ctx != nullptr && out != nullptr, "cfd_ewise: null argument");
         check_dtype(dtype);
-        CFD_REQUIRE(op >= CFD_EW_ADD && op <= CFD_EW_SDIV, "cfd_ewise: unknown op");
+        CFD_REQUIRE(op >= CFD_EW_ADD && op <= CFD_EW_ABS, "cfd_ewise: unknown op");
         CFD_REQUIRE(n >= 0, "cfd_ewise: negative length");
         const bool needs_a = op != CFD_EW_FILL;
         const bool needs_b = op == CFD_EW_ADD || op == CFD_EW_SUB || op == CFD_EW_MUL || op == CFD_EW_DIV ||

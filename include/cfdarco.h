@@ -209,6 +209,13 @@ int cfd_stab_noalloc(cfd_mesh_t mesh, const void* phi, void* out, int axis_mask,
 #define CFD_EW_SUBS 14      /* out = a - s */
 #define CFD_EW_SSUB 15      /* out = s - a */
 #define CFD_EW_SDIV 16      /* out = s / a */
+/* exp and abs of a field (Variable::exp / exp / abs, src/fvm3d.cpp:472-494;
+ * the reference evaluates them on the host only and throws on its device
+ * path).  exp is the device libm's (within 1 ulp of the exact value in FP32
+ * and FP64, so within 2 ulp of any other faithful exp such as the host's);
+ * abs clears the sign bit and is exact. */
+#define CFD_EW_EXP 17       /* out = exp(a) */
+#define CFD_EW_ABS 18       /* out = |a| */
 int cfd_ewise(cfd_ctx_t ctx, int op, int dtype, const void* a, const void* b, const void* aux, double s,
               void* out, int64_t n, void* stream);
 

@@ -120,6 +120,22 @@ int main(int argc, char** argv) {
         };
         Equation(size_t(in.steps)).evaluate(vars, eqs, &dt, false, vars);
         write_output(argv[3], vars);
+    } else if (kase.rfind("expabs", 0) == 0) {
+        // exp and abs on the device (the reference's exp/abs, src/fvm3d.cpp:
+        // 472-494, run on the host only), inside fused stencil expressions
+        // and as assignment operands
+        auto x = Variable(mesh.get(), F.at(0), keep, keep_cu, "x");
+        auto y = Variable(mesh.get(), F.at(1), keep, keep_cu, "y");
+        auto z = Variable(mesh.get(), F.at(2), keep, keep_cu, "z");
+        std::vector<Variable*> vars{&x, &y, &z};
+        auto dt = DT(mesh.get(), UpdatePolicies::constant_dt, UpdatePolicies::constant_dt_cu, dtv, vars);
+        EquationTemplate eqs = {
+            {&y, '=', exp(-abs(x) * 0.5f) * y, jit},
+            {&z, '=', abs(d1dx(x)) - abs(x) * z, jit},
+            {d1t(x), '=', -abs(d1dy(x)), jit},
+        };
+        Equation(size_t(in.steps)).evaluate(vars, eqs, &dt, false, vars);
+        write_output(argv[3], vars);
     } else if (kase.rfind("d2t", 0) == 0) {
         // examples/main_3d_wave.cu: d2t(u) = c*c*(d2dx(u) + d2dy(u)); host
         // boundary callback (host round trip every step)

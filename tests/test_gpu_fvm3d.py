@@ -104,6 +104,34 @@ def test_acoustic_system_through_api(oracle_lib, case):
         assert np.array_equal(g, r), name
 
 
+@pytest.mark.parametrize("case", ["expabs", "expabs_nojit"])
+def test_exp_abs_through_api(oracle_lib, case):
+    """exp and abs on the device.  abs is exact, so x and z, which do not
+    depend on exp, match bit for bit; y = exp(-|x| / 2) * y takes the
+    device exp, within 1 ulp of the exact value, against exp in float64
+    rounded to float32: each of the 4 steps adds at most 1 ulp (exp) and 1/2
+    ulp (rounded product) of relative error on each side, so |got - ref| <=
+    16 ulp of the result bounds it."""
+    n, d = (10, 12, 8), (1 / 16, 1 / 16, 1 / 16)
+    om = _mesh(oracle_lib, n, d)
+    rng = np.random.default_rng(21)
+    N = n[0] * n[1] * n[2]
+    x0 = (3.0 * rng.standard_normal(N)).astype(f32)
+    y0, z0 = rng.standard_normal(N).astype(f32), rng.standard_normal(N).astype(f32)
+    dt, steps = f32(0.01), 4
+    got = _run_case(case, n, d, float(dt), steps, [x0, y0, z0], 3)
+    x, y, z = x0.copy(), y0.copy(), z0.copy()
+    for _ in range(steps):
+        e = np.exp(((-np.abs(x)) * f32(0.5)).astype(np.float64)).astype(f32)
+        y = e * y
+        z = np.abs(om.grad(x, 1)) - np.abs(x) * z
+        x = dt * (-np.abs(om.grad(x, 2))) + x
+    assert np.array_equal(got[0], x)
+    assert np.array_equal(got[2], z)
+    ulps = np.abs(got[1] - y) / np.spacing(np.abs(y))
+    assert np.all(ulps <= 16), float(ulps.max())
+
+
 @pytest.mark.parametrize("case,rule", [("d2t", 0), ("d2t_nojit", 2), ("d2t_nojit_store", 1)])
 def test_second_order_time(oracle_lib, case, rule):
     """d2t: fused equations run two Euler-form steps then leapfrog; operator-by-

@@ -307,6 +307,52 @@ def test_ewise_and_reductions(hip_ctx, oracle_lib):
     assert hip_ctx.count_nonfinite(T(x)) == 3
 
 
+def _ulps(got, ref):
+    """Distance in units in the last place, counted on the ordered integer
+    image of the floats (0 for equal values, NaN == NaN, +0 == -0)."""
+    it = np.int32 if got.dtype == np.float32 else np.int64
+    def key(a):
+        i = a.view(it).astype(np.int64)
+        return np.where(i < 0, np.iinfo(it).min - i, i)
+    d = np.abs(key(got) - key(ref))
+    d[np.isnan(got) & np.isnan(ref)] = 0
+    return d
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_ewise_exp_abs(hip_ctx, dtype):
+    """exp and abs (no device form in the reference, whose exp/abs run on the
+    host only).  abs is exact: bit for bit against numpy.  exp is the device
+    libm's, within 1 ulp of the exact value; it is held to 2 ulp of numpy's
+    exp (in float64, itself within 1 ulp; for FP32 rounded from float64), and
+    exp(0) = 1, exp(-inf) = 0, exp(inf) = inf, overflow to inf and NaN
+    propagation exactly."""
+    rng = np.random.default_rng(17)
+    lo, hi = (-87.0, 88.0) if dtype == np.float32 else (-708.0, 709.0)
+    for n, off in ((4099, 0), (999, 1)):  # vector kernel with a tail; unaligned element kernel
+        a = rng.uniform(lo, hi, n + off)
+        a[off: off + 200] = rng.standard_normal(200)  # the common range as well
+        special = [0.0, -0.0, np.inf, -np.inf, np.nan, 1e4, -1e4, 1.0, -1.0]
+        a[off + 200: off + 200 + len(special)] = special
+        a = a.astype(dtype)
+        ta = _t(a, dtype)[off:]
+        a = a[off:]
+        out = torch.empty(n + off, dtype=ta.dtype, device="cuda")[off:]
+        hip_ctx.ewise("abs", ta, None, None, 0.0, out)
+        got = _np(out)
+        assert np.array_equal(got.view(np.uint8), np.abs(a).view(np.uint8)), (np.dtype(dtype).name, n, off)
+        hip_ctx.ewise("exp", ta, None, None, 0.0, out)
+        got = _np(out)
+        with np.errstate(over="ignore"):
+            ref = np.exp(a.astype(np.float64)).astype(dtype)
+        u = _ulps(got, ref)
+        assert u.max() <= 2, (np.dtype(dtype).name, n, off, int(u.max()), a[np.argmax(u)])
+        for x, want in ((0.0, 1.0), (-0.0, 1.0), (np.inf, np.inf), (-np.inf, 0.0), (1e4, np.inf), (-1e4, 0.0)):
+            i = np.flatnonzero((a == x) & (np.signbit(a) == np.signbit(x)))
+            assert i.size and np.all(got[i] == want), (x, got[i])
+        assert np.all(np.isnan(got[np.isnan(a)]))
+
+
 def test_error_paths(hip_ctx):
     """C-ABI misuse and pool exhaustion come back as status codes."""
     from cfdarcov2_amd import capi
