@@ -60,10 +60,10 @@ __global__ void __launch_bounds__(64 * TYT) batch_step_k(const BatchCaseDev* __r
         if (cs.g.vol_pow2) heat_march_tile<float, TYT, R, true>(cs.g, in[0], out[0], cs.coef, cs.dt, L, zt, yt, xc, lds.h);
         else heat_march_tile<float, TYT, R, false>(cs.g, in[0], out[0], cs.coef, cs.dt, L, zt, yt, xc, lds.h);
     } else if (cs.g.vol_pow2) {
-        wave_march_tile<float, TYT, R, false, 1, true>(cs.g, in[0], in[1], in[2], in[3], out[0], out[1], out[2],
+        wave_march_tile<float, TYT, R, false, true>(cs.g, in[0], in[1], in[2], in[3], out[0], out[1], out[2],
                                                        out[3], cs.coef, nullptr, cs.dt, L, zt, yt, xc, lds.w);
     } else {
-        wave_march_tile<float, TYT, R, false, 1, false>(cs.g, in[0], in[1], in[2], in[3], out[0], out[1], out[2],
+        wave_march_tile<float, TYT, R, false, false>(cs.g, in[0], in[1], in[2], in[3], out[0], out[1], out[2],
                                                         out[3], cs.coef, nullptr, cs.dt, L, zt, yt, xc, lds.w);
     }
 }

@@ -14,8 +14,10 @@
 // (stencil.hip), so ns steps here equal ns single steps bit for bit.
 //
 // Threads: 64 lanes along z (one region row per wave pass) x 16 waves over the
-// region's (x, y) rows (enough waves per CU to hide the LDS and memory
-// latency: the boxes are few, about one workgroup per CU at 64^3).
+// region's (x, y) rows (enough waves per CU to hide the LDS latency: the boxes
+// are few, about one workgroup per CU at 64^3).  A launch lasts a few memory
+// latencies (the region's loads, issued together, and the stores) plus ns
+// passes over LDS.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
@@ -44,9 +46,22 @@ __global__ void __launch_bounds__(64 * kMultiWaves) heat_multi_k(UniformGeo<T> g
     const int nry = gy1 - gy0;
     auto slot = [&](int x, int y, int z) { return ((x - gx0) * R + (y - gy0)) * 64 + (z - gz0); };
 
-    for (int r = wy; r < (gx1 - gx0) * nry; r += kMultiWaves) {
+    // every load of the region is issued before the first LDS store, so the
+    // wave waits for the memory latency once rather than once per row
+    constexpr int NR = (R * R + kMultiWaves - 1) / kMultiWaves;
+    const int nrows = (gx1 - gx0) * nry;
+    T v[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const int r = wy + i * kMultiWaves;
         const int x = gx0 + r / nry, y = gy0 + r % nry, z = gz0 + lane;
-        if (z < gz1) buf[0][slot(x, y, z)] = Tin[(int64_t(x) * Y + y) * Z + z];
+        v[i] = (r < nrows && z < gz1) ? Tin[(int64_t(x) * Y + y) * Z + z] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const int r = wy + i * kMultiWaves;
+        const int x = gx0 + r / nry, y = gy0 + r % nry, z = gz0 + lane;
+        if (r < nrows && z < gz1) buf[0][slot(x, y, z)] = v[i];
     }
     __syncthreads();
 

@@ -74,7 +74,7 @@ template <class T, int TYT, int R> struct MarchLds {
 // batched ensemble kernel.
 // NT: stream the outputs past the L2 (non-temporal stores), leaving its
 // capacity to the halo rows that neighbouring tiles read again.
-template <class T, int TYT, int R, bool NT = false, int PF = 1, bool PW = false>
+template <class T, int TYT, int R, bool NT = false, bool PW = false>
 __device__ inline void wave_march_tile(const UniformGeo<T>& g, const T* __restrict__ p, const T* __restrict__ u,
                                        const T* __restrict__ v, const T* __restrict__ w, T* __restrict__ po,
                                        T* __restrict__ uo, T* __restrict__ vo, T* __restrict__ wo, T K,
@@ -167,7 +167,8 @@ __device__ inline void wave_march_tile(const UniformGeo<T>& g, const T* __restri
         }
     }
 
-    // ---- prefetch registers: the loads of plane x are issued PF planes ahead --
+    // ---- prefetch registers: the loads of plane x + 1 are issued before the
+    // arithmetic of plane x
     struct Fetch {
         T pp2[R], un[R], vn[R], wn[R], ch[R], wlo[R], whi[R];
         T rh, vh;
@@ -194,11 +195,9 @@ __device__ inline void wave_march_tile(const UniformGeo<T>& g, const T* __restri
         for (int r = 0; r < R; ++r) f.pp2[r] = f.un[r] = f.vn[r] = f.wn[r] = f.ch[r] = f.wlo[r] = f.whi[r] = T(0);
         f.rh = f.vh = T(0);
     };
-    Fetch fa, fb;
+    Fetch fa;
     clear(fa);
-    clear(fb);
     fetch(x0, fa);
-    if (PF == 2 && x0 + 1 < x1) fetch(x0 + 1, fb);
 
     int b = 0;
     for (int64_t x = x0; x < x1; ++x, b ^= 1) {
@@ -209,12 +208,7 @@ __device__ inline void wave_march_tile(const UniformGeo<T>& g, const T* __restri
             ch[r] = fa.ch[r]; wlo[r] = fa.wlo[r]; whi[r] = fa.whi[r];
         }
         const T rh = fa.rh, vh = fa.vh;
-        if (PF == 2) {
-            fa = fb;
-            if (x + 2 < x1) fetch(x + 2, fb);
-        } else if (x + 1 < x1) {
-            fetch(x + 1, fa);
-        }
+        if (x + 1 < x1) fetch(x + 1, fa);
 
         T* P = sp[b];
         T* SV = sv[b];

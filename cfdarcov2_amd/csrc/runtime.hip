@@ -888,7 +888,8 @@ int cfd_mesh_set_wave_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk,
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");
         require_lattice(m, "cfd_mesh_set_wave_shape");
-        CFD_REQUIRE(variant >= 0 && variant <= 12, "wave variant must be 0 to 12");
+        CFD_REQUIRE(variant == 0 || variant == 1 || (variant >= 7 && variant <= 9) || variant == 11 || variant == 12,
+                    "wave variant must be 0, 1, 7, 8, 9, 11 or 12");
         const bool row_form = variant >= 7;
         if (variant >= 11) {
             if (m->dtype == CFD_F32)
@@ -900,18 +901,13 @@ int cfd_mesh_set_wave_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk,
                 throw Error(CFD_E_UNSUPPORTED, "the row form needs a z extent that is a multiple of 4 and at most 256");
         } else if (row_form) {
             CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || (tile_rows == 14 && m->dtype == CFD_F32),
-                        "row-form (7 to 10) tile_rows must be 4 or 8 (or 14 in FP32)");
+                        "row-form (7 to 9) tile_rows must be 4 or 8 (or 14 in FP32)");
             if (!wave_row_supported(m->Z))
-                throw Error(CFD_E_UNSUPPORTED, "the row form (variants 7 to 10) needs a z extent that is a multiple of 4 and at most 256");
-        } else if (variant == 5 || variant == 6)
-            CFD_REQUIRE(tile_rows == 4, "variant 5 and 6 tile_rows must be 4");
-        else if (variant == 4)
-            CFD_REQUIRE(tile_rows == 4 || tile_rows == 8, "variant 4 tile_rows must be 4 or 8");
-        else if (variant == 3)
-            CFD_REQUIRE(tile_rows == 2 || tile_rows == 4 || tile_rows == 8, "variant 3 tile_rows must be 2, 4 or 8");
-        else
-            CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || tile_rows == 16 || (tile_rows == 32 && variant != 2),
-                        "tile_rows must be 4, 8, 16 or 32 (4, 8 or 16 for variant 2)");
+                throw Error(CFD_E_UNSUPPORTED, "the row form (variants 7 to 9) needs a z extent that is a multiple of 4 and at most 256");
+        } else {
+            CFD_REQUIRE(tile_rows == 4 || tile_rows == 8 || tile_rows == 16 || tile_rows == 32,
+                        "tile_rows must be 4, 8, 16 or 32");
+        }
         CFD_REQUIRE(chunk >= 1 && chunk <= 65535, "chunk must be in [1, 65535]");
         m->wave_variant = variant;
         m->wave_ty = tile_rows;

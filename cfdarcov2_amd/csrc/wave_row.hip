@@ -1,4 +1,4 @@
-// Row form of the fused acoustic step (variants 7 to 10) on an exactly uniform
+// Row form of the fused acoustic step (variants 7 to 9) on an exactly uniform
 // mesh whose z extent is a multiple of 4 and at most 256.
 //
 // One wave holds whole z rows of a plane: RW = 256 / Z consecutive rows, one

@@ -7,8 +7,8 @@ vertex coordinates, each face's four vertices and its cells, each cell's six
 faces in the local order z-, z+, y-, y+, x-, x+ and its eight vertices.
 
 `lattice_connectivity` builds it for Mesh3D(x, y, z, dx, dy, dz) in the
-reference numbering (src/mesh3d.cpp:323-416), `read_mesh_json` reads a
-mesh.json file.
+reference numbering (src/mesh3d.cpp:323-416), `shuffled` renumbers a mesh at
+random, `read_mesh_json` reads a mesh.json file.
 """
 from __future__ import annotations
 
@@ -54,6 +54,23 @@ def lattice_connectivity(nx, ny, nz, dx, dy, dz, vertex_offsets=None, dtype=np.f
         face_cells[f[~first], 0] = cells[~first]
     return {"vertices": v.astype(np.float64), "face_verts": face_verts, "face_cells": face_cells,
             "cell_faces": cell_faces.astype(np.int64), "cell_verts": cell_verts.astype(np.int64)}
+
+
+def shuffled(conn, seed) -> tuple[dict, np.ndarray]:
+    """The same mesh with cells, faces and vertices renumbered at random (each
+    face keeps its vertex order and each cell its local face order).
+    Returns the new connectivity and the cell map old -> new."""
+    rng = np.random.default_rng(seed)
+    N, F, V = len(conn["cell_faces"]), len(conn["face_verts"]), len(conn["vertices"])
+    pc, pf, pv = rng.permutation(N), rng.permutation(F), rng.permutation(V)   # old -> new
+    out = {k: np.empty_like(conn[k]) for k in KEYS}
+    out["vertices"][pv] = conn["vertices"]
+    out["face_verts"][pf] = pv[conn["face_verts"]]
+    fcells = conn["face_cells"]
+    out["face_cells"][pf] = np.where(fcells >= 0, pc[np.maximum(fcells, 0)], -1)
+    out["cell_faces"][pc] = pf[conn["cell_faces"]]
+    out["cell_verts"][pc] = pv[conn["cell_verts"]]
+    return out, pc
 
 
 def read_mesh_json(path) -> dict:

@@ -367,27 +367,23 @@ int cfd_mesh_geometry(cfd_mesh_t mesh, cfd_geometry* out);
 
 /* Kernel shape of the fused acoustic step on uniform meshes (no counterpart
  * in the reference, whose JIT kernels use fixed 64-thread blocks,
- * src/fvm3d.cpp:1306-1308): variant 0 = one thread per cell, variant 1 = 2.5D
- * marching kernel with `tile_rows` (4, 8, 16 or 32) y-rows of 64 z-cells per
- * workgroup carried by 4 (8 for 32) thread rows, variant 2 = the same kernel
- * with one thread row per tile row (tile_rows 4, 8 or 16), variant 3 = a
- * register form without LDS or barriers in which each wave owns tile_rows (2,
- * 4 or 8) consecutive rows x 64 z-cells, variant 4 = variant 1 (tile_rows 4
- * or 8) with non-temporal output stores, variant 5 = variant 4 with 4 rows
- * limited to 64 VGPRs (eight waves per SIMD), variant 6 = variant 4 with 4
- * rows and the loads of two planes in flight, variant 7 = row form (z extent a
- * multiple of 4 and at most 256): one wave per z row with four cells per lane,
- * `tile_rows` (4 or 8; 14 in FP32) output rows plus two halo rows per
- * workgroup, non-temporal output stores, variant 8 = variant 7 with ordinary
- * stores, variant 9 = variant 7 with non-temporal loads and ordinary stores,
- * variant 10 = non-temporal loads and stores, variant 11 = two steps per
- * launch (`tile_rows` 4, 8 or 10 in FP32, 2 in FP64; steps run in pairs of
- * launches, the remainder and K-field runs as variant 9), variant 12 = variant 11 with
- * non-temporal stores; `chunk` x-planes per workgroup
- * (per wave for variant 3) and XCD-aware tile placement when bit 0 of
- * `xcd_remap` is set; for the row form (7 to 10), bit 1 of `xcd_remap` makes
- * every other step walk its tiles in reverse.  The row form returns
- * CFD_E_UNSUPPORTED for other z extents.
+ * src/fvm3d.cpp:1306-1308):
+ *   0  one thread per cell (the form for small meshes);
+ *   1  2.5D marching kernel, `tile_rows` (4, 8, 16 or 32) y-rows of 64 z-cells
+ *      per workgroup carried by 4 (8 for 32) thread rows; any z extent;
+ *   7  row form (z extent a multiple of 4 and at most 256): one wave per z row
+ *      with four cells per lane, `tile_rows` (4 or 8; 14 in FP32) output rows
+ *      plus two halo rows per workgroup, non-temporal output stores;
+ *   8  variant 7 with ordinary stores;
+ *   9  variant 7 with non-temporal loads and ordinary stores;
+ *   11 two steps per launch (`tile_rows` 4, 8 or 10 in FP32, 2 in FP64; steps
+ *      run in pairs of launches, the remainder and K-field runs as variant 9);
+ *   12 variant 11 with non-temporal stores.
+ * `chunk` x-planes per workgroup; XCD-aware tile placement when bit 0 of
+ * `xcd_remap` is set; for the row forms, bit 1 of `xcd_remap` makes every
+ * other step walk its tiles in reverse.  The row forms return
+ * CFD_E_UNSUPPORTED for other z extents; other variant numbers (2 to 6 and 10
+ * were forms the autotuner never chose) return CFD_E_INVALID.
  * Uniform meshes whose z extent suits the row form start on variant 11 (FP32)
  * or 7 (FP64).
  * All variants give bit-identical results.  Set by the autotuner

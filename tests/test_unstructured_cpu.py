@@ -19,22 +19,8 @@ TABLES = ["ids", "ids_bf", "nx", "ny", "nz", "nax", "nay", "naz", "area", "alpha
 
 
 def shuffle(conn, seed):
-    """The same mesh with cells, faces and vertices renumbered at random (each
-    face keeps its vertex order and each cell its local face order).
-    Returns the new connectivity and the maps old -> new."""
-    rng = np.random.default_rng(seed)
-    N, F, V = len(conn["cell_faces"]), len(conn["face_verts"]), len(conn["vertices"])
-    pc, pf, pv = rng.permutation(N), rng.permutation(F), rng.permutation(V)   # old -> new
-    out = {"vertices": np.empty_like(conn["vertices"]), "face_verts": np.empty_like(conn["face_verts"]),
-           "face_cells": np.empty_like(conn["face_cells"]), "cell_faces": np.empty_like(conn["cell_faces"]),
-           "cell_verts": np.empty_like(conn["cell_verts"])}
-    out["vertices"][pv] = conn["vertices"]
-    out["face_verts"][pf] = pv[conn["face_verts"]]
-    fcells = conn["face_cells"]
-    out["face_cells"][pf] = np.where(fcells >= 0, pc[np.maximum(fcells, 0)], -1)
-    out["cell_faces"][pc] = pf[conn["cell_faces"]]
-    out["cell_verts"][pc] = pv[conn["cell_verts"]]
-    return out, pc
+    """The same mesh in a random numbering (meshio.shuffled)."""
+    return meshio.shuffled(conn, seed)
 
 
 def _check_all(conn, oracle_lib, dtype):
