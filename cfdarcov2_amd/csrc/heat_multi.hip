@@ -3,108 +3,127 @@
 // single step lasts about as long as a launch).
 //
 // A workgroup owns an output box of B x B x BZ cells (BZ = min(Z, 64 - 2K)).
-// It loads its box widened by K cells on every side that is not a wall into
-// an LDS region (at most (B + 2K) x (B + 2K) x 64 cells), then takes `ns` <= K
-// steps there: step s computes the region narrowed by s on its non-wall
-// sides, which is all that step s + 1 reads, ping-ponging between two LDS
-// buffers; at the walls a cell's outer neighbour is the cell itself, as in
-// the single-step kernel.  Finally it stores its output box.  Cells near a
-// box's sides are computed by several workgroups (the redundant work that
-// buys K - 1 fewer launches).  Each cell's arithmetic is heat_uniform_cell's
-// (stencil.hip), so ns steps here equal ns single steps bit for bit.
+// Its region is the box widened by K cells on every side that is not a wall
+// (at most (B + 2K) x (B + 2K) rows of at most 64 z-cells).  Step s computes
+// the region narrowed by s on its non-wall sides, which is all that step
+// s + 1 reads; at the walls a cell's outer neighbour is the cell itself, as in
+// the single-step kernel.  Cells near a box's sides are computed by several
+// workgroups (the redundant work that buys K - 1 fewer launches).  Each cell's
+// arithmetic is heat_uniform_cell's (stencil.hip), in the same order, so ns
+// steps here equal ns single steps bit for bit.
 //
-// Threads: 64 lanes along z (one region row per wave pass) x 16 waves over the
-// region's (x, y) rows (enough waves per CU to hide the LDS latency: the boxes
-// are few, about one workgroup per CU at 64^3).  A launch lasts a few memory
-// latencies (the region's loads, issued together, and the stores) plus ns
-// passes over LDS.
+// Threads: 64 lanes along z x 16 waves.  Every wave owns a fixed set of the
+// region's (x, y) rows for the whole launch and keeps their cells in
+// registers; the region is mirrored in one LDS buffer, from which a cell
+// reads its x and y neighbours.  z neighbours are the adjacent lanes' values
+// (DPP lane shifts).  Per step: compute every owned row of the narrowed
+// region into registers, barrier, write them to LDS, barrier.  The region's
+// loads are issued together before the first LDS store, and the output box
+// is stored from the registers.  Row ownership and the x/y wall tests are
+// wave-uniform; only the z walls are tested per lane.
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "row.hpp"
 
 namespace cfd {
 
 constexpr int kMultiWaves = 16;
 
 template <class T, int B, int K, bool PW>
-__global__ void __launch_bounds__(64 * kMultiWaves) heat_multi_k(UniformGeo<T> g, const T* __restrict__ Tin, T* __restrict__ Tout,
-                                                    T alpha, T dt, int ns, int bz, int nzt, int nyt) {
-    constexpr int R = B + 2 * K;   // region extent along x and y
-    __shared__ T buf[2][R * R * 64];
-    const int64_t X = g.X, Y = g.Y, Z = g.Z;
+__global__ void __launch_bounds__(64 * kMultiWaves)
+    heat_multi_k(UniformGeo<T> g, const T* __restrict__ Tin, T* __restrict__ Tout, T alpha, T dt, int ns, int bz,
+                 int nzt, int nyt) {
+    constexpr int R = B + 2 * K;                                   // region extent along x and y
+    constexpr int NR = (R * R + kMultiWaves - 1) / kMultiWaves;    // rows per wave
+    __shared__ T buf[R * R * 64];
+    const int X = int(g.X), Y = int(g.Y), Z = int(g.Z);
     const int lin = int(blockIdx.x);
     const int zt = lin % nzt, yt = (lin / nzt) % nyt, xt = lin / (nzt * nyt);
-    const int lane = int(threadIdx.x), wy = int(threadIdx.y);
+    const int lane = int(threadIdx.x);
+    const int wy = __builtin_amdgcn_readfirstlane(int(threadIdx.y));
     // output box and region (clipped to the mesh)
-    const int ox0 = xt * B, ox1 = int(min<int64_t>(ox0 + B, X));
-    const int oy0 = yt * B, oy1 = int(min<int64_t>(oy0 + B, Y));
-    const int oz0 = zt * bz, oz1 = int(min<int64_t>(oz0 + bz, Z));
-    const int gx0 = max(ox0 - K, 0), gx1 = int(min<int64_t>(ox1 + K, X));
-    const int gy0 = max(oy0 - K, 0), gy1 = int(min<int64_t>(oy1 + K, Y));
-    const int gz0 = max(oz0 - K, 0), gz1 = int(min<int64_t>(oz1 + K, Z));
-    const int nry = gy1 - gy0;
-    auto slot = [&](int x, int y, int z) { return ((x - gx0) * R + (y - gy0)) * 64 + (z - gz0); };
+    const int ox0 = xt * B, ox1 = min(ox0 + B, X);
+    const int oy0 = yt * B, oy1 = min(oy0 + B, Y);
+    const int oz0 = zt * bz, oz1 = min(oz0 + bz, Z);
+    const int gx0 = max(ox0 - K, 0), gx1 = min(ox1 + K, X);
+    const int gy0 = max(oy0 - K, 0), gy1 = min(oy1 + K, Y);
+    const int gz0 = max(oz0 - K, 0), gz1 = min(oz1 + K, Z);
+    const int nry = gy1 - gy0, nrows = (gx1 - gx0) * nry;
+    const int z = gz0 + lane;
+    const bool zin = z < gz1;
+    const bool zlo = z == 0, zhi = z == Z - 1;
 
-    // every load of the region is issued before the first LDS store, so the
-    // wave waits for the memory latency once rather than once per row
-    constexpr int NR = (R * R + kMultiWaves - 1) / kMultiWaves;
-    const int nrows = (gx1 - gx0) * nry;
-    T v[NR];
+    // the wave's rows (x = -1: no row) and their cells
+    int rx[NR], ry[NR];
+    T cur[NR];
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
         const int r = wy + i * kMultiWaves;
-        const int x = gx0 + r / nry, y = gy0 + r % nry, z = gz0 + lane;
-        v[i] = (r < nrows && z < gz1) ? Tin[(int64_t(x) * Y + y) * Z + z] : T(0);
+        rx[i] = r < nrows ? gx0 + r / nry : -1;
+        ry[i] = r < nrows ? gy0 + r % nry : -1;
+        cur[i] = (rx[i] >= 0 && zin) ? Tin[(int64_t(rx[i]) * Y + ry[i]) * Z + z] : T(0);
     }
+    auto slot = [&](int x, int y) { return ((x - gx0) * R + (y - gy0)) * 64 + lane; };
 #pragma unroll
-    for (int i = 0; i < NR; ++i) {
-        const int r = wy + i * kMultiWaves;
-        const int x = gx0 + r / nry, y = gy0 + r % nry, z = gz0 + lane;
-        if (r < nrows && z < gz1) buf[0][slot(x, y, z)] = v[i];
-    }
+    for (int i = 0; i < NR; ++i)
+        if (rx[i] >= 0) buf[slot(rx[i], ry[i])] = cur[i];
     __syncthreads();
 
-    const int64_t ext[3] = {X, Y, Z};
-    int cur = 0;
+    const T A0 = g.area[0], A1 = g.area[1], A2 = g.area[2];
+    auto axis = [&](T L, T pc, T lo_v, T hi_v, bool lo, bool hi, int a, T A) {
+        const T al_lo = lo ? g.alpha_b[a] : g.alpha[a];
+        const T al_hi = hi ? g.alpha_b[a] : g.alpha[a];
+        const T s_lo = (-pc + (lo ? pc : lo_v)) * al_lo;
+        const T s_hi = (-pc + (hi ? pc : hi_v)) * al_hi;
+        const T num = s_lo * A + s_hi * A;
+        // a power-of-two volume divides exactly: the product is the same number
+        if constexpr (PW) return L + num * g.inv_vol;
+        else return L + num / g.vol;
+    };
     for (int s = 1; s <= ns; ++s) {
         // the region narrowed by s on its non-wall sides
         const int bx0 = gx0 == 0 ? 0 : gx0 + s, bx1 = gx1 == X ? gx1 : gx1 - s;
         const int by0 = gy0 == 0 ? 0 : gy0 + s, by1 = gy1 == Y ? gy1 : gy1 - s;
         const int bz0 = gz0 == 0 ? 0 : gz0 + s, bz1 = gz1 == Z ? gz1 : gz1 - s;
-        const int nby = by1 - by0;
-        const T* P = buf[cur];
-        T* Q = buf[cur ^ 1];
-        for (int r = wy; r < (bx1 - bx0) * nby; r += kMultiWaves) {
-            const int x = bx0 + r / nby, y = by0 + r % nby, z = bz0 + lane;
-            if (z >= bz1) continue;
-            const int c = slot(x, y, z);
-            const T pc = P[c];
-            const int crd[3] = {x, y, z};
-            const int str[3] = {R * 64, 64, 1};
-            T L = T(0);
+        const bool zact = z >= bz0 && z < bz1;
+        T nv[NR];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const bool lo = crd[a] == 0, hi = crd[a] == ext[a] - 1;
-                const T al_lo = lo ? g.alpha_b[a] : g.alpha[a];
-                const T al_hi = hi ? g.alpha_b[a] : g.alpha[a];
-                const T s_lo = (-pc + (lo ? pc : P[c - str[a]])) * al_lo;
-                const T s_hi = (-pc + (hi ? pc : P[c + str[a]])) * al_hi;
-                const T num = s_lo * g.area[a] + s_hi * g.area[a];
-                // a power-of-two volume divides exactly: the product is the same number
-                if constexpr (PW) L = L + num * g.inv_vol;
-                else L = L + num / g.vol;
-            }
-            Q[c] = dt * (alpha * L) + pc;
+        for (int i = 0; i < NR; ++i) {
+            nv[i] = cur[i];
+            const int x = rx[i], y = ry[i];
+            // z neighbours from the adjacent lanes (lanes 0 and 63 keep their
+            // own value; they are walls or outside the narrowed region)
+            const T pc = cur[i];
+            const T zl = row::lane_shift<row::kWaveShr1>(pc, pc);
+            const T zh = row::lane_shift<row::kWaveShl1>(pc, pc);
+            if (x < bx0 || x >= bx1 || y < by0 || y >= by1) continue;   // wave-uniform; x = -1 fails
+            const int c = slot(x, y);
+            const bool xlo = x == 0, xhi = x == X - 1, ylo = y == 0, yhi = y == Y - 1;
+            const T xl = xlo ? pc : buf[c - R * 64], xh = xhi ? pc : buf[c + R * 64];
+            const T yl = ylo ? pc : buf[c - 64], yh = yhi ? pc : buf[c + 64];
+            T L = T(0);
+            L = axis(L, pc, xl, xh, xlo, xhi, 0, A0);
+            L = axis(L, pc, yl, yh, ylo, yhi, 1, A1);
+            L = axis(L, pc, zl, zh, zlo, zhi, 2, A2);
+            if (zact) nv[i] = dt * (alpha * L) + pc;
         }
         __syncthreads();
-        cur ^= 1;
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const int x = rx[i], y = ry[i];
+            if (x >= bx0 && x < bx1 && y >= by0 && y < by1 && zact) buf[slot(x, y)] = nv[i];
+            cur[i] = nv[i];
+        }
+        __syncthreads();
     }
 
-    const int noy = oy1 - oy0;
-    for (int r = wy; r < (ox1 - ox0) * noy; r += kMultiWaves) {
-        const int x = ox0 + r / noy, y = oy0 + r % noy, z = oz0 + lane;
-        if (z < oz1) Tout[(int64_t(x) * Y + y) * Z + z] = buf[cur][slot(x, y, z)];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const int x = rx[i], y = ry[i];
+        if (x >= ox0 && x < ox1 && y >= oy0 && y < oy1 && z >= oz0 && z < oz1)
+            Tout[(int64_t(x) * Y + y) * Z + z] = cur[i];
     }
 }
 
@@ -136,13 +155,8 @@ void launch_heat_multi(const UniformGeo<T>& g, const void* in, void* out, double
         case 4 * 8 + 2: launch_multi<T, 4, 2>(g, in, out, alpha, dt, ns, s); break;
         case 4 * 8 + 4: launch_multi<T, 4, 4>(g, in, out, alpha, dt, ns, s); break;
         case 8 * 8 + 2: launch_multi<T, 8, 2>(g, in, out, alpha, dt, ns, s); break;
-        case 8 * 8 + 4:
-            if constexpr (sizeof(T) == 4) {
-                launch_multi<T, 8, 4>(g, in, out, alpha, dt, ns, s);
-                break;
-            }
-            [[fallthrough]];
-        default: throw Error(CFD_E_INVALID, "multi-step heat boxes: B 4 or 8, K 2 or 4 (B 8 with K 4 in FP32 only)");
+        case 8 * 8 + 4: launch_multi<T, 8, 4>(g, in, out, alpha, dt, ns, s); break;
+        default: throw Error(CFD_E_INVALID, "multi-step heat boxes: B 4 or 8, K 2 or 4");
     }
 }
 

@@ -846,9 +846,8 @@ int cfd_mesh_set_heat_shape(cfd_mesh_t m, int variant, int waves, int chunk) {
         require_lattice(m, "cfd_mesh_set_heat_shape");
         CFD_REQUIRE(variant >= 0 && variant <= 2, "heat variant must be 0, 1 or 2");
         if (variant == 2) {
-            CFD_REQUIRE(heat_multi_supported(waves, chunk) && !(m->dtype == CFD_F64 && waves == 8 && chunk == 4),
-                        "multi-step heat boxes: 4 or 8 cells wide (waves), 2 or 4 steps per launch (chunk); "
-                        "8 with 4 steps in FP32 only");
+            CFD_REQUIRE(heat_multi_supported(waves, chunk),
+                        "multi-step heat boxes: 4 or 8 cells wide (waves), 2 or 4 steps per launch (chunk)");
             CFD_REQUIRE(m->uniform, "the multi-step heat form needs an exactly uniform mesh");
             m->heat_variant = variant;
             m->heat_nw = waves;

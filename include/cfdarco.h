@@ -398,9 +398,8 @@ int cfd_mesh_set_wave_shape(cfd_mesh_t mesh, int variant, int tile_rows, int chu
  * the remaining steps and runs with a diffusivity field take variant 0;
  * variant 2 = up to `chunk` (2 or 4) steps per launch in LDS boxes of `waves`
  * (4 or 8) x `waves` x min(Z, 64 - 2 chunk) output cells, each workgroup
- * computing its box widened by `chunk` cells for the steps in between (8-wide
- * boxes with 4 steps in FP32 only); the form for small meshes whose single
- * step lasts about a launch.  All give bit-identical results.  Variant 1 pays on large meshes (it needs
+ * computing its box widened by `chunk` cells for the steps in between; the
+ * form for small meshes whose single step lasts about a launch.  All give bit-identical results.  Variant 1 pays on large meshes (it needs
  * enough tiles to fill the GPU): meshes of at least 2^23 cells that allow it
  * start on variant 1 (12 waves, 16 planes), others on variant 0;
  * cfdarcov2_amd/autotune.py:tune_heat times both. */
