@@ -176,7 +176,8 @@ def suite_heat(ctx, cpu=True, verbose=False):
     shape = autotune.tune_heat(m, case.alpha, case.dt, verbose=verbose)
     a = torch.from_numpy(case.T0).cuda()
     b = torch.empty_like(a)
-    m.heat_run(a, b, case.alpha, case.dt, 10)
+    # untimed warm-up of the same run (captures the run's block of steps)
+    m.heat_run(a, b, case.alpha, case.dt, case.steps)
     a.copy_(torch.from_numpy(case.T0))
     wall, ev = _timed(lambda: m.heat_run(a, b, case.alpha, case.dt, case.steps))
     N = m.N
@@ -329,7 +330,7 @@ def suite_ensemble(ctx, steps=500, verbose=False):
         b = plain_shapes[c.spec.n]
         c.mesh.set_wave_shape(b["variant"], b["tile_rows"], b["chunk"], b["xcd_remap"])
     ens.reset()
-    ens.run_sequential(2)
+    ens.run_sequential(64)  # untimed warm-up (captures each heat case's block of steps)
     ens.reset()
     t_seq = time_gpu(lambda: ens.run_sequential(steps), reps=1)
     best = tune_batch(ens, steps=20, verbose=verbose)

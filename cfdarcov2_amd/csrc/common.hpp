@@ -150,19 +150,32 @@ struct cfd_mesh_s {
     std::vector<double> vertex_offsets;
     bool has_offsets = false;
     // fused acoustic step on uniform meshes: variant 0 = per-thread kernel,
-    // 1-6 = 2.5D marching tiles, 7-10 = row form (the variants of
-    // cfd_mesh_set_wave_shape); uniform meshes whose z extent suits the row
-    // form start on it
+    // 1 = 2.5D marching tiles, 7-9 = row form, 11-12 = two-step row form (the
+    // variants of cfd_mesh_set_wave_shape); uniform meshes whose z extent
+    // suits the row form start on it
     int wave_variant = 1;
     int wave_ty = 16;
     int wave_L = 32;
     int wave_remap = 1;
     // heat step on uniform meshes: variant 0 = one thread per cell, one step
     // per launch; 1 = two steps per launch in row form (`heat_nw` waves per
-    // workgroup, `heat_L` planes per chunk)
+    // workgroup, `heat_L` planes per chunk); 2 = up to `heat_L` steps per
+    // launch in LDS boxes `heat_nw` cells wide
     int heat_variant = 0;
     int heat_nw = 8;
     int heat_L = 16;
+    // cfd_heat_run's captured block of steps (a HIP graph of the launches of
+    // kHeatGraphSteps steps from Ta back to Ta), the arguments it was captured
+    // with, the library stream it was captured on and the stream of its last
+    // launch
+    hipGraphExec_t heat_exec = nullptr;
+    hipStream_t capture = nullptr;
+    hipStream_t heat_last = nullptr;
+    const void* hg_Ta = nullptr;
+    const void* hg_Tb = nullptr;
+    const void* hg_af = nullptr;
+    double hg_alpha = 0, hg_dt = 0;
+    int hg_shape[3] = {-1, 0, 0};
     // Euler stepper: variant 0 = one thread per cell, 1 = 2.5D march with
     // `euler_ty` rows x 64 columns x `euler_L` planes per workgroup
     int euler_variant = 1;

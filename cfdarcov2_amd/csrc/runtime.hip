@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -363,6 +364,68 @@ static void* scratch_addr(int device) {
 
 using namespace cfd;
 
+// Steps per captured heat block: even, so a block leaves the state where it
+// found it (Ta), and long enough that one graph launch replaces many kernel
+// launches.
+constexpr int64_t kHeatGraphSteps = 32;
+
+namespace {
+
+void drop_heat_graph(cfd_mesh_s* m) {
+    if (!m->heat_exec) return;
+    // a launch of the old graph may still be running
+    if (m->heat_last) CFD_HIP(hipStreamSynchronize(m->heat_last));
+    (void)hipGraphExecDestroy(m->heat_exec);
+    m->heat_exec = nullptr;
+    m->heat_last = nullptr;
+}
+
+// Captures kHeatGraphSteps steps of launch_heat on the mesh's own stream.
+void build_heat_graph(cfd_mesh_s* m, void* Ta, void* Tb, double alpha, const void* af, double dt) {
+    drop_heat_graph(m);
+    if (!m->capture) CFD_HIP(hipStreamCreateWithFlags(&m->capture, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    CFD_HIP(hipStreamBeginCapture(m->capture, hipStreamCaptureModeThreadLocal));
+    try {
+        if (m->dtype == CFD_F32) cfd::launch_heat<float>(m, Ta, Tb, alpha, af, dt, kHeatGraphSteps, m->capture);
+        else cfd::launch_heat<double>(m, Ta, Tb, alpha, af, dt, kHeatGraphSteps, m->capture);
+    } catch (...) {
+        (void)hipStreamEndCapture(m->capture, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    CFD_HIP(hipStreamEndCapture(m->capture, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    CFD_HIP(e);
+    m->heat_exec = exec;
+    m->hg_Ta = Ta;
+    m->hg_Tb = Tb;
+    m->hg_af = af;
+    m->hg_alpha = alpha;
+    m->hg_dt = dt;
+    m->hg_shape[0] = m->heat_variant;
+    m->hg_shape[1] = m->heat_nw;
+    m->hg_shape[2] = m->heat_L;
+}
+
+bool heat_graph_matches(const cfd_mesh_s* m, const void* Ta, const void* Tb, double alpha, const void* af,
+                        double dt) {
+    return m->heat_exec && m->hg_Ta == Ta && m->hg_Tb == Tb && m->hg_af == af && m->hg_alpha == alpha &&
+           m->hg_dt == dt && m->hg_shape[0] == m->heat_variant && m->hg_shape[1] == m->heat_nw &&
+           m->hg_shape[2] == m->heat_L;
+}
+
+bool stream_capturing(hipStream_t s) {
+    if (!s) return false;  // the null stream cannot be captured
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    CFD_HIP(hipStreamIsCapturing(s, &st));
+    return st != hipStreamCaptureStatusNone;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* cfd_last_error(void) { return g_last_error.c_str(); }
@@ -486,6 +549,8 @@ int cfd_mesh_create_hex(cfd_ctx_t ctx, const cfd_hex_desc* desc, cfd_mesh_t* out
 int cfd_mesh_destroy(cfd_mesh_t mesh) {
     return guarded([&] {
         if (!mesh) return;
+        drop_heat_graph(mesh);
+        if (mesh->capture) (void)hipStreamDestroy(mesh->capture);
         if (mesh->dev_block) pool_free(mesh->ctx, mesh->dev_block, nullptr);
         if (mesh->fv_block) pool_free(mesh->ctx, mesh->fv_block, nullptr);
         if (mesh->dev_block || mesh->fv_block) CFD_HIP(hipStreamSynchronize(nullptr));
@@ -745,14 +810,36 @@ int cfd_count_nonfinite(cfd_ctx_t ctx, int dtype, const void* x, int64_t n, int6
     });
 }
 
+
 int cfd_heat_run(cfd_mesh_t m, void* Ta, void* Tb, double alpha, const void* af, double dt, int64_t nsteps,
                  void* stream) {
     return guarded([&] {
         CFD_REQUIRE(m && Ta && Tb, "cfd_heat_run: null argument");
         CFD_REQUIRE(Ta != Tb, "cfd_heat_run: ping-pong buffers must differ");
         CFD_REQUIRE(nsteps >= 0, "cfd_heat_run: negative step count");
-        if (m->unstructured) DISPATCH(m, launch_uheat, m, Ta, Tb, alpha, af, dt, nsteps, as_stream(stream));
-        else DISPATCH(m, launch_heat, m, Ta, Tb, alpha, af, dt, nsteps, as_stream(stream));
+        hipStream_t s = as_stream(stream);
+        if (m->unstructured) {
+            DISPATCH(m, launch_uheat, m, Ta, Tb, alpha, af, dt, nsteps, s);
+            return;
+        }
+        // Runs of at least one block replay a captured block of steps: one
+        // graph launch instead of one kernel launch per step, which is most
+        // of a step's time on small meshes.  A caller that is itself
+        // capturing the stream gets the plain launches in its graph;
+        // CFDARCO_HEAT_GRAPH=0 turns the blocks off.
+        const char* env = std::getenv("CFDARCO_HEAT_GRAPH");
+        const bool blocks = nsteps >= kHeatGraphSteps && !(env && env[0] == '0') && !stream_capturing(s);
+        if (!blocks) {
+            DISPATCH(m, launch_heat, m, Ta, Tb, alpha, af, dt, nsteps, s);
+            return;
+        }
+        if (!heat_graph_matches(m, Ta, Tb, alpha, af, dt)) build_heat_graph(m, Ta, Tb, alpha, af, dt);
+        const int64_t reps = nsteps / kHeatGraphSteps;
+        for (int64_t r = 0; r < reps; ++r) CFD_HIP(hipGraphLaunch(m->heat_exec, s));
+        m->heat_last = s;
+        // the blocks leave the state in Ta; the remaining steps start there
+        const int64_t rest = nsteps - reps * kHeatGraphSteps;
+        if (rest) DISPATCH(m, launch_heat, m, Ta, Tb, alpha, af, dt, rest, s);
     });
 }
 

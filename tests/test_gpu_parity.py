@@ -608,6 +608,34 @@ def test_heat_multi_step_form(hip_ctx, oracle_lib, dtype, case):
     assert np.array_equal(got, ref), float(np.max(np.abs(got - ref)))
 
 
+def test_heat_run_step_blocks_replayed(hip_ctx, oracle_lib):
+    """Runs of 32 steps or more replay a captured block of 32 steps (a HIP
+    graph): repeated runs on the same fields, with eager work and
+    synchronisations between them, each equal to the oracle; a new shape, dt
+    or pair of fields captures again; step counts that leave a remainder
+    (33, 70) and exact multiples (64)."""
+    from cfdarcov2_amd import capi
+    n, d = (20, 18, 24), (1.0 / 16,) * 3
+    rng = np.random.default_rng(29)
+    for dtype in (np.float64, np.float32):
+        om = oracle_lib.OracleMesh(*n, *d, dtype=dtype)
+        gm = capi.HexMesh(hip_ctx, *n, *d, dtype=dtype)
+        T0 = rng.standard_normal(om.N).astype(dtype)
+        alpha = 1e-2
+        for shape in ((0, 8, 16), (2, 4, 2), (2, 8, 4)):
+            gm.set_heat_shape(*shape)
+            for dt in (0.1 * d[0] ** 2 / alpha, 0.15 * d[0] ** 2 / alpha):
+                a = _t(T0, dtype)
+                b = torch.empty_like(a)
+                for steps in (70, 64, 33, 70):
+                    print(f"\n[case] heat blocks {np.dtype(dtype).name} shape={shape} steps={steps}", flush=True)
+                    a.copy_(_t(T0, dtype))
+                    got = _np(gm.heat_run(a, b, alpha, dt, steps))
+                    torch.randn(1 << 16, device="cuda").sum().item()
+                    ref = om.heat_run(T0, alpha, dt, steps)
+                    assert np.array_equal(got, ref), (np.dtype(dtype).name, shape, dt, steps)
+
+
 def test_heat_multi_step_shape_errors(hip_ctx):
     from cfdarcov2_amd import capi
     gm = capi.HexMesh(hip_ctx, 8, 8, 8, 0.125, 0.125, 0.125, dtype=np.float64)
