@@ -234,7 +234,14 @@ int cfd_count_nonfinite(cfd_ctx_t ctx, int dtype, const void* x, int64_t n, int6
  * Heat: {d1t(T), '=', alpha * lapl(T)} (examples/main_3d_heat.cpp:89-91 with
  * the JIT kernel of src/fvm3d.cpp:1219-1394): T <- dt*(alpha*lapl(T)) + T.
  * alpha_field (nullable) is a per-cell coefficient.  Ping-pong between T_a and
- * T_b: after nsteps the state is in T_a when nsteps is even, T_b when odd. */
+ * T_b: after nsteps the state is in T_a when nsteps is even, T_b when odd.
+ * On structured meshes a run of 32 steps or more replays a HIP graph of a
+ * captured 32-step block (captured on the first such run, and again when
+ * the fields, alpha, alpha_field, dt or the heat form change), then runs the
+ * remaining steps; a stream the caller is capturing gets plain launches, and
+ * CFDARCO_HEAT_GRAPH=0 in the environment turns the blocks off.  Results are
+ * the same either way.  Calls on one mesh from several host threads at once
+ * must be serialised by the caller. */
 int cfd_heat_run(cfd_mesh_t mesh, void* T_a, void* T_b, double alpha, const void* alpha_field, double dt,
                  int64_t nsteps, void* stream);
 
