@@ -37,7 +37,7 @@ EXPORTS = [
     "cfd_grad", "cfd_lapl", "cfd_stab", "cfd_ewise", "cfd_cfl_min", "cfd_count_nonfinite",
     "cfd_heat_run", "cfd_wave_run", "cfd_euler_run", "cfd_mesh_set_wave_shape", "cfd_mesh_set_euler_shape",
     "cfd_mesh_set_heat_shape", "cfd_mesh_set_adjoint_shape",
-    "cfd_wave_traces", "cfd_wave_adjoint",
+    "cfd_wave_traces", "cfd_wave_adjoint", "cfd_heat_traces", "cfd_heat_adjoint",
     "cfd_batch_create", "cfd_batch_set_form", "cfd_batch_set_shape", "cfd_batch_run", "cfd_batch_destroy",
     "cfd_program_compile", "cfd_program_compile_only", "cfd_program_launch", "cfd_program_destroy",
     "cfd_mesh_geometry", "cfd_stab_noalloc",
@@ -139,6 +139,8 @@ def load_library(path: str = LIB_PATH):
                             ctypes.POINTER(ctypes.c_int64), i, vp, vp, i64, vp],
         "cfd_wave_adjoint": [vp, ctypes.c_void_p * 4, ctypes.c_void_p * 4, d, vp, d, i64,
                              ctypes.POINTER(ctypes.c_int64), i, vp, vp, i64, vp, vp],
+        "cfd_heat_traces": [vp, vp, vp, d, vp, d, i64, ctypes.POINTER(ctypes.c_int64), i, vp, vp, i64, vp],
+        "cfd_heat_adjoint": [vp, vp, vp, d, vp, d, i64, ctypes.POINTER(ctypes.c_int64), i, vp, vp, i64, vp, vp],
         "cfd_mesh_set_wave_shape": [vp, i, i, i, i],
         "cfd_mesh_set_euler_shape": [vp, i, i, i],
         "cfd_mesh_set_heat_shape": [vp, i, i, i],

@@ -36,6 +36,14 @@ void run_wave_adjoint(const cfd_mesh_s* m, void* const la[4], void* const lb[4],
                       double dt, int64_t nsteps, const SensorList& sens, const void* tgrad, const void* ckpt,
                       int64_t every, void* gK, hipStream_t s);
 
+template <class T>
+void run_heat_traces(const cfd_mesh_s* m, void* Ta, void* Tb, double alpha, const void* af, double dt,
+                     int64_t nsteps, const SensorList& sens, void* traces, void* ckpt, int64_t every, hipStream_t s);
+template <class T>
+void run_heat_adjoint(const cfd_mesh_s* m, void* la, void* lb, double alpha, const void* af, double dt,
+                      int64_t nsteps, const SensorList& sens, const void* tgrad, const void* ckpt, int64_t every,
+                      void* gA, hipStream_t s);
+
 template <class T> void launch_grad(const cfd_mesh_s* m, const void* phi, void* out, int mask, int wall, hipStream_t s);
 template <class T> void launch_lapl(const cfd_mesh_s* m, const void* phi, void* out, int mask, hipStream_t s);
 template <class T>

@@ -914,6 +914,39 @@ int cfd_wave_adjoint(cfd_mesh_t m, void* const adj_a[4], void* const adj_b[4], d
     });
 }
 
+int cfd_heat_traces(cfd_mesh_t m, void* Ta, void* Tb, double alpha, const void* af, double dt, int64_t nsteps,
+                    const int64_t* sensor_cells, int nsensors, void* traces, void* checkpoints, int64_t every,
+                    void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(m && Ta && Tb, "cfd_heat_traces: null argument");
+        require_lattice(m, "cfd_heat_traces");
+        CFD_REQUIRE(Ta != Tb, "cfd_heat_traces: ping-pong buffers must differ");
+        if (!m->uniform) throw Error(CFD_E_UNSUPPORTED, "cfd_heat_traces: uniform meshes only");
+        CFD_REQUIRE(nsteps >= 0, "cfd_heat_traces: negative step count");
+        CFD_REQUIRE(!checkpoints || every >= 1, "cfd_heat_traces: checkpoint interval must be >= 1");
+        const SensorList sens = make_sensors(m, sensor_cells, nsensors);
+        DISPATCH(m, run_heat_traces, m, Ta, Tb, alpha, af, dt, nsteps, sens, traces, checkpoints, every,
+                 as_stream(stream));
+    });
+}
+
+int cfd_heat_adjoint(cfd_mesh_t m, void* adj_a, void* adj_b, double alpha, const void* af, double dt,
+                     int64_t nsteps, const int64_t* sensor_cells, int nsensors, const void* trace_grad,
+                     const void* checkpoints, int64_t every, void* grad_alpha, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(m && adj_a && adj_b, "cfd_heat_adjoint: null argument");
+        require_lattice(m, "cfd_heat_adjoint");
+        CFD_REQUIRE(adj_a != adj_b, "cfd_heat_adjoint: ping-pong buffers must differ");
+        if (!m->uniform) throw Error(CFD_E_UNSUPPORTED, "cfd_heat_adjoint: uniform meshes only");
+        CFD_REQUIRE(nsteps >= 0, "cfd_heat_adjoint: negative step count");
+        CFD_REQUIRE(!grad_alpha || (checkpoints && every >= 1),
+                    "cfd_heat_adjoint: grad_alpha needs the forward checkpoints and their interval");
+        const SensorList sens = make_sensors(m, sensor_cells, nsensors);
+        DISPATCH(m, run_heat_adjoint, m, adj_a, adj_b, alpha, af, dt, nsteps, sens, trace_grad, checkpoints, every,
+                 grad_alpha, as_stream(stream));
+    });
+}
+
 int cfd_mesh_set_euler_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk) {
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");

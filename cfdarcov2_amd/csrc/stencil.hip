@@ -668,7 +668,147 @@ void launch_wave(const cfd_mesh_s* m, void* const a[4], void* const b[4], double
     }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the heat step on uniform meshes (no reference counterpart).
+// The step is H = I + dt K L, K = diag(k) with k the constant alpha or the
+// alpha field, L the Laplacian of heat_uniform_cell.  On a uniform mesh L is
+// symmetric (an interior face couples its two cells with the same
+// alpha_d A / V, a wall face couples nothing), so H^T = I + dt L K:
+//   constant alpha:  H^T = H, the forward kernels themselves;
+//   alpha field:     (H^T mu)_i = dt L(k o mu)_i + mu_i   (heat_adj_field_k).
+// dL/dk_i = sum over steps t of mu_t,i dt L(T_{t-1})_i (heat_grad_alpha_k),
+// the forward states T_{t-1} recomputed from checkpoints.
+// ---------------------------------------------------------------------------
+
+// L(k o f) at cell c (L(f) when kf is null), heat_uniform_cell's operation
+// order; on a wall the neighbour is the cell itself and its term is zero
+template <class T, bool PW>
+__device__ inline T lapl_scaled(const UniformGeo<T>& g, const Cell& c, const T* __restrict__ f,
+                                const T* __restrict__ kf) {
+    auto val = [&](int64_t j) { return kf ? kf[j] * f[j] : f[j]; };
+    const T pc = val(c.id);
+    T L = T(0);
+    for (int a = 0; a < 3; ++a) {
+        const T al_lo = at_lo(g, c, a) ? g.alpha_b[a] : g.alpha[a];
+        const T al_hi = at_hi(g, c, a) ? g.alpha_b[a] : g.alpha[a];
+        const T s_lo = (-pc + val(nb_lo(g, c, a))) * al_lo;
+        const T s_hi = (-pc + val(nb_hi(g, c, a))) * al_hi;
+        const T num = s_lo * g.area[a] + s_hi * g.area[a];
+        if constexpr (PW) L = L + num * g.inv_vol;
+        else L = L + num / g.vol;
+    }
+    return L;
+}
+
+template <class T, bool PW>
+__global__ void __launch_bounds__(256) heat_adj_field_k(UniformGeo<T> g, const T* __restrict__ mu,
+                                                        T* __restrict__ out, const T* __restrict__ kf, T dt) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    out[c.id] = dt * lapl_scaled<T, PW>(g, c, mu, kf) + mu[c.id];
+}
+
+template <class T, bool PW>
+__global__ void __launch_bounds__(256) heat_grad_alpha_k(UniformGeo<T> g, const T* __restrict__ mu,
+                                                         const T* __restrict__ Tprev, T* __restrict__ gk, T dt) {
+    const Cell c = cell_of_thread(g);
+    if (!c.valid) return;
+    gk[c.id] = gk[c.id] + (dt * lapl_scaled<T, PW>(g, c, Tprev, nullptr)) * mu[c.id];
+}
+
+template <class T> __global__ void gather_sensors_k(const T* __restrict__ f, SensorList s, T* __restrict__ out) {
+    const int j = int(threadIdx.x);
+    if (j < s.n) out[j] = f[s.cell[j]];
+}
+
+// nsteps heat steps from Ta, ping-pong with Tb; after step t (1-based) T at
+// the sensors goes to traces[(t-1) * n]; before steps 0, every, 2 every, ...
+// the state is copied to checkpoint k.
+template <class T>
+void run_heat_traces(const cfd_mesh_s* m, void* Ta, void* Tb, double alpha, const void* af, double dt,
+                     int64_t nsteps, const SensorList& sens, void* traces, void* ckpt, int64_t every,
+                     hipStream_t s) {
+    const size_t field = size_t(m->N) * sizeof(T);
+    for (int64_t t = 0; t < nsteps; ++t) {
+        void* in = (t & 1) ? Tb : Ta;
+        void* out = (t & 1) ? Ta : Tb;
+        if (ckpt && t % every == 0)
+            CFD_HIP(hipMemcpyAsync((char*)ckpt + size_t(t / every) * field, in, field, hipMemcpyDeviceToDevice, s));
+        launch_heat<T>(m, in, out, alpha, af, dt, 1, s);
+        if (traces && sens.n) {
+            gather_sensors_k<T><<<1, 64, 0, s>>>((const T*)out, sens, (T*)traces + t * sens.n);
+            check_launch();
+        }
+    }
+}
+
+// The transposed steps in reverse order from adj_a (the adjoint of the final
+// state), the trace gradients of step t added at the sensors before the
+// transposed step t; the result is in adj_a (nsteps even) or adj_b (odd).
+template <class T>
+void run_heat_adjoint(const cfd_mesh_s* m, void* la, void* lb, double alpha, const void* af, double dt,
+                      int64_t nsteps, const SensorList& sens, const void* tgrad, const void* ckpt, int64_t every,
+                      void* gA, hipStream_t s) {
+    const auto& g = ugeo<T>(m);
+    const size_t field = size_t(m->N) * sizeof(T);
+    void* ws = nullptr;  // the forward states of one checkpoint segment
+    if (gA) ws = pool_alloc(m->ctx, int64_t(every) * int64_t(field), s);
+    try {
+        int64_t done = 0;
+        auto step = [&](int64_t t, const void* Tprev) {
+            void* in = (done & 1) ? lb : la;
+            void* out = (done & 1) ? la : lb;
+            if (tgrad) launch_scatter_add<T>(in, sens, (const T*)tgrad + (t - 1) * sens.n, s);
+            if (gA) {
+                if (g.vol_pow2)
+                    heat_grad_alpha_k<T, true><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)in, (const T*)Tprev,
+                                                                                   (T*)gA, T(dt));
+                else
+                    heat_grad_alpha_k<T, false><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)in,
+                                                                                    (const T*)Tprev, (T*)gA, T(dt));
+                check_launch();
+            }
+            if (af) {
+                if (g.vol_pow2)
+                    heat_adj_field_k<T, true><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)in, (T*)out,
+                                                                                  (const T*)af, T(dt));
+                else
+                    heat_adj_field_k<T, false><<<grid_for(g, kBlock), kBlock, 0, s>>>(g, (const T*)in, (T*)out,
+                                                                                   (const T*)af, T(dt));
+                check_launch();
+            } else {
+                launch_heat<T>(m, in, out, alpha, nullptr, dt, 1, s);
+            }
+            ++done;
+        };
+        int64_t t = nsteps;
+        while (t >= 1) {
+            if (!gA) {
+                step(t, nullptr);
+                --t;
+                continue;
+            }
+            // states T_{t0} .. T_{t-1} of this segment, from its checkpoint
+            const int64_t seg = (t - 1) / every, t0 = seg * every, len = t - t0;
+            CFD_HIP(hipMemcpyAsync(ws, (const char*)ckpt + size_t(seg) * field, field, hipMemcpyDeviceToDevice, s));
+            for (int64_t k = 1; k < len; ++k)
+                launch_heat<T>(m, (char*)ws + size_t(k - 1) * field, (char*)ws + size_t(k) * field, alpha, af, dt, 1,
+                               s);
+            for (int64_t tt = t; tt > t0; --tt) step(tt, (const char*)ws + size_t(tt - 1 - t0) * field);
+            t = t0;
+        }
+    } catch (...) {
+        if (ws) pool_free(m->ctx, ws, s);
+        throw;
+    }
+    if (ws) pool_free(m->ctx, ws, s);
+}
+
 #define CFD_INSTANTIATE(T)                                                                                       \
+    template void run_heat_traces<T>(const cfd_mesh_s*, void*, void*, double, const void*, double, int64_t,     \
+                                     const SensorList&, void*, void*, int64_t, hipStream_t);                    \
+    template void run_heat_adjoint<T>(const cfd_mesh_s*, void*, void*, double, const void*, double, int64_t,    \
+                                      const SensorList&, const void*, const void*, int64_t, void*, hipStream_t); \
     template void launch_grad<T>(const cfd_mesh_s*, const void*, void*, int, int, hipStream_t);                \
     template void launch_lapl<T>(const cfd_mesh_s*, const void*, void*, int, hipStream_t);                     \
     template void launch_stab<T>(const cfd_mesh_s*, const void*, void*, int, void*, hipStream_t);              \

@@ -293,6 +293,27 @@ int cfd_wave_adjoint(cfd_mesh_t mesh, void* const adj_a[4], void* const adj_b[4]
                      const void* K_field, double dt, int64_t nsteps, const int64_t* sensor_cells, int nsensors,
                      const void* trace_grad, const void* checkpoints, int64_t every, void* grad_K, void* stream);
 
+/* ---- adjoint of the heat step (uniform meshes; no reference counterpart) --
+ * cfd_heat_traces: nsteps heat steps as cfd_heat_run, recording T after step t
+ * (1-based) at the sensor cells into traces[(t-1)*nsensors + j] (device,
+ * mesh dtype, nullable) and, when checkpoints (device, nullable) is given,
+ * copying T (N values) before steps 0, every, 2*every, ... into checkpoint
+ * k = t / every.  sensor_cells is a host array of at most 64 distinct ids. */
+int cfd_heat_traces(cfd_mesh_t mesh, void* T_a, void* T_b, double alpha, const void* alpha_field, double dt,
+                    int64_t nsteps, const int64_t* sensor_cells, int nsensors, void* traces, void* checkpoints,
+                    int64_t every, void* stream);
+/* cfd_heat_adjoint: the transposed steps H^T = I + dt L diag(k) in reverse
+ * order (L is symmetric on a uniform mesh; k = alpha, or alpha_field).  adj_a
+ * holds the adjoint of the final state (normally zero) and trace_grad
+ * (device, nsteps*nsensors, nullable) the gradient of the loss with respect to
+ * the traces; after the call adj_a (nsteps even) or adj_b (nsteps odd) holds
+ * the gradient with respect to the initial state.  grad_alpha (device, N,
+ * nullable) accumulates dL/dk per cell and needs the checkpoints of
+ * cfd_heat_traces. */
+int cfd_heat_adjoint(cfd_mesh_t mesh, void* adj_a, void* adj_b, double alpha, const void* alpha_field, double dt,
+                     int64_t nsteps, const int64_t* sensor_cells, int nsensors, const void* trace_grad,
+                     const void* checkpoints, int64_t every, void* grad_alpha, void* stream);
+
 /* Kernel form of cfd_wave_traces / cfd_wave_adjoint (no reference
  * counterpart): form 0 = one thread per cell; form 1 = row form (uniform
  * meshes whose z extent is a multiple of 4 and at most 256, else
