@@ -37,7 +37,7 @@ EXPORTS = [
     "cfd_grad", "cfd_lapl", "cfd_stab", "cfd_ewise", "cfd_cfl_min", "cfd_count_nonfinite",
     "cfd_heat_run", "cfd_wave_run", "cfd_euler_run", "cfd_mesh_set_wave_shape", "cfd_mesh_set_euler_shape",
     "cfd_mesh_set_heat_shape", "cfd_mesh_set_adjoint_shape",
-    "cfd_wave_traces", "cfd_wave_adjoint", "cfd_heat_traces", "cfd_heat_adjoint",
+    "cfd_wave_traces", "cfd_wave_adjoint", "cfd_heat_traces", "cfd_heat_adjoint", "cfd_mesh_autotune",
     "cfd_batch_create", "cfd_batch_set_form", "cfd_batch_set_shape", "cfd_batch_run", "cfd_batch_destroy",
     "cfd_program_compile", "cfd_program_compile_only", "cfd_program_launch", "cfd_program_destroy",
     "cfd_mesh_geometry", "cfd_stab_noalloc",
@@ -141,6 +141,7 @@ def load_library(path: str = LIB_PATH):
                              ctypes.POINTER(ctypes.c_int64), i, vp, vp, i64, vp, vp],
         "cfd_heat_traces": [vp, vp, vp, d, vp, d, i64, ctypes.POINTER(ctypes.c_int64), i, vp, vp, i64, vp],
         "cfd_heat_adjoint": [vp, vp, vp, d, vp, d, i64, ctypes.POINTER(ctypes.c_int64), i, vp, vp, i64, vp, vp],
+        "cfd_mesh_autotune": [vp, i, i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), vp],
         "cfd_mesh_set_wave_shape": [vp, i, i, i, i],
         "cfd_mesh_set_euler_shape": [vp, i, i, i],
         "cfd_mesh_set_heat_shape": [vp, i, i, i],
@@ -380,6 +381,17 @@ class HexMesh:
 
     def set_euler_shape(self, variant=1, tile_rows=4, chunk=16):
         check(self.lib.cfd_mesh_set_euler_shape(self.h, int(variant), int(tile_rows), int(chunk)))
+
+    def autotune(self, kind="wave", steps=10):
+        """cfd_mesh_autotune: times every kernel shape of the acoustic or heat
+        step on this mesh and applies the fastest; returns the shape and its
+        time per step."""
+        shape = (ctypes.c_int * 4)()
+        us = ctypes.c_double()
+        check(self.lib.cfd_mesh_autotune(self.h, {"wave": 0, "heat": 1}[kind], int(steps), shape, ctypes.byref(us),
+                                         _stream_ptr()))
+        keys = ("variant", "tile_rows", "chunk", "xcd_remap") if kind == "wave" else ("variant", "waves", "chunk")
+        return dict(zip(keys, list(shape)), us_per_step=us.value)
 
     def set_wave_shape(self, variant=1, tile_rows=16, chunk=32, xcd_remap=1):
         """xcd_remap: bit 0 XCD-contiguous tiles; bit 1 (row forms) every other

@@ -4,11 +4,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 #include "hexmesh.hpp"
@@ -422,6 +425,38 @@ bool stream_capturing(hipStream_t s) {
     hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
     CFD_HIP(hipStreamIsCapturing(s, &st));
     return st != hipStreamCaptureStatusNone;
+}
+
+}  // namespace
+
+namespace {
+
+// The kernel shapes cfd_mesh_autotune times (the forms and tiles of
+// cfd_mesh_set_wave_shape / cfd_mesh_set_heat_shape; shapes a mesh does not
+// allow are skipped).
+std::vector<std::array<int, 4>> tune_candidates(int kind) {
+    std::vector<std::array<int, 4>> c;
+    if (kind == CFD_TUNE_WAVE) {
+        c.push_back({0, 16, 32, 1});
+        for (int ty : {4, 8, 16, 32})
+            for (int L : {4, 8, 16, 32, 64})
+                for (int r : {0, 1}) c.push_back({1, ty, L, r});
+        for (int v : {7, 8, 9})
+            for (int ty : {4, 8, 14})
+                for (int L : {4, 8, 16, 32, 64})
+                    for (int r : {0, 1, 2, 3}) c.push_back({v, ty, L, r});
+        for (int v : {11, 12})
+            for (int ty : {2, 4, 8, 10})
+                for (int L : {8, 16, 32, 64})
+                    for (int r : {0, 1, 2, 3}) c.push_back({v, ty, L, r});
+    } else {
+        c.push_back({0, 8, 16, 0});
+        for (int w : {8, 12, 16})
+            for (int L : {1, 2, 4, 8, 16}) c.push_back({1, w, L, 0});
+        for (int B : {4, 8})
+            for (int K : {2, 4}) c.push_back({2, B, K, 0});
+    }
+    return c;
 }
 
 }  // namespace
@@ -983,6 +1018,75 @@ int cfd_mesh_set_heat_shape(cfd_mesh_t m, int variant, int waves, int chunk) {
         m->heat_variant = variant;
         m->heat_nw = waves;
         m->heat_L = chunk;
+    });
+}
+
+
+int cfd_mesh_autotune(cfd_mesh_t m, int kind, int steps, int* shape_out, double* us_per_step_out, void* stream) {
+    return guarded([&] {
+        CFD_REQUIRE(m != nullptr && shape_out != nullptr, "cfd_mesh_autotune: null argument");
+        require_lattice(m, "cfd_mesh_autotune");
+        CFD_REQUIRE(kind == CFD_TUNE_WAVE || kind == CFD_TUNE_HEAT, "cfd_mesh_autotune: unknown kind");
+        CFD_REQUIRE(steps >= 2 && steps <= 100000, "cfd_mesh_autotune: steps must be in [2, 100000]");
+        CFD_HIP(hipSetDevice(m->ctx->device));
+        hipStream_t s = as_stream(stream);
+        const size_t field = size_t(m->N) * (m->dtype == CFD_F32 ? sizeof(float) : sizeof(double));
+        const int nf = kind == CFD_TUNE_WAVE ? 8 : 2;
+        // scratch fields (zero: the kernels do the same work for any data)
+        char* buf = (char*)pool_alloc(m->ctx, int64_t(field) * nf, s);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto release = [&] {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            pool_free(m->ctx, buf, s);
+        };
+        try {
+            CFD_HIP(hipMemsetAsync(buf, 0, field * nf, s));
+            CFD_HIP(hipEventCreate(&e0));
+            CFD_HIP(hipEventCreate(&e1));
+            void* a[4];
+            void* b[4];
+            for (int f = 0; f < 4; ++f) {
+                a[f] = buf + size_t(f) * field;
+                b[f] = buf + size_t(4 + f) * field;
+            }
+            auto run = [&](int64_t n) {
+                if (kind == CFD_TUNE_WAVE) DISPATCH(m, launch_wave, m, a, b, -1.0, nullptr, 1e-3, n, s);
+                else DISPATCH(m, launch_heat, m, a[0], a[1], 1.0, nullptr, 1e-3, n, s);
+            };
+            std::array<int, 4> best{};
+            float best_ms = std::numeric_limits<float>::infinity();
+            for (const auto& c : tune_candidates(kind)) {
+                const int rc = kind == CFD_TUNE_WAVE ? cfd_mesh_set_wave_shape(m, c[0], c[1], c[2], c[3])
+                                                     : cfd_mesh_set_heat_shape(m, c[0], c[1], c[2]);
+                if (rc != CFD_OK) continue;  // a shape this mesh does not allow
+                run(2);
+                float ms = std::numeric_limits<float>::infinity();
+                for (int rep = 0; rep < 2; ++rep) {
+                    CFD_HIP(hipEventRecord(e0, s));
+                    run(steps);
+                    CFD_HIP(hipEventRecord(e1, s));
+                    CFD_HIP(hipEventSynchronize(e1));
+                    float t = 0;
+                    CFD_HIP(hipEventElapsedTime(&t, e0, e1));
+                    ms = std::min(ms, t / float(steps));
+                }
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = c;
+                }
+            }
+            CFD_REQUIRE(best_ms < std::numeric_limits<float>::infinity(), "cfd_mesh_autotune: no shape runs");
+            const int rc = kind == CFD_TUNE_WAVE ? cfd_mesh_set_wave_shape(m, best[0], best[1], best[2], best[3])
+                                                 : cfd_mesh_set_heat_shape(m, best[0], best[1], best[2]);
+            CFD_REQUIRE(rc == CFD_OK, "cfd_mesh_autotune: the chosen shape was refused");
+            for (int k = 0; k < 4; ++k) shape_out[k] = best[k];
+            if (us_per_step_out) *us_per_step_out = double(best_ms) * 1e3;
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
     });
 }
 

@@ -433,6 +433,18 @@ int cfd_mesh_set_wave_shape(cfd_mesh_t mesh, int variant, int tile_rows, int chu
  * cfdarcov2_amd/autotune.py:tune_heat times both. */
 int cfd_mesh_set_heat_shape(cfd_mesh_t mesh, int variant, int waves, int chunk);
 
+/* Autotuner (no counterpart in the reference, whose JIT kernels have one fixed
+ * shape): times every kernel shape of the fused acoustic step (CFD_TUNE_WAVE,
+ * the variants and tiles of cfd_mesh_set_wave_shape) or of the heat step
+ * (CFD_TUNE_HEAT, cfd_mesh_set_heat_shape) on this mesh, `steps` steps each
+ * (best of two runs, HIP events on the stream), on zeroed scratch fields from
+ * the pool, and applies the fastest.  shape_out receives {variant,
+ * tile_rows or waves, chunk, xcd_remap (0 for heat)}; us_per_step_out
+ * (nullable) its time per step.  Synchronises the stream. */
+#define CFD_TUNE_WAVE 0
+#define CFD_TUNE_HEAT 1
+int cfd_mesh_autotune(cfd_mesh_t mesh, int kind, int steps, int* shape_out, double* us_per_step_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -636,6 +636,37 @@ def test_heat_run_step_blocks_replayed(hip_ctx, oracle_lib):
                     assert np.array_equal(got, ref), (np.dtype(dtype).name, shape, dt, steps)
 
 
+def test_c_abi_autotuner(hip_ctx, oracle_lib):
+    """cfd_mesh_autotune picks and applies a shape the mesh allows, and runs
+    with it still equal the oracle."""
+    from cfdarcov2_amd import capi
+    rng = np.random.default_rng(31)
+    n, d = (40, 36, 64), (1.0 / 64,) * 3
+    gm = capi.HexMesh(hip_ctx, *n, *d, dtype=np.float32)
+    om = oracle_lib.OracleMesh(*n, *d, dtype=np.float32)
+    print("\n[case] C-ABI autotuner, wave", flush=True)
+    best = gm.autotune("wave", steps=4)
+    assert best["variant"] in (0, 1, 7, 8, 9, 11, 12) and best["us_per_step"] > 0, best
+    f = [rng.standard_normal(om.N).astype(np.float32) for _ in range(4)]
+    A = [_t(a, np.float32) for a in f]
+    B = [torch.empty_like(A[0]) for _ in range(4)]
+    got = [_np(t) for t in gm.wave_run(A, B, -1.0, 0.4 / 64, 8)]
+    for name, g, r in zip("puvw", got, om.wave_run(*f, -1.0, 0.4 / 64, 8)):
+        assert np.array_equal(g, r), (name, best)
+    gh = capi.HexMesh(hip_ctx, 32, 32, 32, 1 / 32, 1 / 32, 1 / 32, dtype=np.float64)
+    oh = oracle_lib.OracleMesh(32, 32, 32, 1 / 32, 1 / 32, 1 / 32, dtype=np.float64)
+    print("\n[case] C-ABI autotuner, heat", flush=True)
+    best = gh.autotune("heat", steps=6)
+    assert best["variant"] in (0, 1, 2), best
+    T0 = rng.standard_normal(oh.N)
+    a = _t(T0, np.float64)
+    b = torch.empty_like(a)
+    dt = 0.1 / 32 ** 2
+    assert np.array_equal(_np(gh.heat_run(a, b, 1.0, dt, 9)), oh.heat_run(T0, 1.0, dt, 9)), best
+    gm.close()
+    gh.close()
+
+
 def test_heat_multi_step_shape_errors(hip_ctx):
     from cfdarcov2_amd import capi
     gm = capi.HexMesh(hip_ctx, 8, 8, 8, 0.125, 0.125, 0.125, dtype=np.float64)
