@@ -7,7 +7,9 @@
 // evaluates the six face fluxes in the lower-cell -> upper-cell orientation (so
 // both cells of a face compute bit-identical fluxes and the scheme stays
 // conservative without scatter or atomics on the fields), and updates
-//   U <- U - (dt * net) * (1 / V),  net = ((Gx+ - Gx-) + (Gy+ - Gy-)) + (Gz+ - Gz-).
+//   U <- U - (dt * net) * (1 / V),  net = ((Gx+ - Gx-) + (Gy+ - Gy-)) + (Gz+ - Gz-),
+// every product that is added to something rounded once as a fused
+// multiply-add, in the same nesting as the oracle's stepper.
 //
 // Time step, all on the device: the step kernel reads dt from a rotating slot,
 // and from the cell's new state computes V / sum_f A_f (|u.n_f| + a), reduces
@@ -40,6 +42,18 @@ template <class T> struct EOut { T* u[5]; };
 
 template <class T> struct Prim { T ux, uy, uz, p, a; };
 
+// a * b + c rounded once (the fused multiply-add the scheme's rounding is
+// defined with; the oracle uses std::fma in the same places)
+template <class T> __device__ inline T fmul_add(T a, T b, T c) {
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+    else return __builtin_fma(a, b, c);
+}
+
+// u . n of a velocity and a face normal
+template <class T> __device__ inline T un_of(T ux, T uy, T uz, T nx, T ny, T nz) {
+    return fmul_add(uz, nz, fmul_add(uy, ny, ux * nx));
+}
+
 // Primitive state with one division: the reciprocal density multiplies the
 // momenta and the sound-speed argument (the oracle's euler_prim, operation
 // for operation).
@@ -49,25 +63,27 @@ template <class T> __device__ inline Prim<T> prim(T gamma, const T U[5]) {
     q.ux = U[1] * ir;
     q.uy = U[2] * ir;
     q.uz = U[3] * ir;
-    const T q2 = (q.ux * q.ux + q.uy * q.uy) + q.uz * q.uz;
-    q.p = (gamma - T(1)) * (U[4] - (T(0.5) * U[0]) * q2);
+    const T q2 = fmul_add(q.uz, q.uz, fmul_add(q.uy, q.uy, q.ux * q.ux));
+    q.p = (gamma - T(1)) * fmul_add(-(T(0.5) * U[0]), q2, U[4]);
     q.a = sqrt((gamma * q.p) * ir);
     return q;
 }
 
 // Rusanov flux through a face of unit +axis normal n and area A, L = lower cell
+// (the oracle's rusanov_flux, operation for operation)
 template <class T>
 __device__ inline void rusanov(const T UL[5], const Prim<T>& L, const T UR[5], const Prim<T>& R, T nx, T ny, T nz,
                                T A, T G[5]) {
-    const T unL = (L.ux * nx + L.uy * ny) + L.uz * nz;
-    const T unR = (R.ux * nx + R.uy * ny) + R.uz * nz;
-    const T FL[5] = {UL[0] * unL, UL[1] * unL + L.p * nx, UL[2] * unL + L.p * ny, UL[3] * unL + L.p * nz,
-                     (UL[4] + L.p) * unL};
-    const T FR[5] = {UR[0] * unR, UR[1] * unR + R.p * nx, UR[2] * unR + R.p * ny, UR[3] * unR + R.p * nz,
-                     (UR[4] + R.p) * unR};
+    const T unL = un_of(L.ux, L.uy, L.uz, nx, ny, nz);
+    const T unR = un_of(R.ux, R.uy, R.uz, nx, ny, nz);
+    const T FL[5] = {UL[0] * unL, fmul_add(UL[1], unL, L.p * nx), fmul_add(UL[2], unL, L.p * ny),
+                     fmul_add(UL[3], unL, L.p * nz), (UL[4] + L.p) * unL};
+    const T FR[5] = {UR[0] * unR, fmul_add(UR[1], unR, R.p * nx), fmul_add(UR[2], unR, R.p * ny),
+                     fmul_add(UR[3], unR, R.p * nz), (UR[4] + R.p) * unR};
     const T lam = fmax(fabs(unL) + L.a, fabs(unR) + R.a);
+    const T hA = T(0.5) * A;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) G[k] = (T(0.5) * ((FL[k] + FR[k]) - lam * (UR[k] - UL[k]))) * A;
+    for (int k = 0; k < 5; ++k) G[k] = fmul_add(-lam, UR[k] - UL[k], FL[k] + FR[k]) * hA;
 }
 
 // family and family-local index of local face fc of cell (x, y, z)
@@ -91,8 +107,8 @@ __device__ inline T local_dt(const FvGeo<T>& g, int64_t x, int64_t y, int64_t z,
     for (int fc = 0; fc < 6; ++fc) {
         int fam;
         const int64_t f = fv_face(g, x, y, z, fc, fam);
-        const T un = (q.ux * g.n[fam][0][f] + q.uy * g.n[fam][1][f]) + q.uz * g.n[fam][2][f];
-        s = s + g.A[fam][f] * (fabs(un) + q.a);
+        const T un = un_of(q.ux, q.uy, q.uz, g.n[fam][0][f], g.n[fam][1][f], g.n[fam][2][f]);
+        s = fmul_add(g.A[fam][f], fabs(un) + q.a, s);
     }
     return g.vol[c] / s;
 }
@@ -103,8 +119,8 @@ template <class T> __device__ inline T local_dt_faces(T V, const Prim<T>& q, con
     T s = T(0);
 #pragma unroll
     for (int fc = 0; fc < 6; ++fc) {
-        const T un = (q.ux * f[fc][0] + q.uy * f[fc][1]) + q.uz * f[fc][2];
-        s = s + f[fc][3] * (fabs(un) + q.a);
+        const T un = un_of(q.ux, q.uy, q.uz, f[fc][0], f[fc][1], f[fc][2]);
+        s = fmul_add(f[fc][3], fabs(un) + q.a, s);
     }
     return V / s;
 }
@@ -201,7 +217,7 @@ __global__ void __launch_bounds__(256) euler_step_k(FvGeo<T> g, EIn<T> in, EOut<
         T Un[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            Un[k] = Ui[k] - (dt * net[k]) * iV;
+            Un[k] = fmul_add(-(dt * net[k]), iV, Ui[k]);
             out.u[k][c] = Un[k];
         }
         loc = local_dt(g, x, y, z, c, prim(gamma, Un));
@@ -374,7 +390,7 @@ __global__ void __launch_bounds__(64 * TY) euler_march_k(FvGeo<T> g, EIn<T> in, 
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
                 const T net = acc[k] + (G2[k] - G1[k]);
-                Unew[k] = Uc[k] - (dt * net) * iV;
+                Unew[k] = fmul_add(-(dt * net), iV, Uc[k]);
                 out.u[k][c] = Unew[k];
             }
             loc = fmin(loc, local_dt_faces(V, prim(gamma, Unew), fg));

@@ -593,12 +593,18 @@ void wave_run_fused(const HexMesh<T>& m, T* p, T* u, T* v, T* w, T K, const T* K
 //   un = (u n_x + v n_y) + w n_z,  F(U) = (rho un, rho u un + p n_x,
 //        rho v un + p n_y, rho w un + p n_z, (E + p) un),
 //   lam = max(|un_L| + a_L, |un_R| + a_R),
-//   G_k = (0.5 ((F_L,k + F_R,k) - lam (U_R,k - U_L,k))) A.
+//   G_k = ((F_L,k + F_R,k) - lam (U_R,k - U_L,k)) (A / 2).
 // Update: net_k = ((Gx+ - Gx-) + (Gy+ - Gy-)) + (Gz+ - Gz-),
 //         U_k <- U_k - (dt net_k) (1 / V)  (one division per cell; the
 //         primitive velocities and sound speed use 1 / rho the same way).
 // Time step, taken from the state at the start of each step:
 //   dt = CFL min_i V_i / s_i,  s_i = sum over fc = 0..5 of A_f (|un_f| + a_i).
+// Rounding: every product that is added to something is one fused
+// multiply-add, fma(a, b, c) = a b + c rounded once (std::fma here, the same
+// instruction on the GPU), in the nesting written out in euler_prim,
+// rusanov_flux, euler_dt and euler_step; the scheme is this project's own,
+// and its rounding is defined here for the kernels and this restatement
+// alike.
 // ---------------------------------------------------------------------------
 
 // Geometry of the stepper.  The reference's sum-of-components area and
@@ -653,23 +659,29 @@ template <class T> inline EulerPrim<T> euler_prim(T gamma, T r, T mx, T my, T mz
     q.ux = mx * ir;
     q.uy = my * ir;
     q.uz = mz * ir;
-    const T q2 = (q.ux * q.ux + q.uy * q.uy) + q.uz * q.uz;
-    q.p = (gamma - T(1)) * (E - (T(0.5) * r) * q2);
+    const T q2 = std::fma(q.uz, q.uz, std::fma(q.uy, q.uy, q.ux * q.ux));
+    q.p = (gamma - T(1)) * std::fma(-(T(0.5) * r), q2, E);
     q.a = std::sqrt((gamma * q.p) * ir);
     return q;
+}
+
+// u . n of a velocity and a face normal
+template <class T> inline T euler_un(T ux, T uy, T uz, T nx, T ny, T nz) {
+    return std::fma(uz, nz, std::fma(uy, ny, ux * nx));
 }
 
 template <class T>
 inline void rusanov_flux(const T UL[5], const EulerPrim<T>& L, const T UR[5], const EulerPrim<T>& R, T nx, T ny,
                          T nz, T A, T G[5]) {
-    const T unL = (L.ux * nx + L.uy * ny) + L.uz * nz;
-    const T unR = (R.ux * nx + R.uy * ny) + R.uz * nz;
-    const T FL[5] = {UL[0] * unL, UL[1] * unL + L.p * nx, UL[2] * unL + L.p * ny, UL[3] * unL + L.p * nz,
-                     (UL[4] + L.p) * unL};
-    const T FR[5] = {UR[0] * unR, UR[1] * unR + R.p * nx, UR[2] * unR + R.p * ny, UR[3] * unR + R.p * nz,
-                     (UR[4] + R.p) * unR};
+    const T unL = euler_un(L.ux, L.uy, L.uz, nx, ny, nz);
+    const T unR = euler_un(R.ux, R.uy, R.uz, nx, ny, nz);
+    const T FL[5] = {UL[0] * unL, std::fma(UL[1], unL, L.p * nx), std::fma(UL[2], unL, L.p * ny),
+                     std::fma(UL[3], unL, L.p * nz), (UL[4] + L.p) * unL};
+    const T FR[5] = {UR[0] * unR, std::fma(UR[1], unR, R.p * nx), std::fma(UR[2], unR, R.p * ny),
+                     std::fma(UR[3], unR, R.p * nz), (UR[4] + R.p) * unR};
     const T lam = std::max(std::abs(unL) + L.a, std::abs(unR) + R.a);
-    for (int k = 0; k < 5; ++k) G[k] = (T(0.5) * ((FL[k] + FR[k]) - lam * (UR[k] - UL[k]))) * A;
+    const T hA = T(0.5) * A;
+    for (int k = 0; k < 5; ++k) G[k] = std::fma(-lam, UR[k] - UL[k], FL[k] + FR[k]) * hA;
 }
 
 // +axis unit normal and area of local face fc of cell i
@@ -688,8 +700,8 @@ template <class T> T euler_dt(const HexMesh<T>& m, const T* const U[5], T gamma,
         for (int fc = 0; fc < 6; ++fc) {
             T nx, ny, nz, A;
             face_na(m, i, fc, nx, ny, nz, A);
-            const T un = (q.ux * nx + q.uy * ny) + q.uz * nz;
-            s = s + A * (std::abs(un) + q.a);
+            const T un = euler_un(q.ux, q.uy, q.uz, nx, ny, nz);
+            s = std::fma(A, std::abs(un) + q.a, s);
         }
         mn = std::min(mn, m.svol[i] / s);
     }
@@ -729,7 +741,7 @@ template <class T> void euler_step(const HexMesh<T>& m, T* const U[5], T gamma, 
         for (int k = 0; k < 5; ++k) {
             // fc 4/5 = x-/x+, 2/3 = y-/y+, 0/1 = z-/z+
             const T net = ((Gf[5][k] - Gf[4][k]) + (Gf[3][k] - Gf[2][k])) + (Gf[1][k] - Gf[0][k]);
-            U[k][i] = Ui[k] - (dt * net) * iV;
+            U[k][i] = std::fma(-(dt * net), iV, Ui[k]);
         }
     }
 }
