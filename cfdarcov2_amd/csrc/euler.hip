@@ -286,9 +286,17 @@ __global__ void __launch_bounds__(64 * TY) euler_march_k(FvGeo<T> g, EIn<T> in, 
             hz_slot = (ly + 1) * PZ + (lz == 0 ? 0 : TZ + 1);
         }
     }
-    auto load = [&](int64_t o, T U[5]) {
+    // Addresses: a pointer uniform over the workgroup (the field plus the
+    // plane's offset) and a 32-bit byte offset per lane (the launcher checks
+    // that a plane fits 32-bit byte offsets)
+    auto at = [](const T* base, uint32_t boff) {
+        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+    };
+    const uint32_t bcol = uint32_t(col) * uint32_t(sizeof(T));
+    const uint32_t bzf = valid ? uint32_t(y * (Z + 1) + z) * uint32_t(sizeof(T)) : 0;  // z-face of (y, z)
+    auto load = [&](int64_t plane, uint32_t boff, T U[5]) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) U[k] = in.u[k][o];
+        for (int k = 0; k < 5; ++k) U[k] = at(in.u[k] + plane * SX, boff);
     };
     auto put = [&](T* const* S, int slot, const T U[5], const Prim<T>& q) {
 #pragma unroll
@@ -300,23 +308,26 @@ __global__ void __launch_bounds__(64 * TY) euler_march_k(FvGeo<T> g, EIn<T> in, 
         for (int k = 0; k < 5; ++k) U[k] = S[k][slot];
         q.ux = S[5][slot]; q.uy = S[6][slot]; q.uz = S[7][slot]; q.p = S[8][slot]; q.a = S[9][slot];
     };
-    auto face = [&](int fam, int64_t f, T& nx, T& ny, T& nz, T& A) {
-        nx = g.n[fam][0][f]; ny = g.n[fam][1][f]; nz = g.n[fam][2][f]; A = g.A[fam][f];
+    // face geometry: family fam, faces from `first` (uniform), lane offset boff
+    auto face4 = [&](int fam, int64_t first, uint32_t boff, T (&o)[4]) {
+        o[0] = at(g.n[fam][0] + first, boff);
+        o[1] = at(g.n[fam][1] + first, boff);
+        o[2] = at(g.n[fam][2] + first, boff);
+        o[3] = at(g.A[fam] + first, boff);
     };
-    auto face4 = [&](int fam, int64_t f, T (&o)[4]) { face(fam, f, o[0], o[1], o[2], o[3]); };
 
     T Uc[5] = {}, Un[5] = {}, Gxl[5] = {};
     T fg[6][4] = {};  // this cell's face geometry in local face order; fg[4] carried along x
     Prim<T> qc{}, qn{};
     if (valid) {  // warm-up: state of plane x0 and the flux through its lower x-face
-        load(x0 * SX + col, Uc);
+        load(x0, bcol, Uc);
         qc = prim(gamma, Uc);
-        face4(2, x0 * SX + col, fg[4]);
+        face4(2, x0 * SX, bcol, fg[4]);
         T Ul[5];
         Prim<T> ql;
         if (x0 == 0) mirror(Uc, qc, 0, Ul, ql);
         else {
-            load((x0 - 1) * SX + col, Ul);
+            load(x0 - 1, bcol, Ul);
             ql = prim(gamma, Ul);
         }
         rusanov(Ul, ql, Uc, qc, fg[4][0], fg[4][1], fg[4][2], fg[4][3], Gxl);
@@ -329,17 +340,17 @@ __global__ void __launch_bounds__(64 * TY) euler_march_k(FvGeo<T> g, EIn<T> in, 
 #pragma unroll
         for (int k = 0; k < 10; ++k) S[k] = sh[b][k];
         if (valid && x + 1 < X) {
-            load((x + 1) * SX + col, Un);
+            load(x + 1, bcol, Un);
             qn = prim(gamma, Un);
         }
         if (hy_col >= 0) {
             T Uh[5];
-            load(x * SX + hy_col, Uh);
+            load(x, uint32_t(hy_col) * uint32_t(sizeof(T)), Uh);
             put(S, hy_slot, Uh, prim(gamma, Uh));
         }
         if (hz_col >= 0) {
             T Uh[5];
-            load(x * SX + hz_col, Uh);
+            load(x, uint32_t(hz_col) * uint32_t(sizeof(T)), Uh);
             put(S, hz_slot, Uh, prim(gamma, Uh));
         }
         if (valid) put(S, own, Uc, qc);
@@ -350,7 +361,7 @@ __global__ void __launch_bounds__(64 * TY) euler_march_k(FvGeo<T> g, EIn<T> in, 
             // net = ((Gx+ - Gx-) + (Gy+ - Gy-)) + (Gz+ - Gz-), accumulated one
             // axis at a time; the upper x flux becomes the next plane's lower.
             // The six faces' geometry stays in registers for the next dt.
-            face4(2, (x + 1) * SX + col, fg[5]);
+            face4(2, (x + 1) * SX, bcol, fg[5]);
             if (x == X - 1) {
                 mirror(Uc, qc, 0, Um, qm);
                 rusanov(Uc, qc, Um, qm, fg[5][0], fg[5][1], fg[5][2], fg[5][3], G1);
@@ -363,35 +374,34 @@ __global__ void __launch_bounds__(64 * TY) euler_march_k(FvGeo<T> g, EIn<T> in, 
                 Gxl[k] = G1[k];
             }
             // y faces
-            face4(1, x * (Y + 1) * Z + y * Z + z, fg[2]);
+            face4(1, x * (Y + 1) * Z, bcol, fg[2]);
             if (y == 0) mirror(Uc, qc, 1, Um, qm);
             else get(S, own - PZ, Um, qm);
             rusanov(Um, qm, Uc, qc, fg[2][0], fg[2][1], fg[2][2], fg[2][3], G1);
-            face4(1, x * (Y + 1) * Z + (y + 1) * Z + z, fg[3]);
+            face4(1, x * (Y + 1) * Z + Z, bcol, fg[3]);
             if (y == Y - 1) mirror(Uc, qc, 1, Um, qm);
             else get(S, own + PZ, Um, qm);
             rusanov(Uc, qc, Um, qm, fg[3][0], fg[3][1], fg[3][2], fg[3][3], G2);
 #pragma unroll
             for (int k = 0; k < 5; ++k) acc[k] = acc[k] + (G2[k] - G1[k]);
             // z faces
-            const int64_t fz = x * Y * (Z + 1) + y * (Z + 1) + z;
-            face4(0, fz, fg[0]);
+            const int64_t fz = x * Y * (Z + 1);
+            face4(0, fz, bzf, fg[0]);
             if (z == 0) mirror(Uc, qc, 2, Um, qm);
             else get(S, own - 1, Um, qm);
             rusanov(Um, qm, Uc, qc, fg[0][0], fg[0][1], fg[0][2], fg[0][3], G1);
-            face4(0, fz + 1, fg[1]);
+            face4(0, fz + 1, bzf, fg[1]);
             if (z == Z - 1) mirror(Uc, qc, 2, Um, qm);
             else get(S, own + 1, Um, qm);
             rusanov(Uc, qc, Um, qm, fg[1][0], fg[1][1], fg[1][2], fg[1][3], G2);
-            const int64_t c = x * SX + col;
-            const T V = g.vol[c];
+            const T V = at(g.vol + x * SX, bcol);
             const T iV = T(1) / V;
             T Unew[5];
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
                 const T net = acc[k] + (G2[k] - G1[k]);
                 Unew[k] = fmul_add(-(dt * net), iV, Uc[k]);
-                out.u[k][c] = Unew[k];
+                *reinterpret_cast<T*>(reinterpret_cast<char*>(out.u[k] + x * SX) + bcol) = Unew[k];
             }
             loc = fmin(loc, local_dt_faces(V, prim(gamma, Unew), fg));
 #pragma unroll
@@ -458,6 +468,8 @@ void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamm
         CFD_HIP(hipGetLastError());
         const int ty = m->euler_ty, L = m->euler_L;
         const int nzt = int((m->Z + 63) / 64), nyt = int((m->Y + ty - 1) / ty);
+        CFD_REQUIRE(m->euler_variant == 0 || m->Y * (m->Z + 1) * int64_t(sizeof(T)) < (int64_t(1) << 32),
+                    "Euler march: a plane of this mesh exceeds 32-bit byte offsets (use the per-thread form)");
         const int64_t nwg = int64_t(nzt) * nyt * ((m->X + L - 1) / L);
         CFD_REQUIRE(nwg < (int64_t(1) << 31), "euler march grid too large");
         for (int64_t st = 0; st < nsteps; ++st) {
