@@ -40,6 +40,18 @@ template <int CTRL> __device__ inline double lane_shift(double own, double v) {
     return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
+// lane j's value of v (j wave-uniform)
+__device__ inline int lane_value(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+__device__ inline float lane_value(float v, int j) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), j));
+}
+__device__ inline double lane_value(double v, int j) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(b)), j));
+    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(b >> 32)), j));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
 template <class V4> __device__ inline void put(V4* dst, const V4& v, bool nt) {
     if (nt) __builtin_nontemporal_store(v, dst);
     else *dst = v;
@@ -77,14 +89,14 @@ __device__ inline void z_nbrs_row(const V4& c, bool zlo_wall, bool zhi_wall, V4&
 }
 
 // Extra inputs and outputs of the row tile's trace-recording forward step
-// (MODE 1) and transposed step (MODE 2).
+// (MODE 1) and transposed step (MODE 2, and MODE 3 with the K gradient).
 template <class T> struct RowAux {
-    T dK;              // MODE 2: dt * K; the transposed step's q = dK * lam_p
+    T dK;              // MODE 2/3: dt * K; the transposed step's q = dK * lam_p
     T* trace;          // MODE 1: p' at the sensor cells (nullable)
     T* s_out;          // MODE 1: the velocity divergence S of every cell (nullable)
-    const T* src;      // MODE 2: added to nu_p at the sensor cells (nullable)
-    const T* s_in;     // MODE 2: the forward step's S, read with gK
-    T* gK;             // MODE 2: gK += (lam_p dt) S (nullable)
+    const T* src;      // MODE 2/3: added to nu_p at the sensor cells (nullable)
+    const T* s_in;     // MODE 3: the forward step's S, read with gK
+    T* gK;             // MODE 3: gK += (lam_p dt) S
 };
 
 // One (y tile, x chunk) of the single-step acoustic row form: TY output waves
@@ -102,8 +114,9 @@ template <class T> struct RowAux {
 // which is the fused step with q in place of p, the velocity update's
 // dt * (-G) + u replaced by lam - G(q), and the pressure update's
 // dt * (K * S) + p by dt * S + lam_p (KF must be false); aux->src is added to
-// nu_p at the sensors and aux->gK, when set, gains (lam_p dt) S_t from the
-// forward step's aux->s_in.
+// nu_p at the sensors.  MODE 3 is MODE 2 whose aux->gK also gains
+// (lam_p dt) S_t from the forward step's aux->s_in (a mode of its own, so the
+// plain transposed step keeps no registers for it).
 template <class T, int TY, bool PW, bool KF, bool XB, bool NTL, bool MR = true, int MODE = 0>
 __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p, const T* __restrict__ u,
                                 const T* __restrict__ v, const T* __restrict__ w, T* __restrict__ po,
@@ -111,7 +124,8 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
                                 const T* __restrict__ Kf, T dt, int64_t x0, int64_t x1, int y0, int wv, bool nt,
                                 typename Row4<T>::type (*sp)[64], typename Row4<T>::type (*sv)[64],
                                 const SensorList* sens = nullptr, const RowAux<T>* aux = nullptr) {
-    constexpr bool ADJ = MODE == 2;
+    constexpr bool ADJ = MODE >= 2;
+    constexpr bool GKM = MODE == 3;
     static_assert(!(ADJ && KF), "the transposed row step takes a constant K");
     using V4 = typename Row4<T>::type;
     const int64_t X = g.X;
@@ -169,16 +183,24 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
         if constexpr (ADJ) return q - mggp(mlin(pc, pl), mlin(pc, ph), A, Vd);
         else return mvel(pl, pc, ph, q, A, Vd, dt);
     };
-    // MODE 1/2: the sensors in this output wave's rows and x chunk (bit j =
-    // sensor j; wave-uniform, built once on the scalar unit)
+    // MODE 1/2: lane j loads sensor j's x, y, z (and in MODE 2 its trace
+    // gradient) here (x = -1 past the list); once the first plane's loads are
+    // in flight one ballot turns them into `mine`, the sensors in this output
+    // wave's rows and x chunk (bit j = sensor j, wave-uniform).  The planes
+    // then read a sensor's values from its lane (readlane), so a wave holding
+    // sensors waits on no memory for them.
     const int wy0 = y0 + wv * RW;
     uint64_t mine = 0;
+    int sxl = -1, syl = -1, szl = 0;
+    T srcl = T(0);
+    bool want = false;
     if constexpr (MODE != 0) {
-        const bool want = ADJ ? aux->src != nullptr : aux->trace != nullptr;
-        if (is_out && want) {
-            for (int j = 0; j < sens->n; ++j)
-                if (sens->x[j] >= x0 && sens->x[j] < x1 && sens->y[j] >= wy0 && sens->y[j] < wy0 + RW)
-                    mine |= uint64_t(1) << j;
+        want = is_out && (ADJ ? aux->src != nullptr : aux->trace != nullptr);
+        if (want && lane < sens->n) {
+            sxl = sens->x[lane];
+            syl = sens->y[lane];
+            szl = sens->z[lane];
+            if constexpr (ADJ) srcl = aux->src[lane];
         }
     }
 
@@ -209,8 +231,8 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
         if constexpr (KF) {
             if (out) f.k = ld(Kf, pb + rb);
         }
-        if constexpr (ADJ) {  // the K gradient's inputs: S_t and gK
-            if (out && aux->gK) {
+        if constexpr (GKM) {  // the K gradient's inputs: S_t and gK
+            if (out) {
                 f.k = ld(aux->s_in, pb + rb);
                 f.gk = ld(aux->gK, pb + rb);
             }
@@ -250,6 +272,9 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
     for (int64_t x = x0; x < x1; ++x) {
         const Fetch cur = nx;
         if (x + 1 < x1) fetch(x + 1, nx);
+        if constexpr (MODE != 0) {
+            if (want && x == x0) mine = __ballot(sxl >= x0 && sxl < x1 && syl >= wy0 && syl < wy0 + RW);
+        }
         const V4 Qa = qv(cur.a);  // p(x+2) (output cells) or p(x) (halo cells), as q in MODE 2
 
         // 1. u'(x+1); p into the LDS plane
@@ -321,26 +346,31 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
             if constexpr (MODE != 0) {
                 // sensors of this plane: record p' (MODE 1) or add the trace
                 // gradient (MODE 2) in the lane that holds the cell
+                // (every lane_value is read here, with the whole wave active:
+                // a value reloaded from scratch under a narrower exec mask
+                // would be stale in the lanes outside it)
                 uint64_t m = mine;
                 while (m) {
                     const int j = __builtin_ctzll(m);
                     m &= m - 1;
-                    if (sens->x[j] != x) continue;
-                    const int zz = sens->z[j];
-                    if (out && lp.r == sens->y[j] - wy0 && zz >= lp.zc && zz < lp.zc + 4) {
+                    const int sx = lane_value(sxl, j), sy = lane_value(syl, j), zz = lane_value(szl, j);
+                    T sv = T(0);
+                    if constexpr (ADJ) sv = lane_value(srcl, j);
+                    if (sx != x) continue;
+                    if (out && lp.r == sy - wy0 && zz >= lp.zc && zz < lp.zc + 4) {
                         const int i = zz - lp.zc;
-                        if constexpr (ADJ) pn[i] = pn[i] + aux->src[j];
+                        if constexpr (ADJ) pn[i] = pn[i] + sv;
                         else aux->trace[j] = pn[i];
                     }
                 }
-                if constexpr (ADJ) {
-                    if (out && aux->gK) {
+                if constexpr (GKM) {
+                    if (out) {
                         V4 gk;
 #pragma unroll
                         for (int i = 0; i < 4; ++i) gk[i] = cur.gk[i] + (R0[i] * dt) * cur.k[i];
                         *reinterpret_cast<V4*>(aux->gK + o) = gk;
                     }
-                } else {
+                } else if constexpr (MODE == 1) {
                     if (out && aux->s_out) *reinterpret_cast<V4*>(aux->s_out + o) = Sv;
                 }
             }

@@ -92,12 +92,25 @@ static void launch_row_k(const UniformGeo<T>& g, void* const* in, void* const* o
 template <class T, int TY, bool NTL>
 static void launch_row_ty(const UniformGeo<T>& g, void* const* in, void* const* out, double K, const void* Kf,
                           double dt, int L, int order, bool nt, hipStream_t s) {
-    if (g.vol_pow2) {
-        if (Kf) launch_row_k<T, TY, true, true, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
-        else launch_row_k<T, TY, true, false, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
-    } else {
-        if (Kf) launch_row_k<T, TY, false, true, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
+    if constexpr (sizeof(T) == 8 && TY == 8) {
+        // The FP64 8-row tile with a K field would spill registers, and the
+        // tile reads other lanes' registers (the z neighbours), which a reload
+        // under a partial exec mask leaves stale in the inactive lanes: such
+        // runs take the 4-row tile (same results, every form is bit-exact).
+        if (Kf) {
+            launch_row_ty<T, 4, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
+            return;
+        }
+        if (g.vol_pow2) launch_row_k<T, TY, true, false, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
         else launch_row_k<T, TY, false, false, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
+    } else {
+        if (g.vol_pow2) {
+            if (Kf) launch_row_k<T, TY, true, true, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
+            else launch_row_k<T, TY, true, false, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
+        } else {
+            if (Kf) launch_row_k<T, TY, false, true, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
+            else launch_row_k<T, TY, false, false, NTL>(g, in, out, K, Kf, dt, L, order, nt, s);
+        }
     }
 }
 
@@ -123,7 +136,7 @@ bool launch_wave_row(const UniformGeo<T>& g, void* const* in, void* const* out, 
     switch (ty) {
         case 4: launch_row_c<T, 4>(g, in, out, K, Kf, dt, L, order, cache, s); break;
         case 8: launch_row_c<T, 8>(g, in, out, K, Kf, dt, L, order, cache, s); break;
-        case 14:  // FP32 only: the FP64 form of this tile spills registers
+        case 14:  // FP32 only: the FP64 form of this tile spills registers (see launch_row_ty)
             if constexpr (sizeof(T) == 4) {
                 launch_row_c<T, 14>(g, in, out, K, Kf, dt, L, order, cache, s);
                 break;

@@ -317,13 +317,18 @@ int cfd_heat_adjoint(cfd_mesh_t mesh, void* adj_a, void* adj_b, double alpha, co
 /* Kernel form of cfd_wave_traces / cfd_wave_adjoint (no reference
  * counterpart): form 0 = one thread per cell; form 1 = row form (uniform
  * meshes whose z extent is a multiple of 4 and at most 256, else
- * CFD_E_UNSUPPORTED) with `tile_rows` (4 or 8) output waves and `chunk`
- * x-planes per workgroup.  The row form's forward records the traces in the
+ * CFD_E_UNSUPPORTED) with `tile_rows` (4 or 8; 4 in FP64) output waves and
+ * `chunk` x-planes per workgroup.  The row form's forward records the traces in the
  * fused step itself; its transposed step needs a constant K (runs with a K
  * field take form 0 for the transposed steps) and its K gradient keeps one
- * divergence field per recomputed step instead of three velocity fields.  Both
- * forms give bit-identical traces and gradients.  Uniform meshes that allow it
- * start on form 1 (4 waves, 8 planes). */
+ * divergence field per recomputed step instead of three velocity fields.
+ * Form 2 (FP32 meshes the row form takes, else CFD_E_UNSUPPORTED) advances two
+ * steps per launch, forward (constant K) and transposed (no K gradient), in
+ * runs of four steps; `tile_rows` names its two-step tile (4; or 8 or 10 for
+ * z extents up to 128) and the other steps take the row form with 8 output
+ * waves (tile_rows 8 or 10) or 4.  All forms give bit-identical traces and
+ * gradients.  Uniform meshes that allow it start on form 1 (4 waves, 8
+ * planes). */
 int cfd_mesh_set_adjoint_shape(cfd_mesh_t mesh, int form, int tile_rows, int chunk);
 
 /* ---- batched ensemble (FP32 uniform meshes; no reference counterpart) -----
@@ -401,7 +406,9 @@ int cfd_mesh_geometry(cfd_mesh_t mesh, cfd_geometry* out);
  *      per workgroup carried by 4 (8 for 32) thread rows; any z extent;
  *   7  row form (z extent a multiple of 4 and at most 256): one wave per z row
  *      with four cells per lane, `tile_rows` (4 or 8; 14 in FP32) output rows
- *      plus two halo rows per workgroup, non-temporal output stores;
+ *      plus two halo rows per workgroup, non-temporal output stores (FP64
+ *      runs with a K field take 4 rows where 8 are set: that build would
+ *      spill registers);
  *   8  variant 7 with ordinary stores;
  *   9  variant 7 with non-temporal loads and ordinary stores;
  *   11 two steps per launch (`tile_rows` 4, 8 or 10 in FP32, 2 in FP64; steps
