@@ -176,10 +176,11 @@ ADJOINT_CANDIDATES = ([(0, 4, 8)] + [(1, ty, L) for ty in (4, 8) for L in (2, 4,
                       + [(2, ty, L) for ty in (4, 8, 10) for L in (4, 8, 16, 32)])
 
 
-def tune_adjoint(problem, steps=20, candidates=None, verbose=False) -> dict:
+def tune_adjoint(problem, steps=40, candidates=None, verbose=False, repeats=3) -> dict:
     """Pick the fastest kernel shape of the transposed acoustic step for the
     problem's mesh (constant K, no K gradient: the dL/dp0 path), from runs on
-    scratch fields; forms the mesh does not allow are skipped."""
+    scratch fields (best of `repeats` timed runs per shape, the shapes being
+    a few per cent apart); forms the mesh does not allow are skipped."""
     import torch
 
     from . import adjoint
@@ -196,12 +197,14 @@ def tune_adjoint(problem, steps=20, candidates=None, verbose=False) -> dict:
         except CfdError:
             continue
         adjoint.adjoint(short, g)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        adjoint.adjoint(short, g)
-        e1.record(stream)
-        e1.synchronize()
-        ms = e0.elapsed_time(e1) / steps
+        ms = float("inf")
+        for _ in range(repeats):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            adjoint.adjoint(short, g)
+            e1.record(stream)
+            e1.synchronize()
+            ms = min(ms, e0.elapsed_time(e1) / steps)
         table.append({"form": form, "tile_rows": ty, "chunk": L, "ms_per_step": ms})
         if verbose:
             print(f"  adjoint shape form={form} ty={ty} L={L}: {ms * 1e3:.1f} us/step", flush=True)

@@ -240,13 +240,29 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
     };
 
     // ---- warm-up of the output cells' x queue ---------------------------------
+    // Every load of the warm-up and of the first plane is issued before any
+    // is used (planes clamped into the mesh, the wall cases chosen by
+    // selects), so a workgroup starts after one memory round trip, not one
+    // per plane: with short chunks the start is a large part of its run.
     V4 P0 = zero4, Pp1 = zero4, Um1 = zero4, U0 = zero4;
     V4 R0 = zero4, Rp1 = zero4;  // MODE 2: lam_p(x), lam_p(x+1) as loaded (P0, Pp1 hold q)
+    V4 a0 = zero4, am1 = zero4, ap1 = zero4, am2 = zero4, um = zero4, uc = zero4;
     if (out) {
-        V4 Pm1 = zero4;
-        P0 = ld(p, x0 * SX + rb);
-        if (x0 >= 1) Pm1 = ld(p, (x0 - 1) * SX + rb);
-        if (x0 + 1 < X) Pp1 = ld(p, (x0 + 1) * SX + rb);
+        const int64_t xm1 = x0 >= 1 ? x0 - 1 : x0, xm2 = x0 >= 2 ? x0 - 2 : xm1;
+        const int64_t xp1 = x0 + 1 < X ? x0 + 1 : x0;
+        a0 = ld(p, x0 * SX + rb);
+        am1 = ld(p, xm1 * SX + rb);
+        ap1 = ld(p, xp1 * SX + rb);
+        am2 = ld(p, xm2 * SX + rb);
+        um = ld(u, xm1 * SX + rb);
+        uc = ld(u, x0 * SX + rb);
+    }
+    Fetch nx{zero4, zero4, zero4, zero4, zero4, zero4};
+    fetch(x0, nx);
+    if (out) {
+        P0 = a0;
+        V4 Pm1 = x0 >= 1 ? am1 : zero4;
+        Pp1 = x0 + 1 < X ? ap1 : zero4;
         if constexpr (ADJ) {
             R0 = P0;
             Rp1 = Pp1;
@@ -255,19 +271,14 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
             Pm1 = qv(Pm1);
         }
         if (x0 >= 1) {
-            const V4 Pm2 = (x0 >= 2) ? qv(ld(p, (x0 - 2) * SX + rb)) : Pm1;
-            const V4 um = ld(u, (x0 - 1) * SX + rb);
+            const V4 Pm2 = (x0 >= 2) ? qv(am2) : Pm1;
 #pragma unroll
             for (int i = 0; i < 4; ++i) Um1[i] = vel(Pm2[i], Pm1[i], P0[i], um[i], Ax);
         }
-        const V4 uc = ld(u, x0 * SX + rb);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
             U0[i] = vel(x0 >= 1 ? Pm1[i] : P0[i], P0[i], x0 + 1 < X ? Pp1[i] : P0[i], uc[i], Ax);
     }
-
-    Fetch nx{zero4, zero4, zero4, zero4, zero4, zero4};
-    fetch(x0, nx);
 
     for (int64_t x = x0; x < x1; ++x) {
         const Fetch cur = nx;

@@ -216,13 +216,27 @@ __device__ inline void tile2(const UniformGeo<T>& g, const T* __restrict__ p, co
     };
 
     // ---- warm-up of the first step's x queue (the single-step kernel's) ------
+    // every load of the warm-up and of the first plane issued before any is
+    // used, as the single-step row tile's warm-up
     V4T<T> P0 = zero4, Pp1 = zero4, Um1 = zero4, U0 = zero4;
     V4T<T> RA0 = zero4, RAp1 = zero4;  // MODE 2: lam_p(x), lam_p(x+1) as loaded (P0, Pp1 hold q)
+    V4T<T> a0 = zero4, am1 = zero4, ap1 = zero4, am2 = zero4, um = zero4, uc = zero4;
     if (rA && lv) {
-        V4T<T> Pm1 = zero4;
-        P0 = ld(p, xs * SX + rb);
-        if (xs >= 1) Pm1 = ld(p, (xs - 1) * SX + rb);
-        if (xs + 1 < X) Pp1 = ld(p, (xs + 1) * SX + rb);
+        const int64_t xm1 = xs >= 1 ? xs - 1 : xs, xm2 = xs >= 2 ? xs - 2 : xm1;
+        const int64_t xp1 = xs + 1 < X ? xs + 1 : xs;
+        a0 = ld(p, xs * SX + rb);
+        am1 = ld(p, xm1 * SX + rb);
+        ap1 = ld(p, xp1 * SX + rb);
+        am2 = ld(p, xm2 * SX + rb);
+        um = ld(u, xm1 * SX + rb);
+        uc = ld(u, xs * SX + rb);
+    }
+    Fetch nx{zero4, zero4, zero4, zero4};
+    fetch(xs, nx);
+    if (rA && lv) {
+        P0 = a0;
+        V4T<T> Pm1 = xs >= 1 ? am1 : zero4;
+        Pp1 = xs + 1 < X ? ap1 : zero4;
         if constexpr (ADJ) {
             RA0 = P0;
             RAp1 = Pp1;
@@ -231,11 +245,9 @@ __device__ inline void tile2(const UniformGeo<T>& g, const T* __restrict__ p, co
             Pm1 = qv(Pm1);
         }
         if (xs >= 1) {
-            const V4T<T> Pm2 = (xs >= 2) ? qv(ld(p, (xs - 2) * SX + rb)) : Pm1;
-            const V4T<T> um = ld(u, (xs - 1) * SX + rb);
+            const V4T<T> Pm2 = (xs >= 2) ? qv(am2) : Pm1;
             Um1 = vel(Pm2, Pm1, P0, um, Ax);
         }
-        const V4T<T> uc = ld(u, xs * SX + rb);
         V4T<T> lo = P0, hi = P0;
         if (xs >= 1) lo = Pm1;
         if (xs + 1 < X) hi = Pp1;
@@ -244,9 +256,6 @@ __device__ inline void tile2(const UniformGeo<T>& g, const T* __restrict__ p, co
     // second-step queues: p'(x-2), p'(x-1) (as computed: lam_p' in MODE 2);
     // v', w' of x-2, x-1; u''(x-3), u''(x-2)
     V4T<T> Q2 = zero4, Q1 = zero4, Va2 = zero4, Va1 = zero4, Wa2 = zero4, Wa1 = zero4, R3 = zero4, R2 = zero4;
-
-    Fetch nx{zero4, zero4, zero4, zero4};
-    fetch(xs, nx);
 
     for (int64_t x = xs; x < xe; ++x) {
         const bool s1 = !XB || x < X;  // first step at plane x
@@ -532,13 +541,27 @@ __device__ inline void tile2m(const UniformGeo<T>& g, const T* __restrict__ p, c
         }
     };
 
+    // warm-up: every load of it and of the first plane issued before any is
+    // used, as tile2's
     V4T<T> P0 = zero4, Pp1 = zero4, Um1 = zero4, U0 = zero4;
     V4T<T> RA0 = zero4, RAp1 = zero4;  // MODE 2: lam_p(x), lam_p(x+1) as loaded (P0, Pp1 hold q)
+    V4T<T> a0 = zero4, am1 = zero4, ap1 = zero4, am2 = zero4, um = zero4, uc = zero4;
     if (real) {
-        V4T<T> Pm1 = zero4;
-        P0 = ld(p, xs * SX + rb);
-        if (xs >= 1) Pm1 = ld(p, (xs - 1) * SX + rb);
-        if (xs + 1 < X) Pp1 = ld(p, (xs + 1) * SX + rb);
+        const int64_t xm1 = xs >= 1 ? xs - 1 : xs, xm2 = xs >= 2 ? xs - 2 : xm1;
+        const int64_t xp1 = xs + 1 < X ? xs + 1 : xs;
+        a0 = ld(p, xs * SX + rb);
+        am1 = ld(p, xm1 * SX + rb);
+        ap1 = ld(p, xp1 * SX + rb);
+        am2 = ld(p, xm2 * SX + rb);
+        um = ld(u, xm1 * SX + rb);
+        uc = ld(u, xs * SX + rb);
+    }
+    Fetch nx{zero4, zero4, zero4, zero4};
+    fetch(xs, nx);
+    if (real) {
+        P0 = a0;
+        V4T<T> Pm1 = xs >= 1 ? am1 : zero4;
+        Pp1 = xs + 1 < X ? ap1 : zero4;
         if constexpr (ADJ) {
             RA0 = P0;
             RAp1 = Pp1;
@@ -547,11 +570,9 @@ __device__ inline void tile2m(const UniformGeo<T>& g, const T* __restrict__ p, c
             Pm1 = qv(Pm1);
         }
         if (xs >= 1) {
-            const V4T<T> Pm2 = (xs >= 2) ? qv(ld(p, (xs - 2) * SX + rb)) : Pm1;
-            const V4T<T> um = ld(u, (xs - 1) * SX + rb);
+            const V4T<T> Pm2 = (xs >= 2) ? qv(am2) : Pm1;
             Um1 = vel(Pm2, Pm1, P0, um, Ax);
         }
-        const V4T<T> uc = ld(u, xs * SX + rb);
         V4T<T> lo = P0, hi = P0;
         if (xs >= 1) lo = Pm1;
         if (xs + 1 < X) hi = Pp1;
@@ -560,9 +581,6 @@ __device__ inline void tile2m(const UniformGeo<T>& g, const T* __restrict__ p, c
     // second-step queues: p'(x-2), p'(x-1) (as computed: lam_p' in MODE 2);
     // v', w' of x-2, x-1; u''(x-3), u''(x-2)
     V4T<T> Q2 = zero4, Q1 = zero4, Va2 = zero4, Va1 = zero4, Wa2 = zero4, Wa1 = zero4, R3 = zero4, R2 = zero4;
-
-    Fetch nx{zero4, zero4, zero4, zero4};
-    fetch(xs, nx);
 
     for (int64_t x = xs; x < xe; ++x) {
         const bool s1 = !XB || x < X;
