@@ -102,13 +102,17 @@ def tune_euler(mesh, U, gamma=1.4, cfl=0.3, steps=10, candidates=None, verbose=F
     touched.  Returns the best shape and the table."""
     import torch
 
+    from .capi import CfdError
     cands = candidates or EULER_CANDIDATES
     A = [u.clone() for u in U]
     B = [torch.empty_like(u) for u in U]
     stream = torch.cuda.current_stream()
     table = []
     for variant, ty, L in cands:
-        mesh.set_euler_shape(variant, ty, L)
+        try:
+            mesh.set_euler_shape(variant, ty, L)
+        except CfdError:   # a shape this mesh's precision does not offer
+            continue
         for a, u in zip(A, U):
             a.copy_(u)
         mesh.euler_run(A, B, gamma, cfl, 2)

@@ -494,8 +494,15 @@ void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamm
             switch (ty) {
                 case 2: CFD_EULER_MARCH(2); break;
                 case 4: CFD_EULER_MARCH(4); break;
-                case 8: CFD_EULER_MARCH(8); break;
-                default: throw Error(CFD_E_INVALID, "euler march tile rows must be 2, 4 or 8");
+                case 8:
+                    // FP32 only: the FP64 8-row build spills registers, and the
+                    // kernel's dt reduction reads other lanes' registers
+                    if constexpr (sizeof(T) == 4) {
+                        CFD_EULER_MARCH(8);
+                        break;
+                    }
+                    [[fallthrough]];
+                default: throw Error(CFD_E_INVALID, "euler march tile rows must be 2, 4 or 8 (2 or 4 in FP64)");
             }
 #undef CFD_EULER_MARCH
         }

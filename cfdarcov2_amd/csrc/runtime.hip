@@ -987,7 +987,8 @@ int cfd_mesh_set_euler_shape(cfd_mesh_t m, int variant, int tile_rows, int chunk
         CFD_REQUIRE(m != nullptr, "null mesh");
         require_lattice(m, "cfd_mesh_set_euler_shape");
         CFD_REQUIRE(variant == 0 || variant == 1, "euler variant must be 0 or 1");
-        CFD_REQUIRE(tile_rows == 2 || tile_rows == 4 || tile_rows == 8, "euler tile_rows must be 2, 4 or 8");
+        CFD_REQUIRE(tile_rows == 2 || tile_rows == 4 || (tile_rows == 8 && m->dtype == CFD_F32),
+                    "euler tile_rows must be 2, 4 or 8 (2 or 4 in FP64: the FP64 8-row build spills registers)");
         CFD_REQUIRE(chunk >= 1 && chunk <= 65535, "euler chunk must be in [1, 65535]");
         m->euler_variant = variant;
         m->euler_ty = tile_rows;

@@ -267,7 +267,7 @@ int cfd_euler_run(cfd_mesh_t mesh, void* const state_a[5], void* const state_b[5
                   int64_t nsteps, void* dt_hist, void* stream);
 /* Kernel shape of cfd_euler_run: variant 0 = one thread per cell (every face
  * flux evaluated by both of its cells), 1 = 2.5D march (default) with
- * tile_rows (2, 4 or 8) y-rows x 64 z-cells per workgroup and `chunk`
+ * tile_rows (2, 4 or 8; 2 or 4 in FP64) y-rows x 64 z-cells per workgroup and `chunk`
  * x-planes, each cell's primitives computed once per plane and shared through
  * LDS.  Both give bit-identical results. */
 int cfd_mesh_set_euler_shape(cfd_mesh_t mesh, int variant, int tile_rows, int chunk);

@@ -1,5 +1,6 @@
 """The kernels that read other lanes' registers (DPP lane shifts for z
-neighbours, readlane for the sensors) must not spill registers: a spilled
+neighbours, readlane and ballots for the sensors, shuffles in reductions)
+must not spill registers: a spilled
 value is reloaded only in the lanes active at the reload, so a later read of
 an inactive lane's copy sees a stale value.  (An FP64 8-row build of the
 transposed row step spilled and did exactly that: it reloaded a sensor's
@@ -20,8 +21,8 @@ import pytest
 from cfdarcov2_amd import build
 
 CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cfdarcov2_amd", "csrc")
-CROSS_LANE_UNITS = ["adj_row.hip", "adj_row2.hip", "adjoint.hip", "batch.hip", "heat_multi.hip", "heat_row.hip",
-                    "wave_row.hip", "wave_row2.hip"]
+CROSS_LANE_UNITS = ["adj_row.hip", "adj_row2.hip", "adjoint.hip", "batch.hip", "euler.hip", "ewise.hip",
+                    "heat_multi.hip", "heat_row.hip", "wave_row.hip", "wave_row2.hip"]
 
 
 def _have_hipcc():
@@ -34,7 +35,7 @@ def _units_including_cross_lane_code():
         if not f.endswith(".hip"):
             continue
         src = open(os.path.join(CSRC, f)).read()
-        if re.search(r'#include "row2?\.hpp"', src) or "update_dpp" in src or "readlane" in src:
+        if re.search(r'#include "row2?\.hpp"|update_dpp|readlane|__shfl|__ballot', src):
             out.append(f)
     return out
 
