@@ -130,6 +130,7 @@ template <class T> struct UnstructGeo {
     const T* area;
     const T* alpha;        // alpha_d (the same from both cells)
     const T* vol;          // per cell
+    const T* len;          // per (fc, cell), fc-major: centre-to-face distance
 };
 
 }  // namespace cfd

@@ -306,13 +306,19 @@ void unstructured_table(const UConn& c, const int64_t* perm, const int64_t* iper
 template <class T> void launch_ugrad(const cfd_mesh_s* m, const void* phi, void* out, int mask, int wall, hipStream_t s);
 template <class T> void launch_ulapl(const cfd_mesh_s* m, const void* phi, void* out, int mask, hipStream_t s);
 template <class T>
+void launch_ustab(const cfd_mesh_s* m, const void* phi, void* out, int mask, void* work, hipStream_t s);
+template <class T>
+void launch_uwave(const cfd_mesh_s* m, void* const a[4], void* const b[4], double K, const void* Kf, double dt,
+                  int64_t nsteps, hipStream_t s);
+template <class T>
 void launch_uheat(const cfd_mesh_s* m, void* Ta, void* Tb, double alpha, const void* af, double dt, int64_t nsteps,
                   hipStream_t s);
 
 static void require_lattice(const cfd_mesh_s* m, const char* what) {
     if (m->unstructured)
         throw Error(CFD_E_UNSUPPORTED, std::string(what) +
-                                           ": unstructured meshes take cfd_grad, cfd_lapl and cfd_heat_run only");
+                                           ": unstructured meshes take cfd_grad, cfd_lapl, cfd_stab, cfd_heat_run and "
+                                           "cfd_wave_run only");
 }
 
 static UConn uconn_of(const cfd_umesh_desc* d) {
@@ -703,14 +709,14 @@ int cfd_lapl(cfd_mesh_t m, const void* phi, void* out, int mask, void* stream) {
 int cfd_stab(cfd_mesh_t m, const void* phi, void* out, int mask, void* stream) {
     return guarded([&] {
         CFD_REQUIRE(m && phi && out, "cfd_stab: null argument");
-        require_lattice(m, "cfd_stab");
         CFD_REQUIRE(mask >= 0 && mask <= 7, "cfd_stab: bad axis_mask");
         hipStream_t s = as_stream(stream);
         // the first stage's gradients go through 3N pool entries on general
-        // meshes; uniform meshes recompute them in one pass
-        void* work = m->uniform ? nullptr : pool_alloc(m->ctx, 3 * m->N * elem_size(m->dtype), s);
+        // and unstructured meshes; uniform meshes recompute them in one pass
+        void* work = m->uniform && !m->unstructured ? nullptr : pool_alloc(m->ctx, 3 * m->N * elem_size(m->dtype), s);
         try {
-            DISPATCH(m, launch_stab, m, phi, out, mask, work, s);
+            if (m->unstructured) DISPATCH(m, launch_ustab, m, phi, out, mask, work, s);
+            else DISPATCH(m, launch_stab, m, phi, out, mask, work, s);
         } catch (...) {
             if (work) pool_free(m->ctx, work, s);
             throw;
@@ -882,13 +888,13 @@ int cfd_wave_run(cfd_mesh_t m, void* const a[4], void* const b[4], double K, con
                  int64_t nsteps, void* stream) {
     return guarded([&] {
         CFD_REQUIRE(m && a && b, "cfd_wave_run: null argument");
-        require_lattice(m, "cfd_wave_run");
         for (int i = 0; i < 4; ++i) {
             CFD_REQUIRE(a[i] && b[i], "cfd_wave_run: null state pointer");
             for (int j = 0; j < 4; ++j) CFD_REQUIRE(a[i] != b[j], "cfd_wave_run: state_a and state_b overlap");
         }
         CFD_REQUIRE(nsteps >= 0, "cfd_wave_run: negative step count");
-        DISPATCH(m, launch_wave, m, a, b, K, Kf, dt, nsteps, as_stream(stream));
+        if (m->unstructured) DISPATCH(m, launch_uwave, m, a, b, K, Kf, dt, nsteps, as_stream(stream));
+        else DISPATCH(m, launch_wave, m, a, b, K, Kf, dt, nsteps, as_stream(stream));
     });
 }
 

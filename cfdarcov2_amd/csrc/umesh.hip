@@ -7,7 +7,8 @@
 //   (-1 on the boundary) and the face id; int8: the sign that turns the face
 //   normal into the cell's outward normal;
 //   per face: the normal (first three vertices), area and alpha_d;
-//   per cell: the volume.
+//   per cell: the volume; per (fc, cell): the centre-to-face distance (the
+//   stabilisation's upwind extrapolation).
 // Each cell reads its own six faces and neighbours and writes only itself:
 // no atomics and no scatter.  A face's value is formed by each of its two
 // cells in that cell's own operation order, which is the reference's per
@@ -138,6 +139,150 @@ template <class T> void launch_ulapl(const cfd_mesh_s* m, const void* phi, void*
     ulapl_k<T><<<ugrid(m->N), kUBlock, 0, s>>>(ugeo_u<T>(m), (const T*)phi, (T*)out, mask);
     CFD_HIP(hipGetLastError());
 }
+// StabVar, first stage: the three outward-Gauss gradient components of phi
+// with the reference face rule (neighbour = self on the boundary), each
+// written as computed (no 0 + g), as the reference keeps them per axis.
+template <class T>
+__global__ void __launch_bounds__(kUBlock) ustab_grad_k(UnstructGeo<T> g, const T* __restrict__ phi,
+                                                        T* __restrict__ gx, T* __restrict__ gy, T* __restrict__ gz) {
+    const int64_t i = int64_t(blockIdx.x) * kUBlock + threadIdx.x;
+    if (i >= g.N) return;
+    int32_t nb[6], f[6];
+    int8_t sg[6];
+    uload(g, i, nb, f, sg);
+    const T pc = phi[i];
+    T fv[6];
+#pragma unroll
+    for (int fc = 0; fc < 6; ++fc) fv[fc] = (pc + (nb[fc] < 0 ? pc : phi[nb[fc]])) * T(0.5);
+    gx[i] = ugauss(g, fv, f, sg, 0, i, false);
+    gy[i] = ugauss(g, fv, f, sg, 1, i, false);
+    gz[i] = ugauss(g, fv, f, sg, 2, i, false);
+}
+
+// StabVar, second stage (include/grad_utils.hpp:38-80, src/fvm3d.cpp:915-936):
+// per selected axis d, the upwind face value phi + len * g_d * n_out,d of the
+// cell on face fc against the neighbour's on its opposite local face fc ^ 1
+// (self on the boundary), 0.5 * (own - neighbour's), axis-normal Gauss sum,
+// res = 0 + S_x (+ S_y) (+ S_z).  The neighbour's value is recomputed from its
+// phi, gradient and face geometry, as its own cell forms it.
+template <class T>
+__global__ void __launch_bounds__(kUBlock) ustab_k(UnstructGeo<T> g, const T* __restrict__ phi,
+                                                   const T* __restrict__ gx, const T* __restrict__ gy,
+                                                   const T* __restrict__ gz, T* __restrict__ out, int mask) {
+    const int64_t i = int64_t(blockIdx.x) * kUBlock + threadIdx.x;
+    if (i >= g.N) return;
+    int32_t nb[6], f[6];
+    int8_t sg[6];
+    uload(g, i, nb, f, sg);
+    const T* const gd[3] = {gx, gy, gz};
+    auto upwind = [&](int d, int64_t c, int fc) {
+        const int64_t k = int64_t(fc) * g.N + c;
+        const T n = g.fn[d][g.face[k]];
+        const T na = g.osign[k] < 0 ? -n : n;
+        return phi[c] + g.len[k] * gd[d][c] * na;
+    };
+    T res = T(0);
+    for (int d = 0; d < 3; ++d) {
+        if (!(mask & (1 << d))) continue;
+        T fv[6];
+#pragma unroll
+        for (int fc = 0; fc < 6; ++fc) {
+            const int64_t c = nb[fc] < 0 ? i : int64_t(nb[fc]);
+            const T own = upwind(d, i, fc);
+            const T col = upwind(d, c, fc ^ 1);
+            fv[fc] = T(0.5) * (own - col);
+        }
+        res = res + ugauss(g, fv, f, sg, d, i, true);
+    }
+    out[i] = res;
+}
+
+// The fused acoustic step (the reference operators' composition, as the
+// structured kernels run it): first every cell's velocities
+//   u_d' = dt * (-(0 + G_d(p))) + u_d   (face values (p_c + p_nb) / 2, neighbour = self)
+// then the pressure from the new velocities
+//   p' = dt * (K * ((D_x(u') + D_y(v')) + D_z(w'))) + p
+// with D_a = 0 + the outward Gauss sum of velocity a under the reflective-wall
+// face rule (zero on boundary faces of the axis-a pair of local faces).
+template <class T>
+__global__ void __launch_bounds__(kUBlock) uwave_vel_k(UnstructGeo<T> g, const T* __restrict__ p,
+                                                       const T* __restrict__ u, const T* __restrict__ v,
+                                                       const T* __restrict__ w, T* __restrict__ uo,
+                                                       T* __restrict__ vo, T* __restrict__ wo, T dt) {
+    const int64_t i = int64_t(blockIdx.x) * kUBlock + threadIdx.x;
+    if (i >= g.N) return;
+    int32_t nb[6], f[6];
+    int8_t sg[6];
+    uload(g, i, nb, f, sg);
+    const T pc = p[i];
+    T fv[6];
+#pragma unroll
+    for (int fc = 0; fc < 6; ++fc) fv[fc] = (pc + (nb[fc] < 0 ? pc : p[nb[fc]])) * T(0.5);
+    const T* const vin[3] = {u, v, w};
+    T* const vout[3] = {uo, vo, wo};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const T gd = T(0) + ugauss(g, fv, f, sg, d, i, false);
+        vout[d][i] = dt * (-gd) + vin[d][i];
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kUBlock) uwave_pres_k(UnstructGeo<T> g, const T* __restrict__ p,
+                                                        const T* __restrict__ u, const T* __restrict__ v,
+                                                        const T* __restrict__ w, T* __restrict__ po, T K,
+                                                        const T* __restrict__ Kf, T dt) {
+    const int64_t i = int64_t(blockIdx.x) * kUBlock + threadIdx.x;
+    if (i >= g.N) return;
+    int32_t nb[6], f[6];
+    int8_t sg[6];
+    uload(g, i, nb, f, sg);
+    const T* const vel[3] = {u, v, w};
+    T div[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const T* q = vel[d];
+        const T qc = q[i];
+        T fv[6];
+#pragma unroll
+        for (int fc = 0; fc < 6; ++fc) {
+            const bool bnd = nb[fc] < 0;
+            fv[fc] = (bnd && (fc >> 1) == 2 - d) ? T(0) : (qc + (bnd ? qc : q[nb[fc]])) * T(0.5);
+        }
+        div[d] = T(0) + ugauss(g, fv, f, sg, d, i, false);
+    }
+    const T S = (div[0] + div[1]) + div[2];
+    const T k = Kf ? Kf[i] : K;
+    po[i] = dt * (k * S) + p[i];
+}
+
+template <class T>
+void launch_uwave(const cfd_mesh_s* m, void* const a[4], void* const b[4], double K, const void* Kf, double dt,
+                  int64_t nsteps, hipStream_t s) {
+    for (int64_t t = 0; t < nsteps; ++t) {
+        void* const* in = (t & 1) ? b : a;
+        void* const* out = (t & 1) ? a : b;
+        uwave_vel_k<T><<<ugrid(m->N), kUBlock, 0, s>>>(ugeo_u<T>(m), (const T*)in[0], (const T*)in[1],
+                                                       (const T*)in[2], (const T*)in[3], (T*)out[1], (T*)out[2],
+                                                       (T*)out[3], T(dt));
+        CFD_HIP(hipGetLastError());
+        uwave_pres_k<T><<<ugrid(m->N), kUBlock, 0, s>>>(ugeo_u<T>(m), (const T*)in[0], (const T*)out[1],
+                                                        (const T*)out[2], (const T*)out[3], (T*)out[0], T(K),
+                                                        (const T*)Kf, T(dt));
+        CFD_HIP(hipGetLastError());
+    }
+}
+
+template <class T>
+void launch_ustab(const cfd_mesh_s* m, const void* phi, void* out, int mask, void* work, hipStream_t s) {
+    T* w = static_cast<T*>(work);
+    const int64_t N = m->N;
+    ustab_grad_k<T><<<ugrid(N), kUBlock, 0, s>>>(ugeo_u<T>(m), (const T*)phi, w, w + N, w + 2 * N);
+    CFD_HIP(hipGetLastError());
+    ustab_k<T><<<ugrid(N), kUBlock, 0, s>>>(ugeo_u<T>(m), (const T*)phi, w, w + N, w + 2 * N, (T*)out, mask);
+    CFD_HIP(hipGetLastError());
+}
+
 template <class T>
 void launch_uheat(const cfd_mesh_s* m, void* Ta, void* Tb, double alpha, const void* af, double dt, int64_t nsteps,
                   hipStream_t s) {
@@ -151,7 +296,7 @@ void launch_uheat(const cfd_mesh_s* m, void* Ta, void* Tb, double alpha, const v
 
 // Geometry, numbering and upload of an unstructured mesh.  Device block:
 // nbr (6N int32), face (6N int32), then per face fn (3F), area (F), alpha (F),
-// then vol (N), then osign (6N int8).
+// then vol (N), len (6N), then osign (6N int8).
 template <class T> void create_unstructured(cfd_mesh_s* m, int reorder) {
     const UConn& c = m->uconn;
     check_uconn(c);
@@ -171,7 +316,7 @@ template <class T> void create_unstructured(cfd_mesh_s* m, int reorder) {
     for (int64_t n = 0; n < F; ++n) face_iperm[m->face_perm[n]] = n;
 
     const size_t ib = 6 * size_t(N) * sizeof(int32_t);
-    const size_t tb = (5 * size_t(F) + size_t(N)) * sizeof(T);
+    const size_t tb = (5 * size_t(F) + 7 * size_t(N)) * sizeof(T);
     const size_t sb = 6 * size_t(N);
     const int64_t bytes = int64_t(2 * ib + tb + sb);
     std::vector<char> host(static_cast<size_t>(bytes));
@@ -199,6 +344,9 @@ template <class T> void create_unstructured(cfd_mesh_s* m, int reorder) {
         tv[4 * F + n] = g.falpha[f];
     }
     for (int64_t n = 0; n < N; ++n) tv[5 * F + n] = g.vol[m->cell_perm[n]];
+    T* lv = tv + 5 * F + N;
+    for (int64_t n = 0; n < N; ++n)
+        for (int fc = 0; fc < 6; ++fc) lv[fc * N + n] = g.len[6 * m->cell_perm[n] + fc];
 
     hipStream_t s = nullptr;
     m->dev_block = pool_alloc(m->ctx, bytes, s);
@@ -216,6 +364,7 @@ template <class T> void create_unstructured(cfd_mesh_s* m, int reorder) {
     u.area = dt + 3 * F;
     u.alpha = dt + 4 * F;
     u.vol = dt + 5 * F;
+    u.len = dt + 5 * F + N;
     u.osign = reinterpret_cast<const int8_t*>(d + 2 * ib + tb);
 }
 
@@ -273,6 +422,9 @@ void unstructured_table(const UConn& c, const int64_t* perm, const int64_t* iper
 #define CFD_U_INSTANTIATE(T)                                                                                   \
     template void launch_ugrad<T>(const cfd_mesh_s*, const void*, void*, int, int, hipStream_t);              \
     template void launch_ulapl<T>(const cfd_mesh_s*, const void*, void*, int, hipStream_t);                   \
+    template void launch_ustab<T>(const cfd_mesh_s*, const void*, void*, int, void*, hipStream_t);            \
+    template void launch_uwave<T>(const cfd_mesh_s*, void* const[4], void* const[4], double, const void*,      \
+                                  double, int64_t, hipStream_t);                                               \
     template void launch_uheat<T>(const cfd_mesh_s*, void*, void*, double, const void*, double, int64_t,       \
                                   hipStream_t);                                                                \
     template void create_unstructured<T>(cfd_mesh_s*, int);                                                    \

@@ -1,5 +1,6 @@
 """Unstructured hex meshes on the device (cfd_mesh_create_unstructured): the
-gradient, Laplacian and heat step through the connectivity form, with the
+gradient, Laplacian, stabilisation, heat step and acoustic step through the
+connectivity form, with the
 cells renumbered by reverse Cuthill-McKee, equal the oracle built from the
 same connectivity bit for bit after mapping back to the caller's numbering;
 on a perturbed lattice, in a shuffled numbering, and at 64^3."""
@@ -47,6 +48,7 @@ def test_unstructured_operators_equal_oracle(hip_ctx, oracle_lib, dtype, kind, r
         for wall in (-1, 0, 2):
             assert np.array_equal(back(um.grad(dphi, mask, wall)), om.grad(phi, mask, wall)), (mask, wall)
         assert np.array_equal(back(um.lapl(dphi, mask)), om.lapl(phi, mask)), mask
+        assert np.array_equal(back(um.stab(dphi, mask)), om.stab(phi, mask)), ("stab", mask)
     # the mesh's tables, in its numbering
     for t in ("vol", "alpha", "nax"):
         got = um.table(t)
@@ -69,9 +71,42 @@ def test_unstructured_entry_points_refuse_other_operators(hip_ctx):
     conn = meshio.lattice_connectivity(4, 4, 4, 0.25, 0.25, 0.25)
     um = capi.UnstructuredMesh(hip_ctx, conn, dtype="f32")
     phi = torch.zeros(um.N, dtype=torch.float32, device="cuda")
+    out = torch.empty_like(phi)
     with pytest.raises(capi.CfdError) as e:
-        um.stab(phi)
+        um.stab_noalloc(phi, out)
+    assert e.value.code == -4
+    state = [torch.zeros(um.N, dtype=torch.float32, device="cuda") for _ in range(5)]
+    with pytest.raises(capi.CfdError) as e:
+        um.euler_run(state, [torch.empty_like(t) for t in state], 1.4, 0.3, 2)
     assert e.value.code == -4
     with pytest.raises(capi.CfdError):
         um.set_wave_shape(7, 4, 8, 0)
+    um.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("kind,reorder", [("lattice", True), ("perturbed", True), ("shuffled", True),
+                                          ("shuffled", False)])
+def test_unstructured_wave_equals_oracle(hip_ctx, oracle_lib, dtype, kind, reorder):
+    """The fused acoustic step on an unstructured mesh (velocities, then the
+    pressure from the new velocities, reflective walls): equal to the
+    oracle's operator stepper on the same connectivity, odd and even step
+    counts, constant K and a K field."""
+    from cfdarcov2_amd import capi
+    conn = _mesh_case(kind, dtype)
+    print(f"\n[case] unstructured wave {kind} reorder={reorder} {np.dtype(dtype).name}", flush=True)
+    um = capi.UnstructuredMesh(hip_ctx, conn, dtype=dtype, reorder=reorder)
+    om = oracle_lib.OracleMesh.from_connectivity(conn, dtype=dtype)
+    rng = np.random.default_rng(8)
+    f = [rng.standard_normal(um.N).astype(dtype) for _ in range(4)]
+    kf = (-1.0 - rng.random(um.N)).astype(dtype)
+    dt = 0.2 / 8
+    for steps, K_field in ((3, None), (4, kf)):
+        A = [torch.from_numpy(um.to_mesh_order(x)).cuda() for x in f]
+        B = [torch.empty_like(t) for t in A]
+        kd = None if K_field is None else torch.from_numpy(um.to_mesh_order(K_field)).cuda()
+        got = um.wave_run(A, B, -1.3, dt, steps, K_field=kd)
+        ref = om.wave_run(*[x.copy() for x in f], -1.3, dt, steps, K_field=K_field)
+        for name, g_, r in zip("puvw", got, ref):
+            assert np.array_equal(um.to_caller_order(g_.cpu().numpy()), r), (steps, name)
     um.close()
