@@ -430,7 +430,8 @@ def suite_unstructured(ctx, n=128, steps=200):
     mesh kept in its random numbering is timed beside it.  Algorithmic bytes
     per step: per cell the state read and written (16 B), six neighbour and
     six face ids (48 B) and the volume (8 B); per face its normal, area and
-    alpha_d (40 B), each face counted once."""
+    alpha_d (40 B), each face counted once.  The fused acoustic step on the
+    renumbered mesh is timed after it (`wave_rcm`)."""
     import numpy as np
     import torch
 
@@ -462,6 +463,26 @@ def suite_unstructured(ctx, n=128, steps=200):
         if reorder:
             rec["algorithmic_bytes_per_step"] = nbytes
             rec["kernel"] = "uheat_k"
+            # the acoustic step on the same mesh (K = -1, dt = 0.4 dx), two
+            # gathers per step: the velocities, then the pressure from them.
+            # Bytes: the state's 4 fields read and written (64 B per cell),
+            # and per pass the cell's ids, signs and volume (62 B) and each
+            # face's normal and area (32 B)
+            st = [torch.from_numpy(um.to_mesh_order(T0)).cuda()] + [torch.zeros_like(a) for _ in range(3)]
+            sb = [torch.empty_like(x) for x in st]
+            um.wave_run(st, sb, -1.0, 0.4 * d, 4)
+            wsteps = steps // 2 * 2
+            wwall, wev = _timed(lambda: um.wave_run(st, sb, -1.0, 0.4 * d, wsteps))
+            if ctx.count_nonfinite(st[0]):
+                raise RuntimeError("non-finite acoustic state on the unstructured mesh")
+            wbytes = N * (64 + 2 * 62) + F * 2 * 32
+            wus = wev * 1e3 / wsteps
+            rec["wave_rcm"] = {"mcups": N * wsteps / wwall / 1e6, "kernel_us_per_step": wus,
+                               "algorithmic_bytes_per_step": wbytes,
+                               "gbs_algorithmic": wbytes / (wus * 1e-6) / 1e9,
+                               "roofline_frac": wbytes / (wus * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                               "kernels": "uwave_vel_k + uwave_pres_k"}
+            del st, sb
         um.close()
         del a, b
     rec["rcm_speedup"] = rec["random_order"]["kernel_us_per_step"] / rec["rcm"]["kernel_us_per_step"]
