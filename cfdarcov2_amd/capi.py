@@ -435,8 +435,8 @@ class UnstructuredMesh(HexMesh):
     """cfd_mesh_create_unstructured: a hex mesh given by its connectivity (a
     cfdarcov2_amd.meshio dict).  With reorder, the cells are renumbered by
     reverse Cuthill-McKee: fields are in that numbering, `perm[new] = old`
-    maps them to the caller's.  grad, lapl, stab, heat_run, wave_run and table
-    apply."""
+    maps them to the caller's.  grad, lapl, stab, heat_run, wave_run,
+    euler_run and table apply."""
 
     def __init__(self, ctx: Context, conn, dtype="f64", reorder=True):
         self.ctx = ctx

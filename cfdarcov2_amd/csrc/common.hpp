@@ -131,6 +131,11 @@ template <class T> struct UnstructGeo {
     const T* alpha;        // alpha_d (the same from both cells)
     const T* vol;          // per cell
     const T* len;          // per (fc, cell), fc-major: centre-to-face distance
+    // the Rusanov stepper's geometry, built on its first use (null before):
+    // per face the +axis unit vector-area normal and area, per cell the volume
+    const T* sn[3] = {nullptr, nullptr, nullptr};
+    const T* sa = nullptr;
+    const T* svol = nullptr;
 };
 
 }  // namespace cfd

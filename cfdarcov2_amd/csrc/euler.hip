@@ -26,6 +26,7 @@
 #include "common.hpp"
 #include "hexmesh.hpp"
 #include "kernels.hpp"
+#include "umesh.hpp"
 
 namespace cfd {
 
@@ -513,6 +514,182 @@ void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamm
     }
     pool_free(m->ctx, slots, s);
 }
+
+// ---------------------------------------------------------------------------
+// The same stepper on a mesh given by its connectivity (cfd_mesh_create_
+// unstructured): one thread per cell in the mesh's reverse Cuthill-McKee
+// numbering, the neighbours through the (local face, cell) ids, the
+// stepper's geometry per face (build_ufv_geometry).  Each cell evaluates its
+// six face fluxes with the lower cell of the face's axis pair as L, in the
+// per-thread kernel's order, so results equal the oracle's on the same
+// connectivity.
+// ---------------------------------------------------------------------------
+
+__device__ inline int64_t ucell() { return int64_t(blockIdx.x) * 256 + threadIdx.y * 64 + threadIdx.x; }
+
+// V / sum over fc = 0..5 of A_f (|u.n_f| + a), the faces through the ids
+template <class T> __device__ inline T ulocal_dt(const UnstructGeo<T>& g, int64_t i, const Prim<T>& q) {
+    T s = T(0);
+#pragma unroll
+    for (int fc = 0; fc < 6; ++fc) {
+        const int32_t f = g.face[int64_t(fc) * g.N + i];
+        const T un = un_of(q.ux, q.uy, q.uz, g.sn[0][f], g.sn[1][f], g.sn[2][f]);
+        s = fmul_add(g.sa[f], fabs(un) + q.a, s);
+    }
+    return g.svol[i] / s;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) ueuler_dt_k(UnstructGeo<T> g, EIn<T> in, T gamma, Bits<T>* slot) {
+    const int64_t i = ucell();
+    T loc = std::numeric_limits<T>::infinity();
+    if (i < g.N) {
+        T U[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) U[k] = in.u[k][i];
+        loc = ulocal_dt(g, i, prim(gamma, U));
+    }
+    block_min_into(loc, slot);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) ueuler_step_k(UnstructGeo<T> g, EIn<T> in, EOut<T> out, T gamma, T cfl,
+                                                    const Bits<T>* dt_cur, Bits<T>* dt_next, Bits<T>* dt_reset,
+                                                    T* dt_hist, int64_t step) {
+    const int64_t i = ucell();
+    const T dt = cfl * from_bits(*dt_cur);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+        *dt_reset = inf_bits<T>();
+        if (dt_hist) dt_hist[step] = dt;
+    }
+    T loc = std::numeric_limits<T>::infinity();
+    if (i < g.N) {
+        T Ui[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) Ui[k] = in.u[k][i];
+        const Prim<T> qi = prim(gamma, Ui);
+        T net[5] = {T(0), T(0), T(0), T(0), T(0)};
+        // axis pairs in the order x, y, z: net = ((x+ - x-) + (y+ - y-)) + (z+ - z-)
+#pragma unroll
+        for (int pair = 2; pair >= 0; --pair) {
+            T Gp[2][5];
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const int fc = 2 * pair + side;
+                const int64_t k6 = int64_t(fc) * g.N + i;
+                const int32_t nb = g.nbr[k6];
+                T Un[5];
+                if (nb < 0) {  // wall: the mirrored state
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) Un[k] = Ui[k];
+                    Un[3 - pair] = -Ui[3 - pair];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) Un[k] = in.u[k][nb];
+                }
+                const Prim<T> qn = prim(gamma, Un);
+                const int32_t f = g.face[k6];
+                const T nx = g.sn[0][f], ny = g.sn[1][f], nz = g.sn[2][f], A = g.sa[f];
+                if (side == 1) rusanov(Ui, qi, Un, qn, nx, ny, nz, A, Gp[side]);
+                else rusanov(Un, qn, Ui, qi, nx, ny, nz, A, Gp[side]);
+            }
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const T d = Gp[1][k] - Gp[0][k];
+                net[k] = pair == 2 ? d : net[k] + d;
+            }
+        }
+        const T iV = T(1) / g.svol[i];
+        T Un[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            Un[k] = fmul_add(-(dt * net[k]), iV, Ui[k]);
+            out.u[k][i] = Un[k];
+        }
+        loc = ulocal_dt(g, i, prim(gamma, Un));
+    }
+    block_min_into(loc, dt_next);
+}
+
+namespace {
+
+template <class T> UnstructGeo<T>& ugeo_e(cfd_mesh_s* m) {
+    if constexpr (sizeof(T) == 4) return m->uu32; else return m->uu64;
+}
+
+// the stepper's geometry of an unstructured mesh, built and uploaded on first
+// use into one pool block (faces and cells in the mesh's numbering)
+template <class T> const UnstructGeo<T>& ufv_geometry(cfd_mesh_s* m, hipStream_t s) {
+    UnstructGeo<T>& u = ugeo_e<T>(m);
+    if (u.svol) return u;
+    const UConn& c = m->uconn;
+    UGeometry<T> g;
+    build_ugeometry<T>(g, c);
+    std::vector<T> sv, svol;
+    build_ufv_geometry<T>(c, g, sv, svol);
+    const int64_t N = c.N, F = c.F;
+    std::vector<T> host(size_t(4 * F + N));
+    for (int64_t n = 0; n < F; ++n) {
+        const int64_t f = m->face_perm[n];
+        for (int k = 0; k < 4; ++k) host[size_t(k * F + n)] = sv[size_t(k * F + f)];
+    }
+    for (int64_t n = 0; n < N; ++n) host[size_t(4 * F + n)] = svol[size_t(m->cell_perm[n])];
+    const int64_t bytes = int64_t(host.size() * sizeof(T));
+    void* dev = pool_alloc(m->ctx, bytes, s);
+    hipError_t e = hipMemcpyAsync(dev, host.data(), size_t(bytes), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        pool_free(m->ctx, dev, s);
+        CFD_HIP(e);
+    }
+    const T* d = static_cast<const T*>(dev);
+    for (int k = 0; k < 3; ++k) u.sn[k] = d + k * F;
+    u.sa = d + 3 * F;
+    u.svol = d + 4 * F;
+    m->fv_block = dev;
+    m->fv_bytes = bytes;
+    return u;
+}
+
+}  // namespace
+
+template <class T>
+void launch_ueuler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamma, double cfl, int64_t nsteps,
+                   void* dt_hist, hipStream_t s) {
+    const UnstructGeo<T>& g = ufv_geometry<T>(m, s);
+    Bits<T>* slots = (Bits<T>*)pool_alloc(m->ctx, 3 * sizeof(Bits<T>), s);
+    try {
+        const dim3 block(64, 4, 1);
+        const dim3 grid(unsigned((m->N + 255) / 256));
+        const Bits<T> inf = sizeof(T) == 4 ? Bits<T>(0x7f800000u) : Bits<T>(0x7ff0000000000000ull);
+        fill_bits_k<Bits<T>><<<1, 64, 0, s>>>(slots, 3, inf);
+        EIn<T> in{};
+        EOut<T> out{};
+        for (int k = 0; k < 5; ++k) in.u[k] = (const T*)a[k];
+        ueuler_dt_k<T><<<grid, block, 0, s>>>(g, in, T(gamma), slots);
+        CFD_HIP(hipGetLastError());
+        for (int64_t st = 0; st < nsteps; ++st) {
+            void* const* src = (st & 1) ? b : a;
+            void* const* dst = (st & 1) ? a : b;
+            for (int k = 0; k < 5; ++k) {
+                in.u[k] = (const T*)src[k];
+                out.u[k] = (T*)dst[k];
+            }
+            ueuler_step_k<T><<<grid, block, 0, s>>>(g, in, out, T(gamma), T(cfl), slots + (st % 3),
+                                                    slots + ((st + 1) % 3), slots + ((st + 2) % 3), (T*)dt_hist, st);
+            CFD_HIP(hipGetLastError());
+        }
+    } catch (...) {
+        pool_free(m->ctx, slots, s);
+        throw;
+    }
+    pool_free(m->ctx, slots, s);
+}
+
+template void launch_ueuler<float>(cfd_mesh_s*, void* const[5], void* const[5], double, double, int64_t, void*,
+                                   hipStream_t);
+template void launch_ueuler<double>(cfd_mesh_s*, void* const[5], void* const[5], double, double, int64_t, void*,
+                                    hipStream_t);
 
 template void launch_euler<float>(cfd_mesh_s*, void* const[5], void* const[5], double, double, int64_t, void*,
                                   hipStream_t);

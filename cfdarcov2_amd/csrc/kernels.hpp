@@ -92,6 +92,9 @@ void launch_heat_multi(const UniformGeo<T>& g, const void* in, void* out, double
 template <class T>
 void launch_euler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamma, double cfl, int64_t nsteps,
                   void* dt_hist, hipStream_t s);
+template <class T>
+void launch_ueuler(cfd_mesh_s* m, void* const a[5], void* const b[5], double gamma, double cfl, int64_t nsteps,
+                   void* dt_hist, hipStream_t s);
 
 template <class T>
 void launch_ewise(int op, const void* a, const void* b, const void* aux, double s, void* out, int64_t n,

@@ -317,8 +317,8 @@ void launch_uheat(const cfd_mesh_s* m, void* Ta, void* Tb, double alpha, const v
 static void require_lattice(const cfd_mesh_s* m, const char* what) {
     if (m->unstructured)
         throw Error(CFD_E_UNSUPPORTED, std::string(what) +
-                                           ": unstructured meshes take cfd_grad, cfd_lapl, cfd_stab, cfd_heat_run and "
-                                           "cfd_wave_run only");
+                                           ": unstructured meshes take cfd_grad, cfd_lapl, cfd_stab, cfd_heat_run, "
+                                           "cfd_wave_run and cfd_euler_run only");
 }
 
 static UConn uconn_of(const cfd_umesh_desc* d) {
@@ -902,7 +902,6 @@ int cfd_euler_run(cfd_mesh_t m, void* const a[5], void* const b[5], double gamma
                   void* dt_hist, void* stream) {
     return guarded([&] {
         CFD_REQUIRE(m && a && b, "cfd_euler_run: null argument");
-        require_lattice(m, "cfd_euler_run");
         for (int i = 0; i < 5; ++i) {
             CFD_REQUIRE(a[i] && b[i], "cfd_euler_run: null state pointer");
             for (int j = 0; j < 5; ++j) CFD_REQUIRE(a[i] != b[j], "cfd_euler_run: state_a and state_b overlap");
@@ -910,7 +909,8 @@ int cfd_euler_run(cfd_mesh_t m, void* const a[5], void* const b[5], double gamma
         CFD_REQUIRE(gamma > 1.0 && std::isfinite(gamma), "cfd_euler_run: gamma must be finite and > 1");
         CFD_REQUIRE(cfl > 0.0 && std::isfinite(cfl), "cfd_euler_run: cfl must be finite and positive");
         CFD_REQUIRE(nsteps >= 0, "cfd_euler_run: negative step count");
-        DISPATCH(m, launch_euler, m, a, b, gamma, cfl, nsteps, dt_hist, as_stream(stream));
+        if (m->unstructured) DISPATCH(m, launch_ueuler, m, a, b, gamma, cfl, nsteps, dt_hist, as_stream(stream));
+        else DISPATCH(m, launch_euler, m, a, b, gamma, cfl, nsteps, dt_hist, as_stream(stream));
     });
 }
 

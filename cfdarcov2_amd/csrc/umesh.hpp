@@ -145,6 +145,54 @@ template <class T> void build_ugeometry(UGeometry<T>& g, const UConn& c) {
     }
 }
 
+// Geometry of the Rusanov stepper on a mesh given by its connectivity, in the
+// caller's numbering (the structured builder's formulas, build_fv_geometry):
+// per face the vector area S = 0.5 (p2 - p0) x (p3 - p1) oriented along +axis,
+// the axis being that of the face's local pair in its first cell (fc >> 1 =
+// 0 z, 1 y, 2 x), stored as unit normal and area (sv: nx F, ny F, nz F, A F);
+// per cell V = (1/3) sum over fc of (x_f - x_c) . S_out with S_out = -S on
+// the lower face of a pair and +S on the upper.
+template <class T>
+void build_ufv_geometry(const UConn& c, const UGeometry<T>& g, std::vector<T>& sv, std::vector<T>& svol) {
+    const int64_t N = c.N, F = c.F;
+    std::vector<int8_t> axis(F, 0);  // 0 x, 1 y, 2 z
+    for (int64_t i = 0; i < N; ++i)
+        for (int fc = 0; fc < 6; ++fc) {
+            const int64_t f = c.cell_faces[6 * i + fc];
+            if (c.face_cells[2 * f] == i) axis[f] = int8_t(2 - (fc >> 1));
+        }
+    auto vert = [&](int64_t v) { return P3<T>{T(c.verts[3 * v]), T(c.verts[3 * v + 1]), T(c.verts[3 * v + 2])}; };
+    sv.assign(4 * size_t(F), T(0));
+    svol.assign(size_t(N), T(0));
+#pragma omp parallel for schedule(static)
+    for (int64_t f = 0; f < F; ++f) {
+        const P3<T> p0 = vert(c.face_verts[4 * f]), p1 = vert(c.face_verts[4 * f + 1]);
+        const P3<T> p2 = vert(c.face_verts[4 * f + 2]), p3 = vert(c.face_verts[4 * f + 3]);
+        const P3<T> cr = cross3(sub3(p2, p0), sub3(p3, p1));
+        P3<T> S{T(0.5) * cr.x, T(0.5) * cr.y, T(0.5) * cr.z};
+        const T along = axis[f] == 2 ? S.z : (axis[f] == 1 ? S.y : S.x);
+        if (along < T(0)) S = {-S.x, -S.y, -S.z};
+        const T A = std::sqrt(dotf(S, S));
+        sv[f] = S.x / A;
+        sv[F + f] = S.y / A;
+        sv[2 * F + f] = S.z / A;
+        sv[3 * F + f] = A;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        const P3<T> xc{g.ccx[i], g.ccy[i], g.ccz[i]};
+        T acc = T(0);
+        for (int fc = 0; fc < 6; ++fc) {
+            const int64_t f = c.cell_faces[6 * i + fc];
+            const T sgn = (fc & 1) ? T(1) : T(-1);
+            const T A = sv[3 * F + f];
+            const P3<T> So{sgn * (sv[f] * A), sgn * (sv[F + f] * A), sgn * (sv[2 * F + f] * A)};
+            acc = acc + dotf(sub3(P3<T>{g.fcx[f], g.fcy[f], g.fcz[f]}, xc), So);
+        }
+        svol[i] = acc / T(3);
+    }
+}
+
 // Reverse Cuthill-McKee order of the cells (perm[new] = old), every connected
 // component in turn, each started from an unvisited cell of least degree.
 inline std::vector<int64_t> rcm_order(const UConn& c) {

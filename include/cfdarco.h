@@ -110,9 +110,10 @@ int cfd_mesh_get_info(cfd_mesh_t mesh, cfd_mesh_info* info);
  * renumbers the cells by reverse Cuthill-McKee and the faces by their lowest
  * new cell; fields passed to the operators are in that numbering
  * (cfd_mesh_permutation gives perm[new] = old; reorder = 0 keeps the
- * caller's).  cfd_grad, cfd_lapl, cfd_stab, cfd_heat_run, cfd_wave_run and
- * cfd_mesh_table take such a mesh; the other entry points return
- * CFD_E_UNSUPPORTED for it.  Results equal
+ * caller's).  cfd_grad, cfd_lapl, cfd_stab, cfd_heat_run, cfd_wave_run,
+ * cfd_euler_run (one thread per cell; the face axis of the Rusanov geometry is
+ * that of the face's local pair in its first cell) and cfd_mesh_table take
+ * such a mesh; the other entry points return CFD_E_UNSUPPORTED for it.  Results equal
  * the reference semantics bit for bit in any numbering. */
 typedef struct cfd_umesh_desc {
     int64_t n_cells, n_faces, n_vertices;

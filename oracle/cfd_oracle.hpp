@@ -615,10 +615,18 @@ void wave_run_fused(const HexMesh<T>& m, T* p, T* u, T* v, T* w, T K, const T* K
 // (exact for a bilinear face; the six vector areas of a cell sum to zero, so a
 // uniform state is preserved) and the cell volume
 //   V = (1/3) sum over fc = 0..5 of (x_f - x_c) . S_out,f
-// with x_f the face centre and x_c the cell centre (mean of the vertices).
+// with x_f the face centre and x_c the cell centre (mean of the vertices).  A
+// face's axis is that of its local face pair in its first cell (fc >> 1 = 0 z,
+// 1 y, 2 x), which on a lattice is its face family, so the same definition
+// holds for a mesh given by its connectivity.
 template <class T> void compute_fv_geometry(HexMesh<T>& m) {
     const int64_t F = m.F, N = m.N;
-    const int64_t nzf = m.X * m.Y * (m.Z + 1), nyf = m.X * (m.Y + 1) * m.Z;
+    std::vector<int8_t> axis(F, 0);  // 0 x, 1 y, 2 z
+    for (int64_t i = 0; i < N; ++i)
+        for (int fc = 0; fc < 6; ++fc) {
+            const int64_t f = m.cell_faces[6 * i + fc];
+            if (m.face_cells[2 * f] == i) axis[f] = int8_t(2 - (fc >> 1));
+        }
     m.svx.resize(F); m.svy.resize(F); m.svz.resize(F); m.sva.resize(F); m.svol.resize(N);
 #pragma omp parallel for schedule(static)
     for (int64_t f = 0; f < F; ++f) {
@@ -629,7 +637,7 @@ template <class T> void compute_fv_geometry(HexMesh<T>& m) {
         }
         const V3<T> c = vcross(vsub(p[2], p[0]), vsub(p[3], p[1]));
         V3<T> S{T(0.5) * c.x, T(0.5) * c.y, T(0.5) * c.z};
-        const T along = f < nzf ? S.z : (f < nzf + nyf ? S.y : S.x);
+        const T along = axis[f] == 2 ? S.z : (axis[f] == 1 ? S.y : S.x);
         if (along < T(0)) S = {-S.x, -S.y, -S.z};
         const T A = std::sqrt(dot_fixed(S, S));
         m.svx[f] = S.x / A; m.svy[f] = S.y / A; m.svz[f] = S.z / A; m.sva[f] = A;

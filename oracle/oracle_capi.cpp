@@ -98,9 +98,11 @@ void* oracle_mesh_create_conn(int64_t N, int64_t F, int64_t V, const double* ver
     if (dtype == 0) {
         cfdo::build_from_connectivity<float>(om->f, N, F, V, verts, face_verts, face_cells, cell_faces, cell_verts);
         cfdo::compute_geometry<float>(om->f);
+        cfdo::compute_fv_geometry<float>(om->f);
     } else {
         cfdo::build_from_connectivity<double>(om->d, N, F, V, verts, face_verts, face_cells, cell_faces, cell_verts);
         cfdo::compute_geometry<double>(om->d);
+        cfdo::compute_fv_geometry<double>(om->d);
     }
     return om;
 }

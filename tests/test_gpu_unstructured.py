@@ -1,6 +1,6 @@
 """Unstructured hex meshes on the device (cfd_mesh_create_unstructured): the
-gradient, Laplacian, stabilisation, heat step and acoustic step through the
-connectivity form, with the
+gradient, Laplacian, stabilisation, heat step, acoustic step and Rusanov step
+through the connectivity form, with the
 cells renumbered by reverse Cuthill-McKee, equal the oracle built from the
 same connectivity bit for bit after mapping back to the caller's numbering;
 on a perturbed lattice, in a shuffled numbering, and at 64^3."""
@@ -75,9 +75,8 @@ def test_unstructured_entry_points_refuse_other_operators(hip_ctx):
     with pytest.raises(capi.CfdError) as e:
         um.stab_noalloc(phi, out)
     assert e.value.code == -4
-    state = [torch.zeros(um.N, dtype=torch.float32, device="cuda") for _ in range(5)]
     with pytest.raises(capi.CfdError) as e:
-        um.euler_run(state, [torch.empty_like(t) for t in state], 1.4, 0.3, 2)
+        um.set_euler_shape(1, 4, 16)
     assert e.value.code == -4
     with pytest.raises(capi.CfdError):
         um.set_wave_shape(7, 4, 8, 0)
@@ -109,4 +108,32 @@ def test_unstructured_wave_equals_oracle(hip_ctx, oracle_lib, dtype, kind, reord
         ref = om.wave_run(*[x.copy() for x in f], -1.3, dt, steps, K_field=K_field)
         for name, g_, r in zip("puvw", got, ref):
             assert np.array_equal(um.to_caller_order(g_.cpu().numpy()), r), (steps, name)
+    um.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("kind,reorder", [("lattice", True), ("perturbed", True), ("perturbed", False),
+                                          ("shuffled", True)])
+def test_unstructured_euler_equals_oracle(hip_ctx, oracle_lib, dtype, kind, reorder):
+    """The Rusanov stepper on an unstructured mesh (face axis from the first
+    cell's local face pair, walls mirrored, the time step from the device
+    minimum): fields and every step's dt equal the oracle's on the same
+    connectivity."""
+    from cfdarcov2_amd import capi
+    conn = _mesh_case(kind, dtype)
+    print(f"\n[case] unstructured euler {kind} reorder={reorder} {np.dtype(dtype).name}", flush=True)
+    um = capi.UnstructuredMesh(hip_ctx, conn, dtype=dtype, reorder=reorder)
+    om = oracle_lib.OracleMesh.from_connectivity(conn, dtype=dtype)
+    U0 = workloads.sod_state(om.table("ccx"), x0=0.4, dtype=dtype)
+    rng = np.random.default_rng(12)
+    U0[1:4] = (0.05 * rng.standard_normal((3, om.N)) * U0[0]).astype(dtype)
+    steps = 7
+    ref, ref_dt = om.euler_run(U0, 1.4, 0.3, steps)
+    A = [torch.from_numpy(um.to_mesh_order(U0[k])).cuda() for k in range(5)]
+    B = [torch.empty_like(t) for t in A]
+    hist = torch.empty(steps, dtype=A[0].dtype, device="cuda")
+    got = um.euler_run(A, B, 1.4, 0.3, steps, dt_hist=hist)
+    assert np.array_equal(hist.cpu().numpy(), ref_dt)
+    for k in range(5):
+        assert np.array_equal(um.to_caller_order(got[k].cpu().numpy()), ref[k]), k
     um.close()
