@@ -430,12 +430,13 @@ def suite_unstructured(ctx, n=128, steps=200):
     mesh kept in its random numbering is timed beside it.  Algorithmic bytes
     per step: per cell the state read and written (16 B), six neighbour and
     six face ids (48 B) and the volume (8 B); per face its normal, area and
-    alpha_d (40 B), each face counted once.  The fused acoustic step on the
-    renumbered mesh is timed after it (`wave_rcm`)."""
+    alpha_d (40 B), each face counted once.  The fused acoustic step and the
+    Rusanov step on the renumbered mesh are timed after it (`wave_rcm`,
+    `euler_rcm`)."""
     import numpy as np
     import torch
 
-    from cfdarcov2_amd import capi, meshio
+    from cfdarcov2_amd import capi, meshio, workloads
     d = 1.0 / n
     conn, _ = meshio.shuffled(meshio.lattice_connectivity(n, n, n, d, d, d, dtype=np.float64), 7)
     rng = np.random.default_rng(9)
@@ -483,6 +484,27 @@ def suite_unstructured(ctx, n=128, steps=200):
                                "roofline_frac": wbytes / (wus * 1e-6) / 1e9 / HBM_PEAK_GBS,
                                "kernels": "uwave_vel_k + uwave_pres_k"}
             del st, sb
+            # the Rusanov step on the same mesh: a Sod tube along x, CFL 0.3.
+            # Bytes: the five state fields read and written (80 B per cell),
+            # the cell's ids and signs (54 B) and each face's normal and area
+            # (32 B)
+            ccx = um.to_mesh_order(np.asarray(conn["vertices"])[np.asarray(conn["cell_verts"])][:, :, 0].mean(axis=1))
+            U0 = workloads.sod_state(ccx, dtype=np.float64)
+            ea = [torch.from_numpy(np.ascontiguousarray(U0[k])).cuda() for k in range(5)]
+            eb = [torch.empty_like(x) for x in ea]
+            um.euler_run(ea, eb, 1.4, 0.3, 2)
+            esteps = 100
+            ewall, eev = _timed(lambda: um.euler_run(ea, eb, 1.4, 0.3, esteps))
+            if any(ctx.count_nonfinite(x) for x in ea):
+                raise RuntimeError("non-finite Rusanov state on the unstructured mesh")
+            ebytes = N * (80 + 54) + F * 32
+            eus = eev * 1e3 / esteps
+            rec["euler_rcm"] = {"mcups": N * esteps / ewall / 1e6, "kernel_us_per_step": eus,
+                                "algorithmic_bytes_per_step": ebytes,
+                                "gbs_algorithmic": ebytes / (eus * 1e-6) / 1e9,
+                                "roofline_frac": ebytes / (eus * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                                "kernel": "ueuler_step_k"}
+            del ea, eb
         um.close()
         del a, b
     rec["rcm_speedup"] = rec["random_order"]["kernel_us_per_step"] / rec["rcm"]["kernel_us_per_step"]
