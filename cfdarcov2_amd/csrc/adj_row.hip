@@ -14,6 +14,8 @@
 // (adjoint.hip) operation for operation, so the two forms agree bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "common.hpp"
 #include "kernels.hpp"
 #include "march.hpp"
@@ -47,7 +49,7 @@ wave_row_aux_k(UniformGeo<T> g, const T* __restrict__ p, const T* __restrict__ u
     const bool inner = x0 >= 1 && x1 + 2 <= X;
 #define CFD_AUX(XB, MR)                                                                                         \
     row::row_tile<T, TY, PW, KF, XB, false, MR, MODE>(g, p, u, v, w, po, uo, vo, wo, K, Kf, dt, x0, x1, y0, wv, \
-                                                      false, sp, sv, sens, &aux)
+                                                      false, sp, sv, sens, &aux, lin)
     if (row::rows_per_wave(g.Z) == 1) {
         if (inner) CFD_AUX(false, false);
         else CFD_AUX(true, false);
@@ -104,6 +106,13 @@ bool aligned4(const void* p, size_t elem) { return (reinterpret_cast<uintptr_t>(
 
 }  // namespace
 
+// Output rows of the single-step row tile for `m`'s adjoint shape: form 2's
+// tile_rows name its two-step tiles, and its single steps take the 8-row tile
+// from 8 rows up, else the 4-row one
+template <class T> int adj_row_tile_rows(const cfd_mesh_s* m) {
+    return m->adj_form == 2 ? (m->adj_ty >= 8 ? 8 : 4) : m->adj_ty;
+}
+
 // Whether the row form takes this mesh and these fields (16-byte aligned rows
 // in FP32, 32-byte in FP64).  Adjoint form 2 runs its single steps (K
 // gradients, K fields, remainders) in this form too.
@@ -115,6 +124,18 @@ bool adj_row_usable(const cfd_mesh_s* m, const void* const* fields, int n) {
     return true;
 }
 
+// Per-tile sensor masks for the row-form launches of `m`'s adjoint shape:
+// tile = x chunk * (y tiles) + y tile, as the kernel numbers its tiles.
+template <class T> std::vector<uint64_t> adj_row_tile_sensors(const cfd_mesh_s* m, const SensorList& sens) {
+    const int ty = adj_row_tile_rows<T>(m);
+    const int64_t rows = int64_t(ty) * row::rows_per_wave(m->Z);
+    const int64_t nyt = (m->Y + rows - 1) / rows, nxc = (m->X + m->adj_L - 1) / m->adj_L;
+    std::vector<uint64_t> mask(size_t(nyt * nxc), 0);
+    for (int j = 0; j < sens.n; ++j)
+        mask[size_t((sens.x[j] / m->adj_L) * nyt + sens.y[j] / rows)] |= uint64_t(1) << j;
+    return mask;
+}
+
 template <class T>
 void launch_wave_row_aux(const cfd_mesh_s* m, int mode, const void* const in[4], void* const out[4], double K,
                          const void* Kf, double dt, const SensorList* sens, const row::RowAux<T>& aux, int rev,
@@ -122,9 +143,7 @@ void launch_wave_row_aux(const cfd_mesh_s* m, int mode, const void* const in[4],
     const UniformGeo<T>& g = ugeo_of<T>(m);
     const int L = m->adj_L;
     CFD_REQUIRE(mode == 1 || (mode == 2 && Kf == nullptr), "transposed row step: constant K only");
-    // form 2's tile_rows name its two-step tiles; its single steps take the
-    // 8-row tile from 8 rows up, else the 4-row one
-    const int ty = m->adj_form == 2 ? (m->adj_ty >= 8 ? 8 : 4) : m->adj_ty;
+    const int ty = adj_row_tile_rows<T>(m);
     if constexpr (sizeof(T) == 8) {
         // FP64 takes the 4-row tile only: the 8-row builds spill registers,
         // and these kernels read other lanes' registers (z neighbours, the
@@ -145,6 +164,7 @@ void launch_wave_row_aux(const cfd_mesh_s* m, int mode, const void* const in[4],
 
 #define CFD_AUX_INSTANTIATE(T)                                                                                   \
     template bool adj_row_usable<T>(const cfd_mesh_s*, const void* const*, int);                                \
+    template std::vector<uint64_t> adj_row_tile_sensors<T>(const cfd_mesh_s*, const SensorList&);              \
     template void launch_wave_row_aux<T>(const cfd_mesh_s*, int, const void* const[4], void* const[4], double,  \
                                          const void*, double, const SensorList*, const row::RowAux<T>&, int,     \
                                          hipStream_t);

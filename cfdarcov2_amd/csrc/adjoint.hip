@@ -49,16 +49,32 @@ void launch_wave_row2_aux(const cfd_mesh_s* m, int mode, const void* const in[4]
 
 // A device copy of the sensor list for the row-form kernels, which read it
 // from memory the caches keep (the kernel-argument copy of 1.5 KB costs each
-// workgroup a slow first read)
-static const SensorList* upload_sensors(const cfd_mesh_s* m, const SensorList& sens, hipStream_t s) {
-    void* d = pool_alloc(m->ctx, int64_t(sizeof(SensorList)), s);
-    try {
-        CFD_HIP(hipMemcpyAsync(d, &sens, sizeof(SensorList), hipMemcpyHostToDevice, s));
-    } catch (...) {
-        pool_free(m->ctx, d, s);
-        throw;
+// workgroup a slow first read), followed by the single-step row tiles' per-tile
+// sensor masks.  One pool block; the copy is waited for, so the host staging
+// vector may go.
+template <class T> std::vector<uint64_t> adj_row_tile_sensors(const cfd_mesh_s* m, const SensorList& sens);
+struct DevSensors {
+    const SensorList* list = nullptr;
+    const uint64_t* tiles = nullptr;
+    void* block = nullptr;
+};
+template <class T> static DevSensors upload_sensors(const cfd_mesh_s* m, const SensorList& sens, hipStream_t s) {
+    const std::vector<uint64_t> tiles = adj_row_tile_sensors<T>(m, sens);
+    const size_t lb = (sizeof(SensorList) + 255) / 256 * 256;
+    std::vector<char> host(lb + tiles.size() * sizeof(uint64_t), 0);
+    std::memcpy(host.data(), &sens, sizeof(SensorList));
+    std::memcpy(host.data() + lb, tiles.data(), tiles.size() * sizeof(uint64_t));
+    DevSensors d;
+    d.block = pool_alloc(m->ctx, int64_t(host.size()), s);
+    hipError_t e = hipMemcpyAsync(d.block, host.data(), host.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        pool_free(m->ctx, d.block, s);
+        CFD_HIP(e);
     }
-    return static_cast<const SensorList*>(d);
+    d.list = static_cast<const SensorList*>(d.block);
+    d.tiles = reinterpret_cast<const uint64_t*>(static_cast<char*>(d.block) + lb);
+    return d;
 }
 
 namespace {
@@ -301,7 +317,8 @@ void run_wave_traces(const cfd_mesh_s* m, void* const a[4], void* const b[4], do
     const bool rowf = adj_row_usable<T>(m, fields, 9);
     const bool two = !Kf && adj_row2_usable<T>(m, fields, 8);
     auto trace_row = [&](int64_t t) -> T* { return traces ? (T*)traces + t * sens.n : nullptr; };
-    const SensorList* dsens = (rowf || two) ? upload_sensors(m, sens, s) : nullptr;
+    const DevSensors ds = (rowf || two) ? upload_sensors<T>(m, sens, s) : DevSensors{};
+    const SensorList* dsens = ds.list;
     try {
         for (int64_t t = 0; t < nsteps; ++t) {
             void* const* in = (t & 1) ? b : a;
@@ -324,7 +341,7 @@ void run_wave_traces(const cfd_mesh_s* m, void* const a[4], void* const b[4], do
             }
             T* tr = trace_row(t);
             if (rowf) {
-                const row::RowAux<T> aux{T(0), tr, nullptr, nullptr, nullptr, nullptr};
+                const row::RowAux<T> aux{T(0), tr, nullptr, nullptr, nullptr, nullptr, ds.tiles};
                 launch_wave_row_aux<T>(m, 1, (const void* const*)in, out, K, Kf, dt, dsens, aux, int(t & 1), s);
             } else {
                 launch_wave_trace_step<T>(m, (const void* const*)in, out, K, Kf, dt, sens, tr, s);
@@ -332,10 +349,10 @@ void run_wave_traces(const cfd_mesh_s* m, void* const a[4], void* const b[4], do
         }
         CFD_HIP(hipGetLastError());
     } catch (...) {
-        pool_free(m->ctx, (void*)dsens, s);
+        pool_free(m->ctx, ds.block, s);
         throw;
     }
-    pool_free(m->ctx, (void*)dsens, s);
+    pool_free(m->ctx, ds.block, s);
 }
 
 // Reverse loop in row form (constant K).  With grad_K, each checkpoint
@@ -352,7 +369,8 @@ static void run_wave_adjoint_row(const cfd_mesh_s* m, void* const la[4], void* c
     const bool two = !gK && adj_row2_usable<T>(m, fields, 8);
     void* ws = nullptr;     // S of each step of one segment: every fields
     void* st = nullptr;     // recomputed state: 2 x {p, u, v, w}
-    const SensorList* dsens = upload_sensors(m, sens, s);
+    const DevSensors ds = upload_sensors<T>(m, sens, s);
+    const SensorList* dsens = ds.list;
     try {
         if (gK) {
             ws = pool_alloc(m->ctx, int64_t(every) * int64_t(field), s);
@@ -367,7 +385,7 @@ static void run_wave_adjoint_row(const cfd_mesh_s* m, void* const la[4], void* c
             void* const* in = (done & 1) ? lb : la;
             void* const* out = (done & 1) ? la : lb;
             const T* src = src_of(t);
-            const row::RowAux<T> aux{dK, nullptr, nullptr, src, (const T*)s_in, s_in ? (T*)gK : nullptr};
+            const row::RowAux<T> aux{dK, nullptr, nullptr, src, (const T*)s_in, s_in ? (T*)gK : nullptr, ds.tiles};
             launch_wave_row_aux<T>(m, 2, (const void* const*)in, out, K, nullptr, dt, dsens, aux, int(done & 1), s);
             ++done;
         };
@@ -413,12 +431,12 @@ static void run_wave_adjoint_row(const cfd_mesh_s* m, void* const la[4], void* c
     } catch (...) {
         pool_free(m->ctx, ws, s);
         pool_free(m->ctx, st, s);
-        pool_free(m->ctx, (void*)dsens, s);
+        pool_free(m->ctx, ds.block, s);
         throw;
     }
     pool_free(m->ctx, ws, s);
     pool_free(m->ctx, st, s);
-    pool_free(m->ctx, (void*)dsens, s);
+    pool_free(m->ctx, ds.block, s);
 }
 
 // Reverse loop.  adj_a holds the adjoint of the final state (without the last

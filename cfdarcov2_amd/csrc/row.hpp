@@ -97,6 +97,10 @@ template <class T> struct RowAux {
     const T* src;      // MODE 2/3: added to nu_p at the sensor cells (nullable)
     const T* s_in;     // MODE 3: the forward step's S, read with gK
     T* gK;             // MODE 3: gK += (lam_p dt) S
+    // MODE 1-3: per tile (the kernel's tile index: x chunk * y tiles + y tile)
+    // the sensors inside it (bit j = sensor j), built on the host; a tile
+    // without sensors reads nothing else of the list
+    const uint64_t* tile_sensors;
 };
 
 // One (y tile, x chunk) of the single-step acoustic row form: TY output waves
@@ -123,7 +127,8 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
                                 T* __restrict__ uo, T* __restrict__ vo, T* __restrict__ wo, T K,
                                 const T* __restrict__ Kf, T dt, int64_t x0, int64_t x1, int y0, int wv, bool nt,
                                 typename Row4<T>::type (*sp)[64], typename Row4<T>::type (*sv)[64],
-                                const SensorList* sens = nullptr, const RowAux<T>* aux = nullptr) {
+                                const SensorList* sens = nullptr, const RowAux<T>* aux = nullptr,
+                                int64_t tile = 0) {
     constexpr bool ADJ = MODE >= 2;
     constexpr bool GKM = MODE == 3;
     static_assert(!(ADJ && KF), "the transposed row step takes a constant K");
@@ -183,24 +188,28 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
         if constexpr (ADJ) return q - mggp(mlin(pc, pl), mlin(pc, ph), A, Vd);
         else return mvel(pl, pc, ph, q, A, Vd, dt);
     };
-    // MODE 1/2: lane j loads sensor j's x, y, z (and in MODE 2 its trace
-    // gradient) here (x = -1 past the list); once the first plane's loads are
-    // in flight one ballot turns them into `mine`, the sensors in this output
-    // wave's rows and x chunk (bit j = sensor j, wave-uniform).  The planes
-    // then read a sensor's values from its lane (readlane), so a wave holding
-    // sensors waits on no memory for them.
+    // MODE 1-3: the tile's sensors come from a per-tile mask (one scalar load;
+    // most tiles have none and read nothing more).  In a tile with sensors,
+    // lane j loads sensor j's x, y, z (and in MODE 2/3 its trace gradient)
+    // and one ballot gives `mine`, the sensors in this output wave's rows
+    // (bit j = sensor j, wave-uniform); the planes then read a sensor's values
+    // from its lane (readlane), so a wave holding sensors waits on no memory
+    // for them.
     const int wy0 = y0 + wv * RW;
     uint64_t mine = 0;
     int sxl = -1, syl = -1, szl = 0;
     T srcl = T(0);
-    bool want = false;
     if constexpr (MODE != 0) {
-        want = is_out && (ADJ ? aux->src != nullptr : aux->trace != nullptr);
-        if (want && lane < sens->n) {
-            sxl = sens->x[lane];
-            syl = sens->y[lane];
-            szl = sens->z[lane];
-            if constexpr (ADJ) srcl = aux->src[lane];
+        const bool want = is_out && (ADJ ? aux->src != nullptr : aux->trace != nullptr);
+        const uint64_t tmask = want ? aux->tile_sensors[tile] : 0;
+        if (tmask) {
+            if (lane < sens->n) {
+                sxl = sens->x[lane];
+                syl = sens->y[lane];
+                szl = sens->z[lane];
+                if constexpr (ADJ) srcl = aux->src[lane];
+            }
+            mine = __ballot(((tmask >> lane) & 1) && syl >= wy0 && syl < wy0 + RW);
         }
     }
 
@@ -283,9 +292,6 @@ __device__ inline void row_tile(const UniformGeo<T>& g, const T* __restrict__ p,
     for (int64_t x = x0; x < x1; ++x) {
         const Fetch cur = nx;
         if (x + 1 < x1) fetch(x + 1, nx);
-        if constexpr (MODE != 0) {
-            if (want && x == x0) mine = __ballot(sxl >= x0 && sxl < x1 && syl >= wy0 && syl < wy0 + RW);
-        }
         const V4 Qa = qv(cur.a);  // p(x+2) (output cells) or p(x) (halo cells), as q in MODE 2
 
         // 1. u'(x+1); p into the LDS plane
