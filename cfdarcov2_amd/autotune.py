@@ -176,8 +176,7 @@ def tune_heat(mesh, alpha, dt, steps=100, candidates=None, verbose=False) -> dic
     return dict(best, table=table)
 
 
-ADJOINT_CANDIDATES = ([(0, 4, 8)] + [(1, ty, L) for ty in (4, 8) for L in (2, 4, 8, 16, 32)]
-                      + [(2, ty, L) for ty in (4, 8, 10) for L in (4, 8, 16, 32)])
+ADJOINT_CANDIDATES = [(0, 4, 8)] + [(1, ty, L) for ty in (4, 8) for L in (2, 4, 8, 16, 32)]
 
 
 def tune_adjoint(problem, steps=40, candidates=None, verbose=False, repeats=3) -> dict:

@@ -403,7 +403,7 @@ class HexMesh:
 
     def set_adjoint_shape(self, form=1, tile_rows=4, chunk=8):
         """Kernel form of wave_traces / wave_adjoint: 0 = one thread per cell,
-        1 = row form, 2 = two steps per launch in row form (FP32)."""
+        1 = row form."""
         check(self.lib.cfd_mesh_set_adjoint_shape(self.h, int(form), int(tile_rows), int(chunk)))
 
     def wave_run(self, state_a, state_b, K, dt, nsteps, K_field=None):

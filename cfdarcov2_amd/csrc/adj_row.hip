@@ -106,19 +106,14 @@ bool aligned4(const void* p, size_t elem) { return (reinterpret_cast<uintptr_t>(
 
 }  // namespace
 
-// Output rows of the single-step row tile for `m`'s adjoint shape: form 2's
-// tile_rows name its two-step tiles, and its single steps take the 8-row tile
-// from 8 rows up, else the 4-row one
-template <class T> int adj_row_tile_rows(const cfd_mesh_s* m) {
-    return m->adj_form == 2 ? (m->adj_ty >= 8 ? 8 : 4) : m->adj_ty;
-}
+// Output rows of the row tile for `m`'s adjoint shape
+template <class T> int adj_row_tile_rows(const cfd_mesh_s* m) { return m->adj_ty; }
 
 // Whether the row form takes this mesh and these fields (16-byte aligned rows
-// in FP32, 32-byte in FP64).  Adjoint form 2 runs its single steps (K
-// gradients, K fields, remainders) in this form too.
+// in FP32, 32-byte in FP64).
 template <class T>
 bool adj_row_usable(const cfd_mesh_s* m, const void* const* fields, int n) {
-    if (m->adj_form < 1 || !m->uniform || !wave_row_supported(m->Z)) return false;
+    if (m->adj_form != 1 || !m->uniform || !wave_row_supported(m->Z)) return false;
     for (int i = 0; i < n; ++i)
         if (fields[i] && !aligned4(fields[i], sizeof(T))) return false;
     return true;

@@ -31,7 +31,6 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "row.hpp"
-#include "row2.hpp"
 
 namespace cfd {
 
@@ -41,11 +40,6 @@ template <class T>
 void launch_wave_row_aux(const cfd_mesh_s* m, int mode, const void* const in[4], void* const out[4], double K,
                          const void* Kf, double dt, const SensorList* sens, const row::RowAux<T>& aux, int rev,
                          hipStream_t s);
-// two-step row form (adj_row2.hip)
-template <class T> bool adj_row2_usable(const cfd_mesh_s* m, const void* const* fields, int n);
-template <class T>
-void launch_wave_row2_aux(const cfd_mesh_s* m, int mode, const void* const in[4], void* const out[4], double K,
-                          double dt, const SensorList* sens, const row2::Aux2<T>& aux, int rev, hipStream_t s);
 
 // A device copy of the sensor list for the row-form kernels, which read it
 // from memory the caches keep (the kernel-argument copy of 1.5 KB costs each
@@ -304,10 +298,7 @@ template <class T> void launch_scatter_add(void* lp, const SensorList& sens, con
 
 // nsteps forward steps from state_a, ping-pong with state_b; after step t
 // (1-based) p at the sensors goes to traces[(t-1) * n]; before steps
-// 0, every, 2 every, ... the state {p,u,v,w} is copied to checkpoint k.  The
-// two-step form (constant K) takes runs of four steps as two launches
-// a -> b -> a, so the state ends where single steps leave it; runs with a
-// checkpoint inside and the last nsteps % 4 steps are single steps.
+// 0, every, 2 every, ... the state {p,u,v,w} is copied to checkpoint k.
 template <class T>
 void run_wave_traces(const cfd_mesh_s* m, void* const a[4], void* const b[4], double K, const void* Kf, double dt,
                      int64_t nsteps, const SensorList& sens, void* traces, void* ckpt, int64_t every,
@@ -315,9 +306,8 @@ void run_wave_traces(const cfd_mesh_s* m, void* const a[4], void* const b[4], do
     const size_t field = size_t(m->N) * sizeof(T);
     const void* fields[9] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3], Kf};
     const bool rowf = adj_row_usable<T>(m, fields, 9);
-    const bool two = !Kf && adj_row2_usable<T>(m, fields, 8);
     auto trace_row = [&](int64_t t) -> T* { return traces ? (T*)traces + t * sens.n : nullptr; };
-    const DevSensors ds = (rowf || two) ? upload_sensors<T>(m, sens, s) : DevSensors{};
+    const DevSensors ds = rowf ? upload_sensors<T>(m, sens, s) : DevSensors{};
     const SensorList* dsens = ds.list;
     try {
         for (int64_t t = 0; t < nsteps; ++t) {
@@ -327,17 +317,6 @@ void run_wave_traces(const cfd_mesh_s* m, void* const a[4], void* const b[4], do
                 char* dst = (char*)ckpt + size_t(t / every) * 4 * field;
                 for (int f = 0; f < 4; ++f)
                     CFD_HIP(hipMemcpyAsync(dst + f * field, in[f], field, hipMemcpyDeviceToDevice, s));
-            }
-            if (two && t + 4 <= nsteps &&
-                !(ckpt && ((t + 1) % every == 0 || (t + 2) % every == 0 || (t + 3) % every == 0))) {
-                for (int h = 0; h < 2; ++h) {  // steps t+2h and t+2h+1
-                    const int64_t tt = t + 2 * h;
-                    const row2::Aux2<T> aux{T(0), {trace_row(tt), trace_row(tt + 1)}, {nullptr, nullptr}};
-                    launch_wave_row2_aux<T>(m, 1, (const void* const*)(h ? out : in), h ? in : out, K, dt, dsens, aux,
-                                            int((tt >> 1) & 1), s);
-                }
-                t += 3;
-                continue;
             }
             T* tr = trace_row(t);
             if (rowf) {
@@ -365,8 +344,6 @@ static void run_wave_adjoint_row(const cfd_mesh_s* m, void* const la[4], void* c
                                  int64_t every, void* gK, hipStream_t s) {
     const size_t field = size_t(m->N) * sizeof(T);
     const T dK = T(T(dt) * T(K));
-    const void* fields[8] = {la[0], la[1], la[2], la[3], lb[0], lb[1], lb[2], lb[3]};
-    const bool two = !gK && adj_row2_usable<T>(m, fields, 8);
     void* ws = nullptr;     // S of each step of one segment: every fields
     void* st = nullptr;     // recomputed state: 2 x {p, u, v, w}
     const DevSensors ds = upload_sensors<T>(m, sens, s);
@@ -392,21 +369,6 @@ static void run_wave_adjoint_row(const cfd_mesh_s* m, void* const la[4], void* c
         int64_t t = nsteps;
         while (t >= 1) {
             if (!gK) {
-                if (two && t >= 4) {
-                    // transposed steps t .. t-3 as two launches la -> lb -> la
-                    // (relative to the current parity), as four single steps
-                    void* const* in = (done & 1) ? lb : la;
-                    void* const* out = (done & 1) ? la : lb;
-                    for (int h = 0; h < 2; ++h) {
-                        const int64_t tt = t - 2 * h;
-                        const row2::Aux2<T> aux{dK, {nullptr, nullptr}, {src_of(tt), src_of(tt - 1)}};
-                        launch_wave_row2_aux<T>(m, 2, (const void* const*)(h ? out : in), h ? in : out, K, dt, dsens,
-                                                aux, int(((done >> 1) + h) & 1), s);
-                    }
-                    done += 4;
-                    t -= 4;
-                    continue;
-                }
                 adj_step(t, nullptr);
                 --t;
                 continue;

@@ -190,9 +190,7 @@ struct cfd_mesh_s {
     // adjoint of the acoustic step (cfd_wave_traces / cfd_wave_adjoint):
     // form 0 = one thread per cell, 1 = row form with `adj_ty` output waves
     // and `adj_L` planes per workgroup (where the mesh allows it; constant K
-    // for the transposed step), 2 = two steps per launch in row form (FP32,
-    // constant K, no K gradient; `adj_ty` names the two-step tile, single
-    // steps take the row form)
+    // for the transposed step)
     int adj_form = 1;
     int adj_ty = 4;
     int adj_L = 8;

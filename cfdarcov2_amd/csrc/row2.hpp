@@ -42,29 +42,8 @@ __device__ inline V4T<T> vel4(const V4T<T>& lo, const V4T<T>& c, const V4T<T>& h
     return r;
 }
 
-// the transposed step's velocity update lam - G(q), q = dK * lam_p in the
-// pressure slots (the single-step transposed row tile's operation order)
+// pressure update of a row vector: the single-step kernels' flux sums
 template <class T, bool PW>
-__device__ inline V4T<T> vel4t(const V4T<T>& lo, const V4T<T>& c, const V4T<T>& hi, const V4T<T>& q, T A,
-                               const VolDiv<T, PW>& Vd) {
-    V4T<T> r;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) r[i] = q[i] - mggp(mlin(c[i], lo[i]), mlin(c[i], hi[i]), A, Vd);
-    return r;
-}
-
-// Extra inputs and outputs of a two-step pass that records the sensor traces
-// (MODE 1) or runs two transposed steps (MODE 2); entry 0 belongs to the
-// pass's first step, entry 1 to its second.
-template <class T> struct Aux2 {
-    T dK;              // MODE 2: the pressure slots hold q = dK * lam_p
-    T* trace[2];       // MODE 1: p' at the sensor cells after each step (nullable)
-    const T* src[2];   // MODE 2: added to nu_p at the sensor cells after each step (nullable)
-};
-
-// pressure update of a row vector: the single-step kernels' flux sums (ADJ:
-// the transposed step's dt * S + lam_p)
-template <class T, bool PW, bool ADJ = false>
 __device__ inline V4T<T> pres4(const V4T<T>& P, const V4T<T>& Um, const V4T<T>& U, const V4T<T>& Up, bool xlo,
                                bool xhi, const V4T<T>& vn, const V4T<T>& vyl, const V4T<T>& vyh, bool first, bool last,
                                const V4T<T>& wn, int lane, int nl, T Ax, T Ay, T Az, const VolDiv<T, PW>& Vd, T K,
@@ -83,23 +62,16 @@ __device__ inline V4T<T> pres4(const V4T<T>& P, const V4T<T>& Um, const V4T<T>& 
         T S = mggp(fx_lo, fx_hi, Ax, Vd);
         S = S + mggp(fy_lo, fy_hi, Ay, Vd);
         S = S + mggp(fz_lo, fz_hi, Az, Vd);
-        if constexpr (ADJ) pn[i] = dt * S + P[i];
-        else pn[i] = dt * (K * S) + P[i];
+        pn[i] = dt * (K * S) + P[i];
     }
     return pn;
 }
 
-// One (y tile, x chunk) of the two-step row form with one z row of 256 cells
-// per wave (wave_row2.hip describes the wave roles and the pipeline).  MODE 1
-// and MODE 2 are tile2m's: the traces recorded after each step, or two
-// transposed steps with the trace gradients added after each.
-template <class T, int TY, bool PW, bool XB, bool NTL, int MODE = 0>
+template <class T, int TY, bool PW, bool XB, bool NTL>
 __device__ inline void tile2(const UniformGeo<T>& g, const T* __restrict__ p, const T* __restrict__ u,
                              const T* __restrict__ v, const T* __restrict__ w, T* __restrict__ po, T* __restrict__ uo,
                              T* __restrict__ vo, T* __restrict__ wo, T K, T dt, int64_t x0, int64_t x1, int y0, int j,
-                             bool nt, V4T<T> (*P1)[64], V4T<T> (*V1)[64], V4T<T> (*P2)[64], V4T<T> (*V2)[64],
-                             const SensorList* sens = nullptr, const Aux2<T>* aux = nullptr) {
-    constexpr bool ADJ = MODE == 2;
+                             bool nt, V4T<T> (*P1)[64], V4T<T> (*V1)[64], V4T<T> (*P2)[64], V4T<T> (*V2)[64]) {
     const int64_t X = g.X;
     const int Y = int(g.Y), Z = int(g.Z);
     const int64_t SX = int64_t(Y) * Z;
@@ -149,77 +121,16 @@ __device__ inline void tile2(const UniformGeo<T>& g, const T* __restrict__ p, co
         f.c = ld(v, pb + rb);
         if (rA) f.d = ld(w, pb + rb);
     };
-    // MODE 2: a pressure value as the stencil's q = dK * lam_p
-    T dK = T(0);
-    if constexpr (ADJ) dK = aux->dK;
-    auto qv = [&](const V4T<T>& a) -> V4T<T> {
-        if constexpr (ADJ) {
-            V4T<T> r;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) r[i] = dK * a[i];
-            return r;
-        } else {
-            return a;
-        }
-    };
-    auto vel = [&](const V4T<T>& lo, const V4T<T>& c, const V4T<T>& hi, const V4T<T>& q, T A) -> V4T<T> {
-        if constexpr (ADJ) return vel4t(lo, c, hi, q, A, Vd);
-        else return vel4(lo, c, hi, q, A, Vd, dt);
-    };
 
     const int64_t xs = x0 >= 2 ? x0 - 2 : 0;  // first plane of the first step
     const int64_t xe = x1 + 2;                // end of the iterations
     const int64_t xs1 = xe < X ? xe : X;      // end of the first step's planes
 
-    // MODE 1/2: the sensors of the wave's row, as tile2m's masks (loaded here,
-    // tested once the first plane's loads are in flight, read by readlane)
-    uint64_t m1 = 0, m2 = 0;
-    int sxl = -1, syl = -1, szl = 0;
-    T src0 = T(0), src1 = T(0);
-    bool w0 = false, w1 = false;
-    if constexpr (MODE != 0) {
-        w0 = ADJ ? aux->src[0] != nullptr : aux->trace[0] != nullptr;
-        w1 = ADJ ? aux->src[1] != nullptr : aux->trace[1] != nullptr;
-        if ((w0 || w1) && lane < sens->n) {
-            sxl = sens->x[lane];
-            syl = sens->y[lane];
-            szl = sens->z[lane];
-            if constexpr (ADJ) {
-                if (w0) src0 = aux->src[0][lane];
-                if (w1) src1 = aux->src[1][lane];
-            }
-        }
-    }
-    auto sensor_masks = [&]() {
-        if constexpr (MODE != 0) {
-            const bool in_out = rO && syl == y && sxl >= x0 && sxl < x1;
-            m1 = __ballot(w0 && (ADJ ? (rA && syl == y && sxl >= xs && sxl < xs1) : in_out));
-            m2 = __ballot(w1 && in_out);
-        }
-    };
-    auto at_sensors = [&](uint64_t mask, int64_t xx, int k, V4T<T>& val) {
-        if constexpr (MODE != 0) {
-            while (mask) {  // lane values read with the whole wave active, as the row tile's
-                const int s = __builtin_ctzll(mask);
-                mask &= mask - 1;
-                const int sx = row::lane_value(sxl, s), zz = row::lane_value(szl, s);
-                T sv = T(0);
-                if constexpr (ADJ) sv = row::lane_value(k ? src1 : src0, s);
-                if (sx != xx) continue;
-                if (zz >= 4 * lane && zz < 4 * lane + 4) {
-                    const int i = zz - 4 * lane;
-                    if constexpr (ADJ) val[i] = val[i] + sv;
-                    else aux->trace[k][s] = val[i];
-                }
-            }
-        }
-    };
-
     // ---- warm-up of the first step's x queue (the single-step kernel's) ------
     // every load of the warm-up and of the first plane issued before any is
-    // used, as the single-step row tile's warm-up
+    // used (planes clamped into the mesh, the wall cases chosen by selects),
+    // as the single-step row tile's warm-up
     V4T<T> P0 = zero4, Pp1 = zero4, Um1 = zero4, U0 = zero4;
-    V4T<T> RA0 = zero4, RAp1 = zero4;  // MODE 2: lam_p(x), lam_p(x+1) as loaded (P0, Pp1 hold q)
     V4T<T> a0 = zero4, am1 = zero4, ap1 = zero4, am2 = zero4, um = zero4, uc = zero4;
     if (rA && lv) {
         const int64_t xm1 = xs >= 1 ? xs - 1 : xs, xm2 = xs >= 2 ? xs - 2 : xm1;
@@ -235,26 +146,18 @@ __device__ inline void tile2(const UniformGeo<T>& g, const T* __restrict__ p, co
     fetch(xs, nx);
     if (rA && lv) {
         P0 = a0;
-        V4T<T> Pm1 = xs >= 1 ? am1 : zero4;
+        const V4T<T> Pm1 = xs >= 1 ? am1 : zero4;
         Pp1 = xs + 1 < X ? ap1 : zero4;
-        if constexpr (ADJ) {
-            RA0 = P0;
-            RAp1 = Pp1;
-            P0 = qv(P0);
-            Pp1 = qv(Pp1);
-            Pm1 = qv(Pm1);
-        }
         if (xs >= 1) {
-            const V4T<T> Pm2 = (xs >= 2) ? qv(am2) : Pm1;
-            Um1 = vel(Pm2, Pm1, P0, um, Ax);
+            const V4T<T> Pm2 = (xs >= 2) ? am2 : Pm1;
+            Um1 = vel4(Pm2, Pm1, P0, um, Ax, Vd, dt);
         }
         V4T<T> lo = P0, hi = P0;
         if (xs >= 1) lo = Pm1;
         if (xs + 1 < X) hi = Pp1;
-        U0 = vel(lo, P0, hi, uc, Ax);
+        U0 = vel4(lo, P0, hi, uc, Ax, Vd, dt);
     }
-    // second-step queues: p'(x-2), p'(x-1) (as computed: lam_p' in MODE 2);
-    // v', w' of x-2, x-1; u''(x-3), u''(x-2)
+    // second-step queues: p'(x-2), p'(x-1); v', w' of x-2, x-1; u''(x-3), u''(x-2)
     V4T<T> Q2 = zero4, Q1 = zero4, Va2 = zero4, Va1 = zero4, Wa2 = zero4, Wa1 = zero4, R3 = zero4, R2 = zero4;
 
     for (int64_t x = xs; x < xe; ++x) {
@@ -265,23 +168,20 @@ __device__ inline void tile2(const UniformGeo<T>& g, const T* __restrict__ p, co
         const bool su = m >= x0 - 1 && m >= 0 && m < X;
         const Fetch cur = nx;
         if (x + 1 < xs1) fetch(x + 1, nx);
-        if (x == xs) sensor_masks();
-        const V4T<T> Qa = qv(cur.a);   // p(x+2) (d <= 2) or p(x) (d == 3), as q in MODE 2
-        const V4T<T> qQ2 = qv(Q2);     // p'(x-2), as q in MODE 2
 
         // A. u'(x+1); p(x) and p'(x-2) into LDS
         V4T<T> Up1 = zero4;
         if (rA) {
             if (s1) {
                 if (!XB || x + 1 < X) {
-                    const V4T<T> hi = (!XB || x + 2 < X) ? Qa : Pp1;
-                    Up1 = vel(P0, Pp1, hi, cur.b, Ax);
+                    const V4T<T> hi = (!XB || x + 2 < X) ? cur.a : Pp1;
+                    Up1 = vel4(P0, Pp1, hi, cur.b, Ax, Vd, dt);
                 }
                 P1[j][lane] = P0;
             }
-            if (s2) P2[j][lane] = qQ2;
+            if (s2) P2[j][lane] = Q2;
         } else if (rH && s1) {
-            P1[j][lane] = Qa;
+            P1[j][lane] = cur.a;
         }
         __syncthreads();
 
@@ -293,25 +193,24 @@ __device__ inline void tile2(const UniformGeo<T>& g, const T* __restrict__ p, co
                 const V4T<T> pyh = P1[jh][lane];
                 V4T<T> pzl, pzh;
                 z_nbrs(P0, lane, nl, pzl, pzh);
-                vn = vel(pyl, P0, pyh, cur.c, Ay);
-                wn = vel(pzl, P0, pzh, cur.d, Az);
+                vn = vel4(pyl, P0, pyh, cur.c, Ay, Vd, dt);
+                wn = vel4(pzl, P0, pzh, cur.d, Az, Vd, dt);
                 V1[j][lane] = vn;
             }
             if (rB && s2) {
                 const V4T<T> qyl = P2[jl][lane];
                 const V4T<T> qyh = P2[jh][lane];
                 V4T<T> qzl, qzh;
-                z_nbrs(qQ2, lane, nl, qzl, qzh);
-                v2 = vel(qyl, qQ2, qyh, Va2, Ay);
-                if (rO) w2 = vel(qzl, qQ2, qzh, Wa2, Az);
+                z_nbrs(Q2, lane, nl, qzl, qzh);
+                v2 = vel4(qyl, Q2, qyh, Va2, Ay, Vd, dt);
+                if (rO) w2 = vel4(qzl, Q2, qzh, Wa2, Az, Vd, dt);
                 V2[j][lane] = v2;
             }
         } else if (rH && s1) {
             const V4T<T> in = P1[jin][lane];
-            const V4T<T> qb = qv(cur.b);
-            const V4T<T> pl = lowerH ? qb : in;
-            const V4T<T> ph = lowerH ? in : qb;
-            V1[j][lane] = vel(pl, Qa, ph, cur.c, Ay);
+            const V4T<T> pl = lowerH ? cur.b : in;
+            const V4T<T> ph = lowerH ? in : cur.b;
+            V1[j][lane] = vel4(pl, cur.a, ph, cur.c, Ay, Vd, dt);
         }
         __syncthreads();
 
@@ -321,24 +220,21 @@ __device__ inline void tile2(const UniformGeo<T>& g, const T* __restrict__ p, co
             if (s1) {
                 const V4T<T> vyl = first ? vn : V1[j - 1][lane];
                 const V4T<T> vyh = last ? vn : V1[j + 1][lane];
-                Q0 = pres4<T, PW, ADJ>(ADJ ? RA0 : P0, Um1, U0, Up1, !XB || x > 0, !XB || x < X - 1, vn, vyl, vyh,
-                                       first, last, wn, lane, nl, Ax, Ay, Az, Vd, K, dt);
-                at_sensors(m1, x, 0, Q0);
+                Q0 = pres4(P0, Um1, U0, Up1, !XB || x > 0, !XB || x < X - 1, vn, vyl, vyh, first, last, wn, lane,
+                           nl, Ax, Ay, Az, Vd, K, dt);
             }
             V4T<T> R1 = zero4;
             if (rO) {
                 if (su) {
-                    const V4T<T> qQ1 = qv(Q1);
-                    const V4T<T> lo = (!XB || m >= 1) ? qQ2 : qQ1;
-                    const V4T<T> hi = (!XB || x < X) ? qv(Q0) : qQ1;
-                    R1 = vel(lo, qQ1, hi, Um1, Ax);
+                    const V4T<T> lo = (!XB || m >= 1) ? Q2 : Q1;
+                    const V4T<T> hi = (!XB || x < X) ? Q0 : Q1;
+                    R1 = vel4(lo, Q1, hi, Um1, Ax, Vd, dt);
                 }
                 if (s2) {
                     const V4T<T> vyl = first ? v2 : V2[j - 1][lane];
                     const V4T<T> vyh = last ? v2 : V2[j + 1][lane];
-                    V4T<T> pn = pres4<T, PW, ADJ>(Q2, R3, R2, R1, !XB || xp > 0, !XB || xp < X - 1, v2, vyl, vyh,
-                                                  first, last, w2, lane, nl, Ax, Ay, Az, Vd, K, dt);
-                    at_sensors(m2, xp, 1, pn);
+                    const V4T<T> pn = pres4(Q2, R3, R2, R1, !XB || xp > 0, !XB || xp < X - 1, v2, vyl, vyh, first, last,
+                                        w2, lane, nl, Ax, Ay, Az, Vd, K, dt);
                     if (lv) {
                         const int64_t o = xp * SX + rb;
                         row::put(reinterpret_cast<V4T<T>*>(po + o) + lane, pn, nt);
@@ -349,11 +245,7 @@ __device__ inline void tile2(const UniformGeo<T>& g, const T* __restrict__ p, co
                 }
             }
             P0 = Pp1;
-            Pp1 = Qa;
-            if constexpr (ADJ) {
-                RA0 = RAp1;
-                RAp1 = cur.a;
-            }
+            Pp1 = cur.a;
             Um1 = U0;
             U0 = Up1;
             Q2 = Q1;
@@ -370,8 +262,8 @@ __device__ inline void tile2(const UniformGeo<T>& g, const T* __restrict__ p, co
 
 // Pressure update of a lane's four cells when a wave holds several rows: the
 // z walls are a row's first and last lane (zlo, zhi), first / last are the
-// lane's own y walls (ADJ: the transposed step's dt * S + lam_p).
-template <class T, bool PW, bool ADJ = false>
+// lane's own y walls.
+template <class T, bool PW>
 __device__ inline V4T<T> pres4m(const V4T<T>& P, const V4T<T>& Um, const V4T<T>& U, const V4T<T>& Up, bool xlo,
                                 bool xhi, const V4T<T>& vn, const V4T<T>& vyl, const V4T<T>& vyh, bool first, bool last,
                                 const V4T<T>& wn, bool zlo, bool zhi, T Ax, T Ay, T Az, const VolDiv<T, PW>& Vd, T K,
@@ -390,8 +282,7 @@ __device__ inline V4T<T> pres4m(const V4T<T>& P, const V4T<T>& Um, const V4T<T>&
         T S = mggp(fx_lo, fx_hi, Ax, Vd);
         S = S + mggp(fy_lo, fy_hi, Ay, Vd);
         S = S + mggp(fz_lo, fz_hi, Az, Vd);
-        if constexpr (ADJ) pn[i] = dt * S + P[i];
-        else pn[i] = dt * (K * S) + P[i];
+        pn[i] = dt * (K * S) + P[i];
     }
     return pn;
 }
@@ -404,21 +295,11 @@ __device__ inline V4T<T> pres4m(const V4T<T>& P, const V4T<T>& Um, const V4T<T>&
 // output rows, which alone are stored.  The x pipeline (first step at plane x,
 // second at x-2, register queues) and the four LDS planes are those of tile2,
 // with row j of the band at element j*Z.
-//
-// MODE 1 also records p' at the sensor cells after each of the two steps
-// (aux->trace[0], [1]); MODE 2 runs two transposed steps (the single-step
-// transposed row tile's arithmetic, row.hpp): the pressure slots of the
-// stencil hold q = dK * lam_p while the pressure update adds dt * S to lam_p
-// itself (kept beside q in RA0 / RAp1 and the raw p' queue), and
-// aux->src[0], [1] are added to nu_p at the sensors after each step, in every
-// lane that computes the cell, so the second step reads the sum.
-template <class T, int NW, bool PW, bool XB, bool NTL, int MODE = 0>
+template <class T, int NW, bool PW, bool XB, bool NTL>
 __device__ inline void tile2m(const UniformGeo<T>& g, const T* __restrict__ p, const T* __restrict__ u,
                               const T* __restrict__ v, const T* __restrict__ w, T* __restrict__ po, T* __restrict__ uo,
                               T* __restrict__ vo, T* __restrict__ wo, T K, T dt, int64_t x0, int64_t x1, int y0, int wv,
-                              bool nt, V4T<T> (*sP1)[64], V4T<T> (*sV1)[64], V4T<T> (*sP2)[64], V4T<T> (*sV2)[64],
-                              const SensorList* sens = nullptr, const Aux2<T>* aux = nullptr) {
-    constexpr bool ADJ = MODE == 2;
+                              bool nt, V4T<T> (*sP1)[64], V4T<T> (*sV1)[64], V4T<T> (*sP2)[64], V4T<T> (*sV2)[64]) {
     const int64_t X = g.X;
     const int Y = int(g.Y), Z = int(g.Z);
     const int64_t SX = int64_t(Y) * Z;
@@ -466,85 +347,14 @@ __device__ inline void tile2m(const UniformGeo<T>& g, const T* __restrict__ p, c
         f.c = ld(v, pb + rb);
         f.d = ld(w, pb + rb);
     };
-    // MODE 2: a pressure value as the stencil's q = dK * lam_p
-    T dK = T(0);
-    if constexpr (ADJ) dK = aux->dK;
-    auto qv = [&](const V4T<T>& a) -> V4T<T> {
-        if constexpr (ADJ) {
-            V4T<T> r;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) r[i] = dK * a[i];
-            return r;
-        } else {
-            return a;
-        }
-    };
-    auto vel = [&](const V4T<T>& lo, const V4T<T>& c, const V4T<T>& hi, const V4T<T>& q, T A) -> V4T<T> {
-        if constexpr (ADJ) return vel4t(lo, c, hi, q, A, Vd);
-        else return vel4(lo, c, hi, q, A, Vd, dt);
-    };
 
     const int64_t xs = x0 >= 2 ? x0 - 2 : 0;
     const int64_t xe = x1 + 2;
     const int64_t xs1 = xe < X ? xe : X;
 
-    // MODE 1/2: the sensors of the wave's rows (bit j = sensor j;
-    // wave-uniform): lane j loads sensor j's x, y, z and trace gradients here
-    // (x = -1 past the list) and, once the first plane's loads are in flight,
-    // a ballot gathers the tests; the planes read a sensor's values from its
-    // lane.  m1: sensors the first step touches (MODE 1: recorded from the output
-    // rows and planes [x0, x1) only; MODE 2: added on every plane of the first
-    // step, [xs, xs1)); m2: the second step's, output rows and planes [x0, x1).
-    uint64_t m1 = 0, m2 = 0;
-    int sxl = -1, syl = -1, szl = 0;
-    T src0 = T(0), src1 = T(0);
-    bool w0 = false, w1 = false;
-    if constexpr (MODE != 0) {
-        w0 = ADJ ? aux->src[0] != nullptr : aux->trace[0] != nullptr;
-        w1 = ADJ ? aux->src[1] != nullptr : aux->trace[1] != nullptr;
-        if ((w0 || w1) && lane < sens->n) {
-            sxl = sens->x[lane];
-            syl = sens->y[lane];
-            szl = sens->z[lane];
-            if constexpr (ADJ) {
-                if (w0) src0 = aux->src[0][lane];
-                if (w1) src1 = aux->src[1][lane];
-            }
-        }
-    }
-    auto sensor_masks = [&]() {
-        if constexpr (MODE != 0) {
-            const int wy0 = y0 - 4 + wv * RW;
-            const int oy0 = wy0 > y0 ? wy0 : y0, oy1 = (wy0 + RW < y0 + TR) ? wy0 + RW : y0 + TR;
-            const bool in_out = syl >= oy0 && syl < oy1 && sxl >= x0 && sxl < x1;
-            m1 = __ballot(w0 && (ADJ ? (syl >= wy0 && syl < wy0 + RW && sxl >= xs && sxl < xs1) : in_out));
-            m2 = __ballot(w1 && in_out);
-        }
-    };
-    // the sensors of plane xx in a lane's four cells: record (MODE 1) or add
-    // (MODE 2) entry k of the aux arrays
-    auto at_sensors = [&](uint64_t mask, int64_t xx, int k, bool mine, V4T<T>& val) {
-        if constexpr (MODE != 0) {
-            while (mask) {  // lane values read with the whole wave active, as the row tile's
-                const int j = __builtin_ctzll(mask);
-                mask &= mask - 1;
-                const int sx = row::lane_value(sxl, j), sy = row::lane_value(syl, j), zz = row::lane_value(szl, j);
-                T sv = T(0);
-                if constexpr (ADJ) sv = row::lane_value(k ? src1 : src0, j);
-                if (sx != xx) continue;
-                if (mine && y == sy && zz >= lp.zc && zz < lp.zc + 4) {
-                    const int i = zz - lp.zc;
-                    if constexpr (ADJ) val[i] = val[i] + sv;
-                    else aux->trace[k][j] = val[i];
-                }
-            }
-        }
-    };
-
     // warm-up: every load of it and of the first plane issued before any is
     // used, as tile2's
     V4T<T> P0 = zero4, Pp1 = zero4, Um1 = zero4, U0 = zero4;
-    V4T<T> RA0 = zero4, RAp1 = zero4;  // MODE 2: lam_p(x), lam_p(x+1) as loaded (P0, Pp1 hold q)
     V4T<T> a0 = zero4, am1 = zero4, ap1 = zero4, am2 = zero4, um = zero4, uc = zero4;
     if (real) {
         const int64_t xm1 = xs >= 1 ? xs - 1 : xs, xm2 = xs >= 2 ? xs - 2 : xm1;
@@ -560,26 +370,17 @@ __device__ inline void tile2m(const UniformGeo<T>& g, const T* __restrict__ p, c
     fetch(xs, nx);
     if (real) {
         P0 = a0;
-        V4T<T> Pm1 = xs >= 1 ? am1 : zero4;
+        const V4T<T> Pm1 = xs >= 1 ? am1 : zero4;
         Pp1 = xs + 1 < X ? ap1 : zero4;
-        if constexpr (ADJ) {
-            RA0 = P0;
-            RAp1 = Pp1;
-            P0 = qv(P0);
-            Pp1 = qv(Pp1);
-            Pm1 = qv(Pm1);
-        }
         if (xs >= 1) {
-            const V4T<T> Pm2 = (xs >= 2) ? qv(am2) : Pm1;
-            Um1 = vel(Pm2, Pm1, P0, um, Ax);
+            const V4T<T> Pm2 = (xs >= 2) ? am2 : Pm1;
+            Um1 = vel4(Pm2, Pm1, P0, um, Ax, Vd, dt);
         }
         V4T<T> lo = P0, hi = P0;
         if (xs >= 1) lo = Pm1;
         if (xs + 1 < X) hi = Pp1;
-        U0 = vel(lo, P0, hi, uc, Ax);
+        U0 = vel4(lo, P0, hi, uc, Ax, Vd, dt);
     }
-    // second-step queues: p'(x-2), p'(x-1) (as computed: lam_p' in MODE 2);
-    // v', w' of x-2, x-1; u''(x-3), u''(x-2)
     V4T<T> Q2 = zero4, Q1 = zero4, Va2 = zero4, Va1 = zero4, Wa2 = zero4, Wa1 = zero4, R3 = zero4, R2 = zero4;
 
     for (int64_t x = xs; x < xe; ++x) {
@@ -590,21 +391,18 @@ __device__ inline void tile2m(const UniformGeo<T>& g, const T* __restrict__ p, c
         const bool su = m >= x0 - 1 && m >= 0 && m < X;
         const Fetch cur = nx;
         if (x + 1 < xs1) fetch(x + 1, nx);
-        if (x == xs) sensor_masks();
-        const V4T<T> Qa = qv(cur.a);   // p(x+2), as q in MODE 2
-        const V4T<T> qQ2 = qv(Q2);     // p'(x-2), as q in MODE 2
 
         // A. u'(x+1); p(x) and p'(x-2) into LDS
         V4T<T> Up1 = zero4;
         if (live) {
             if (s1) {
                 if (!XB || x + 1 < X) {
-                    const V4T<T> hi = (!XB || x + 2 < X) ? Qa : Pp1;
-                    Up1 = vel(P0, Pp1, hi, cur.b, Ax);
+                    const V4T<T> hi = (!XB || x + 2 < X) ? cur.a : Pp1;
+                    Up1 = vel4(P0, Pp1, hi, cur.b, Ax, Vd, dt);
                 }
                 if (lp.span) P1[own] = P0;
             }
-            if (s2 && lp.span) P2[own] = qQ2;
+            if (s2 && lp.span) P2[own] = Q2;
         }
         __syncthreads();
 
@@ -614,15 +412,15 @@ __device__ inline void tile2m(const UniformGeo<T>& g, const T* __restrict__ p, c
             if (s1) {
                 V4T<T> pzl, pzh;
                 row::z_nbrs_row(P0, zlo, zhi, pzl, pzh);
-                vn = vel(P1[ilo], P0, P1[ihi], cur.c, Ay);
-                wn = vel(pzl, P0, pzh, cur.d, Az);
+                vn = vel4(P1[ilo], P0, P1[ihi], cur.c, Ay, Vd, dt);
+                wn = vel4(pzl, P0, pzh, cur.d, Az, Vd, dt);
                 if (lp.span) V1[own] = vn;
             }
             if (s2) {
                 V4T<T> qzl, qzh;
-                row::z_nbrs_row(qQ2, zlo, zhi, qzl, qzh);
-                v2 = vel(P2[ilo], qQ2, P2[ihi], Va2, Ay);
-                w2 = vel(qzl, qQ2, qzh, Wa2, Az);
+                row::z_nbrs_row(Q2, zlo, zhi, qzl, qzh);
+                v2 = vel4(P2[ilo], Q2, P2[ihi], Va2, Ay, Vd, dt);
+                w2 = vel4(qzl, Q2, qzh, Wa2, Az, Vd, dt);
                 if (lp.span) V2[own] = v2;
             }
         }
@@ -631,22 +429,18 @@ __device__ inline void tile2m(const UniformGeo<T>& g, const T* __restrict__ p, c
         // C. p'(x); u''(x-1); p''(x-2) and the outputs
         if (live) {
             V4T<T> Q0 = zero4;
-            if (s1) {
-                Q0 = pres4m<T, PW, ADJ>(ADJ ? RA0 : P0, Um1, U0, Up1, !XB || x > 0, !XB || x < X - 1, vn, V1[ilo],
-                                        V1[ihi], first, last, wn, zlo, zhi, Ax, Ay, Az, Vd, K, dt);
-                at_sensors(m1, x, 0, ADJ ? real : out, Q0);
-            }
+            if (s1)
+                Q0 = pres4m(P0, Um1, U0, Up1, !XB || x > 0, !XB || x < X - 1, vn, V1[ilo], V1[ihi], first, last, wn,
+                            zlo, zhi, Ax, Ay, Az, Vd, K, dt);
             V4T<T> R1 = zero4;
             if (su) {
-                const V4T<T> qQ1 = qv(Q1);
-                const V4T<T> lo = (!XB || m >= 1) ? qQ2 : qQ1;
-                const V4T<T> hi = (!XB || x < X) ? qv(Q0) : qQ1;
-                R1 = vel(lo, qQ1, hi, Um1, Ax);
+                const V4T<T> lo = (!XB || m >= 1) ? Q2 : Q1;
+                const V4T<T> hi = (!XB || x < X) ? Q0 : Q1;
+                R1 = vel4(lo, Q1, hi, Um1, Ax, Vd, dt);
             }
             if (s2) {
-                V4T<T> pn = pres4m<T, PW, ADJ>(Q2, R3, R2, R1, !XB || xp > 0, !XB || xp < X - 1, v2, V2[ilo],
-                                               V2[ihi], first, last, w2, zlo, zhi, Ax, Ay, Az, Vd, K, dt);
-                at_sensors(m2, xp, 1, out, pn);
+                const V4T<T> pn = pres4m(Q2, R3, R2, R1, !XB || xp > 0, !XB || xp < X - 1, v2, V2[ilo], V2[ihi], first,
+                                     last, w2, zlo, zhi, Ax, Ay, Az, Vd, K, dt);
                 if (out) {
                     const int64_t o = xp * SX + rb;
                     row::put(reinterpret_cast<V4T<T>*>(po + o), pn, nt);
@@ -656,11 +450,7 @@ __device__ inline void tile2m(const UniformGeo<T>& g, const T* __restrict__ p, c
                 }
             }
             P0 = Pp1;
-            Pp1 = Qa;
-            if constexpr (ADJ) {
-                RA0 = RAp1;
-                RAp1 = cur.a;
-            }
+            Pp1 = cur.a;
             Um1 = U0;
             U0 = Up1;
             Q2 = Q1;

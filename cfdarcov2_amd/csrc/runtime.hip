@@ -1101,17 +1101,13 @@ int cfd_mesh_set_adjoint_shape(cfd_mesh_t m, int form, int tile_rows, int chunk)
     return guarded([&] {
         CFD_REQUIRE(m != nullptr, "null mesh");
         require_lattice(m, "cfd_mesh_set_adjoint_shape");
-        CFD_REQUIRE(form >= 0 && form <= 2, "adjoint form must be 0, 1 or 2");
+        CFD_REQUIRE(form == 0 || form == 1, "adjoint form must be 0 or 1");
         CFD_REQUIRE(chunk >= 1 && chunk <= 65535, "adjoint chunk must be in [1, 65535]");
-        if (form >= 1 && !(m->uniform && wave_row_supported(m->Z)))
+        if (form == 1 && !(m->uniform && wave_row_supported(m->Z)))
             throw Error(CFD_E_UNSUPPORTED,
-                        "the adjoint row forms need an exactly uniform mesh whose z extent is a multiple of 4 and "
+                        "the adjoint row form needs an exactly uniform mesh whose z extent is a multiple of 4 and "
                         "at most 256");
-        if (form == 2) {
-            if (m->dtype != CFD_F32) throw Error(CFD_E_UNSUPPORTED, "the two-step adjoint form is FP32 only");
-            CFD_REQUIRE(tile_rows == 4 || (m->Z <= 128 && (tile_rows == 8 || tile_rows == 10)),
-                        "two-step adjoint tiles have 4 rows, or 4, 8 or 10 rows for z extents up to 128");
-        } else if (m->dtype == CFD_F32) {
+        if (m->dtype == CFD_F32) {
             CFD_REQUIRE(tile_rows == 4 || tile_rows == 8, "adjoint row tiles have 4 or 8 output waves");
         } else {
             // the FP64 8-row builds spill registers (see launch_wave_row_aux)

@@ -324,12 +324,7 @@ int cfd_heat_adjoint(cfd_mesh_t mesh, void* adj_a, void* adj_b, double alpha, co
  * fused step itself; its transposed step needs a constant K (runs with a K
  * field take form 0 for the transposed steps) and its K gradient keeps one
  * divergence field per recomputed step instead of three velocity fields.
- * Form 2 (FP32 meshes the row form takes, else CFD_E_UNSUPPORTED) advances two
- * steps per launch, forward (constant K) and transposed (no K gradient), in
- * runs of four steps; `tile_rows` names its two-step tile (4; or 8 or 10 for
- * z extents up to 128) and the other steps take the row form with 8 output
- * waves (tile_rows 8 or 10) or 4.  All forms give bit-identical traces and
- * gradients.  Uniform meshes that allow it start on form 1 (4 waves, 8
+ * Both forms give bit-identical traces and gradients.  Uniform meshes that allow it start on form 1 (4 waves, 8
  * planes). */
 int cfd_mesh_set_adjoint_shape(cfd_mesh_t mesh, int form, int tile_rows, int chunk);
 

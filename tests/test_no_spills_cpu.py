@@ -21,7 +21,7 @@ import pytest
 from cfdarcov2_amd import build
 
 CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cfdarcov2_amd", "csrc")
-CROSS_LANE_UNITS = ["adj_row.hip", "adj_row2.hip", "adjoint.hip", "batch.hip", "euler.hip", "ewise.hip",
+CROSS_LANE_UNITS = ["adj_row.hip", "adjoint.hip", "batch.hip", "euler.hip", "ewise.hip",
                     "heat_multi.hip", "heat_row.hip", "wave_row.hip", "wave_row2.hip"]
 
 
